@@ -1,0 +1,140 @@
+// Host side of the GPU path tracer: plugin loading and the progressive batch loop, restating
+// the reference GpuPathIntegrator (src/integrators/gpu_path.cpp:38-101, 458-513) for Linux.
+#include "prender_host.h"
+#include "scene_build.h"
+
+#include <dlfcn.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+
+namespace {
+
+void SetErr(char *err, int len, const std::string &m) {
+    if (err && len > 0) std::snprintf(err, size_t(len), "%s", m.c_str());
+}
+
+struct Plugin {
+    void *handle = nullptr;
+    int (*version)() = nullptr;
+    int (*listDevices)(prgpu::DeviceInfo *, int) = nullptr;
+    int (*render)(int, const prgpu::SceneDesc *, const prgpu::RenderParams *, float *, char *, int) = nullptr;
+    int (*extVersion)() = nullptr;
+    int (*renderEx)(int, const prgpu::SceneDescEx *, const prgpu::RenderParamsEx *, double *, double *, char *, int) =
+        nullptr;
+};
+
+struct GeneratedScene {
+    phost::SceneBuilder sb;
+};
+
+}  // namespace
+
+PH_API void *ph_generate(int config, int width, int height, float detail, char *err, int errLen) {
+    auto gs = std::make_unique<GeneratedScene>();
+    phost::GenOptions o;
+    o.width = width;
+    o.height = height;
+    o.detail = detail > 0 ? detail : 1.f;
+    std::string e;
+    if (!phost::GenerateConfig(config, o, &gs->sb, &e)) {
+        SetErr(err, errLen, e);
+        return nullptr;
+    }
+    gs->sb.Desc();
+    return gs.release();
+}
+
+PH_API void ph_scene_free(void *s) { delete static_cast<GeneratedScene *>(s); }
+
+PH_API const prgpu::SceneDescEx *ph_scene_desc(void *s) { return &static_cast<GeneratedScene *>(s)->sb.Desc(); }
+
+PH_API void ph_scene_defaults(void *s, int32_t *out, uint64_t *seed) {
+    const phost::SceneBuilder &sb = static_cast<GeneratedScene *>(s)->sb;
+    out[0] = sb.spp;
+    out[1] = sb.maxDepth;
+    out[2] = sb.rrStartDepth;
+    *seed = sb.seed;
+}
+
+PH_API void *ph_plugin_open(const char *path, char *err, int errLen) {
+    auto p = std::make_unique<Plugin>();
+    p->handle = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+    if (!p->handle) {
+        const char *e = dlerror();
+        SetErr(err, errLen, std::string("cannot load GPU plugin: ") + (e ? e : path));
+        return nullptr;
+    }
+    p->version = reinterpret_cast<int (*)()>(dlsym(p->handle, "prgpu_api_version"));
+    p->listDevices = reinterpret_cast<decltype(p->listDevices)>(dlsym(p->handle, "prgpu_list_devices"));
+    p->render = reinterpret_cast<decltype(p->render)>(dlsym(p->handle, "prgpu_render"));
+    if (!p->version || !p->listDevices || !p->render || p->version() != prgpu::kApiVersion) {
+        SetErr(err, errLen, "GPU plugin has an incompatible interface");
+        dlclose(p->handle);
+        return nullptr;
+    }
+    p->extVersion = reinterpret_cast<int (*)()>(dlsym(p->handle, "prgpu_api_ext_version"));
+    if (p->extVersion && p->extVersion() == prgpu::kExtVersion)
+        p->renderEx = reinterpret_cast<decltype(p->renderEx)>(dlsym(p->handle, "prgpu_render_ex"));
+    return p.release();
+}
+
+PH_API void ph_plugin_close(void *pl) {
+    auto *p = static_cast<Plugin *>(pl);
+    if (!p) return;
+    if (p->handle) dlclose(p->handle);
+    delete p;
+}
+
+PH_API int ph_plugin_has_ext(void *pl) { return static_cast<Plugin *>(pl)->renderEx ? 1 : 0; }
+
+PH_API int ph_plugin_list_devices(void *pl, prgpu::DeviceInfo *out, int max) {
+    return static_cast<Plugin *>(pl)->listDevices(out, max);
+}
+
+PH_API int ph_render(void *pl, int device, const prgpu::SceneDescEx *scene, const prgpu::RenderParamsEx *base,
+                     int targetSpp, double timeLimit, double *sum, double *sumsq, int32_t *sppDone, double *seconds,
+                     char *err, int errLen) {
+    auto *p = static_cast<Plugin *>(pl);
+    if (!p || !p->renderEx) {
+        SetErr(err, errLen, "GPU plugin lacks the extension interface");
+        return 1;
+    }
+    const int target = timeLimit > 0 ? std::numeric_limits<int>::max() : std::max(1, targetSpp);
+    int done = 0, batch = 1;
+    char msg[1024] = {0};
+    auto start = std::chrono::steady_clock::now();
+    while (done < target) {
+        prgpu::RenderParamsEx rp = *base;
+        rp.spp = std::min(batch, target - done);
+        rp.firstSample = base->firstSample + done;
+        auto t0 = std::chrono::steady_clock::now();
+        if (p->renderEx(device, scene, &rp, sum, sumsq, msg, sizeof(msg)) != 0) {
+            SetErr(err, errLen, std::string("GPU render failed: ") + msg);
+            return 1;
+        }
+        double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        done += rp.spp;
+        if (dt < 0.15) batch = std::min(batch * 2, 256);
+        else if (dt > 0.5 && batch > 1) batch /= 2;
+        double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+        if (timeLimit > 0 && elapsed >= timeLimit) break;
+    }
+    if (sppDone) *sppDone = done;
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+    return 0;
+}
+
+PH_API void ph_resolve_srgb(const double *xyz, int64_t pixels, double scale, float *rgb) {
+    for (int64_t i = 0; i < pixels; ++i) phost::XYZToLinearSRGB(xyz + 3 * i, scale, rgb + 3 * i);
+}
+
+PH_API void ph_fit_sigmoid(const float *rgb, float *coeffs) { phost::FitRGBSigmoid(rgb, coeffs); }
+
+PH_API float ph_spectrum_value(const prgpu::SpectrumEx *s, const float *knots, float lambda) {
+    return phost::SpectrumValue(*s, knots, lambda);
+}

@@ -1,0 +1,139 @@
+// Host-side scene assembly for the prender GPU plugin: a flat-array builder that produces the
+// plugin's SceneDescEx, the spectral helpers a scene needs (blackbodies normalised to unit
+// luminance, named metals and glasses, RGB reflectance uplift), procedural meshes, and the
+// seeded generators of the benchmark scenes.
+//
+// This is the part of the reference that builds pr::Scene before GpuPathIntegrator exports it
+// (src/scene/scene_loader.cpp, src/geometry/shape.cpp:302-353, src/core/noise.cpp); it feeds the
+// plugin through the C ABI in include/prender_gpu.h.
+#pragma once
+
+#include "prender_gpu.h"
+
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace phost {
+
+struct F3 {
+    float x = 0, y = 0, z = 0;
+};
+inline F3 f3(float x, float y, float z) { return {x, y, z}; }
+inline F3 operator+(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline F3 operator-(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline F3 operator*(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline F3 norm3(F3 v) {
+    float l = std::sqrt(dot3(v, v));
+    float inv = 1.f / l;
+    return {v.x * inv, v.y * inv, v.z * inv};
+}
+inline F3 crossd(F3 a, F3 b) {
+    double ax = a.x, ay = a.y, az = a.z, bx = b.x, by = b.y, bz = b.z;
+    return {float(ay * bz - az * by), float(az * bx - ax * bz), float(ax * by - ay * bx)};
+}
+inline prgpu::Float3 pf(F3 v) { return {v.x, v.y, v.z}; }
+
+// PCG32 seeded like the reference's RNG(seqIndex) (src/core/rng.h:19): scene generators draw
+// their random parameters from RNG(config number).
+struct SceneRng {
+    uint64_t state = 0, inc = 1;
+    explicit SceneRng(uint64_t seq);
+    uint32_t next();
+    float uniform();
+    float uniform(float a, float b) { return a + (b - a) * uniform(); }
+};
+
+struct Mesh {
+    std::vector<F3> p, n;
+    std::vector<float> uv;  // 2 per vertex
+    std::vector<int> idx;
+};
+
+Mesh MakeUVSphere(F3 c, float r, int segments, int rings);
+Mesh MakeIcosphere(F3 c, float r, int subdivisions);
+Mesh MakeQuad(F3 corner, F3 edgeU, F3 edgeV);  // normal = normalize(edgeU x edgeV) ... see .cpp
+Mesh MakeBox(F3 center, F3 size);
+
+// Improved Perlin noise and fBm (src/core/noise.cpp:36-60).
+float PerlinNoise(F3 p);
+float FBm(F3 p, int octaves, float lacunarity = 2.f, float gain = 0.5f);
+
+// Camera-to-world for a camera at pos looking at target, +z forward (src/core/math.cpp:87-105).
+void LookAt(F3 pos, F3 target, F3 up, float out12[12]);
+
+class SceneBuilder {
+  public:
+    // ---- spectra ----
+    int Constant(float v);
+    int Sigmoid(float c0, float c1, float c2, float scale = 1.f);
+    int RGBAlbedo(float r, float g, float b);     // smooth reflectance spectrum for an sRGB albedo
+    int RGBUnbounded(float r, float g, float b);  // unbounded positive spectrum (scattering coefficients)
+    int BlackbodyUnitY(float T);                  // Planck at T normalised to luminance Y = 1
+    int Metal(const std::string &name, bool k);   // Au, Ag, Cu, Al: n (k = false) or k
+    int Glass(const std::string &name);           // Sellmeier IOR: BK7, SF11, fused_silica, diamond, sapphire
+    int Sellmeier(const float *B, const float *C, int n);
+    int Piecewise(const std::vector<float> &lambdas, const std::vector<float> &values, float scale = 1.f);
+
+    // ---- materials ----
+    int Diffuse(int reflectance);
+    int Conductor(int eta, int k, float alphaX, float alphaY);
+    int ConductorReflectance(int reflectance, float alphaX, float alphaY);
+    int Dielectric(int eta, float alphaX, float alphaY, bool thin = false);
+    int Interface();
+
+    // ---- geometry ----
+    int AddMesh(const Mesh &m, int material, int mediumInside = -1, int mediumOutside = -1, bool normals = true);
+    int AddSphere(F3 c, float r, int material, int mediumInside = -1, int mediumOutside = -1);
+    // Every triangle of the mesh becomes its own area light (src/scene/scene_loader.cpp:785-801).
+    void AddEmitter(const Mesh &m, int spectrum, float scale, bool twoSided = false);
+
+    // ---- lights, media, camera ----
+    int AddPointLight(F3 p, int spectrum, float scale);
+    int AddDistantLight(F3 towardsLight, int spectrum, float scale);
+    int AddEnvironment(int spectrum, float scale);
+    int AddHomogeneous(int sigmaA, int sigmaS, float scale, float g);
+    int AddGrid(int sigmaA, int sigmaS, float scale, float g, int nx, int ny, int nz, const std::vector<float> &data,
+                F3 boundsMin, F3 boundsMax);
+    void SetCamera(F3 pos, F3 target, float fovYDeg, int width, int height, float lensRadius = 0, float focalDistance = 0);
+    void SetFilter(prgpu::FilterType t, float radius);
+
+    const prgpu::SceneDescEx &Desc();
+    float SpectrumY(int id) const;
+
+    std::vector<prgpu::Primitive> prims;
+    std::vector<prgpu::Material> materials;
+    std::vector<prgpu::SpectrumEx> spectra;
+    std::vector<float> knots;
+    std::vector<prgpu::LightEx> lights;
+    std::vector<prgpu::MediumEx> media;
+    std::vector<float> grid;
+    prgpu::CameraEx camera{};
+    int cameraMedium = -1;
+
+    // Render defaults that come with a generated scene.
+    int spp = 16, maxDepth = -1, rrStartDepth = 1;
+    uint64_t seed = 1;
+
+  private:
+    int AddSpectrum(const prgpu::SpectrumEx &s);
+    prgpu::SceneDescEx desc_{};
+};
+
+float SpectrumValue(const prgpu::SpectrumEx &s, const float *knots, float lambda);
+// Fits sigmoid-polynomial coefficients (c0 l^2 + c1 l + c2 inside S) reproducing an sRGB
+// reflectance (Jakob & Hanika 2019, as src/core/rgb2spec.cpp:35-165 fits its table nodes).
+void FitRGBSigmoid(const float rgb[3], float coeffs[3]);
+// Linear sRGB of a film XYZ value, with the renderer's spectral white adaptation (src/core/film.cpp:170-175).
+void XYZToLinearSRGB(const double xyz[3], double scale, float rgb[3]);
+
+struct GenOptions {
+    int width = 0, height = 0;  // 0: the configuration's resolution
+    float detail = 1.f;         // geometry density factor (1 = full configuration)
+};
+// Builds benchmark configuration 1..5. Returns false for unknown configurations.
+bool GenerateConfig(int config, const GenOptions &opt, SceneBuilder *sb, std::string *err);
+
+}  // namespace phost

@@ -1,0 +1,272 @@
+// C-ABI entry points of the MI355X plugin (include/prender_gpu.h), device management and the
+// ray-batch query kernel. Every entry point takes the one global lock, never throws across the
+// ABI and reports failures as a nonzero return with a message (reference
+// src/gpu/prender_gpu.cpp:1576-1730 for the contract).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "device.h"
+#include "pt_host.h"
+#include "trace_dev.h"
+
+namespace pgpu {
+
+namespace {
+
+std::mutex gMutex;
+std::vector<std::unique_ptr<Device>> gDevices;
+bool gEnumerated = false;
+
+void SetErr(char *err, int len, const std::string &m) {
+    if (err && len > 0) std::snprintf(err, size_t(len), "%s", m.c_str());
+}
+
+void Enumerate() {
+    if (gEnumerated) return;
+    gEnumerated = true;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    for (int i = 0; i < n; ++i) {
+        auto d = std::make_unique<Device>();
+        d->id = i;
+        gDevices.push_back(std::move(d));
+    }
+}
+
+Device &Get(int index) {
+    if (index < 0 || index >= int(gDevices.size())) throw std::runtime_error("no such GPU device");
+    Device &d = *gDevices[size_t(index)];
+    PG_CHECK(hipSetDevice(d.id));
+    if (!d.stream) {
+        PG_CHECK(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
+        hipDeviceProp_t prop;
+        PG_CHECK(hipGetDeviceProperties(&prop, d.id));
+        d.cus = prop.multiProcessorCount;
+        d.flags.ensure(64);
+        PG_CHECK(hipMemset(d.flags.p, 0, 64));
+    }
+    return d;
+}
+
+constexpr int kTraceBlocksPerCU = 6;  // LDS: kStackLds * 256 * 4 B = 24 KiB per block
+
+__global__ __launch_bounds__(kTraceBlock) void k_trace_rays(TraceCtx c, const prgpu::RayEx *rays, prgpu::HitEx *hits,
+                                                            int64_t n, int anyHit) {
+    __shared__ uint32_t lds[kStackLds * kTraceBlock];
+    const int tid = int(blockIdx.x * blockDim.x + threadIdx.x);
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t i = tid; i < n; i += stride) {
+        const prgpu::RayEx rr = rays[i];
+        DRay r{rr.o.x, rr.o.y, rr.o.z, rr.d.x, rr.d.y, rr.d.z};
+        DHit h;
+        TraceStats st;
+        bool hit = anyHit ? dev_trace<true, false>(c, r, rr.tMax, &h, lds, tid, &st)
+                          : dev_trace<false, false>(c, r, rr.tMax, &h, lds, tid, &st);
+        prgpu::HitEx o;
+        o.t = hit ? h.t : 0.f;
+        o.prim = hit ? h.prim : -1;
+        o.b1 = hit ? h.b1 : 0.f;
+        o.b2 = hit ? h.b2 : 0.f;
+        hits[i] = o;
+    }
+}
+
+}  // namespace
+
+// Content fingerprint of the geometry: counts, pointers and a strided sample of the primitives.
+// The reference keys its upload cache on the host pointer and counts only; a strided content
+// sample additionally catches buffers that are reused with new contents.
+uint64_t GeometryKey(const prgpu::Primitive *prims, int count) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void *p, size_t n) {
+        const unsigned char *b = static_cast<const unsigned char *>(p);
+        for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    };
+    mix(&count, sizeof(count));
+    uintptr_t ptr = uintptr_t(prims);
+    mix(&ptr, sizeof(ptr));
+    if (count > 0) {
+        int step = std::max(1, count / 4096);
+        for (int i = 0; i < count; i += step) mix(&prims[i], sizeof(prgpu::Primitive));
+        mix(&prims[count - 1], sizeof(prgpu::Primitive));
+    }
+    return h;
+}
+
+void EnsureGeometry(Device &d, const prgpu::Primitive *prims, int count) {
+    uint64_t key = GeometryKey(prims, count);
+    if (key == d.geomKey && d.nodes.p) return;
+    WideBVH bvh;
+    BuildWideBVH(prims, count, &bvh);
+    if (bvh.nodes.empty()) {  // empty scene: a root with no children
+        bvh.nodes.emplace_back();
+        std::memset(&bvh.nodes[0], 0, sizeof(WNode));
+        bvh.depth = 1;
+    }
+    d.nodes.upload(bvh.nodes.data(), bvh.nodes.size(), d.stream);
+    d.recs.upload(bvh.recs.data(), std::max<size_t>(1, bvh.recs.size()), d.stream);
+    d.depth = bvh.depth;
+    d.nodeCount = int(bvh.nodes.size());
+    d.recCount = int(bvh.recs.size());
+    for (int a = 0; a < 3; ++a) {
+        d.sceneLo[a] = bvh.lo[a];
+        d.sceneHi[a] = bvh.hi[a];
+    }
+    // At most 7 entries per level are pending on a traversal stack.
+    int need = 7 * bvh.depth + 8;
+    d.spillCap = std::max(0, need - kStackLds);
+    d.spillStride = d.cus * kTraceBlocksPerCU * kTraceBlock;
+    if (d.spillCap > 0) d.spill.ensure(size_t(d.spillCap) * size_t(d.spillStride) * sizeof(uint32_t));
+    PG_CHECK(hipStreamSynchronize(d.stream));
+    d.geomKey = key;
+}
+
+TraceCtx MakeTraceCtx(Device &d) {
+    TraceCtx c;
+    c.nodes = d.nodes.as<WNode>();
+    c.recs = d.recs.as<TriRec>();
+    c.spill = d.spill.as<uint32_t>();
+    c.spillStride = d.spillStride;
+    c.spillCap = d.spillCap;
+    c.errorFlag = d.flags.as<int>();
+    return c;
+}
+
+int TraceGrid(const Device &d, int64_t n) {
+    int64_t blocks = (n + kTraceBlock - 1) / kTraceBlock;
+    return int(std::max<int64_t>(1, std::min<int64_t>(blocks, int64_t(d.cus) * kTraceBlocksPerCU)));
+}
+
+void CheckTraceFlags(Device &d) {
+    int f = 0;
+    PG_CHECK(hipMemcpyAsync(&f, d.flags.p, sizeof(int), hipMemcpyDeviceToHost, d.stream));
+    PG_CHECK(hipStreamSynchronize(d.stream));
+    if (f) {
+        PG_CHECK(hipMemsetAsync(d.flags.p, 0, sizeof(int), d.stream));
+        throw std::runtime_error("BVH traversal stack overflow");
+    }
+}
+
+}  // namespace pgpu
+
+using namespace pgpu;
+
+PRGPU_API int prgpu_api_version() { return prgpu::kApiVersion; }
+PRGPU_API int prgpu_api_ext_version() { return prgpu::kExtVersion; }
+
+PRGPU_API int prgpu_list_devices(prgpu::DeviceInfo *out, int max) {
+    std::lock_guard<std::mutex> lock(gMutex);
+    Enumerate();
+    int n = 0;
+    for (const auto &d : gDevices) {
+        if (n >= max) break;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, d->id) != hipSuccess) continue;
+        prgpu::DeviceInfo &di = out[n++];
+        std::memset(&di, 0, sizeof(di));
+        std::snprintf(di.name, sizeof(di.name), "%s (%s)", prop.name, prop.gcnArchName);
+        di.globalMemory = uint64_t(prop.totalGlobalMem);
+        di.computeUnits = prop.multiProcessorCount;
+        di.isGpu = 1;
+        std::snprintf(di.backend, sizeof(di.backend), "hip");
+    }
+    return n;
+}
+
+PRGPU_API int prgpu_trace_rays(int deviceIndex, const prgpu::SceneDescEx *sd, const prgpu::RayEx *rays, int64_t n,
+                               int anyHit, prgpu::HitEx *hits, char *err, int errLen) {
+    std::lock_guard<std::mutex> lock(gMutex);
+    try {
+        Enumerate();
+        if (!sd || sd->apiVersion != prgpu::kApiVersion || sd->extVersion != prgpu::kExtVersion)
+            throw std::runtime_error("API version mismatch");
+        if (n < 0 || (n > 0 && (!rays || !hits))) throw std::runtime_error("invalid ray batch");
+        Device &d = Get(deviceIndex);
+        EnsureGeometry(d, sd->prims, sd->primCount);
+        if (n == 0) return 0;
+        d.rays.upload(rays, size_t(n), d.stream);
+        d.hits.ensure(size_t(n) * sizeof(prgpu::HitEx));
+        hipLaunchKernelGGL(k_trace_rays, dim3(TraceGrid(d, n)), dim3(kTraceBlock), 0, d.stream, MakeTraceCtx(d),
+                           d.rays.as<prgpu::RayEx>(), d.hits.as<prgpu::HitEx>(), n, anyHit);
+        PG_CHECK(hipGetLastError());
+        PG_CHECK(hipMemcpyAsync(hits, d.hits.p, size_t(n) * sizeof(prgpu::HitEx), hipMemcpyDeviceToHost, d.stream));
+        CheckTraceFlags(d);
+        return 0;
+    } catch (const std::exception &e) {
+        SetErr(err, errLen, e.what());
+        return 1;
+    }
+}
+
+PRGPU_API int prgpu_render_ex(int deviceIndex, const prgpu::SceneDescEx *sd, const prgpu::RenderParamsEx *rp,
+                              double *sum, double *sumsq, char *err, int errLen) {
+    std::lock_guard<std::mutex> lock(gMutex);
+    try {
+        Enumerate();
+        if (!sd || sd->apiVersion != prgpu::kApiVersion || sd->extVersion != prgpu::kExtVersion)
+            throw std::runtime_error("API version mismatch");
+        if (!rp || !sum) throw std::runtime_error("invalid arguments");
+        Device &d = Get(deviceIndex);
+        EnsureGeometry(d, sd->prims, sd->primCount);
+        RenderExImpl(d, *sd, *rp, sum, sumsq);
+        return 0;
+    } catch (const std::exception &e) {
+        SetErr(err, errLen, e.what());
+        return 1;
+    }
+}
+
+PRGPU_API int prgpu_sample_radiance(int deviceIndex, const prgpu::SceneDescEx *sd, const prgpu::RenderParamsEx *rp,
+                                    prgpu::SampleRecordEx *out, char *err, int errLen) {
+    std::lock_guard<std::mutex> lock(gMutex);
+    try {
+        Enumerate();
+        if (!sd || sd->apiVersion != prgpu::kApiVersion || sd->extVersion != prgpu::kExtVersion)
+            throw std::runtime_error("API version mismatch");
+        if (!rp || !out) throw std::runtime_error("invalid arguments");
+        Device &d = Get(deviceIndex);
+        EnsureGeometry(d, sd->prims, sd->primCount);
+        SampleRadianceImpl(d, *sd, *rp, out);
+        return 0;
+    } catch (const std::exception &e) {
+        SetErr(err, errLen, e.what());
+        return 1;
+    }
+}
+
+PRGPU_API int prgpu_render(int deviceIndex, const prgpu::SceneDesc *sd, const prgpu::RenderParams *rp, float *outXYZ,
+                           char *err, int errLen) {
+    std::lock_guard<std::mutex> lock(gMutex);
+    try {
+        Enumerate();
+        if (deviceIndex < 0 || deviceIndex >= int(gDevices.size())) throw std::runtime_error("no such GPU device");
+        if (!sd || sd->apiVersion != prgpu::kApiVersion) throw std::runtime_error("API version mismatch");
+        if (!rp || !outXYZ) throw std::runtime_error("invalid arguments");
+        Device &d = Get(deviceIndex);
+        RenderV4Impl(d, *sd, *rp, outXYZ);
+        return 0;
+    } catch (const std::exception &e) {
+        SetErr(err, errLen, e.what());
+        return 1;
+    }
+}
+
+PRGPU_API int prgpu_release(int deviceIndex) {
+    std::lock_guard<std::mutex> lock(gMutex);
+    if (deviceIndex < 0 || deviceIndex >= int(gDevices.size())) return 1;
+    Device &d = *gDevices[size_t(deviceIndex)];
+    if (hipSetDevice(d.id) != hipSuccess) return 1;
+    if (d.stream) (void)hipStreamSynchronize(d.stream);
+    ReleasePT(d);
+    d.nodes.release();
+    d.recs.release();
+    d.spill.release();
+    d.rays.release();
+    d.hits.release();
+    d.geomKey = d.ptKey = 0;
+    return 0;
+}

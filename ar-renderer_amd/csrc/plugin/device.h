@@ -1,0 +1,74 @@
+// Per-device state of the plugin: HIP stream, the scene resident in HBM (rebuilt when the
+// host's scene changes) and reusable scratch buffers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "prender_gpu.h"
+#include "wbvh.h"
+
+namespace pgpu {
+
+#define PG_CHECK(x)                                                                                 \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    // Grows to at least n bytes (contents are not preserved).
+    void ensure(size_t n) {
+        if (n <= bytes && p) return;
+        release();
+        PG_CHECK(hipMalloc(&p, n ? n : 16));
+        bytes = n ? n : 16;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+    template <class T> void upload(const T *src, size_t count, hipStream_t s) {
+        ensure(count * sizeof(T));
+        if (count) PG_CHECK(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+struct PTScene;  // path-tracing data (pt_scene.h)
+
+struct Device {
+    int id = 0;
+    int cus = 0;
+    hipStream_t stream = nullptr;
+    // Geometry
+    uint64_t geomKey = 0;
+    DevBuf nodes, recs;
+    int depth = 0;
+    int nodeCount = 0, recCount = 0;
+    float sceneLo[3] = {0, 0, 0}, sceneHi[3] = {0, 0, 0};
+    // Traversal scratch
+    DevBuf spill;
+    int spillCap = 0, spillStride = 0;
+    DevBuf flags;  // [0] traversal stack overflow
+    // Path tracing
+    uint64_t ptKey = 0;
+    std::unique_ptr<PTScene> pt;
+    DevBuf rays, hits;
+    Device();
+    ~Device();
+};
+
+}  // namespace pgpu

@@ -1,0 +1,221 @@
+// Device traversal of the 8-wide quantised BVH (gfx950, wave64).
+//
+// One ray per lane. A lane's traversal stack lives in LDS, interleaved by lane so that the 64
+// lanes of a wave touch 64 consecutive banks; entries beyond the LDS part spill to a per-thread
+// HBM slab (sized by the host from the tree depth: at most 7 entries per level are pending).
+// Children that a ray's slab test hits are ordered near-to-far with an 8-key sorting network
+// (key = distance bits | slot), the nearest is taken next and the rest are pushed far-to-near.
+//
+// The triangle test is the reference's Moller-Trumbore in float (src/geometry/shape.cpp:202-222)
+// operation for operation; the file is compiled without FMA contraction so that the hit
+// distances, and therefore the closest-hit primitive ids, are bit-identical to the CPU's.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "wbvh.h"
+
+namespace pgpu {
+
+constexpr int kTraceBlock = 256;
+constexpr int kStackLds = 24;  // LDS entries per lane
+
+struct DRay {
+    float ox, oy, oz, dx, dy, dz;
+};
+
+struct DHit {
+    float t;
+    int prim;  // original primitive index, -1 = miss
+    float b1, b2;
+};
+
+__device__ __forceinline__ float dev_inv(float d) {
+    // 1/d, with +-FLT_MAX instead of +-inf so that 0 * inv never produces a NaN in the slab test.
+    float r = 1.f / d;
+    return __builtin_isinf(r) ? __builtin_copysignf(3.402823466e38f, d) : r;
+}
+
+// Reference Moller-Trumbore, no contraction. Returns true and updates t/b1/b2 on a closer hit.
+__device__ __forceinline__ bool dev_hit_triangle(const DRay &r, float p0x, float p0y, float p0z, float e1x, float e1y,
+                                                 float e1z, float e2x, float e2y, float e2z, float tMax, float *tOut,
+                                                 float *uOut, float *vOut) {
+    // pvec = cross(d, e2)
+    float px = r.dy * e2z - r.dz * e2y;
+    float py = r.dz * e2x - r.dx * e2z;
+    float pz = r.dx * e2y - r.dy * e2x;
+    float det = e1x * px + e1y * py + e1z * pz;
+    if (det == 0.f || !__builtin_isfinite(det)) return false;
+    float inv = 1.f / det;
+    float tx = r.ox - p0x, ty = r.oy - p0y, tz = r.oz - p0z;
+    float u = (tx * px + ty * py + tz * pz) * inv;
+    if (u < 0.f || u > 1.f) return false;
+    // qvec = cross(tvec, e1)
+    float qx = ty * e1z - tz * e1y;
+    float qy = tz * e1x - tx * e1z;
+    float qz = tx * e1y - ty * e1x;
+    float v = (r.dx * qx + r.dy * qy + r.dz * qz) * inv;
+    if (v < 0.f || u + v > 1.f) return false;
+    float t = (e2x * qx + e2y * qy + e2z * qz) * inv;
+    if (t <= 0.f || t >= tMax) return false;
+    *tOut = t;
+    *uOut = u;
+    *vOut = v;
+    return true;
+}
+
+// Robust quadratic in double precision (src/geometry/shape.cpp:42-61).
+__device__ __forceinline__ bool dev_hit_sphere(const DRay &r, float cx, float cy, float cz, float rad, float tMax,
+                                               float *tOut) {
+    double ox = r.ox - cx, oy = r.oy - cy, oz = r.oz - cz;
+    double dx = r.dx, dy = r.dy, dz = r.dz;
+    double a = dx * dx + dy * dy + dz * dz;
+    double b = ox * dx + oy * dy + oz * dz;
+    double c = ox * ox + oy * oy + oz * oz - double(rad) * rad;
+    double k = b / a;
+    double lx = ox - k * dx, ly = oy - k * dy, lz = oz - k * dz;
+    double disc = double(rad) * rad - (lx * lx + ly * ly + lz * lz);
+    if (disc < 0) return false;
+    double q = -b - __builtin_copysign(__builtin_sqrt(a * disc), b);
+    double t0 = c / q, t1 = q / a;
+    if (t0 > t1) {
+        double s = t0;
+        t0 = t1;
+        t1 = s;
+    }
+    double t = t0;
+    if (t <= 0) t = t1;
+    if (t <= 0 || t >= tMax) return false;
+    *tOut = float(t);
+    return true;
+}
+
+__device__ __forceinline__ void cswap(uint32_t &a, uint32_t &b) {
+    uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    a = lo;
+    b = hi;
+}
+
+// Optimal 19-comparator network for 8 keys.
+__device__ __forceinline__ void sort8(uint32_t k[8]) {
+    cswap(k[0], k[2]); cswap(k[1], k[3]); cswap(k[4], k[6]); cswap(k[5], k[7]);
+    cswap(k[0], k[4]); cswap(k[1], k[5]); cswap(k[2], k[6]); cswap(k[3], k[7]);
+    cswap(k[0], k[1]); cswap(k[2], k[3]); cswap(k[4], k[5]); cswap(k[6], k[7]);
+    cswap(k[2], k[4]); cswap(k[3], k[5]);
+    cswap(k[1], k[4]); cswap(k[3], k[6]);
+    cswap(k[1], k[2]); cswap(k[3], k[4]); cswap(k[5], k[6]);
+}
+
+__device__ __forceinline__ float pow2f(int e) { return __int_as_float((e + 127) << 23); }
+
+struct TraceCtx {
+    const WNode *nodes;
+    const TriRec *recs;
+    uint32_t *spill;       // per-thread spill slab: entry k at spill[k * spillStride + tid]
+    int spillStride;       // number of threads sharing the slab layout
+    int spillCap;          // entries available per thread in the slab
+    int *errorFlag;        // set to 1 if a traversal stack would overflow its slab
+};
+
+// Counters for the algorithmic-traffic figures (nodes fetched, leaf records tested).
+struct TraceStats {
+    uint32_t nodes = 0, recs = 0;
+};
+
+// Closest (kAny = false) or any (kAny = true) hit along r in (0, tMax). `lds` is this block's
+// stack area (kStackLds * blockDim.x entries), `tid` the global thread index for the spill slab.
+template <bool kAny, bool kStats>
+__device__ __forceinline__ bool dev_trace(const TraceCtx &c, const DRay &r, float tMax, DHit *hit, uint32_t *lds,
+                                          int tid, TraceStats *st) {
+    const float ix = dev_inv(r.dx), iy = dev_inv(r.dy), iz = dev_inv(r.dz);
+    const uint32_t lane = threadIdx.x;
+    const uint32_t bd = blockDim.x;
+    int sp = 0;
+    uint32_t cur = 0;  // root node
+    bool found = false;
+    hit->prim = -1;
+    while (true) {
+        if (!(cur & kLeafBit)) {
+            const uint4 *np = reinterpret_cast<const uint4 *>(c.nodes + cur);
+            uint4 h0 = np[0], c0 = np[1], c1 = np[2], q0 = np[3], q1 = np[4], q2 = np[5];
+            if (kStats) st->nodes++;
+            float ox = __uint_as_float(h0.x), oy = __uint_as_float(h0.y), oz = __uint_as_float(h0.z);
+            int ex = int(int8_t(h0.w & 0xff)), ey = int(int8_t((h0.w >> 8) & 0xff)), ez = int(int8_t((h0.w >> 16) & 0xff));
+            int nch = int(h0.w >> 24);
+            float sx = pow2f(ex), sy = pow2f(ey), sz = pow2f(ez);
+            // Per-axis origin terms shared by all children.
+            float bx = (ox - r.ox), by = (oy - r.oy), bz = (oz - r.oz);
+            uint32_t child[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+            uint32_t qlx[2] = {q0.x, q0.y}, qly[2] = {q0.z, q0.w}, qlz[2] = {q1.x, q1.y};
+            uint32_t qhx[2] = {q1.z, q1.w}, qhy[2] = {q2.x, q2.y}, qhz[2] = {q2.z, q2.w};
+            uint32_t key[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int w = i >> 2, sh = (i & 3) * 8;
+                float lx = float((qlx[w] >> sh) & 0xff) * sx + bx, hx = float((qhx[w] >> sh) & 0xff) * sx + bx;
+                float ly = float((qly[w] >> sh) & 0xff) * sy + by, hy = float((qhy[w] >> sh) & 0xff) * sy + by;
+                float lz = float((qlz[w] >> sh) & 0xff) * sz + bz, hz = float((qhz[w] >> sh) & 0xff) * sz + bz;
+                float t0x = lx * ix, t1x = hx * ix, t0y = ly * iy, t1y = hy * iy, t0z = lz * iz, t1z = hz * iz;
+                float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+                float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+                // Conservative widening as the reference slab test (bvh.cpp:137-156), on both ends.
+                tf = tf * (1.f + 2.f * 1.2e-7f * 3.f);
+                tn = tn - fabsf(tn) * (2.f * 1.2e-7f * 3.f);
+                bool h = (i < nch) && tn <= tf && tn < tMax && tf >= 0.f;
+                key[i] = h ? ((__float_as_uint(fmaxf(tn, 0.f)) & ~7u) | uint32_t(i)) : 0xffffffffu;
+            }
+            sort8(key);
+            if (key[0] == 0xffffffffu) {
+                if (sp == 0) break;
+                --sp;
+                cur = sp < kStackLds ? lds[sp * bd + lane] : c.spill[size_t(sp - kStackLds) * c.spillStride + tid];
+                continue;
+            }
+            // Push hit children far to near; continue with the nearest.
+#pragma unroll
+            for (int i = 7; i >= 1; --i) {
+                if (key[i] != 0xffffffffu) {
+                    uint32_t ref = child[key[i] & 7u];
+                    if (sp < kStackLds) {
+                        lds[sp * bd + lane] = ref;
+                    } else if (sp - kStackLds < c.spillCap) {
+                        c.spill[size_t(sp - kStackLds) * c.spillStride + tid] = ref;
+                    } else {
+                        *c.errorFlag = 1;
+                        --sp;  // keep the stack in bounds; the error flag reports the lost subtree
+                    }
+                    ++sp;
+                }
+            }
+            cur = child[key[0] & 7u];
+            continue;
+        }
+        // Leaf: test its records.
+        const uint32_t first = cur & 0x07ffffffu, cnt = ((cur >> 27) & 7u) + 1u;
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const float4 *rp = reinterpret_cast<const float4 *>(c.recs + first + k);
+            float4 a = rp[0], b = rp[1], d = rp[2];
+            if (kStats) st->recs++;
+            uint32_t prim = __float_as_uint(d.y);
+            float t, u = 0.f, v = 0.f;
+            bool h;
+            if (prim & kSphereBit) h = dev_hit_sphere(r, a.x, a.y, a.z, a.w, tMax, &t);
+            else h = dev_hit_triangle(r, a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, d.x, tMax, &t, &u, &v);
+            if (h) {
+                found = true;
+                tMax = t;
+                hit->t = t;
+                hit->prim = int(prim & ~kSphereBit);
+                hit->b1 = u;
+                hit->b2 = v;
+                if (kAny) return true;
+            }
+        }
+        if (sp == 0) break;
+        --sp;
+        cur = sp < kStackLds ? lds[sp * bd + lane] : c.spill[size_t(sp - kStackLds) * c.spillStride + tid];
+    }
+    return found;
+}
+
+}  // namespace pgpu

@@ -1,0 +1,320 @@
+"""Python binding of the MI355X prender plugin and its host library.
+
+The plugin (libprender_gpu.so) exposes the reference's v4 C ABI (src/gpu/gpu_api.h) plus the
+extension declared in include/prender_gpu.h; the host library (libprender_host.so) generates
+the benchmark scenes and runs the progressive loop of the reference's GpuPathIntegrator
+(src/integrators/gpu_path.cpp). This module mirrors those C structs with ctypes / numpy dtypes
+and wraps the entry points. Loading fails loudly when a library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PLUGIN_PATH = os.path.join(HERE, "libprender_gpu.so")
+HOST_PATH = os.path.join(HERE, "libprender_host.so")
+
+API_VERSION = 4
+EXT_VERSION = 1
+
+# ---- enums (include/prender_gpu.h) ---------------------------------------------------------------
+PRIM_TRIANGLE, PRIM_SPHERE = 0, 1
+MAT_DIFFUSE, MAT_CONDUCTOR, MAT_DIELECTRIC, MAT_BLACK, MAT_INTERFACE = 0, 1, 2, 3, 4
+SPEC_CONSTANT, SPEC_SIGMOID, SPEC_BLACKBODY, SPEC_PIECEWISE, SPEC_SELLMEIER, SPEC_CAUCHY, SPEC_TABLE95 = range(7)
+LIGHT_AREA, LIGHT_POINT, LIGHT_DISTANT, LIGHT_ENV = 0, 1, 2, 3
+MEDIUM_HOMOGENEOUS, MEDIUM_GRID = 0, 1
+FILTER_BOX, FILTER_GAUSSIAN, FILTER_BH = 0, 1, 2
+LIGHT_SAMPLER_BVH, LIGHT_SAMPLER_POWER = 0, 1
+
+# ---- numpy layouts of the POD arrays -------------------------------------------------------------
+PRIM_DTYPE = np.dtype({
+    "names": ["type", "material", "light", "mediumInside", "mediumOutside", "pad", "p0", "p1", "p2",
+              "n0", "n1", "n2", "uv", "r", "flip"],
+    "formats": ["<u4", "<i4", "<i4", "<i4", "<i4", "<u4", ("<f4", 3), ("<f4", 3), ("<f4", 3),
+                ("<f4", 3), ("<f4", 3), ("<f4", 3), ("<f4", 6), "<f4", "<u4"],
+    "offsets": [0, 4, 8, 12, 16, 20, 24, 36, 48, 60, 72, 84, 96, 120, 124],
+    "itemsize": 128})
+MATERIAL_DTYPE = np.dtype({
+    "names": ["type", "albedo", "albedoB", "checkerScale", "eta", "k", "alphaX", "alphaY", "dispersive", "thin",
+              "coatEta", "coatAlpha", "thickness", "coatG", "coatAlbedo", "coatDispersive", "coatMaxDepth",
+              "coatSamples", "sheenColor", "sheenRoughness", "sheenWeight", "pad"],
+    "formats": ["<u4", "<i4", "<i4", "<f4", "<i4", "<i4", "<f4", "<f4", "<u4", "<u4", "<i4", "<f4", "<f4", "<f4",
+                "<i4", "<u4", "<i4", "<i4", "<i4", "<f4", "<f4", "<u4"],
+    "itemsize": 88})
+SPECTRUM_DTYPE = np.dtype({"names": ["kind", "scale", "offset", "count", "p"],
+                           "formats": ["<u4", "<f4", "<i4", "<i4", ("<f4", 8)], "itemsize": 48})
+LIGHT_DTYPE = np.dtype({"names": ["kind", "primitive", "spectrum", "scale", "position", "direction", "twoSided",
+                                  "cosPower"],
+                        "formats": ["<u4", "<i4", "<i4", "<f4", ("<f4", 3), ("<f4", 3), "<u4", "<f4"],
+                        "itemsize": 48})
+MEDIUM_DTYPE = np.dtype({"names": ["kind", "sigmaA", "sigmaS", "scale", "g", "nx", "ny", "nz", "gridOffset",
+                                   "maxDensity", "boundsMin", "boundsMax", "worldToMedium"],
+                         "formats": ["<u4", "<i4", "<i4", "<f4", "<f4", "<i4", "<i4", "<i4", "<i8", "<f4",
+                                     ("<f4", 3), ("<f4", 3), ("<f4", 12)],
+                         "offsets": [0, 4, 8, 12, 16, 20, 24, 28, 32, 40, 44, 56, 68], "itemsize": 120})
+RAY_DTYPE = np.dtype({"names": ["o", "d", "tMax"], "formats": [("<f4", 3), ("<f4", 3), "<f4"], "itemsize": 28})
+HIT_DTYPE = np.dtype({"names": ["t", "prim", "b1", "b2"], "formats": ["<f4", "<i4", "<f4", "<f4"], "itemsize": 16})
+SAMPLE_DTYPE = np.dtype({"names": ["L", "lam", "pdf", "pFilm", "weight", "flags"],
+                         "formats": [("<f4", 4), ("<f4", 4), ("<f4", 4), ("<f4", 2), "<f4", "<u4"],
+                         "itemsize": 64})
+
+
+class Float3(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+
+class CameraEx(C.Structure):
+    _fields_ = [("cameraToWorld", C.c_float * 12), ("tanHalfFov", C.c_float), ("lensRadius", C.c_float),
+                ("focalDistance", C.c_float), ("width", C.c_int32), ("height", C.c_int32),
+                ("filterType", C.c_int32), ("filterRadius", C.c_float), ("pad", C.c_int32)]
+
+
+class SceneDescEx(C.Structure):
+    _fields_ = [("apiVersion", C.c_int32), ("extVersion", C.c_int32),
+                ("prims", C.c_void_p), ("primCount", C.c_int32),
+                ("materials", C.c_void_p), ("materialCount", C.c_int32),
+                ("spectra", C.c_void_p), ("spectrumCount", C.c_int32),
+                ("knots", C.c_void_p), ("knotCount", C.c_int32),
+                ("lights", C.c_void_p), ("lightCount", C.c_int32),
+                ("media", C.c_void_p), ("mediumCount", C.c_int32),
+                ("gridData", C.c_void_p), ("gridCount", C.c_int64),
+                ("cameraMedium", C.c_int32), ("pad", C.c_int32),
+                ("camera", CameraEx)]
+
+
+class RenderParamsEx(C.Structure):
+    _fields_ = [("spp", C.c_int32), ("firstSample", C.c_int32), ("seed", C.c_uint64), ("maxDepth", C.c_int32),
+                ("clampLuminance", C.c_float), ("rrStartDepth", C.c_int32), ("russianRoulette", C.c_int32),
+                ("lightSampler", C.c_int32), ("flags", C.c_int32)]
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 256), ("globalMemory", C.c_uint64), ("computeUnits", C.c_int32),
+                ("isGpu", C.c_int32), ("backend", C.c_char * 32)]
+
+
+assert C.sizeof(SceneDescEx) == 208 and C.sizeof(RenderParamsEx) == 40 and C.sizeof(DeviceInfo) == 304
+
+
+def render_params(spp=1, first_sample=0, seed=1, max_depth=-1, rr_start=1, rr=True, light_sampler=LIGHT_SAMPLER_BVH,
+                  clamp=0.0):
+    return RenderParamsEx(spp, first_sample, seed, max_depth, clamp, rr_start, 1 if rr else 0, light_sampler, 0)
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None and len(a) else None
+
+
+class Scene:
+    """A flat scene: numpy arrays laid out as the C structs, plus the SceneDescEx pointing at them.
+
+    Either filled by the host library's generators (``generate``) or assembled in Python for
+    small test scenes. The arrays must stay alive while the descriptor is in use.
+    """
+
+    def __init__(self, prims, materials, spectra, knots, lights, media=None, grid=None, camera=None,
+                 camera_medium=-1, defaults=None, handle=None, host=None):
+        self.prims = np.ascontiguousarray(prims, dtype=PRIM_DTYPE)
+        self.materials = np.ascontiguousarray(materials, dtype=MATERIAL_DTYPE)
+        self.spectra = np.ascontiguousarray(spectra, dtype=SPECTRUM_DTYPE)
+        self.knots = np.ascontiguousarray(knots if knots is not None else np.zeros(0), dtype=np.float32)
+        self.lights = np.ascontiguousarray(lights if lights is not None else np.zeros(0, LIGHT_DTYPE),
+                                           dtype=LIGHT_DTYPE)
+        self.media = np.ascontiguousarray(media if media is not None else np.zeros(0, MEDIUM_DTYPE),
+                                          dtype=MEDIUM_DTYPE)
+        self.grid = np.ascontiguousarray(grid if grid is not None else np.zeros(0), dtype=np.float32)
+        self.camera = camera if camera is not None else CameraEx()
+        self.defaults = defaults or {"spp": 16, "maxDepth": -1, "rrStartDepth": 1, "seed": 1}
+        self._handle = handle
+        self._host = host
+        self.desc = SceneDescEx()
+        d = self.desc
+        d.apiVersion, d.extVersion = API_VERSION, EXT_VERSION
+        d.prims, d.primCount = _ptr(self.prims), len(self.prims)
+        d.materials, d.materialCount = _ptr(self.materials), len(self.materials)
+        d.spectra, d.spectrumCount = _ptr(self.spectra), len(self.spectra)
+        d.knots, d.knotCount = _ptr(self.knots), len(self.knots)
+        d.lights, d.lightCount = _ptr(self.lights), len(self.lights)
+        d.media, d.mediumCount = _ptr(self.media), len(self.media)
+        d.gridData, d.gridCount = _ptr(self.grid), len(self.grid)
+        d.cameraMedium = camera_medium
+        d.camera = self.camera
+
+    @property
+    def width(self):
+        return self.camera.width
+
+    @property
+    def height(self):
+        return self.camera.height
+
+    def margin(self):
+        r = self.camera.filterRadius if self.camera.filterType != FILTER_BOX else 0.5
+        return int(np.ceil(r - 0.5))
+
+    def desc_ptr(self):
+        return C.byref(self.desc)
+
+    def __del__(self):
+        if self._handle is not None and self._host is not None:
+            self._host.ph_scene_free(self._handle)
+            self._handle = None
+
+
+def _load(path, what):
+    if not os.path.exists(path):
+        raise RuntimeError(f"{what} is not built: {path} is missing (run __graft_entry__.build())")
+    return C.CDLL(path)
+
+
+class Host:
+    """libprender_host.so: scene generation, plugin loading, progressive render."""
+
+    def __init__(self, path=HOST_PATH):
+        L = self.lib = _load(path, "the prender host library")
+        L.ph_generate.restype = C.c_void_p
+        L.ph_generate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_char_p, C.c_int]
+        L.ph_scene_free.argtypes = [C.c_void_p]
+        L.ph_scene_desc.restype = C.POINTER(SceneDescEx)
+        L.ph_scene_desc.argtypes = [C.c_void_p]
+        L.ph_scene_defaults.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_uint64)]
+        L.ph_plugin_open.restype = C.c_void_p
+        L.ph_plugin_open.argtypes = [C.c_char_p, C.c_char_p, C.c_int]
+        L.ph_plugin_close.argtypes = [C.c_void_p]
+        L.ph_plugin_has_ext.argtypes = [C.c_void_p]
+        L.ph_plugin_list_devices.argtypes = [C.c_void_p, C.POINTER(DeviceInfo), C.c_int]
+        L.ph_render.argtypes = [C.c_void_p, C.c_int, C.POINTER(SceneDescEx), C.POINTER(RenderParamsEx), C.c_int,
+                                C.c_double, C.c_void_p, C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                C.c_char_p, C.c_int]
+        L.ph_resolve_srgb.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.c_void_p]
+        L.ph_fit_sigmoid.argtypes = [C.c_void_p, C.c_void_p]
+        L.ph_spectrum_value.restype = C.c_float
+        L.ph_spectrum_value.argtypes = [C.c_void_p, C.c_void_p, C.c_float]
+
+    def generate(self, config, width=0, height=0, detail=1.0):
+        err = C.create_string_buffer(512)
+        h = self.lib.ph_generate(config, width, height, detail, err, 512)
+        if not h:
+            raise RuntimeError(err.value.decode())
+        d = self.lib.ph_scene_desc(h).contents
+
+        def arr(ptr, n, dt):
+            if n == 0:
+                return np.zeros(0, dt)
+            buf = (C.c_char * (n * dt.itemsize)).from_address(ptr)
+            return np.frombuffer(buf, dtype=dt).copy()
+
+        vals = (C.c_int32 * 3)()
+        seed = C.c_uint64()
+        self.lib.ph_scene_defaults(h, vals, C.byref(seed))
+        defaults = {"spp": vals[0], "maxDepth": vals[1], "rrStartDepth": vals[2], "seed": seed.value}
+        cam = CameraEx.from_buffer_copy(d.camera)
+        s = Scene(arr(d.prims, d.primCount, PRIM_DTYPE), arr(d.materials, d.materialCount, MATERIAL_DTYPE),
+                  arr(d.spectra, d.spectrumCount, SPECTRUM_DTYPE), arr(d.knots, d.knotCount, np.dtype("<f4")),
+                  arr(d.lights, d.lightCount, LIGHT_DTYPE), arr(d.media, d.mediumCount, MEDIUM_DTYPE),
+                  arr(d.gridData, d.gridCount, np.dtype("<f4")), cam, d.cameraMedium, defaults)
+        self.lib.ph_scene_free(h)
+        return s
+
+    def resolve_srgb(self, xyz, scale):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        out = np.zeros(xyz.shape, np.float32)
+        self.lib.ph_resolve_srgb(xyz.ctypes.data, xyz.size // 3, scale, out.ctypes.data)
+        return out
+
+    def fit_sigmoid(self, rgb):
+        rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+        out = np.zeros(3, np.float32)
+        self.lib.ph_fit_sigmoid(rgb.ctypes.data, out.ctypes.data)
+        return out
+
+    def spectrum_value(self, spec_row, knots, lam):
+        s = np.ascontiguousarray(np.array([spec_row], dtype=SPECTRUM_DTYPE))
+        k = np.ascontiguousarray(knots, dtype=np.float32)
+        return self.lib.ph_spectrum_value(s.ctypes.data, k.ctypes.data if len(k) else None, lam)
+
+    def open_plugin(self, path=PLUGIN_PATH):
+        err = C.create_string_buffer(512)
+        h = self.lib.ph_plugin_open(path.encode(), err, 512)
+        if not h:
+            raise RuntimeError(err.value.decode())
+        return h
+
+    def render(self, plugin, scene, params, spp, device=0, time_limit=0.0, variance=True):
+        n = scene.width * scene.height * 3
+        s = np.zeros(n, np.float64)
+        sq = np.zeros(n, np.float64) if variance else None
+        done = C.c_int32()
+        secs = C.c_double()
+        err = C.create_string_buffer(1024)
+        rc = self.lib.ph_render(plugin, device, scene.desc_ptr(), C.byref(params), spp, time_limit, s.ctypes.data,
+                                sq.ctypes.data if sq is not None else None, C.byref(done), C.byref(secs), err, 1024)
+        if rc != 0:
+            raise RuntimeError(err.value.decode())
+        return s.reshape(scene.height, scene.width, 3), (sq.reshape(scene.height, scene.width, 3)
+                                                          if sq is not None else None), done.value, secs.value
+
+
+class Plugin:
+    """Direct binding of libprender_gpu.so's C ABI."""
+
+    def __init__(self, path=PLUGIN_PATH):
+        L = self.lib = _load(path, "the MI355X plugin")
+        L.prgpu_api_version.restype = C.c_int
+        L.prgpu_api_ext_version.restype = C.c_int
+        L.prgpu_list_devices.argtypes = [C.POINTER(DeviceInfo), C.c_int]
+        L.prgpu_trace_rays.argtypes = [C.c_int, C.POINTER(SceneDescEx), C.c_void_p, C.c_int64, C.c_int, C.c_void_p,
+                                       C.c_char_p, C.c_int]
+        L.prgpu_render_ex.argtypes = [C.c_int, C.POINTER(SceneDescEx), C.POINTER(RenderParamsEx), C.c_void_p,
+                                      C.c_void_p, C.c_char_p, C.c_int]
+        L.prgpu_sample_radiance.argtypes = [C.c_int, C.POINTER(SceneDescEx), C.POINTER(RenderParamsEx), C.c_void_p,
+                                            C.c_char_p, C.c_int]
+        L.prgpu_render.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_int]
+        L.prgpu_release.argtypes = [C.c_int]
+
+    def version(self):
+        return self.lib.prgpu_api_version(), self.lib.prgpu_api_ext_version()
+
+    def devices(self):
+        arr = (DeviceInfo * 16)()
+        n = self.lib.prgpu_list_devices(arr, 16)
+        return [arr[i] for i in range(n)]
+
+    @staticmethod
+    def _check(rc, err):
+        if rc != 0:
+            raise RuntimeError(err.value.decode())
+
+    def trace(self, scene, rays, any_hit=False, device=0):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        err = C.create_string_buffer(1024)
+        self._check(self.lib.prgpu_trace_rays(device, scene.desc_ptr(), rays.ctypes.data if len(rays) else None,
+                                              len(rays), 1 if any_hit else 0,
+                                              hits.ctypes.data if len(hits) else None, err, 1024), err)
+        return hits
+
+    def render_ex(self, scene, params, device=0, variance=True):
+        n = scene.width * scene.height * 3
+        s = np.zeros(n, np.float64)
+        sq = np.zeros(n, np.float64) if variance else None
+        err = C.create_string_buffer(1024)
+        self._check(self.lib.prgpu_render_ex(device, scene.desc_ptr(), C.byref(params), s.ctypes.data,
+                                             sq.ctypes.data if sq is not None else None, err, 1024), err)
+        shape = (scene.height, scene.width, 3)
+        return s.reshape(shape), (sq.reshape(shape) if sq is not None else None)
+
+    def sample_radiance(self, scene, params, device=0):
+        m = scene.margin()
+        n = (scene.width + 2 * m) * (scene.height + 2 * m)
+        out = np.zeros(n, SAMPLE_DTYPE)
+        err = C.create_string_buffer(1024)
+        self._check(self.lib.prgpu_sample_radiance(device, scene.desc_ptr(), C.byref(params), out.ctypes.data, err,
+                                                   1024), err)
+        return out
+
+    def release(self, device=0):
+        return self.lib.prgpu_release(device)

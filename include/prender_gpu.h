@@ -1,0 +1,344 @@
+#pragma once
+// C ABI of the prender GPU plugin (libprender_gpu.so), MI355X / gfx950 implementation.
+//
+// Part 1 is the version-4 interface that PRender's host loads for `--device gpu`
+// (reference: src/gpu/gpu_api.h). Every struct keeps the reference's exact LP64 layout so
+// an unmodified v4 host can pass its buffers straight through; the layout is pinned by the
+// static_asserts at the end of this file.
+//
+// Part 2 is an extension, discovered with dlsym("prgpu_api_ext_version"), that carries what
+// v4 cannot express: analytic spectra (exact per-wavelength evaluation instead of 5 nm
+// tables), distant lights, grid media, the light-BVH sampler, the Russian-roulette start
+// depth, FP64 sum and sum-of-squares film output, a ray-batch query used by the closest-hit
+// parity tests, and a per-sample radiance query used by the same-RNG-stream parity tests.
+// The v4 entry points are unchanged by it.
+
+#include <stdint.h>
+
+#define PRGPU_API extern "C" __attribute__((visibility("default")))
+
+namespace prgpu {
+
+// ---------------------------------------------------------------------------------------------
+// Part 1: v4 interface (reference src/gpu/gpu_api.h:15-173).
+
+constexpr int kApiVersion = 4;             // gpu_api.h:17
+constexpr int kSpectrumSamples = 95;       // gpu_api.h:20 (360..830 nm every 5 nm)
+constexpr float kSpectrumMin = 360.f, kSpectrumStep = 5.f;
+constexpr int kFilterBins = 64;            // gpu_api.h:23
+constexpr int kSheenMu = 32, kSheenAlpha = 16;  // gpu_api.h:25
+
+struct Float3 {
+    float x, y, z;
+};
+
+struct Spectrum {  // gpu_api.h:31-33: samples at kSpectrumMin + kSpectrumStep * i, linear interpolation
+    float v[kSpectrumSamples];
+};
+
+struct BVHNode {  // gpu_api.h:35-40: the host's binary BVH, depth-first order
+    Float3 bmin, bmax;
+    int32_t offset;   // leaf: first slot in primOrder; interior: index of the second child
+    uint16_t nPrims;  // 0 for interior nodes
+    uint8_t axis, pad;
+};
+
+enum PrimType : uint32_t { PrimTriangle = 0, PrimSphere = 1 };
+
+struct Primitive {  // gpu_api.h:44-56
+    uint32_t type;
+    int32_t material;                     // index into materials
+    int32_t light;                        // index into lights for area emitters, else -1
+    int32_t mediumInside, mediumOutside;  // -1 = vacuum
+    uint32_t pad;
+    Float3 p0, p1, p2;                    // triangle vertices; sphere: p0 = centre
+    Float3 n0, n1, n2;                    // per-vertex normals, all zero = flat
+    float u0, v0, u1, v1, u2, v2;         // texture coordinates
+    float r;                              // sphere radius
+    uint32_t flip;                        // sphere: normals point inwards
+};
+
+enum MaterialType : uint32_t {
+    MatDiffuse = 0, MatConductor = 1, MatDielectric = 2, MatBlack = 3, MatInterface = 4,
+    MatCoatedDiffuse = 5, MatCoatedConductor = 6
+};
+
+struct Material {  // gpu_api.h:61-81
+    uint32_t type;
+    int32_t albedo;        // spectrum: diffuse reflectance / conductor artist reflectance, -1 none
+    int32_t albedoB;       // second checker spectrum, -1 none
+    float checkerScale;    // checker frequency in uv, 0 = no checker
+    int32_t eta, k;        // spectra: conductor n and k, dielectric IOR
+    float alphaX, alphaY;  // GGX alpha = roughness^2; max < 1e-3 is smooth
+    uint32_t dispersive;   // IOR varies with wavelength (secondary wavelengths terminate)
+    uint32_t thin;
+    int32_t coatEta;
+    float coatAlpha, thickness, coatG;
+    int32_t coatAlbedo;
+    uint32_t coatDispersive;
+    int32_t coatMaxDepth, coatSamples;
+    int32_t sheenColor;
+    float sheenRoughness, sheenWeight;
+    uint32_t pad;
+};
+
+struct Medium {  // gpu_api.h:84-87: homogeneous, sigma = spectrum * scale, Henyey-Greenstein g
+    int32_t sigmaA, sigmaS;
+    float scale, g;
+};
+
+enum FilterType : uint32_t { FilterBox = 0, FilterGaussian = 1, FilterBlackmanHarris = 2 };
+
+struct Filter {  // gpu_api.h:91-95
+    uint32_t type;
+    float radius, sigma, norm;
+    float cdf[kFilterBins + 1];
+};
+
+enum LightType : uint32_t { LightArea = 0, LightPoint = 1 };
+
+struct Light {  // gpu_api.h:99-110
+    uint32_t type;
+    int32_t primitive;  // area light: emitting primitive
+    int32_t spectrum;   // radiance (area) or intensity (point)
+    float scale;
+    Float3 position;    // point light
+    uint32_t twoSided;
+    float cosPower;
+    float area;
+    float pmf;          // power-based selection probability
+    uint32_t pad;
+};
+
+struct Camera {  // gpu_api.h:112-118
+    float cameraToWorld[12];  // row-major 3x4
+    float tanHalfFov, aspect;
+    float lensRadius, focalDistance;
+    int32_t width, height, margin;
+    uint32_t pad;
+};
+
+struct SceneDesc {  // gpu_api.h:120-146
+    int32_t apiVersion;
+    const BVHNode *nodes;
+    int32_t nodeCount;
+    const int32_t *primOrder;
+    const Primitive *prims;
+    int32_t primCount;
+    const Material *materials;
+    int32_t materialCount;
+    const Light *lights;
+    int32_t lightCount;
+    const Spectrum *spectra;
+    int32_t spectrumCount;
+    const Medium *media;
+    int32_t mediumCount;
+    int32_t cameraMedium;
+    int32_t envSpectrum;
+    float envScale;
+    float envPmf;
+    Float3 worldCenter;
+    float worldRadius;
+    Camera camera;
+    Filter filter;
+    const float *sheenAlbedo;
+    const float *cie;
+    float cieYIntegral;
+};
+
+struct RenderParams {  // gpu_api.h:148-154
+    int32_t spp;
+    int32_t firstSample;
+    uint64_t seed;
+    int32_t maxDepth;      // <= 0: unlimited
+    float clampLuminance;  // 0 = off
+};
+
+struct DeviceInfo {  // gpu_api.h:156-162
+    char name[256];
+    uint64_t globalMemory;
+    int32_t computeUnits;
+    int32_t isGpu;
+    char backend[32];
+};
+
+// ---------------------------------------------------------------------------------------------
+// Part 2: extension (no counterpart in the reference ABI; see the file comment).
+
+constexpr int kExtVersion = 1;
+
+// Analytic spectrum: value(lambda) = scale * base(lambda). Mirrors the reference Spectrum
+// classes (src/core/spectrum.h:151-231, src/core/rgb2spec.h:13-74) so the device evaluates
+// exactly what the CPU path tracer evaluates at each hero wavelength.
+enum SpectrumKind : uint32_t {
+    SpecConstant = 0,   // base = p[0]
+    SpecSigmoid = 1,    // base = S(p[0] l^2 + p[1] l + p[2]), S(x) = 1/2 + x / (2 sqrt(1 + x^2))
+    SpecBlackbody = 2,  // base = Planck(l, T = p[0]) * p[1]   (p[1] = 1 / Planck(peak))
+    SpecPiecewise = 3,  // base = piecewise-linear over knots[offset .. offset+2*count): lambdas then values
+    SpecSellmeier = 4,  // base = sqrt(max(1, 1 + sum_i p[i] l2 / (l2 - p[3+i]))), l in um, count terms (<= 3)
+    SpecCauchy = 5,     // base = p[0] + p[1] / l^2
+    SpecTable95 = 6,    // base = linear interpolation of knots[offset .. offset+95) at 5 nm from 360 nm (v4 tables)
+};
+
+struct SpectrumEx {  // 48 bytes
+    uint32_t kind;
+    float scale;
+    int32_t offset, count;
+    float p[8];
+};
+
+enum LightKindEx : uint32_t {
+    LightExArea = 0,         // DiffuseAreaLight on primitive `primitive` (reference src/lights/light.cpp:10-103)
+    LightExPoint = 1,        // PointLight at `position`
+    LightExDistant = 2,      // DistantLight arriving from `direction` (src/lights/sky.cpp:52-60)
+    LightExEnvironment = 3,  // constant EnvironmentLight (src/lights/light.cpp:215-297)
+};
+
+struct LightEx {  // 48 bytes
+    uint32_t kind;
+    int32_t primitive;
+    int32_t spectrum;
+    float scale;
+    Float3 position;   // point light
+    Float3 direction;  // distant light: unit vector towards the light
+    uint32_t twoSided;
+    float cosPower;
+};
+
+enum MediumKindEx : uint32_t { MediumHomogeneous = 0, MediumGrid = 1 };
+
+struct MediumEx {  // 120 bytes
+    uint32_t kind;
+    int32_t sigmaA, sigmaS;  // spectra
+    float scale, g;
+    int32_t nx, ny, nz;      // grid resolution (cell-centred samples, x fastest)
+    int64_t gridOffset;      // first density value in SceneDescEx::gridData
+    float maxDensity;
+    Float3 boundsMin, boundsMax;    // density bounds in medium space
+    float worldToMedium[12];        // row-major 3x4 affine
+};
+
+struct CameraEx {  // thin-lens perspective camera (reference src/cameras/camera.cpp:7-48)
+    float cameraToWorld[12];  // row-major 3x4
+    float tanHalfFov;
+    float lensRadius, focalDistance;
+    int32_t width, height;    // image resolution
+    int32_t filterType;       // FilterType
+    float filterRadius;       // pixels; the film margin is ceil(radius - 0.5)
+    int32_t pad;
+};
+
+struct SceneDescEx {
+    int32_t apiVersion;  // kApiVersion
+    int32_t extVersion;  // kExtVersion
+    const Primitive *prims;
+    int32_t primCount;
+    const Material *materials;  // spectrum indices refer to `spectra`
+    int32_t materialCount;
+    const SpectrumEx *spectra;
+    int32_t spectrumCount;
+    const float *knots;
+    int32_t knotCount;
+    const LightEx *lights;
+    int32_t lightCount;
+    const MediumEx *media;
+    int32_t mediumCount;
+    const float *gridData;
+    int64_t gridCount;
+    int32_t cameraMedium;
+    int32_t pad;
+    CameraEx camera;
+};
+
+enum LightSamplerKind : int32_t { LightSamplerBVH = 0, LightSamplerPower = 1 };
+
+struct RenderParamsEx {
+    int32_t spp;               // passes rendered by this call
+    int32_t firstSample;       // sample index of the first pass
+    uint64_t seed;
+    int32_t maxDepth;          // <= 0: unlimited
+    float clampLuminance;      // firefly clamp on sample luminance, 0 = off
+    int32_t rrStartDepth;      // Russian roulette once depth > rrStartDepth (reference: 1)
+    int32_t russianRoulette;   // 0 disables
+    int32_t lightSampler;      // LightSamplerKind
+    int32_t flags;             // reserved, 0
+};
+
+struct RayEx {  // one query ray for prgpu_trace_rays
+    Float3 o, d;
+    float tMax;
+};
+
+struct HitEx {
+    float t;
+    int32_t prim;  // original primitive index, -1 = miss
+    float b1, b2;  // triangle barycentrics
+};
+
+// One camera sample's result for prgpu_sample_radiance: the radiance that the CPU path
+// tracer's Li returns for the same (pixel, sample index, seed), before the film.
+struct SampleRecordEx {
+    float L[4];
+    float lambda[4];
+    float pdf[4];
+    float pFilmX, pFilmY;
+    float weight;        // camera ray weight; 0 = no ray
+    uint32_t flags;      // bit 0: L holds NaN/Inf (the film drops the sample)
+};
+
+} // namespace prgpu
+
+// ---- v4 entry points (reference gpu_api.h:166-173) ------------------------------------------
+// Number of devices written to out (at most max).
+PRGPU_API int prgpu_list_devices(prgpu::DeviceInfo *out, int max);
+// Renders params->spp samples per pixel and ADDS the unnormalised XYZ sums (filter importance
+// sampled, margin 0) into outXYZ[3 * (y * width + x)]. 0 on success, else a message in err.
+PRGPU_API int prgpu_render(int deviceIndex, const prgpu::SceneDesc *scene, const prgpu::RenderParams *params,
+                           float *outXYZ, char *err, int errLen);
+PRGPU_API int prgpu_api_version();
+
+// ---- extension entry points (no reference counterpart) --------------------------------------
+PRGPU_API int prgpu_api_ext_version();
+// Renders params->spp passes over the extended film. Each pass splats every sample with the
+// film's reconstruction filter (as the CPU film does) and ADDS, per image pixel and channel,
+// the pass value to sumXYZ and its square to sumSqXYZ (both width*height*3, may be null
+// for sumSqXYZ). 0 on success.
+PRGPU_API int prgpu_render_ex(int deviceIndex, const prgpu::SceneDescEx *scene, const prgpu::RenderParamsEx *params,
+                              double *sumXYZ, double *sumSqXYZ, char *err, int errLen);
+// Closest-hit (anyHit = 0) or any-hit (anyHit = 1) queries for n rays; rays and hits are
+// host arrays. For any-hit queries only HitEx::prim (>= 0: occluded) is meaningful.
+PRGPU_API int prgpu_trace_rays(int deviceIndex, const prgpu::SceneDescEx *scene, const prgpu::RayEx *rays, int64_t n,
+                               int anyHit, prgpu::HitEx *hits, char *err, int errLen);
+// Radiance of one camera sample per extended-film pixel for sample index params->firstSample
+// (params->spp is ignored): out has (width + 2m) * (height + 2m) records in raster order.
+PRGPU_API int prgpu_sample_radiance(int deviceIndex, const prgpu::SceneDescEx *scene, const prgpu::RenderParamsEx *params,
+                                    prgpu::SampleRecordEx *out, char *err, int errLen);
+// Frees every device buffer the plugin holds for the device.
+PRGPU_API int prgpu_release(int deviceIndex);
+
+// ---- layout pins (measured on the reference's LP64 build) -----------------------------------
+static_assert(sizeof(prgpu::Float3) == 12, "Float3");
+static_assert(sizeof(prgpu::Spectrum) == 380, "Spectrum");
+static_assert(sizeof(prgpu::BVHNode) == 32, "BVHNode");
+static_assert(sizeof(prgpu::Primitive) == 128, "Primitive");
+static_assert(__builtin_offsetof(prgpu::Primitive, p0) == 24, "Primitive.p0");
+static_assert(__builtin_offsetof(prgpu::Primitive, n0) == 60, "Primitive.n0");
+static_assert(__builtin_offsetof(prgpu::Primitive, u0) == 96, "Primitive.u0");
+static_assert(__builtin_offsetof(prgpu::Primitive, r) == 120, "Primitive.r");
+static_assert(sizeof(prgpu::Material) == 88, "Material");
+static_assert(sizeof(prgpu::Medium) == 16, "Medium");
+static_assert(sizeof(prgpu::Filter) == 276, "Filter");
+static_assert(sizeof(prgpu::Light) == 48, "Light");
+static_assert(sizeof(prgpu::Camera) == 80, "Camera");
+static_assert(sizeof(prgpu::SceneDesc) == 520, "SceneDesc");
+static_assert(__builtin_offsetof(prgpu::SceneDesc, camera) == 140, "SceneDesc.camera");
+static_assert(__builtin_offsetof(prgpu::SceneDesc, filter) == 220, "SceneDesc.filter");
+static_assert(__builtin_offsetof(prgpu::SceneDesc, sheenAlbedo) == 496, "SceneDesc.sheenAlbedo");
+static_assert(__builtin_offsetof(prgpu::SceneDesc, cie) == 504, "SceneDesc.cie");
+static_assert(sizeof(prgpu::RenderParams) == 24, "RenderParams");
+static_assert(__builtin_offsetof(prgpu::RenderParams, seed) == 8, "RenderParams.seed");
+static_assert(sizeof(prgpu::DeviceInfo) == 304, "DeviceInfo");
+static_assert(sizeof(prgpu::SpectrumEx) == 48, "SpectrumEx");
+static_assert(sizeof(prgpu::LightEx) == 48, "LightEx");
+static_assert(sizeof(prgpu::MediumEx) == 120, "MediumEx");
+static_assert(sizeof(prgpu::SampleRecordEx) == 64, "SampleRecordEx");

@@ -1,0 +1,158 @@
+"""ctypes binding of the oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+The oracle is a CPU restatement of the reference CPU path tracer (src/integrators/path.cpp and
+everything under it). Only tests/, the cpu_baseline leg of bench.py and __graft_entry__.smoke()
+use it, as the checker. Struct layouts come from the product binding (prender_amd), which
+mirrors include/prender_gpu.h.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(ROOT, "ar-renderer_amd"))
+import prender_amd as pa  # noqa: E402
+
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        vp, fp, ip = C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_int32)
+        L.or_scene_create.restype = vp
+        L.or_scene_create.argtypes = [C.POINTER(pa.SceneDescEx)]
+        L.or_scene_destroy.argtypes = [vp]
+        L.or_scene_info.argtypes = [vp, vp]
+        L.or_world_sphere.argtypes = [vp, vp]
+        L.or_bvh_dump.argtypes = [vp, vp, vp]
+        L.or_trace.argtypes = [vp, vp, C.c_int64, C.c_int, vp, C.c_int]
+        L.or_trace_brute.argtypes = [vp, vp, C.c_int64, vp, vp, C.c_int]
+        L.or_light_bvh_dump.argtypes = [vp, vp, vp]
+        L.or_light_sample.argtypes = [vp, C.c_int, vp, vp, C.c_float, fp]
+        L.or_light_pmf.restype = C.c_float
+        L.or_light_pmf.argtypes = [vp, C.c_int, vp, vp, C.c_int]
+        L.or_camera_ray.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_float, vp]
+        L.or_sample_radiance.argtypes = [vp, C.POINTER(pa.RenderParamsEx), vp, C.c_int]
+        L.or_render.restype = C.c_double
+        L.or_render.argtypes = [vp, C.POINTER(pa.RenderParamsEx), vp, vp, C.c_int]
+        L.or_rng_stream.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, vp]
+        L.or_sample_visible.argtypes = [C.c_float, vp, vp]
+        L.or_cie.argtypes = [C.c_float, vp]
+        L.or_cie_y_integral.restype = C.c_float
+        L.or_to_xyz.argtypes = [vp, vp, vp, vp]
+        for f in ("or_spectrum_eval",):
+            getattr(L, f).restype = C.c_float
+            getattr(L, f).argtypes = [vp, vp, C.c_int, C.c_float]
+        for f in ("or_spectrum_max", "or_spectrum_y"):
+            getattr(L, f).restype = C.c_float
+            getattr(L, f).argtypes = [vp, vp, C.c_int]
+        L.or_bxdf.argtypes = [vp, C.c_int, vp, vp, C.c_float, vp, vp]
+        for f in ("or_fr_dielectric", "or_fr_complex", "or_hg"):
+            getattr(L, f).restype = C.c_float
+        L.or_fr_dielectric.argtypes = [C.c_float, C.c_float]
+        L.or_fr_complex.argtypes = [C.c_float, C.c_float, C.c_float]
+        L.or_hg.argtypes = [C.c_float, C.c_float]
+        L.or_sample_hg.argtypes = [vp, C.c_float, vp, vp, vp]
+        L.or_filter.argtypes = [C.c_int, C.c_float, vp, C.c_int, vp, vp]
+        L.or_light_sample_li.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp]
+        L.or_light_pdf_li.restype = C.c_float
+        L.or_light_pdf_li.argtypes = [vp, C.c_int, vp, vp]
+        L.or_light_power.restype = C.c_float
+        L.or_light_power.argtypes = [vp, C.c_int]
+        L.or_medium_tr.argtypes = [vp, C.c_int, vp, vp, C.c_float, vp, vp, vp]
+        L.or_medium_sample.argtypes = [vp, C.c_int, vp, vp, C.c_float, C.c_float, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def f32(*v):
+    return np.ascontiguousarray(np.array(v, dtype=np.float32).ravel())
+
+
+class OracleScene:
+    """The oracle's view of a prender_amd.Scene (its own binary BVH and light BVH)."""
+
+    def __init__(self, scene):
+        self.scene = scene
+        self.h = lib().or_scene_create(C.byref(scene.desc))
+        if not self.h:
+            raise RuntimeError("oracle scene creation failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_scene_destroy(self.h)
+            self.h = None
+
+    def info(self):
+        out = np.zeros(4, np.int64)
+        lib().or_scene_info(self.h, out.ctypes.data)
+        return {"bvh_nodes": int(out[0]), "light_bvh_nodes": int(out[1]), "infinite_lights": int(out[2]),
+                "margin": int(out[3])}
+
+    def world_sphere(self):
+        out = np.zeros(4, np.float32)
+        lib().or_world_sphere(self.h, out.ctypes.data)
+        return out
+
+    def trace(self, rays, any_hit=False, threads=0):
+        rays = np.ascontiguousarray(rays, dtype=pa.RAY_DTYPE)
+        hits = np.zeros(len(rays), pa.HIT_DTYPE)
+        lib().or_trace(self.h, rays.ctypes.data, len(rays), 1 if any_hit else 0, hits.ctypes.data,
+                       threads or os.cpu_count())
+        return hits
+
+    def trace_brute(self, rays, threads=0):
+        rays = np.ascontiguousarray(rays, dtype=pa.RAY_DTYPE)
+        hits = np.zeros(len(rays), pa.HIT_DTYPE)
+        ties = np.zeros(len(rays), np.int32)
+        lib().or_trace_brute(self.h, rays.ctypes.data, len(rays), hits.ctypes.data, ties.ctypes.data,
+                             threads or os.cpu_count())
+        return hits, ties
+
+    def light_bvh(self):
+        n = self.info()["light_bvh_nodes"]
+        f = np.zeros((max(n, 1), 12), np.float32)
+        iv = np.zeros((max(n, 1), 3), np.int32)
+        lib().or_light_bvh_dump(self.h, f.ctypes.data, iv.ctypes.data)
+        return f[:n], iv[:n]
+
+    def light_sample(self, p, n, u, bvh=True):
+        pmf = C.c_float()
+        li = lib().or_light_sample(self.h, 1 if bvh else 0, f32(*p).ctypes.data, f32(*n).ctypes.data, u,
+                                   C.byref(pmf))
+        return li, pmf.value
+
+    def light_pmf(self, p, n, light, bvh=True):
+        return lib().or_light_pmf(self.h, 1 if bvh else 0, f32(*p).ctypes.data, f32(*n).ctypes.data, light)
+
+    def camera_ray(self, fx, fy, lx=0.5, ly=0.5):
+        out = np.zeros(6, np.float32)
+        lib().or_camera_ray(self.h, fx, fy, lx, ly, out.ctypes.data)
+        return out
+
+    def sample_radiance(self, params, threads=0):
+        m = self.scene.margin()
+        n = (self.scene.width + 2 * m) * (self.scene.height + 2 * m)
+        out = np.zeros(n, pa.SAMPLE_DTYPE)
+        lib().or_sample_radiance(self.h, C.byref(params), out.ctypes.data, threads or os.cpu_count())
+        return out
+
+    def render(self, params, variance=True, threads=0):
+        n = self.scene.width * self.scene.height * 3
+        s = np.zeros(n, np.float64)
+        sq = np.zeros(n, np.float64) if variance else None
+        secs = lib().or_render(self.h, C.byref(params), s.ctypes.data, sq.ctypes.data if sq is not None else None,
+                               threads or os.cpu_count())
+        shape = (self.scene.height, self.scene.width, 3)
+        return s.reshape(shape), (sq.reshape(shape) if sq is not None else None), secs

@@ -1,0 +1,345 @@
+// Driver over the compiled reference components (TEST INFRASTRUCTURE ONLY).
+//
+// Built by Makefile.ref from the reference's own sources under /root/reference/src (those that
+// compile unmodified with the image's g++) into oracle/_ref/libprref.so. It exposes C entry
+// points mirroring the oracle's component probes (rf_* vs or_*), so the golden-vector script in
+// tests/golden/ can record the reference's outputs and the tests can check the oracle against
+// them. Nothing here is shipped or called by the product.
+#include "cameras/camera.h"
+#include "core/film.h"
+#include "core/math.h"
+#include "core/rgb2spec.h"
+#include "core/sampling.h"
+#include "core/spectrum.h"
+#include "geometry/bvh.h"
+#include "geometry/shape.h"
+#include "integrators/integrator.h"
+#include "lights/light.h"
+#include "lights/light_bvh.h"
+#include "materials/bxdf.h"
+#include "scene/scene.h"
+
+#include "../include/prender_gpu.h"
+
+#include <cstring>
+#include <memory>
+#include <unordered_map>
+#include <vector>
+
+using namespace pr;
+
+namespace {
+
+// Reference spectrum objects for the analytic descriptors the generators emit.
+struct SigmoidSpec : Spectrum {
+    RGBSigmoidPolynomial p;
+    explicit SigmoidSpec(RGBSigmoidPolynomial q) : p(q) {}
+    float operator()(float l) const override { return p(l); }
+    float MaxValue() const override { return std::numeric_limits<float>::quiet_NaN(); }  // not exercised
+};
+
+SpectrumPtr MakeSpectrum(const prgpu::SpectrumEx &s, const float *knots) {
+    SpectrumPtr base;
+    switch (s.kind) {
+    case prgpu::SpecConstant: base = std::make_shared<ConstantSpectrum>(s.p[0]); break;
+    case prgpu::SpecSigmoid: base = std::make_shared<SigmoidSpec>(RGBSigmoidPolynomial(s.p[0], s.p[1], s.p[2])); break;
+    case prgpu::SpecBlackbody: base = std::make_shared<BlackbodySpectrum>(s.p[0]); break;
+    case prgpu::SpecPiecewise: {
+        std::vector<float> l(knots + s.offset, knots + s.offset + s.count);
+        std::vector<float> v(knots + s.offset + s.count, knots + s.offset + 2 * s.count);
+        base = std::make_shared<PiecewiseLinearSpectrum>(l, v);
+        break;
+    }
+    case prgpu::SpecSellmeier: {
+        std::vector<float> B(s.p, s.p + s.count), C(s.p + 3, s.p + 3 + s.count);
+        base = std::make_shared<SellmeierIOR>(B, C);
+        break;
+    }
+    case prgpu::SpecCauchy: base = std::make_shared<CauchyIOR>(s.p[0], s.p[1]); break;
+    default: return nullptr;
+    }
+    if (s.scale != 1.f) return std::make_shared<ScaledSpectrum>(s.scale, base);
+    return base;
+}
+
+Vec3f V(const prgpu::Float3 &f) { return {f.x, f.y, f.z}; }
+
+struct RefScene {
+    Scene scene;
+    std::vector<const Light *> lightOrder;
+    std::unordered_map<const Light *, int> lightIndex;
+};
+
+SampledWavelengths Lambdas(float u, int terminate) {
+    SampledWavelengths w = SampledWavelengths::SampleVisible(u);
+    if (terminate) w.TerminateSecondary();
+    return w;
+}
+
+}  // namespace
+
+extern "C" {
+
+void rf_rng_stream(uint64_t pixel, uint64_t sampleIndex, uint64_t seed, int n, float *out) {
+    IndependentSampler s;
+    s.StartPixelSample(pixel, sampleIndex, seed);
+    for (int i = 0; i < n; ++i) out[i] = s.Get1D();
+}
+
+void rf_sample_visible(float u, float *l, float *pdf) {
+    SampledWavelengths w = SampledWavelengths::SampleVisible(u);
+    SampledSpectrum p = w.PDF();
+    for (int i = 0; i < 4; ++i) {
+        l[i] = w[i];
+        pdf[i] = p[i];
+    }
+}
+
+void rf_cie(float l, float *xyz) {
+    xyz[0] = CIE_X(l);
+    xyz[1] = CIE_Y(l);
+    xyz[2] = CIE_Z(l);
+}
+float rf_cie_y_integral() { return CIE_Y_Integral(); }
+
+// XYZ of radiance L carried by SampleVisible(u) (secondary wavelengths terminated when asked).
+void rf_to_xyz(const float *L, float u, int terminate, float *xyz) {
+    SampledWavelengths w = Lambdas(u, terminate);
+    SampledSpectrum s;
+    for (int i = 0; i < 4; ++i) s[i] = L[i];
+    XYZ x = ToXYZ(s, w);
+    xyz[0] = x.x;
+    xyz[1] = x.y;
+    xyz[2] = x.z;
+}
+
+float rf_spectrum_eval(const prgpu::SpectrumEx *s, const float *knots, float l) { return (*MakeSpectrum(*s, knots))(l); }
+float rf_spectrum_y(const prgpu::SpectrumEx *s, const float *knots) { return SpectrumToY(*MakeSpectrum(*s, knots)); }
+float rf_spectrum_max(const prgpu::SpectrumEx *s, const float *knots) { return MakeSpectrum(*s, knots)->MaxValue(); }
+// Named data as the reference loader builds it: metal n/k and glass Sellmeier IOR.
+float rf_named_metal(const char *name, int k, float l) {
+    SpectrumPtr eta, kk;
+    if (!GetNamedMetal(name, &eta, &kk)) return -1;
+    return k ? (*kk)(l) : (*eta)(l);
+}
+float rf_named_ior(const char *name, float l) {
+    SpectrumPtr s = GetNamedIOR(name);
+    return s ? (*s)(l) : -1;
+}
+float rf_blackbody_unit_y(float T, float l) {
+    auto bb = std::make_shared<BlackbodySpectrum>(T);
+    float y = SpectrumToY(*bb);
+    return ScaledSpectrum(1 / y, bb)(l);
+}
+
+// BxDF probe with the same descriptor as or_bxdf: [type, alphaX, alphaY, etaD, R0..3, eta0..3, k0..3].
+void rf_bxdf(const float *desc, int op, const float *wo, const float *wi, float uc, const float *u, float *out) {
+    int type = int(desc[0]);
+    TrowbridgeReitzDistribution d(desc[1], desc[2]);
+    SampledSpectrum R, eta, k;
+    for (int i = 0; i < 4; ++i) {
+        R[i] = desc[4 + i];
+        eta[i] = desc[8 + i];
+        k[i] = desc[12 + i];
+    }
+    std::unique_ptr<BxDF> b;
+    if (type == 1) b = std::make_unique<DiffuseBxDF>(R);
+    else if (type == 2) b = std::make_unique<ConductorBxDF>(d, eta, k);
+    else if (type == 3) b = std::make_unique<DielectricBxDF>(desc[3], d);
+    else b = std::make_unique<ThinDielectricBxDF>(desc[3]);
+    Vec3f o(wo[0], wo[1], wo[2]);
+    if (op == 0) {
+        SampledSpectrum f = b->f(o, Vec3f(wi[0], wi[1], wi[2]), TransportMode::Radiance);
+        for (int i = 0; i < 4; ++i) out[i] = f[i];
+    } else if (op == 1) {
+        out[0] = b->PDF(o, Vec3f(wi[0], wi[1], wi[2]), TransportMode::Radiance, BxDFReflTransFlags::All);
+    } else {
+        auto s = b->Sample_f(o, uc, Vec2f(u[0], u[1]), TransportMode::Radiance, BxDFReflTransFlags::All);
+        for (int i = 0; i < 11; ++i) out[i] = 0;
+        if (s) {
+            for (int i = 0; i < 4; ++i) out[i] = s->f[i];
+            out[4] = s->wi.x;
+            out[5] = s->wi.y;
+            out[6] = s->wi.z;
+            out[7] = s->pdf;
+            out[8] = float(uint32_t(s->flags));
+            out[9] = s->eta;
+            out[10] = 1;
+        }
+    }
+}
+
+float rf_fr_dielectric(float c, float eta) { return FrDielectric(c, eta); }
+float rf_fr_complex(float c, float eta, float k) {
+    return FrComplex(c, SampledSpectrum(eta), SampledSpectrum(k))[0];
+}
+float rf_hg(float c, float g) { return HenyeyGreenstein(c, g); }
+void rf_sample_hg(const float *wo, float g, const float *u, float *wi, float *pdf) {
+    Vec3f r = SampleHenyeyGreenstein(Vec3f(wo[0], wo[1], wo[2]), g, Vec2f(u[0], u[1]), pdf);
+    wi[0] = r.x;
+    wi[1] = r.y;
+    wi[2] = r.z;
+}
+void rf_filter(int type, float radius, const float *x, int n, float *out, float *norm) {
+    FilterSettings fs;
+    fs.type = type == 0 ? FilterType::Box : (type == 1 ? FilterType::Gaussian : FilterType::BlackmanHarris);
+    fs.radius = radius;
+    Filter f(fs);
+    for (int i = 0; i < n; ++i) out[i] = f.Eval1D(x[i]);
+    *norm = f.Norm();
+}
+
+// ---- scene-level probes: the reference BVH, light BVH, lights and camera on a flat scene ----
+
+void *rf_scene_create(const prgpu::SceneDescEx *d, float fovY) {
+    auto rs = std::make_unique<RefScene>();
+    Scene &sc = rs->scene;
+    sc.primitives.resize(size_t(d->primCount));
+    for (int i = 0; i < d->primCount; ++i) {
+        const prgpu::Primitive &p = d->prims[i];
+        std::unique_ptr<Shape> shape;
+        if (p.type == prgpu::PrimSphere) {
+            shape = std::make_unique<Sphere>(V(p.p0), p.r, p.flip != 0);
+        } else {
+            auto m = std::make_shared<TriangleMesh>();
+            m->p = {V(p.p0), V(p.p1), V(p.p2)};
+            bool normals = !(p.n0.x == 0 && p.n0.y == 0 && p.n0.z == 0 && p.n1.x == 0 && p.n1.y == 0 && p.n1.z == 0 &&
+                             p.n2.x == 0 && p.n2.y == 0 && p.n2.z == 0);
+            if (normals) m->n = {V(p.n0), V(p.n1), V(p.n2)};
+            m->uv = {Vec2f(p.u0, p.v0), Vec2f(p.u1, p.v1), Vec2f(p.u2, p.v2)};
+            m->indices = {0, 1, 2};
+            sc.meshes.push_back(m);
+            shape = std::make_unique<Triangle>(m, 0);
+        }
+        sc.primitives[size_t(i)].shape = shape.get();
+        sc.shapes.push_back(std::move(shape));
+    }
+    for (int i = 0; i < d->lightCount; ++i) {
+        const prgpu::LightEx &l = d->lights[i];
+        SpectrumPtr sp = MakeSpectrum(d->spectra[l.spectrum], d->knots);
+        std::shared_ptr<Light> light;
+        if (l.kind == prgpu::LightExArea) {
+            Primitive &pr = sc.primitives[size_t(l.primitive)];
+            auto al = std::make_shared<DiffuseAreaLight>(pr.shape, sp, l.scale, l.twoSided != 0, l.cosPower,
+                                                         &pr.mediumInterface);
+            pr.areaLight = al.get();
+            light = al;
+        } else if (l.kind == prgpu::LightExPoint) {
+            light = std::make_shared<PointLight>(V(l.position), sp, l.scale);
+        } else if (l.kind == prgpu::LightExDistant) {
+            light = std::make_shared<DistantLight>(V(l.direction), sp, l.scale);
+        } else {
+            light = std::make_shared<EnvironmentLight>(sp, l.scale);
+        }
+        rs->lightIndex[light.get()] = i;
+        rs->lightOrder.push_back(light.get());
+        sc.lights.push_back(light);
+    }
+    const prgpu::CameraEx &c = d->camera;
+    Mat4 m;
+    for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 4; ++k) m.m[r][k] = c.cameraToWorld[r * 4 + k];
+    CameraParams cp;
+    cp.cameraToWorld = Transform(m);
+    cp.fovY = fovY;
+    cp.lensRadius = c.lensRadius;
+    cp.focalDistance = c.focalDistance;
+    FilterSettings fs;
+    fs.type = c.filterType == 0 ? FilterType::Box : (c.filterType == 1 ? FilterType::Gaussian : FilterType::BlackmanHarris);
+    fs.radius = c.filterRadius;
+    Filter filt(fs);
+    CameraFilmInfo info{c.width, c.height, int(std::ceil(filt.Radius() - 0.5f))};
+    sc.camera = std::make_unique<PerspectiveCamera>(cp, info);
+    sc.Build();
+    return rs.release();
+}
+
+void rf_scene_destroy(void *h) { delete static_cast<RefScene *>(h); }
+
+void rf_trace(void *h, const prgpu::RayEx *rays, int64_t n, int anyHit, prgpu::HitEx *hits) {
+    const Scene &sc = static_cast<RefScene *>(h)->scene;
+    for (int64_t i = 0; i < n; ++i) {
+        Ray r(V(rays[i].o), V(rays[i].d));
+        prgpu::HitEx &o = hits[i];
+        o.t = 0;
+        o.b1 = o.b2 = 0;
+        if (anyHit) {
+            o.prim = sc.GetBVH().IntersectP(r, rays[i].tMax) ? 0 : -1;
+            continue;
+        }
+        BVH::Hit bh;
+        if (sc.GetBVH().Intersect(r, rays[i].tMax, &bh)) {
+            o.t = bh.t;
+            o.prim = bh.prim;
+            o.b1 = bh.info.b1;
+            o.b2 = bh.info.b2;
+        } else {
+            o.prim = -1;
+        }
+    }
+}
+
+int rf_light_sample(void *h, const float *p, const float *n, float u, float *pmf) {
+    RefScene &rs = *static_cast<RefScene *>(h);
+    *pmf = 0;
+    const Light *l = rs.scene.lightBVH.Sample(Vec3f(p[0], p[1], p[2]), Vec3f(n[0], n[1], n[2]), u, pmf);
+    return l ? rs.lightIndex[l] : -1;
+}
+
+float rf_light_pmf(void *h, const float *p, const float *n, int light) {
+    RefScene &rs = *static_cast<RefScene *>(h);
+    return rs.scene.lightBVH.PMF(Vec3f(p[0], p[1], p[2]), Vec3f(n[0], n[1], n[2]), rs.lightOrder[size_t(light)]);
+}
+
+float rf_light_power_pmf(void *h, int light) {
+    RefScene &rs = *static_cast<RefScene *>(h);
+    return rs.scene.lightSampler.PMF(rs.lightOrder[size_t(light)]);
+}
+
+void rf_camera_ray(void *h, float fx, float fy, float lx, float ly, float *out) {
+    const Scene &sc = static_cast<RefScene *>(h)->scene;
+    CameraSample cs{Vec2f(fx, fy), Vec2f(lx, ly)};
+    auto r = sc.camera->GenerateRay(cs);
+    out[0] = r->o.x;
+    out[1] = r->o.y;
+    out[2] = r->o.z;
+    out[3] = r->d.x;
+    out[4] = r->d.y;
+    out[5] = r->d.z;
+}
+
+// Li sample from a light for hero wavelengths SampleVisible(ul): out = L0..3, wi, pdf, ok.
+void rf_light_sample_li(void *h, int light, const float *p, const float *n, const float *u, float ul, float *out) {
+    RefScene &rs = *static_cast<RefScene *>(h);
+    SampledWavelengths w = SampledWavelengths::SampleVisible(ul);
+    LightSampleContext ctx;
+    ctx.p = Vec3f(p[0], p[1], p[2]);
+    ctx.n = ctx.ns = Vec3f(n[0], n[1], n[2]);
+    for (int i = 0; i < 9; ++i) out[i] = 0;
+    auto ls = rs.lightOrder[size_t(light)]->SampleLi(ctx, Vec2f(u[0], u[1]), w);
+    if (!ls) return;
+    for (int i = 0; i < 4; ++i) out[i] = ls->L[i];
+    out[4] = ls->wi.x;
+    out[5] = ls->wi.y;
+    out[6] = ls->wi.z;
+    out[7] = ls->pdf;
+    out[8] = 1;
+}
+
+float rf_light_pdf_li(void *h, int light, const float *p, const float *n, const float *wi) {
+    RefScene &rs = *static_cast<RefScene *>(h);
+    LightSampleContext ctx;
+    ctx.p = Vec3f(p[0], p[1], p[2]);
+    ctx.n = ctx.ns = Vec3f(n[0], n[1], n[2]);
+    return rs.lightOrder[size_t(light)]->PDF_Li(ctx, Vec3f(wi[0], wi[1], wi[2]));
+}
+
+void rf_world_sphere(void *h, float *out) {
+    const Scene &sc = static_cast<RefScene *>(h)->scene;
+    out[0] = sc.worldCenter.x;
+    out[1] = sc.worldCenter.y;
+    out[2] = sc.worldCenter.z;
+    out[3] = sc.worldRadius;
+}
+
+}  // extern "C"
