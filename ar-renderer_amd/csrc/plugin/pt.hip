@@ -1,21 +1,1155 @@
-// Wavefront path tracer (placeholder until the kernels land).
+// Wavefront spectral path tracer for gfx950.
+//
+// A batch holds P passes x (sample width x sample height) camera samples, one path slot each.
+// Per bounce: k_intersect (closest hit of every queued path), k_shade (one iteration of the
+// reference's Li loop: media, emission, interfaces, BSDF, next-event estimation, Russian
+// roulette; surviving paths are appended to the next queue, shadow rays to the shadow queue),
+// k_shadow (transmittance of the shadow rays; visible contributions are added to their path).
+// When the queue drains, k_finish turns each path's radiance into a film sample and k_gather
+// splats the samples of every pass onto the image with the film's reconstruction filter,
+// adding the pass value and its square to the FP64 sum and sum-of-squares accumulators.
+//
+// Path state is structure-of-arrays in HBM (float4 / int4 per field) so that the lanes of a
+// wave read and write consecutive 16-byte words.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <optional>
+#include <unordered_map>
+#include <vector>
+
+#include "pt_dev.h"
 #include "pt_host.h"
 
 namespace pgpu {
-struct PTScene {};
+
+// =============================================================================================
+// Host: light BVH (restates src/lights/light_bvh.cpp:10-177 so its PMF matches the CPU's)
+
+namespace {
+
+struct HostLB {
+    Box3 bounds;
+    V3 w{0, 0, 1};
+    float phi = 0, cosTheta_o = 1, cosTheta_e = 0;
+    bool twoSided = false;
+};
+
+struct HostCone {
+    V3 w{0, 0, 1};
+    float cosTheta = kInfF;
+    HostCone() = default;
+    HostCone(V3 ww, float c) : w(normalize(ww)), cosTheta(c) {}
+    bool empty() const { return cosTheta == kInfF; }
+};
+
+V3 RotateVector(float degrees, V3 axis, V3 v) {  // src/core/math.cpp:71-86
+    V3 a = normalize(axis);
+    float rad = (kPi / 180.f) * degrees;
+    float s = std::sin(rad), c = std::cos(rad);
+    float m00 = a.x * a.x + (1 - a.x * a.x) * c, m01 = a.x * a.y * (1 - c) - a.z * s, m02 = a.x * a.z * (1 - c) + a.y * s;
+    float m10 = a.x * a.y * (1 - c) + a.z * s, m11 = a.y * a.y + (1 - a.y * a.y) * c, m12 = a.y * a.z * (1 - c) - a.x * s;
+    float m20 = a.x * a.z * (1 - c) - a.y * s, m21 = a.y * a.z * (1 - c) + a.x * s, m22 = a.z * a.z + (1 - a.z * a.z) * c;
+    return {m00 * v.x + m01 * v.y + m02 * v.z, m10 * v.x + m11 * v.y + m12 * v.z, m20 * v.x + m21 * v.y + m22 * v.z};
+}
+
+float AngleBetween(V3 a, V3 b) {
+    if (dot(a, b) < 0) return kPi - 2 * safe_asin(len(a + b) / 2);
+    return 2 * safe_asin(len(b - a) / 2);
+}
+
+HostCone ConeUnion(const HostCone &a, const HostCone &b) {
+    if (a.empty()) return b;
+    if (b.empty()) return a;
+    float ta = safe_acos(a.cosTheta), tb = safe_acos(b.cosTheta);
+    float td = AngleBetween(a.w, b.w);
+    if (std::min(td + tb, kPi) <= ta) return a;
+    if (std::min(td + ta, kPi) <= tb) return b;
+    float to = (ta + td + tb) / 2;
+    if (to >= kPi) return HostCone(V3(0, 0, 1), -1);
+    float tr = to - ta;
+    V3 wr = cross_d(a.w, b.w);
+    if (len2(wr) == 0) return HostCone(V3(0, 0, 1), -1);
+    V3 w = RotateVector((180.f / kPi) * tr, wr, a.w);
+    return HostCone(w, std::cos(to));
+}
+
+HostLB LBUnion(const HostLB &a, const HostLB &b) {
+    if (a.phi == 0) return b;
+    if (b.phi == 0) return a;
+    HostCone c = ConeUnion(HostCone(a.w, a.cosTheta_o), HostCone(b.w, b.cosTheta_o));
+    HostLB r;
+    r.bounds = join(a.bounds, b.bounds);
+    r.w = c.w;
+    r.phi = a.phi + b.phi;
+    r.cosTheta_o = c.cosTheta;
+    r.cosTheta_e = std::min(a.cosTheta_e, b.cosTheta_e);
+    r.twoSided = a.twoSided || b.twoSided;
+    return r;
+}
+
+float LBCost(const HostLB &b, const Box3 &bounds, int dim) {
+    float t_o = safe_acos(b.cosTheta_o), t_e = safe_acos(b.cosTheta_e);
+    float t_w = std::min(t_o + t_e, kPi);
+    float s_o = safe_sqrt(1 - sq(b.cosTheta_o));
+    float M = 2 * kPi * (1 - b.cosTheta_o) +
+              kPi / 2 * (2 * t_w * s_o - std::cos(t_o - 2 * t_w) - 2 * t_o * s_o + b.cosTheta_o);
+    V3 d = bounds.diag();
+    float Kr = d[dim] > 0 ? max_comp(d) / d[dim] : 1.f;
+    return b.phi * M * Kr * b.bounds.area();
+}
+
+struct LightTreeBuilder {
+    std::vector<int> bounded, infinite;
+    std::vector<HostLB> lbs;
+    std::vector<int> child1, light, parent;
+
+    int Rec(std::vector<std::pair<int, HostLB>> &items, int start, int end, int par) {
+        if (end - start == 1) {
+            int ni = int(lbs.size());
+            lbs.push_back(items[size_t(start)].second);
+            child1.push_back(-1);
+            light.push_back(bounded[size_t(items[size_t(start)].first)]);
+            parent.push_back(par);
+            return ni;
+        }
+        Box3 bounds, cb;
+        for (int i = start; i < end; ++i) {
+            bounds = join(bounds, items[size_t(i)].second.bounds);
+            cb = join(cb, items[size_t(i)].second.bounds.centroid());
+        }
+        float minCost = kInfF;
+        int minBucket = -1, minDim = -1;
+        constexpr int kB = 12;
+        for (int dim = 0; dim < 3; ++dim) {
+            if (cb.hi[dim] == cb.lo[dim]) continue;
+            HostLB bucket[kB];
+            for (int i = start; i < end; ++i) {
+                V3 pc = items[size_t(i)].second.bounds.centroid();
+                int b = int(kB * cb.offset(pc)[dim]);
+                b = clampi(b, 0, kB - 1);
+                bucket[b] = LBUnion(bucket[b], items[size_t(i)].second);
+            }
+            for (int i = 0; i < kB - 1; ++i) {
+                HostLB b0, b1;
+                for (int j = 0; j <= i; ++j) b0 = LBUnion(b0, bucket[j]);
+                for (int j = i + 1; j < kB; ++j) b1 = LBUnion(b1, bucket[j]);
+                float cost = LBCost(b0, bounds, dim) + LBCost(b1, bounds, dim);
+                if (cost > 0 && cost < minCost) {
+                    minCost = cost;
+                    minBucket = i;
+                    minDim = dim;
+                }
+            }
+        }
+        int mid;
+        if (minDim == -1) {
+            mid = (start + end) / 2;
+        } else {
+            auto it = std::partition(items.begin() + start, items.begin() + end, [&](const std::pair<int, HostLB> &l) {
+                int b = int(kB * cb.offset(l.second.bounds.centroid())[minDim]);
+                return clampi(b, 0, kB - 1) <= minBucket;
+            });
+            mid = int(it - items.begin());
+            if (mid == start || mid == end) mid = (start + end) / 2;
+        }
+        int ni = int(lbs.size());
+        lbs.emplace_back();
+        child1.push_back(-1);
+        light.push_back(-1);
+        parent.push_back(par);
+        Rec(items, start, mid, ni);
+        int c1 = Rec(items, mid, end, ni);
+        child1[size_t(ni)] = c1;
+        lbs[size_t(ni)] = LBUnion(lbs[size_t(ni + 1)], lbs[size_t(c1)]);
+        return ni;
+    }
+};
+
+float PrimArea(const prgpu::Primitive &p) {
+    if (p.type == prgpu::PrimSphere) return 4 * kPi * p.r * p.r;
+    V3 a = f3(p.p0), b = f3(p.p1), c = f3(p.p2);
+    return 0.5f * len(cross_d(b - a, c - a));
+}
+
+Box3 PrimBounds(const prgpu::Primitive &p) {
+    if (p.type == prgpu::PrimSphere) {
+        V3 c = f3(p.p0);
+        return {c - V3(p.r, p.r, p.r), c + V3(p.r, p.r, p.r)};
+    }
+    V3 a = f3(p.p0), b = f3(p.p1), c = f3(p.p2);
+    return join(Box3(vmin(a, b), vmax(a, b)), c);
+}
+
+}  // namespace
+
+// =============================================================================================
+// Device-resident path-tracing scene and batch buffers
+
+struct PTScene {
+    DevBuf prims, mats, specs, knots, lights, lightArea, lnodes, leafOf, infinite, powerCdf, powerFunc, media, grid;
+    int lightCount = 0, lnodeCount = 0, infCount = 0, powerCount = 0;
+    float powerInt = 0;
+    float worldR = 1;
+    int hasInterfaces = 0;
+    float cieYInt = 1;
+    // host copies for inspection
+    std::vector<LNode> hostNodes;
+    // batch buffers
+    size_t cap = 0;
+    DevBuf o, d, rng, lam, lpdf, beta, rpath, L, Lh, prevP, prevN, st, hit, q0, q1, pfilm, xyz;
+    DevBuf so, sd, sp, sn, sf, sL, sb, sq;
+    int *countsHost = nullptr;
+    DevBuf counts;
+    ~PTScene() {
+        if (countsHost) (void)hipHostFree(countsHost);
+    }
+};
+
 Device::Device() = default;
 Device::~Device() {
     pt.reset();
     if (stream) (void)hipStreamDestroy(stream);
 }
 void ReleasePT(Device &d) { d.pt.reset(); }
-void RenderExImpl(Device &, const prgpu::SceneDescEx &, const prgpu::RenderParamsEx &, double *, double *) {
-    throw std::runtime_error("render_ex not implemented");
+
+namespace {
+
+uint64_t PTKey(const prgpu::SceneDescEx &sd) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void *p, size_t n) {
+        const unsigned char *b = static_cast<const unsigned char *>(p);
+        for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    };
+    mix(&sd.primCount, sizeof(int));
+    if (sd.materialCount) mix(sd.materials, sizeof(prgpu::Material) * size_t(sd.materialCount));
+    if (sd.spectrumCount) mix(sd.spectra, sizeof(prgpu::SpectrumEx) * size_t(sd.spectrumCount));
+    if (sd.knotCount) mix(sd.knots, sizeof(float) * size_t(sd.knotCount));
+    if (sd.lightCount) mix(sd.lights, sizeof(prgpu::LightEx) * size_t(sd.lightCount));
+    if (sd.mediumCount) mix(sd.media, sizeof(prgpu::MediumEx) * size_t(sd.mediumCount));
+    mix(&sd.gridCount, sizeof(sd.gridCount));
+    if (sd.gridCount > 0) {
+        int64_t step = std::max<int64_t>(1, sd.gridCount / 4096);
+        for (int64_t i = 0; i < sd.gridCount; i += step) mix(&sd.gridData[i], sizeof(float));
+    }
+    mix(&sd.cameraMedium, sizeof(int));
+    mix(&sd.camera, sizeof(sd.camera));
+    return h ^ GeometryKey(sd.prims, sd.primCount);
 }
-void SampleRadianceImpl(Device &, const prgpu::SceneDescEx &, const prgpu::RenderParamsEx &, prgpu::SampleRecordEx *) {
-    throw std::runtime_error("sample_radiance not implemented");
+
+void Validate(const prgpu::SceneDescEx &sd) {
+    auto bad = [](const char *m) { throw std::runtime_error(std::string("invalid scene: ") + m); };
+    if (sd.primCount < 0 || (sd.primCount > 0 && !sd.prims)) bad("primitives");
+    if (sd.camera.width <= 0 || sd.camera.height <= 0) bad("film size");
+    for (int i = 0; i < sd.primCount; ++i) {
+        const prgpu::Primitive &p = sd.prims[i];
+        if (p.material >= sd.materialCount || p.light >= sd.lightCount || p.mediumInside >= sd.mediumCount ||
+            p.mediumOutside >= sd.mediumCount)
+            bad("primitive references out of range");
+    }
+    auto specOK = [&](int s, bool optional) { return (optional && s < 0) || (s >= 0 && s < sd.spectrumCount); };
+    for (int i = 0; i < sd.materialCount; ++i) {
+        const prgpu::Material &m = sd.materials[i];
+        if (m.type == prgpu::MatDiffuse && !specOK(m.albedo, false)) bad("diffuse reflectance");
+        if (m.type == prgpu::MatDiffuse && m.albedoB >= sd.spectrumCount) bad("checker spectrum");
+        if (m.type == prgpu::MatConductor && !((specOK(m.eta, false) && specOK(m.k, false)) || specOK(m.albedo, false)))
+            bad("conductor spectra");
+        if (m.type == prgpu::MatDielectric && !specOK(m.eta, false)) bad("dielectric IOR");
+        if (m.type == prgpu::MatCoatedDiffuse || m.type == prgpu::MatCoatedConductor)
+            bad("coated materials are not supported by this plugin version");
+    }
+    for (int i = 0; i < sd.spectrumCount; ++i) {
+        const prgpu::SpectrumEx &s = sd.spectra[i];
+        if (s.kind == prgpu::SpecPiecewise && (s.count < 1 || s.offset < 0 || s.offset + 2 * s.count > sd.knotCount))
+            bad("piecewise spectrum knots");
+        if (s.kind == prgpu::SpecTable95 && (s.offset < 0 || s.offset + prgpu::kSpectrumSamples > sd.knotCount))
+            bad("tabulated spectrum");
+        if (s.kind == prgpu::SpecSellmeier && (s.count < 0 || s.count > 3)) bad("Sellmeier terms");
+        if (s.kind > prgpu::SpecTable95) bad("spectrum kind");
+    }
+    for (int i = 0; i < sd.lightCount; ++i) {
+        const prgpu::LightEx &l = sd.lights[i];
+        if (!specOK(l.spectrum, false)) bad("light spectrum");
+        if (l.kind == prgpu::LightExArea && (l.primitive < 0 || l.primitive >= sd.primCount)) bad("area light primitive");
+        if (l.kind > prgpu::LightExEnvironment) bad("light kind");
+    }
+    for (int i = 0; i < sd.mediumCount; ++i) {
+        const prgpu::MediumEx &m = sd.media[i];
+        if (!specOK(m.sigmaA, false) || !specOK(m.sigmaS, false)) bad("medium spectra");
+        if (m.kind == prgpu::MediumGrid &&
+            (m.nx <= 0 || m.ny <= 0 || m.nz <= 0 || m.gridOffset < 0 ||
+             m.gridOffset + int64_t(m.nx) * m.ny * m.nz > sd.gridCount))
+            bad("grid medium data");
+    }
+    if (sd.cameraMedium >= sd.mediumCount) bad("camera medium");
 }
+
+// Uploads count elements, or one zeroed element when the array is empty (kernels never index it then).
+template <class T> void UploadOrDummy(DevBuf &b, const T *src, int count, hipStream_t s) {
+    if (count > 0 && src) {
+        b.upload(src, size_t(count), s);
+    } else {
+        T zero{};
+        std::memset(&zero, 0, sizeof(T));
+        b.upload(&zero, 1, s);
+        PG_CHECK(hipStreamSynchronize(s));
+    }
+}
+
+void EnsurePTScene(Device &dev, const prgpu::SceneDescEx &sd) {
+    uint64_t key = PTKey(sd);
+    if (dev.pt && dev.ptKey == key) return;
+    Validate(sd);
+    auto pt = std::make_unique<PTScene>();
+    hipStream_t s = dev.stream;
+    UploadOrDummy(pt->prims, sd.prims, sd.primCount, s);
+    UploadOrDummy(pt->mats, sd.materials, sd.materialCount, s);
+    UploadOrDummy(pt->specs, sd.spectra, sd.spectrumCount, s);
+    std::vector<float> knots(sd.knots, sd.knots + sd.knotCount);
+    knots.push_back(0.f);
+    pt->knots.upload(knots.data(), knots.size(), s);
+    UploadOrDummy(pt->media, sd.media, sd.mediumCount, s);
+    std::vector<float> grid(sd.gridData, sd.gridData + sd.gridCount);
+    grid.push_back(0.f);
+    pt->grid.upload(grid.data(), grid.size(), s);
+    pt->cieYInt = host_cie_y_integral();
+
+    // World sphere: the primitives' bounds joined with the camera position (src/scene/scene.cpp:9-27).
+    Box3 b(V3(dev.sceneLo[0], dev.sceneLo[1], dev.sceneLo[2]), V3(dev.sceneHi[0], dev.sceneHi[1], dev.sceneHi[2]));
+    if (sd.primCount == 0) b = Box3();
+    const float *m = sd.camera.cameraToWorld;
+    V3 camPos(m[0] * 0.f + m[1] * 0.f + m[2] * 0.f + m[3], m[4] * 0.f + m[5] * 0.f + m[6] * 0.f + m[7],
+              m[8] * 0.f + m[9] * 0.f + m[10] * 0.f + m[11]);
+    b = join(b, camPos);
+    V3 wc = b.centroid();
+    float wr = b.empty() ? 0.f : dist(wc, b.hi);
+    pt->worldR = std::max(wr, 1e-3f);
+
+    // Lights: areas, bounds, powers; light BVH and power distribution.
+    SpecTable st{sd.spectra, sd.knots};
+    std::vector<float> areas(size_t(std::max(1, sd.lightCount)), 0.f);
+    std::vector<std::optional<HostLB>> lbs;
+    std::vector<float> power;
+    for (int i = 0; i < sd.lightCount; ++i) {
+        const prgpu::LightEx &l = sd.lights[i];
+        std::optional<HostLB> lb;
+        float pw = 0;
+        if (l.kind == prgpu::LightExArea) {
+            const prgpu::Primitive &p = sd.prims[l.primitive];
+            areas[size_t(i)] = PrimArea(p);
+            HostLB x;
+            x.bounds = PrimBounds(p);
+            if (p.type == prgpu::PrimSphere) {
+                x.w = V3(0, 0, 1);
+                x.cosTheta_o = -1;
+            } else {
+                V3 n = normalize(cross_f(f3(p.p1) - f3(p.p0), f3(p.p2) - f3(p.p0)));
+                if (prim_has_normals(p)) n = face_forward(n, f3(p.n0) + f3(p.n1) + f3(p.n2));
+                x.w = n;
+                x.cosTheta_o = 1;
+            }
+            x.cosTheta_e = 0;
+            x.twoSided = l.twoSided != 0;
+            x.phi = l.scale * spec_max(st, l.spectrum) * areas[size_t(i)] * kPi * (l.twoSided ? 2 : 1);
+            lb = x;
+            float lobe = 2 * kPi / (l.cosPower + 2);
+            pw = l.scale * host_spectrum_y(st, l.spectrum, pt->cieYInt) * areas[size_t(i)] * lobe * (l.twoSided ? 2 : 1);
+        } else if (l.kind == prgpu::LightExPoint) {
+            HostLB x;
+            x.bounds = Box3(f3(l.position), f3(l.position));
+            x.phi = 4 * kPi * l.scale * spec_max(st, l.spectrum);
+            x.cosTheta_o = -1;
+            x.cosTheta_e = 0;
+            lb = x;
+            pw = 4 * kPi * l.scale * host_spectrum_y(st, l.spectrum, pt->cieYInt);
+        } else if (l.kind == prgpu::LightExDistant) {
+            pw = l.scale * host_spectrum_y(st, l.spectrum, pt->cieYInt) * kPi * pt->worldR * pt->worldR;
+        } else {
+            pw = 4 * kPi * kPi * pt->worldR * pt->worldR * l.scale * host_spectrum_y(st, l.spectrum, pt->cieYInt);
+        }
+        lbs.push_back(lb);
+        power.push_back(std::isfinite(pw) && pw > 0 ? pw : 0.f);
+    }
+    LightTreeBuilder tb;
+    std::vector<std::pair<int, HostLB>> items;
+    for (size_t i = 0; i < lbs.size(); ++i) {
+        if (!lbs[i]) {
+            tb.infinite.push_back(int(i));
+        } else if (lbs[i]->phi > 0) {
+            items.push_back({int(tb.bounded.size()), *lbs[i]});
+            tb.bounded.push_back(int(i));
+        }
+    }
+    if (!items.empty()) tb.Rec(items, 0, int(items.size()), -1);
+    std::vector<LNode> nodes(tb.lbs.size());
+    std::vector<int> leafOf(size_t(std::max(1, sd.lightCount)), -1);
+    for (size_t i = 0; i < nodes.size(); ++i) {
+        LNode &n = nodes[i];
+        const HostLB &x = tb.lbs[i];
+        n.lo[0] = x.bounds.lo.x; n.lo[1] = x.bounds.lo.y; n.lo[2] = x.bounds.lo.z;
+        n.hi[0] = x.bounds.hi.x; n.hi[1] = x.bounds.hi.y; n.hi[2] = x.bounds.hi.z;
+        n.w[0] = x.w.x; n.w[1] = x.w.y; n.w[2] = x.w.z;
+        n.phi = x.phi;
+        n.cosTheta_o = x.cosTheta_o;
+        n.cosTheta_e = x.cosTheta_e;
+        n.child1 = tb.child1[i];
+        n.light = tb.light[i];
+        n.parent = tb.parent[i];
+        n.twoSided = x.twoSided ? 1u : 0u;
+        if (n.light >= 0) leafOf[size_t(n.light)] = int(i);
+    }
+    pt->hostNodes = nodes;
+    pt->lnodeCount = int(nodes.size());
+    pt->lnodes.upload(nodes.data(), std::max<size_t>(1, nodes.size()), s);
+    pt->leafOf.upload(leafOf.data(), leafOf.size(), s);
+    std::vector<int> inf = tb.infinite;
+    pt->infCount = int(inf.size());
+    inf.push_back(-1);
+    pt->infinite.upload(inf.data(), inf.size(), s);
+    pt->lightCount = sd.lightCount;
+    UploadOrDummy(pt->lights, sd.lights, sd.lightCount, s);
+    pt->lightArea.upload(areas.data(), areas.size(), s);
+    // Power sampler: Distribution1D over max(power, 1e-4 max) (src/lights/light.cpp:326-340, sampling.cpp:5-27).
+    {
+        float maxP = 0;
+        for (float p : power) maxP = std::max(maxP, p);
+        for (float &p : power) p = std::max(p, maxP > 0 ? maxP * 1e-4f : 1.f);
+        int n = int(power.size());
+        std::vector<float> cdf(size_t(n + 1), 0.f);
+        double acc = 0;
+        for (int i = 1; i < n + 1; ++i) {
+            acc += double(power[size_t(i - 1)]) / n;
+            cdf[size_t(i)] = float(acc);
+        }
+        float funcInt = float(acc);
+        if (funcInt == 0) {
+            for (int i = 1; i < n + 1; ++i) cdf[size_t(i)] = float(i) / float(n);
+        } else {
+            for (int i = 1; i < n + 1; ++i) cdf[size_t(i)] = float(cdf[size_t(i)] / acc);
+        }
+        if (n > 0) cdf[size_t(n)] = 1;
+        power.push_back(0.f);
+        pt->powerCdf.upload(cdf.data(), cdf.size(), s);
+        pt->powerFunc.upload(power.data(), power.size(), s);
+        pt->powerInt = funcInt;
+        pt->powerCount = n;
+    }
+    pt->hasInterfaces = 0;
+    for (int i = 0; i < sd.primCount && !pt->hasInterfaces; ++i) {
+        const prgpu::Primitive &p = sd.prims[i];
+        bool noMat = p.material < 0 || sd.materials[p.material].type == prgpu::MatInterface ||
+                     sd.materials[p.material].type == prgpu::MatBlack;
+        if (noMat && p.light < 0) pt->hasInterfaces = 1;
+    }
+    PG_CHECK(hipHostMalloc(reinterpret_cast<void **>(&pt->countsHost), 64, hipHostMallocDefault));
+    pt->counts.ensure(64);
+    PG_CHECK(hipStreamSynchronize(s));
+    dev.pt = std::move(pt);
+    dev.ptKey = key;
+}
+
+float FilterEval1D(int type, float radius, float sigma, float x) {  // src/core/film.cpp:24-39
+    x = std::abs(x);
+    if (x > radius) return 0;
+    if (type == prgpu::FilterBox) return 1;
+    if (type == prgpu::FilterGaussian)
+        return std::max(0.f, std::exp(-x * x / (2 * sigma * sigma)) - std::exp(-radius * radius / (2 * sigma * sigma)));
+    float t = 0.5f + x / (2 * radius);
+    const float a0 = 0.35875f, a1 = 0.48829f, a2 = 0.14128f, a3 = 0.01168f;
+    return a0 - a1 * std::cos(2 * kPi * t) + a2 * std::cos(4 * kPi * t) - a3 * std::cos(6 * kPi * t);
+}
+
+DevScene MakeDevScene(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::RenderParamsEx &rp) {
+    PTScene &pt = *dev.pt;
+    DevScene s{};
+    s.prims = pt.prims.as<prgpu::Primitive>();
+    s.mats = pt.mats.as<prgpu::Material>();
+    s.spec = SpecTable{pt.specs.as<prgpu::SpectrumEx>(), pt.knots.as<float>()};
+    s.lights = pt.lights.as<prgpu::LightEx>();
+    s.lightArea = pt.lightArea.as<float>();
+    s.lightCount = pt.lightCount;
+    s.lnodes = pt.lnodes.as<LNode>();
+    s.lnodeCount = pt.lnodeCount;
+    s.leafOf = pt.leafOf.as<int>();
+    s.infinite = pt.infinite.as<int>();
+    s.infCount = pt.infCount;
+    s.powerCdf = pt.powerCdf.as<float>();
+    s.powerFunc = pt.powerFunc.as<float>();
+    s.powerInt = pt.powerInt;
+    s.powerCount = pt.powerCount;
+    s.lightSampler = rp.lightSampler;
+    s.media = pt.media.as<prgpu::MediumEx>();
+    s.grid = pt.grid.as<float>();
+    s.cameraMedium = sd.cameraMedium;
+    s.worldR = pt.worldR;
+    s.hasInterfaces = pt.hasInterfaces;
+    std::memcpy(s.c2w, sd.camera.cameraToWorld, sizeof(s.c2w));
+    s.tanHalf = sd.camera.tanHalfFov;
+    s.aspect = float(sd.camera.width) / float(sd.camera.height);
+    s.lensR = std::max(0.f, sd.camera.lensRadius);
+    s.focal = std::max(1e-4f, sd.camera.focalDistance);
+    s.width = sd.camera.width;
+    s.height = sd.camera.height;
+    // Film filter and margin (src/core/film.cpp:9-22, 42).
+    int ft = sd.camera.filterType;
+    float r = sd.camera.filterRadius;
+    float sigma = 0.5f;
+    if (ft == prgpu::FilterBox) r = std::max(0.5f, std::min(r, 0.5f));
+    if (ft == prgpu::FilterGaussian) sigma = r / 3.f;
+    const int n = 4096;
+    double s1 = 0;
+    for (int i = 0; i < n; ++i) s1 += FilterEval1D(ft, r, sigma, -r + (i + 0.5f) * (2 * r / n));
+    s1 *= 2 * r / n;
+    s.filterType = ft;
+    s.filterRadius = r;
+    s.filterSigma = sigma;
+    s.filterNorm = float(1.0 / (s1 * s1));
+    s.margin = int(std::ceil(r - 0.5f));
+    s.xyzNorm = 1.f / (4 * pt.cieYInt);
+    s.trace = MakeTraceCtx(dev);
+    return s;
+}
+
+// =============================================================================================
+// Kernels
+
+struct Params {
+    int maxDepth;  // already mapped: <= 0 -> INT_MAX
+    int rrStart;
+    int rr;
+    float clampY;
+    uint64_t seed;
+    int firstSample;
+};
+
+struct Bufs {
+    float4 *o, *d;
+    ulonglong2 *rng;
+    float4 *lam, *lpdf, *beta, *rpath, *L, *Lh, *prevP, *prevN;
+    int4 *st;
+    float4 *hit;
+    int *q0, *q1;
+    int *counts;  // [0] queue A, [1] queue B, [2] shadow queue
+    float2 *pfilm;
+    float4 *xyz;
+    // shadow records
+    float4 *so, *sd, *sp, *sn, *sf, *sL, *sb;
+};
+
+constexpr int kBlock = 256;
+constexpr uint32_t kSpecularFlag = 1;
+
+__device__ __forceinline__ Pcg load_rng(const Bufs &b, int slot) {
+    ulonglong2 r = b.rng[slot];
+    Pcg p;
+    p.state = r.x;
+    p.inc = r.y;
+    return p;
+}
+__device__ __forceinline__ V2 get2(Pcg &r) {
+    float a = r.uniform();
+    float c = r.uniform();
+    return {a, c};
+}
+
+// Camera samples (src/integrators/path.cpp:61-70, src/cameras/camera.cpp:36-48).
+__global__ __launch_bounds__(kBlock) void k_camera(DevScene s, Params p, Bufs b, int n, int npix, int sw) {
+    int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= n) return;
+    int pass = slot / npix, pix = slot - pass * npix;
+    int py = pix / sw, px = pix - py * sw;
+    Pcg rng;
+    uint64_t words[2] = {uint64_t(py) * uint64_t(sw) + uint64_t(px), p.seed};
+    uint64_t si = uint64_t(p.firstSample + pass);
+    rng.set_sequence(murmur_words(words, 2, false, 0), mix_bits(si * 0x9E3779B97F4A7C15ull + 1));
+    Lambda lam = sample_visible(rng.uniform());
+    V2 u = get2(rng);
+    float fx = px - s.margin + u.x, fy = py - s.margin + u.y;
+    V2 lens = get2(rng);
+    float x = (2 * fx / s.width - 1) * s.tanHalf * s.aspect;
+    float y = (1 - 2 * fy / s.height) * s.tanHalf;
+    V3 dir = normalize(V3(x, y, 1));
+    V3 o(0, 0, 0);
+    if (s.lensR > 0) {
+        V2 pl = disk_concentric(lens);
+        pl.x = pl.x * s.lensR;
+        pl.y = pl.y * s.lensR;
+        float ft = s.focal / dir.z;
+        V3 pFocus = dir * ft;
+        o = V3(pl.x, pl.y, 0);
+        dir = normalize(pFocus - o);
+    }
+    const float *m = s.c2w;
+    V3 wo(m[0] * o.x + m[1] * o.y + m[2] * o.z + m[3], m[4] * o.x + m[5] * o.y + m[6] * o.z + m[7],
+          m[8] * o.x + m[9] * o.y + m[10] * o.z + m[11]);
+    V3 wd = normalize(V3(m[0] * dir.x + m[1] * dir.y + m[2] * dir.z, m[4] * dir.x + m[5] * dir.y + m[6] * dir.z,
+                         m[8] * dir.x + m[9] * dir.y + m[10] * dir.z));
+    b.o[slot] = make_float4(wo.x, wo.y, wo.z, 1.f);           // w: prevPdf
+    b.d[slot] = make_float4(wd.x, wd.y, wd.z, 1.f);           // w: etaScale
+    b.rng[slot] = make_ulonglong2(rng.state, rng.inc);
+    b.lam[slot] = make_float4(lam.l[0], lam.l[1], lam.l[2], lam.l[3]);
+    b.lpdf[slot] = make_float4(lam.pdf[0], lam.pdf[1], lam.pdf[2], lam.pdf[3]);
+    b.beta[slot] = make_float4(1, 1, 1, 1);
+    b.rpath[slot] = make_float4(1, 1, 1, 1);
+    b.L[slot] = make_float4(0, 0, 0, 0);
+    b.Lh[slot] = make_float4(0, 0, 0, 0);
+    b.prevP[slot] = make_float4(0, 0, 0, 0);
+    b.prevN[slot] = make_float4(0, 0, 0, 0);
+    b.st[slot] = make_int4(0, 0, s.cameraMedium, 0);
+    b.pfilm[slot] = make_float2(fx, fy);
+    b.q0[slot] = slot;
+}
+
+__global__ __launch_bounds__(kTraceBlock) void k_intersect(DevScene s, Bufs b, const int *queue, const int *count) {
+    __shared__ uint32_t lds[kStackLds * kTraceBlock];
+    const int n = *count;
+    const int tid = int(blockIdx.x * blockDim.x + threadIdx.x);
+    const int stride = int(gridDim.x * blockDim.x);
+    for (int i = tid; i < n; i += stride) {
+        int slot = queue[i];
+        float4 o = b.o[slot], d = b.d[slot];
+        DRay r{o.x, o.y, o.z, d.x, d.y, d.z};
+        DHit h;
+        TraceStats st;
+        bool hit = dev_trace<false, false>(s.trace, r, kInfF, &h, lds, tid, &st);
+        b.hit[slot] = hit ? make_float4(h.t, __int_as_float(h.prim), h.b1, h.b2) : make_float4(0, __int_as_float(-1), 0, 0);
+    }
+}
+
+struct PathRegs {
+    V3 ro, rd;
+    float prevPdf, etaScale;
+    Pcg rng;
+    Lambda lam;
+    SS beta, rpath, L, Lh;
+    V3 prevP, prevN;
+    int depth, flags, medium, crossings;
+};
+
+__device__ __forceinline__ void add_contrib(PathRegs &r, const SS &c) {  // path.cpp:131-135
+    r.Lh = r.Lh + c;
+    float a = r.rpath.avg();
+    if (a > 0 && is_finite(a)) r.L = r.L + c / a;
+}
+
+// Russian roulette (path.cpp:181-188, 228-235); true when the path is terminated.
+__device__ __forceinline__ bool roulette(const Params &p, PathRegs &r) {
+    if (p.rr && r.depth > p.rrStart) {
+        SS eff;
+        if (r.lam.terminated()) {
+            eff = r.beta;
+        } else {
+            float a = r.rpath.avg();
+            eff = a > 0 && is_finite(a) ? r.beta / a : SS(0.f);
+        }
+        float m = eff.maxc() * r.etaScale;
+        if (m < 1) {
+            float q = maxf(0.f, 1 - m);
+            if (r.rng.uniform() < q) return true;
+            r.beta = r.beta / (1 - q);
+        }
+    }
+    return false;
+}
+
+// Next-event estimation up to the transmittance (path.cpp:82-111): samples a light and, when
+// the unoccluded contribution is nonzero, appends a shadow record that k_shadow completes.
+__device__ void sample_ld(const DevScene &s, const Bufs &b, int slot, PathRegs &r, V3 p, V3 n, V3 wo, const BSDF *bsdf,
+                          float g, V3 ns, bool surface, int mIn, int mOut) {
+    float uLight = r.rng.uniform();
+    V2 uL = get2(r.rng);
+    float pmf;
+    int li = light_sample(s, p, n, uLight, &pmf);
+    if (li < 0 || pmf == 0) return;
+    LightSample ls;
+    if (!sample_li(s, li, p, uL, r.lam, &ls) || !ls.L.nonzero() || ls.pdf == 0) return;
+    V3 wi = ls.wi;
+    SS f;
+    float scatterPdf;
+    if (bsdf) {
+        f = bsdf->f(wo, wi) * absdot(wi, ns);
+        if (!f.nonzero()) return;
+        scatterPdf = bsdf->pdf(wo, wi);
+    } else {
+        float ph = hg_phase(dot(wo, wi), g);
+        f = SS(ph);
+        scatterPdf = ph;
+    }
+    float lightPdf = pmf * ls.pdf;
+    float k = light_is_delta(s, li) ? 1.f / lightPdf : power_heuristic(lightPdf, scatterPdf) / lightPdf;
+    // Shadow ray p0.SpawnRayTo(pLight) (src/scene/interaction.h:43-47) and its starting medium.
+    V3 o = surface ? offset_origin(p, n, ls.pl - p) : p;
+    V3 target = offset_origin(ls.pl, ls.nl, o - ls.pl);
+    V3 d = target - o;
+    int medium = surface ? (dot(d, n) > 0 ? mOut : mIn) : r.medium;
+    int rec = atomicAdd(&b.counts[2], 1);
+    b.so[rec] = make_float4(o.x, o.y, o.z, __int_as_float(medium));
+    b.sd[rec] = make_float4(d.x, d.y, d.z, __int_as_float(slot));
+    b.sp[rec] = make_float4(ls.pl.x, ls.pl.y, ls.pl.z, k);
+    b.sn[rec] = make_float4(ls.nl.x, ls.nl.y, ls.nl.z, r.rpath.avg());
+    st4(b.sf, rec, f);
+    st4(b.sL, rec, ls.L);
+    st4(b.sb, rec, r.beta);
+}
+
+__device__ void add_emitted(const DevScene &s, PathRegs &r, int li, const SS &Le, V3 dir) {  // path.cpp:143-153
+    if (!Le.nonzero()) return;
+    if (r.depth == 0 || (r.flags & kSpecularFlag)) {
+        add_contrib(r, r.beta * Le);
+    } else {
+        float pmf = light_pmf(s, r.prevP, r.prevN, li);
+        float lightPdf = pmf * pdf_li(s, li, r.prevP, dir);
+        float w = power_heuristic(r.prevPdf, lightPdf);
+        add_contrib(r, r.beta * Le * w);
+    }
+}
+
+// One iteration of the reference's Li loop (path.cpp:155-236) for every queued path.
+__global__ __launch_bounds__(kBlock) void k_shade(DevScene s, Params p, Bufs b, const int *queue, const int *count,
+                                                  int *nextQueue, int *nextCount) {
+    const int n = *count;
+    const int stride = int(gridDim.x * blockDim.x);
+    for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += stride) {
+        const int slot = queue[i];
+        PathRegs r;
+        {
+            float4 o = b.o[slot], d = b.d[slot];
+            r.ro = V3(o.x, o.y, o.z);
+            r.prevPdf = o.w;
+            r.rd = V3(d.x, d.y, d.z);
+            r.etaScale = d.w;
+            r.rng = load_rng(b, slot);
+            float4 l = b.lam[slot], lp = b.lpdf[slot];
+            r.lam = {{l.x, l.y, l.z, l.w}, {lp.x, lp.y, lp.z, lp.w}};
+            r.beta = ld4(b.beta, slot);
+            r.rpath = ld4(b.rpath, slot);
+            r.L = ld4(b.L, slot);
+            r.Lh = ld4(b.Lh, slot);
+            float4 pp = b.prevP[slot], pn = b.prevN[slot];
+            r.prevP = V3(pp.x, pp.y, pp.z);
+            r.prevN = V3(pn.x, pn.y, pn.z);
+            int4 st = b.st[slot];
+            r.depth = st.x;
+            r.flags = st.y;
+            r.medium = st.z;
+            r.crossings = st.w;
+        }
+        const float4 hv = b.hit[slot];
+        const int prim = __float_as_int(hv.y);
+        const bool hit = prim >= 0;
+        const float tHit = hv.x;
+        bool alive = true;
+        bool done = false;  // this bounce has been fully handled
+
+        if (r.medium >= 0) {
+            DistSample ds = medium_sample(s, r.medium, r.ro, r.rd, hit ? tHit : kInfF, r.rng.uniform(), r.lam);
+            r.beta = r.beta * ds.weight;
+            r.rpath = r.rpath * ds.pdfRatio;
+            if (!r.beta.nonzero()) {
+                alive = false;
+                done = true;
+            } else if (ds.scattered) {
+                done = true;
+                if (r.depth++ >= p.maxDepth) {
+                    alive = false;
+                } else {
+                    V3 mp = r.ro + r.rd * ds.t;
+                    V3 mwo = -r.rd;
+                    float g = clampf(s.media[r.medium].g, -0.99f, 0.99f);
+                    sample_ld(s, b, slot, r, mp, V3(), mwo, nullptr, g, V3(), false, -1, -1);
+                    float phasePdf;
+                    V3 wi = sample_hg(mwo, g, get2(r.rng), &phasePdf);
+                    if (phasePdf == 0) {
+                        alive = false;
+                    } else {
+                        r.prevPdf = phasePdf;
+                        r.prevP = mp;
+                        r.prevN = V3();
+                        r.flags &= ~kSpecularFlag;
+                        r.ro = mp;
+                        r.rd = wi;
+                        if (roulette(p, r)) alive = false;
+                    }
+                }
+            }
+        }
+
+        if (!done) {
+            if (!hit) {
+                for (int li = 0; li < s.lightCount; ++li)
+                    if (s.lights[li].kind == prgpu::LightExEnvironment)
+                        add_emitted(s, r, li, spec_sample(s.spec, s.lights[li].spectrum, r.lam) * s.lights[li].scale, r.rd);
+                alive = false;
+            } else {
+                const prgpu::Primitive &pr = s.prims[prim];
+                Surf si;
+                interaction(pr, r.ro, r.rd, tHit, hv.z, hv.w, &si);
+                if (pr.light >= 0) add_emitted(s, r, pr.light, area_L(s, s.lights[pr.light], si.n, -r.rd, r.lam), r.rd);
+                if (is_interface(s, pr)) {
+                    if (++r.crossings > 1000) {
+                        alive = false;
+                    } else {
+                        r.medium = dot(r.rd, si.n) > 0 ? pr.mediumOutside : pr.mediumInside;
+                        r.ro = offset_origin(si.p, si.n, r.rd);
+                    }
+                } else if (!has_bsdf(s, pr)) {
+                    alive = false;
+                } else {
+                    BSDF bsdf = eval_surface(s, s.mats[pr.material], si, r.lam);
+                    if (r.depth++ >= p.maxDepth) {
+                        alive = false;
+                    } else {
+                        if (bsdf.non_specular())
+                            sample_ld(s, b, slot, r, si.p, si.n, si.wo, &bsdf, 0, si.ns, true, pr.mediumInside,
+                                      pr.mediumOutside);
+                        float uc = r.rng.uniform();
+                        V2 u2 = get2(r.rng);
+                        BSample bs;
+                        if (!bsdf.sample(si.wo, uc, u2, &bs)) {
+                            alive = false;
+                        } else {
+                            r.beta = r.beta * (bs.f * (absdot(bs.wi, si.ns) / bs.pdf));
+                            r.prevPdf = bs.pdf;
+                            r.flags = (bs.flags & kSpecular) ? (r.flags | kSpecularFlag) : (r.flags & ~kSpecularFlag);
+                            if (bs.flags & kTrans) r.etaScale *= sq(bs.eta);
+                            r.prevP = si.p;
+                            r.prevN = si.n;
+                            r.ro = offset_origin(si.p, si.n, bs.wi);
+                            r.rd = bs.wi;
+                            r.medium = dot(bs.wi, si.n) > 0 ? pr.mediumOutside : pr.mediumInside;
+                            if (!r.beta.nonzero() || r.beta.has_nan()) alive = false;
+                            else if (roulette(p, r)) alive = false;
+                        }
+                    }
+                }
+            }
+        }
+
+        // Store the state this kernel changed.
+        b.o[slot] = make_float4(r.ro.x, r.ro.y, r.ro.z, r.prevPdf);
+        b.d[slot] = make_float4(r.rd.x, r.rd.y, r.rd.z, r.etaScale);
+        b.rng[slot] = make_ulonglong2(r.rng.state, r.rng.inc);
+        b.lpdf[slot] = make_float4(r.lam.pdf[0], r.lam.pdf[1], r.lam.pdf[2], r.lam.pdf[3]);
+        st4(b.beta, slot, r.beta);
+        st4(b.rpath, slot, r.rpath);
+        st4(b.L, slot, r.L);
+        st4(b.Lh, slot, r.Lh);
+        b.prevP[slot] = make_float4(r.prevP.x, r.prevP.y, r.prevP.z, 0.f);
+        b.prevN[slot] = make_float4(r.prevN.x, r.prevN.y, r.prevN.z, 0.f);
+        b.st[slot] = make_int4(r.depth, r.flags, r.medium, r.crossings);
+        if (alive) nextQueue[atomicAdd(nextCount, 1)] = slot;
+    }
+}
+
+// Transmittance of the shadow records (src/scene/scene.cpp:41-59) and the deferred add().
+__global__ __launch_bounds__(kTraceBlock) void k_shadow(DevScene s, Bufs b, const int *count) {
+    __shared__ uint32_t lds[kStackLds * kTraceBlock];
+    const int n = *count;
+    const int tid = int(blockIdx.x * blockDim.x + threadIdx.x);
+    const int stride = int(gridDim.x * blockDim.x);
+    for (int i = tid; i < n; i += stride) {
+        float4 so = b.so[i], sd = b.sd[i], sp = b.sp[i], sn = b.sn[i];
+        const int slot = __float_as_int(sd.w);
+        V3 ro(so.x, so.y, so.z), rd(sd.x, sd.y, sd.z), pl(sp.x, sp.y, sp.z), nl(sn.x, sn.y, sn.z);
+        int medium = __float_as_int(so.w);
+        Lambda lam;
+        {
+            float4 l = b.lam[slot], lp = b.lpdf[slot];
+            lam = {{l.x, l.y, l.z, l.w}, {lp.x, lp.y, lp.z, lp.w}};
+        }
+        SS T(1.f);
+        const float tMax = 1 - 1e-4f;
+        for (int iter = 0;; ++iter) {
+            if (iter == 256) {
+                T = SS(0.f);
+                break;
+            }
+            DRay r{ro.x, ro.y, ro.z, rd.x, rd.y, rd.z};
+            DHit h;
+            TraceStats ts;
+            if (!s.hasInterfaces) {
+                if (dev_trace<true, false>(s.trace, r, tMax, &h, lds, tid, &ts)) {
+                    T = SS(0.f);
+                } else if (medium >= 0) {
+                    float l = len(rd);
+                    T = T * medium_tr(s, medium, ro, rd / l, 1.f * l, lam);
+                }
+                break;
+            }
+            bool hit = dev_trace<false, false>(s.trace, r, tMax, &h, lds, tid, &ts);
+            const prgpu::Primitive *pr = hit ? &s.prims[h.prim] : nullptr;
+            if (hit && !is_interface(s, *pr)) {
+                T = SS(0.f);
+                break;
+            }
+            if (medium >= 0) {
+                float l = len(rd);
+                T = T * medium_tr(s, medium, ro, rd / l, (hit ? h.t : 1.f) * l, lam);
+            }
+            if (!T.nonzero() || !hit) break;
+            Surf si;
+            interaction(*pr, ro, rd, h.t, h.b1, h.b2, &si);
+            medium = dot(rd, si.n) > 0 ? pr->mediumOutside : pr->mediumInside;
+            ro = offset_origin(si.p, si.n, pl - si.p);
+            V3 target = offset_origin(pl, nl, ro - pl);
+            rd = target - ro;
+        }
+        if (!T.nonzero()) continue;
+        // beta * (f * Tr * L * k), then add() with the rPath average at sampling time.
+        SS c = ld4(b.sb, i) * (ld4(b.sf, i) * T * ld4(b.sL, i) * sp.w);
+        float a = sn.w;
+        SS Lh = ld4(b.Lh, slot) + c;
+        st4(b.Lh, slot, Lh);
+        if (a > 0 && is_finite(a)) st4(b.L, slot, ld4(b.L, slot) + c / a);
+    }
+}
+
+// Final radiance per sample: film XYZ (path.cpp:71-73, film.cpp:133-139) or a parity record.
+__global__ __launch_bounds__(kBlock) void k_finish(DevScene s, Params p, Bufs b, int n, prgpu::SampleRecordEx *rec) {
+    int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= n) return;
+    float4 lp = b.lpdf[slot], l = b.lam[slot];
+    Lambda lam = {{l.x, l.y, l.z, l.w}, {lp.x, lp.y, lp.z, lp.w}};
+    SS L = lam.terminated() ? ld4(b.Lh, slot) : ld4(b.L, slot);
+    L = L * 1.f;  // camera ray weight
+    float2 pf = b.pfilm[slot];
+    if (rec) {
+        prgpu::SampleRecordEx &o = rec[slot];
+        for (int k = 0; k < 4; ++k) {
+            o.L[k] = L.v[k];
+            o.lambda[k] = lam.l[k];
+            o.pdf[k] = lam.pdf[k];
+        }
+        o.pFilmX = pf.x;
+        o.pFilmY = pf.y;
+        o.weight = 1.f;
+        o.flags = L.has_nan() ? 1u : 0u;
+        return;
+    }
+    float4 out = make_float4(0, 0, 0, 0);
+    if (!L.has_nan() && L.nonzero()) {
+        double X = 0, Y = 0, Z = 0;  // ToXYZ (src/core/spectrum.cpp:68-80)
+        for (int k = 0; k < 4; ++k) {
+            if (lam.pdf[k] == 0) continue;
+            float v = L.v[k] / lam.pdf[k];
+            X += cie_x(lam.l[k]) * v;
+            Y += cie_y(lam.l[k]) * v;
+            Z += cie_z(lam.l[k]) * v;
+        }
+        float x = float(X) * s.xyzNorm, y = float(Y) * s.xyzNorm, z = float(Z) * s.xyzNorm;
+        if (p.clampY > 0 && y > p.clampY) {
+            float kk = p.clampY / y;
+            x *= kk;
+            y *= kk;
+            z *= kk;
+        }
+        if (is_finite(x) && is_finite(y) && is_finite(z)) out = make_float4(x, y, z, 1.f);
+    }
+    b.xyz[slot] = out;
+}
+
+__device__ __forceinline__ float filter_1d(const DevScene &s, float x) {
+    x = fabsf(x);
+    if (x > s.filterRadius) return 0;
+    if (s.filterType == prgpu::FilterBox) return 1;
+    if (s.filterType == prgpu::FilterGaussian)
+        return maxf(0.f, expf(-x * x / (2 * s.filterSigma * s.filterSigma)) -
+                             expf(-s.filterRadius * s.filterRadius / (2 * s.filterSigma * s.filterSigma)));
+    float t = 0.5f + x / (2 * s.filterRadius);
+    const float a0 = 0.35875f, a1 = 0.48829f, a2 = 0.14128f, a3 = 0.01168f;
+    return a0 - a1 * cosf(2 * kPi * t) + a2 * cosf(4 * kPi * t) - a3 * cosf(6 * kPi * t);
+}
+
+// Film: each output pixel gathers the samples whose filter footprint covers it (the CPU film's
+// AddXYZ splat, film.cpp:141-157, evaluated per pixel so no atomics are needed), one pass at a
+// time, and adds the pass value and its square to the FP64 accumulators.
+__global__ __launch_bounds__(kBlock) void k_gather(DevScene s, Bufs b, int passes, int npix, int sw, int reach,
+                                                   double *sum, double *sumsq) {
+    int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= s.width * s.height) return;
+    int y = pix / s.width, x = pix - y * s.width;
+    const int sh = npix / sw;
+    const float r = s.filterRadius;
+    double accX = 0, accY = 0, accZ = 0, sqX = 0, sqY = 0, sqZ = 0;
+    for (int pass = 0; pass < passes; ++pass) {
+        double eX = 0, eY = 0, eZ = 0;
+        for (int sy = y + s.margin - reach; sy <= y + s.margin + reach; ++sy) {
+            if (sy < 0 || sy >= sh) continue;
+            for (int sx = x + s.margin - reach; sx <= x + s.margin + reach; ++sx) {
+                if (sx < 0 || sx >= sw) continue;
+                int slot = pass * npix + sy * sw + sx;
+                float4 v = b.xyz[slot];
+                if (v.w == 0.f) continue;
+                float2 pf = b.pfilm[slot];
+                int x0 = max(0, int(ceilf(pf.x - r - 0.5f))), x1 = min(s.width - 1, int(floorf(pf.x + r - 0.5f)));
+                int y0 = max(0, int(ceilf(pf.y - r - 0.5f))), y1 = min(s.height - 1, int(floorf(pf.y + r - 0.5f)));
+                if (x < x0 || x > x1 || y < y0 || y > y1) continue;
+                float w = filter_1d(s, pf.x - (x + 0.5f)) * filter_1d(s, pf.y - (y + 0.5f)) * s.filterNorm;
+                if (w == 0) continue;
+                eX += double(v.x) * w;
+                eY += double(v.y) * w;
+                eZ += double(v.z) * w;
+            }
+        }
+        accX += eX;
+        accY += eY;
+        accZ += eZ;
+        sqX += eX * eX;
+        sqY += eY * eY;
+        sqZ += eZ * eZ;
+    }
+    size_t idx = 3 * size_t(pix);
+    sum[idx] += accX;
+    sum[idx + 1] += accY;
+    sum[idx + 2] += accZ;
+    if (sumsq) {
+        sumsq[idx] += sqX;
+        sumsq[idx + 1] += sqY;
+        sumsq[idx + 2] += sqZ;
+    }
+}
+
+// =============================================================================================
+// Host driver
+
+void EnsureBatch(PTScene &pt, size_t n) {
+    if (pt.cap >= n) return;
+    auto f4 = [&](DevBuf &b) { b.ensure(n * sizeof(float4)); };
+    f4(pt.o); f4(pt.d); f4(pt.lam); f4(pt.lpdf); f4(pt.beta); f4(pt.rpath); f4(pt.L); f4(pt.Lh);
+    f4(pt.prevP); f4(pt.prevN); f4(pt.hit); f4(pt.xyz);
+    f4(pt.so); f4(pt.sd); f4(pt.sp); f4(pt.sn); f4(pt.sf); f4(pt.sL); f4(pt.sb);
+    pt.rng.ensure(n * sizeof(ulonglong2));
+    pt.st.ensure(n * sizeof(int4));
+    pt.q0.ensure(n * sizeof(int));
+    pt.q1.ensure(n * sizeof(int));
+    pt.pfilm.ensure(n * sizeof(float2));
+    pt.cap = n;
+}
+
+Bufs MakeBufs(PTScene &pt) {
+    Bufs b;
+    b.o = pt.o.as<float4>(); b.d = pt.d.as<float4>(); b.rng = pt.rng.as<ulonglong2>();
+    b.lam = pt.lam.as<float4>(); b.lpdf = pt.lpdf.as<float4>(); b.beta = pt.beta.as<float4>();
+    b.rpath = pt.rpath.as<float4>(); b.L = pt.L.as<float4>(); b.Lh = pt.Lh.as<float4>();
+    b.prevP = pt.prevP.as<float4>(); b.prevN = pt.prevN.as<float4>(); b.st = pt.st.as<int4>();
+    b.hit = pt.hit.as<float4>(); b.q0 = pt.q0.as<int>(); b.q1 = pt.q1.as<int>(); b.counts = pt.counts.as<int>();
+    b.pfilm = pt.pfilm.as<float2>(); b.xyz = pt.xyz.as<float4>();
+    b.so = pt.so.as<float4>(); b.sd = pt.sd.as<float4>(); b.sp = pt.sp.as<float4>(); b.sn = pt.sn.as<float4>();
+    b.sf = pt.sf.as<float4>(); b.sL = pt.sL.as<float4>(); b.sb = pt.sb.as<float4>();
+    return b;
+}
+
+Params MakeParams(const prgpu::RenderParamsEx &rp) {
+    Params p;
+    p.maxDepth = rp.maxDepth > 0 ? rp.maxDepth : 0x7fffffff;
+    p.rrStart = rp.rrStartDepth;
+    p.rr = rp.russianRoulette;
+    p.clampY = rp.clampLuminance;
+    p.seed = rp.seed;
+    p.firstSample = rp.firstSample;
+    return p;
+}
+
+// Traces one batch of `passes` passes (samples firstSample .. firstSample + passes - 1) to
+// completion; the per-sample results are left in the xyz / record buffers.
+void TraceBatch(Device &dev, const DevScene &s, Params p, int passes, int npix, int sw) {
+    PTScene &pt = *dev.pt;
+    hipStream_t st = dev.stream;
+    const int n = passes * npix;
+    Bufs b = MakeBufs(pt);
+    hipLaunchKernelGGL(k_camera, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s, p, b, n, npix, sw);
+    PG_CHECK(hipGetLastError());
+    int active = n;
+    pt.countsHost[0] = n;
+    PG_CHECK(hipMemcpyAsync(b.counts, pt.countsHost, sizeof(int), hipMemcpyHostToDevice, st));
+    int *cur = b.q0, *nxt = b.q1;
+    int curIdx = 0;
+    const int shadeGrid = std::max(1, std::min((active + kBlock - 1) / kBlock, dev.cus * 16));
+    for (int bounce = 0; active > 0; ++bounce) {
+        const int nxtIdx = 1 - curIdx;
+        PG_CHECK(hipMemsetAsync(b.counts + nxtIdx, 0, sizeof(int), st));
+        PG_CHECK(hipMemsetAsync(b.counts + 2, 0, sizeof(int), st));
+        hipLaunchKernelGGL(k_intersect, dim3(TraceGrid(dev, active)), dim3(kTraceBlock), 0, st, s, b, cur,
+                           b.counts + curIdx);
+        int grid = std::max(1, std::min((active + kBlock - 1) / kBlock, shadeGrid));
+        hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, st, s, p, b, cur, b.counts + curIdx, nxt,
+                           b.counts + nxtIdx);
+        hipLaunchKernelGGL(k_shadow, dim3(TraceGrid(dev, active)), dim3(kTraceBlock), 0, st, s, b, b.counts + 2);
+        PG_CHECK(hipGetLastError());
+        PG_CHECK(hipMemcpyAsync(pt.countsHost, b.counts, 3 * sizeof(int), hipMemcpyDeviceToHost, st));
+        PG_CHECK(hipStreamSynchronize(st));
+        active = pt.countsHost[nxtIdx];
+        std::swap(cur, nxt);
+        curIdx = nxtIdx;
+    }
+    CheckTraceFlags(dev);
+}
+
+// Passes per batch: bounded by a path budget so that the state fits comfortably in HBM.
+int PassesPerBatch(int npix, int spp) {
+    const int64_t budget = int64_t(8) << 20;  // paths in flight
+    return int(std::max<int64_t>(1, std::min<int64_t>(spp, budget / std::max(1, npix))));
+}
+
+}  // namespace
+
+void RenderExImpl(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::RenderParamsEx &rp, double *sum,
+                  double *sumsq) {
+    if (rp.spp <= 0) return;
+    EnsurePTScene(dev, sd);
+    DevScene s = MakeDevScene(dev, sd, rp);
+    const int sw = s.width + 2 * s.margin, sh = s.height + 2 * s.margin, npix = sw * sh;
+    const int per = PassesPerBatch(npix, rp.spp);
+    PTScene &pt = *dev.pt;
+    EnsureBatch(pt, size_t(per) * size_t(npix));
+    const size_t filmN = size_t(s.width) * s.height * 3;
+    DevBuf dsum, dsq;
+    dsum.upload(sum, filmN, dev.stream);
+    if (sumsq) dsq.upload(sumsq, filmN, dev.stream);
+    const int reach = int(std::ceil(s.filterRadius + 0.5f));
+    Params p = MakeParams(rp);
+    for (int done = 0; done < rp.spp; done += per) {
+        int passes = std::min(per, rp.spp - done);
+        p.firstSample = rp.firstSample + done;
+        TraceBatch(dev, s, p, passes, npix, sw);
+        Bufs b = MakeBufs(pt);
+        int n = passes * npix;
+        hipLaunchKernelGGL(k_finish, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, dev.stream, s, p, b, n,
+                           static_cast<prgpu::SampleRecordEx *>(nullptr));
+        int pixels = s.width * s.height;
+        hipLaunchKernelGGL(k_gather, dim3((pixels + kBlock - 1) / kBlock), dim3(kBlock), 0, dev.stream, s, b, passes,
+                           npix, sw, reach, dsum.as<double>(), sumsq ? dsq.as<double>() : static_cast<double *>(nullptr));
+        PG_CHECK(hipGetLastError());
+    }
+    PG_CHECK(hipMemcpyAsync(sum, dsum.p, filmN * sizeof(double), hipMemcpyDeviceToHost, dev.stream));
+    if (sumsq) PG_CHECK(hipMemcpyAsync(sumsq, dsq.p, filmN * sizeof(double), hipMemcpyDeviceToHost, dev.stream));
+    PG_CHECK(hipStreamSynchronize(dev.stream));
+}
+
+void SampleRadianceImpl(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::RenderParamsEx &rp,
+                        prgpu::SampleRecordEx *out) {
+    EnsurePTScene(dev, sd);
+    DevScene s = MakeDevScene(dev, sd, rp);
+    const int sw = s.width + 2 * s.margin, sh = s.height + 2 * s.margin, npix = sw * sh;
+    PTScene &pt = *dev.pt;
+    EnsureBatch(pt, size_t(npix));
+    Params p = MakeParams(rp);
+    TraceBatch(dev, s, p, 1, npix, sw);
+    DevBuf rec;
+    rec.ensure(size_t(npix) * sizeof(prgpu::SampleRecordEx));
+    Bufs b = MakeBufs(pt);
+    hipLaunchKernelGGL(k_finish, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, dev.stream, s, p, b, npix,
+                       rec.as<prgpu::SampleRecordEx>());
+    PG_CHECK(hipGetLastError());
+    PG_CHECK(hipMemcpyAsync(out, rec.p, size_t(npix) * sizeof(prgpu::SampleRecordEx), hipMemcpyDeviceToHost, dev.stream));
+    PG_CHECK(hipStreamSynchronize(dev.stream));
+}
+
 void RenderV4Impl(Device &, const prgpu::SceneDesc &, const prgpu::RenderParams &, float *) {
     throw std::runtime_error("v4 render not implemented");
 }
+
 }  // namespace pgpu
