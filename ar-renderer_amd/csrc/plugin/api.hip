@@ -209,7 +209,7 @@ PRGPU_API int prgpu_render_ex(int deviceIndex, const prgpu::SceneDescEx *sd, con
         Enumerate();
         if (!sd || sd->apiVersion != prgpu::kApiVersion || sd->extVersion != prgpu::kExtVersion)
             throw std::runtime_error("API version mismatch");
-        if (!rp || !sum) throw std::runtime_error("invalid arguments");
+        if (!rp || (!sum && sumsq)) throw std::runtime_error("invalid arguments");
         Device &d = Get(deviceIndex);
         EnsureGeometry(d, sd->prims, sd->primCount);
         RenderExImpl(d, *sd, *rp, sum, sumsq);
@@ -255,6 +255,92 @@ PRGPU_API int prgpu_render(int deviceIndex, const prgpu::SceneDesc *sd, const pr
     }
 }
 
+PRGPU_API int prgpu_film_reset(int deviceIndex, int width, int height) {
+    std::lock_guard<std::mutex> lock(gMutex);
+    try {
+        Enumerate();
+        if (width <= 0 || height <= 0) throw std::runtime_error("invalid film size");
+        Device &d = Get(deviceIndex);
+        size_t n = size_t(width) * size_t(height) * 3 * sizeof(double);
+        d.filmSum.ensure(n);
+        d.filmSq.ensure(n);
+        PG_CHECK(hipMemsetAsync(d.filmSum.p, 0, n, d.stream));
+        PG_CHECK(hipMemsetAsync(d.filmSq.p, 0, n, d.stream));
+        PG_CHECK(hipStreamSynchronize(d.stream));
+        d.filmW = width;
+        d.filmH = height;
+        return 0;
+    } catch (const std::exception &) {
+        return 1;
+    }
+}
+
+PRGPU_API int prgpu_film_read(int deviceIndex, double *sum, double *sumsq) {
+    std::lock_guard<std::mutex> lock(gMutex);
+    try {
+        Enumerate();
+        Device &d = Get(deviceIndex);
+        if (!d.filmSum.p || !sum) return 1;
+        size_t n = size_t(d.filmW) * size_t(d.filmH) * 3 * sizeof(double);
+        PG_CHECK(hipMemcpyAsync(sum, d.filmSum.p, n, hipMemcpyDeviceToHost, d.stream));
+        if (sumsq) PG_CHECK(hipMemcpyAsync(sumsq, d.filmSq.p, n, hipMemcpyDeviceToHost, d.stream));
+        PG_CHECK(hipStreamSynchronize(d.stream));
+        return 0;
+    } catch (const std::exception &) {
+        return 1;
+    }
+}
+
+PRGPU_API int prgpu_profile(int deviceIndex, int mode) {
+    std::lock_guard<std::mutex> lock(gMutex);
+    try {
+        Enumerate();
+        Device &d = Get(deviceIndex);
+        d.profileMode = mode < 0 ? 0 : (mode > 2 ? 2 : mode);
+        if (d.profileMode >= 2 && !d.travCounters.p) {
+            d.travCounters.ensure(8 * sizeof(unsigned long long));
+            PG_CHECK(hipMemsetAsync(d.travCounters.p, 0, 8 * sizeof(unsigned long long), d.stream));
+            PG_CHECK(hipStreamSynchronize(d.stream));
+        }
+        return 0;
+    } catch (const std::exception &) {
+        return 1;
+    }
+}
+
+PRGPU_API int prgpu_profile_read(int deviceIndex, prgpu::ProfileStatsEx *out, int reset) {
+    std::lock_guard<std::mutex> lock(gMutex);
+    try {
+        Enumerate();
+        Device &d = Get(deviceIndex);
+        if (!out) return 1;
+        unsigned long long c[8] = {0};
+        if (d.travCounters.p) {
+            PG_CHECK(hipMemcpyAsync(c, d.travCounters.p, sizeof(c), hipMemcpyDeviceToHost, d.stream));
+            PG_CHECK(hipStreamSynchronize(d.stream));
+        }
+        *out = d.prof;
+        out->closestRays = int64_t(c[0]);
+        out->closestNodes = int64_t(c[1]);
+        out->closestRecs = int64_t(c[2]);
+        out->shadowRays = int64_t(c[3]);
+        out->shadowNodes = int64_t(c[4]);
+        out->shadowRecs = int64_t(c[5]);
+        out->nodeBytes = int64_t(sizeof(WNode));
+        out->recBytes = int64_t(sizeof(TriRec));
+        if (reset) {
+            d.prof = prgpu::ProfileStatsEx{};
+            if (d.travCounters.p) {
+                PG_CHECK(hipMemsetAsync(d.travCounters.p, 0, 8 * sizeof(unsigned long long), d.stream));
+                PG_CHECK(hipStreamSynchronize(d.stream));
+            }
+        }
+        return 0;
+    } catch (const std::exception &) {
+        return 1;
+    }
+}
+
 PRGPU_API int prgpu_release(int deviceIndex) {
     std::lock_guard<std::mutex> lock(gMutex);
     if (deviceIndex < 0 || deviceIndex >= int(gDevices.size())) return 1;
@@ -267,6 +353,12 @@ PRGPU_API int prgpu_release(int deviceIndex) {
     d.spill.release();
     d.rays.release();
     d.hits.release();
+    d.filmSum.release();
+    d.filmSq.release();
+    d.filmW = d.filmH = 0;
+    d.travCounters.release();
+    for (hipEvent_t e : d.eventPool) (void)hipEventDestroy(e);
+    d.eventPool.clear();
     d.geomKey = d.ptKey = 0;
     return 0;
 }

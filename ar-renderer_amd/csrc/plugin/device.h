@@ -67,6 +67,14 @@ struct Device {
     uint64_t ptKey = 0;
     std::unique_ptr<PTScene> pt;
     DevBuf rays, hits;
+    // Device-resident FP64 film (prgpu_film_reset / prgpu_film_read)
+    DevBuf filmSum, filmSq;
+    int filmW = 0, filmH = 0;
+    // Profiling (prgpu_profile)
+    int profileMode = 0;
+    prgpu::ProfileStatsEx prof{};
+    DevBuf travCounters;  // 6 x uint64: closest rays/nodes/recs, shadow rays/nodes/recs
+    std::vector<hipEvent_t> eventPool;
     Device();
     ~Device();
 };

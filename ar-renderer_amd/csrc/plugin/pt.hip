@@ -194,13 +194,15 @@ struct PTScene {
     int lightCount = 0, lnodeCount = 0, infCount = 0, powerCount = 0;
     float powerInt = 0;
     float worldR = 1;
-    int hasInterfaces = 0;
+    int hasInterfaces = 0, hasMedia = 0;
     float cieYInt = 1;
     // host copies for inspection
     std::vector<LNode> hostNodes;
     // batch buffers
     size_t cap = 0;
-    DevBuf o, d, rng, lam, lpdf, beta, rpath, L, Lh, prevP, prevN, st, hit, q0, q1, pfilm, xyz;
+    DevBuf o, d, rng, lam, lpdf, beta, rpath, L, Lh, prevP, prevN, st, hit, q0, q1, qm, pfilm, xyz;
+    DevBuf h0, h1, h2, h3;
+    DevBuf sceneBuf;  // the DevScene the kernels read through a pointer
     DevBuf so, sd, sp, sn, sf, sL, sb, sq;
     int *countsHost = nullptr;
     DevBuf counts;
@@ -444,8 +446,12 @@ void EnsurePTScene(Device &dev, const prgpu::SceneDescEx &sd) {
                      sd.materials[p.material].type == prgpu::MatBlack;
         if (noMat && p.light < 0) pt->hasInterfaces = 1;
     }
+    pt->hasMedia = (sd.cameraMedium >= 0) ? 1 : 0;
+    for (int i = 0; i < sd.primCount && !pt->hasMedia; ++i)
+        if (sd.prims[i].mediumInside >= 0 || sd.prims[i].mediumOutside >= 0) pt->hasMedia = 1;
     PG_CHECK(hipHostMalloc(reinterpret_cast<void **>(&pt->countsHost), 64, hipHostMallocDefault));
     pt->counts.ensure(64);
+    PG_CHECK(hipMemsetAsync(pt->counts.p, 0, 64, s));
     PG_CHECK(hipStreamSynchronize(s));
     dev.pt = std::move(pt);
     dev.ptKey = key;
@@ -513,6 +519,13 @@ DevScene MakeDevScene(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::Re
     return s;
 }
 
+// The kernels read the scene through a pointer into HBM: passing the struct by value and then
+// referencing its fields would force a private copy of the kernel argument.
+const DevScene *UploadDevScene(Device &dev, const DevScene &s) {
+    dev.pt->sceneBuf.upload(&s, 1, dev.stream);
+    return dev.pt->sceneBuf.as<DevScene>();
+}
+
 // =============================================================================================
 // Kernels
 
@@ -531,8 +544,11 @@ struct Bufs {
     float4 *lam, *lpdf, *beta, *rpath, *L, *Lh, *prevP, *prevN;
     int4 *st;
     float4 *hit;
+    float4 *h0, *h1, *h2, *h3;  // surface hit records for the material stage
     int *q0, *q1;
-    int *counts;  // [0] queue A, [1] queue B, [2] shadow queue
+    int *qm;     // material queues, QCount x qcap
+    int qcap;
+    int *counts;  // [0] queue A, [1] queue B, [2] shadow queue, [3..5] material queues
     float2 *pfilm;
     float4 *xyz;
     // shadow records
@@ -541,6 +557,18 @@ struct Bufs {
 
 constexpr int kBlock = 256;
 constexpr uint32_t kSpecularFlag = 1;
+
+// Queue append aggregated per wave: the active lanes take consecutive entries with one atomic.
+__device__ __forceinline__ int wave_append(int *counter) {
+    const unsigned long long mask = __ballot(1);
+    const int lane = __lane_id();
+    const int leader = __ffsll(mask) - 1;
+    const int prefix = __popcll(mask & ((1ull << lane) - 1ull));
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader);
+    return base + prefix;
+}
 
 __device__ __forceinline__ Pcg load_rng(const Bufs &b, int slot) {
     ulonglong2 r = b.rng[slot];
@@ -556,7 +584,8 @@ __device__ __forceinline__ V2 get2(Pcg &r) {
 }
 
 // Camera samples (src/integrators/path.cpp:61-70, src/cameras/camera.cpp:36-48).
-__global__ __launch_bounds__(kBlock) void k_camera(DevScene s, Params p, Bufs b, int n, int npix, int sw) {
+__global__ __launch_bounds__(kBlock) void k_camera(const DevScene *__restrict__ sp, Params p, Bufs b, int n, int npix, int sw) {
+    const DevScene &s = *sp;
     int slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= n) return;
     int pass = slot / npix, pix = slot - pass * npix;
@@ -603,19 +632,29 @@ __global__ __launch_bounds__(kBlock) void k_camera(DevScene s, Params p, Bufs b,
     b.q0[slot] = slot;
 }
 
-__global__ __launch_bounds__(kTraceBlock) void k_intersect(DevScene s, Bufs b, const int *queue, const int *count) {
+template <bool kStats>
+__global__ __launch_bounds__(kTraceBlock) void k_intersect(const DevScene *__restrict__ sp, Bufs b, const int *queue,
+                                                           const int *count, unsigned long long *counters) {
+    const DevScene &s = *sp;
     __shared__ uint32_t lds[kStackLds * kTraceBlock];
     const int n = *count;
     const int tid = int(blockIdx.x * blockDim.x + threadIdx.x);
     const int stride = int(gridDim.x * blockDim.x);
+    TraceStats st;
+    uint32_t rays = 0;
     for (int i = tid; i < n; i += stride) {
         int slot = queue[i];
         float4 o = b.o[slot], d = b.d[slot];
         DRay r{o.x, o.y, o.z, d.x, d.y, d.z};
         DHit h;
-        TraceStats st;
-        bool hit = dev_trace<false, false>(s.trace, r, kInfF, &h, lds, tid, &st);
+        bool hit = dev_trace<false, kStats>(s.trace, r, kInfF, &h, lds, tid, &st);
         b.hit[slot] = hit ? make_float4(h.t, __int_as_float(h.prim), h.b1, h.b2) : make_float4(0, __int_as_float(-1), 0, 0);
+        ++rays;
+    }
+    if (kStats && rays) {
+        atomicAdd(&counters[0], (unsigned long long)rays);
+        atomicAdd(&counters[1], (unsigned long long)st.nodes);
+        atomicAdd(&counters[2], (unsigned long long)st.recs);
     }
 }
 
@@ -636,8 +675,8 @@ __device__ __forceinline__ void add_contrib(PathRegs &r, const SS &c) {  // path
 }
 
 // Russian roulette (path.cpp:181-188, 228-235); true when the path is terminated.
-__device__ __forceinline__ bool roulette(const Params &p, PathRegs &r) {
-    if (p.rr && r.depth > p.rrStart) {
+__device__ __forceinline__ bool roulette(int rr, int rrStart, PathRegs &r) {
+    if (rr && r.depth > rrStart) {
         SS eff;
         if (r.lam.terminated()) {
             eff = r.beta;
@@ -657,7 +696,8 @@ __device__ __forceinline__ bool roulette(const Params &p, PathRegs &r) {
 
 // Next-event estimation up to the transmittance (path.cpp:82-111): samples a light and, when
 // the unoccluded contribution is nonzero, appends a shadow record that k_shadow completes.
-__device__ void sample_ld(const DevScene &s, const Bufs &b, int slot, PathRegs &r, V3 p, V3 n, V3 wo, const BSDF *bsdf,
+template <bool kMedia>
+__device__ __forceinline__ void sample_ld(const DevScene &s, const Bufs &b, int slot, PathRegs &r, V3 p, V3 n, V3 wo, const BSDF *bsdf,
                           float g, V3 ns, bool surface, int mIn, int mOut) {
     float uLight = r.rng.uniform();
     V2 uL = get2(r.rng);
@@ -684,8 +724,8 @@ __device__ void sample_ld(const DevScene &s, const Bufs &b, int slot, PathRegs &
     V3 o = surface ? offset_origin(p, n, ls.pl - p) : p;
     V3 target = offset_origin(ls.pl, ls.nl, o - ls.pl);
     V3 d = target - o;
-    int medium = surface ? (dot(d, n) > 0 ? mOut : mIn) : r.medium;
-    int rec = atomicAdd(&b.counts[2], 1);
+    int medium = kMedia ? (surface ? (dot(d, n) > 0 ? mOut : mIn) : r.medium) : -1;
+    int rec = wave_append(&b.counts[2]);
     b.so[rec] = make_float4(o.x, o.y, o.z, __int_as_float(medium));
     b.sd[rec] = make_float4(d.x, d.y, d.z, __int_as_float(slot));
     b.sp[rec] = make_float4(ls.pl.x, ls.pl.y, ls.pl.z, k);
@@ -695,7 +735,7 @@ __device__ void sample_ld(const DevScene &s, const Bufs &b, int slot, PathRegs &
     st4(b.sb, rec, r.beta);
 }
 
-__device__ void add_emitted(const DevScene &s, PathRegs &r, int li, const SS &Le, V3 dir) {  // path.cpp:143-153
+__device__ __forceinline__ void add_emitted(const DevScene &s, PathRegs &r, int li, const SS &Le, V3 dir) {  // path.cpp:143-153
     if (!Le.nonzero()) return;
     if (r.depth == 0 || (r.flags & kSpecularFlag)) {
         add_contrib(r, r.beta * Le);
@@ -707,44 +747,51 @@ __device__ void add_emitted(const DevScene &s, PathRegs &r, int li, const SS &Le
     }
 }
 
-// One iteration of the reference's Li loop (path.cpp:155-236) for every queued path.
-__global__ __launch_bounds__(kBlock) void k_shade(DevScene s, Params p, Bufs b, const int *queue, const int *count,
-                                                  int *nextQueue, int *nextCount) {
+// ---- stage 1 of a bounce: everything up to the BSDF (path.cpp:155-210) ----------------------------
+// Medium sampling (and, for scattering events, the whole medium vertex), escape and area
+// emission with MIS, interfaces, the depth limit. Paths that continue at a surface are
+// compacted into one queue per BSDF family together with their hit record.
+enum MatQueue : int { QDiffuse = 0, QConductor = 1, QDielectric = 2, QCount = 3 };
+
+template <bool kMedia, bool kInterfaces>
+__global__ __launch_bounds__(kBlock) void k_surface(const DevScene *__restrict__ sp, Params p, Bufs b, const int *queue,
+                                                    const int *count, int *nextQueue, int *nextCount) {
+    const DevScene &s = *sp;
     const int n = *count;
     const int stride = int(gridDim.x * blockDim.x);
     for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += stride) {
         const int slot = queue[i];
         PathRegs r;
+        float4 o = b.o[slot], d = b.d[slot];
+        r.ro = V3(o.x, o.y, o.z);
+        r.prevPdf = o.w;
+        r.rd = V3(d.x, d.y, d.z);
+        r.etaScale = d.w;
+        float4 l = b.lam[slot], lp = b.lpdf[slot];
+        r.lam = {{l.x, l.y, l.z, l.w}, {lp.x, lp.y, lp.z, lp.w}};
+        r.beta = ld4(b.beta, slot);
+        r.rpath = ld4(b.rpath, slot);
+        r.L = ld4(b.L, slot);
+        r.Lh = ld4(b.Lh, slot);
+        int4 st = b.st[slot];
+        r.depth = st.x;
+        r.flags = st.y;
+        r.medium = st.z;
+        r.crossings = st.w;
         {
-            float4 o = b.o[slot], d = b.d[slot];
-            r.ro = V3(o.x, o.y, o.z);
-            r.prevPdf = o.w;
-            r.rd = V3(d.x, d.y, d.z);
-            r.etaScale = d.w;
-            r.rng = load_rng(b, slot);
-            float4 l = b.lam[slot], lp = b.lpdf[slot];
-            r.lam = {{l.x, l.y, l.z, l.w}, {lp.x, lp.y, lp.z, lp.w}};
-            r.beta = ld4(b.beta, slot);
-            r.rpath = ld4(b.rpath, slot);
-            r.L = ld4(b.L, slot);
-            r.Lh = ld4(b.Lh, slot);
             float4 pp = b.prevP[slot], pn = b.prevN[slot];
             r.prevP = V3(pp.x, pp.y, pp.z);
             r.prevN = V3(pn.x, pn.y, pn.z);
-            int4 st = b.st[slot];
-            r.depth = st.x;
-            r.flags = st.y;
-            r.medium = st.z;
-            r.crossings = st.w;
         }
         const float4 hv = b.hit[slot];
         const int prim = __float_as_int(hv.y);
         const bool hit = prim >= 0;
         const float tHit = hv.x;
-        bool alive = true;
-        bool done = false;  // this bounce has been fully handled
+        bool alive = true, toMaterial = false, done = false;
+        int mq = 0;
 
-        if (r.medium >= 0) {
+        if (kMedia && r.medium >= 0) {
+            r.rng = load_rng(b, slot);
             DistSample ds = medium_sample(s, r.medium, r.ro, r.rd, hit ? tHit : kInfF, r.rng.uniform(), r.lam);
             r.beta = r.beta * ds.weight;
             r.rpath = r.rpath * ds.pdfRatio;
@@ -759,7 +806,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, Params p, Bufs b, 
                     V3 mp = r.ro + r.rd * ds.t;
                     V3 mwo = -r.rd;
                     float g = clampf(s.media[r.medium].g, -0.99f, 0.99f);
-                    sample_ld(s, b, slot, r, mp, V3(), mwo, nullptr, g, V3(), false, -1, -1);
+                    sample_ld<kMedia>(s, b, slot, r, mp, V3(), mwo, nullptr, g, V3(), false, -1, -1);
                     float phasePdf;
                     V3 wi = sample_hg(mwo, g, get2(r.rng), &phasePdf);
                     if (phasePdf == 0) {
@@ -771,24 +818,27 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, Params p, Bufs b, 
                         r.flags &= ~kSpecularFlag;
                         r.ro = mp;
                         r.rd = wi;
-                        if (roulette(p, r)) alive = false;
+                        if (roulette(p.rr, p.rrStart, r)) alive = false;
                     }
                 }
             }
+            b.rng[slot] = make_ulonglong2(r.rng.state, r.rng.inc);
         }
 
         if (!done) {
             if (!hit) {
-                for (int li = 0; li < s.lightCount; ++li)
-                    if (s.lights[li].kind == prgpu::LightExEnvironment)
-                        add_emitted(s, r, li, spec_sample(s.spec, s.lights[li].spectrum, r.lam) * s.lights[li].scale, r.rd);
+                for (int li = 0; li < s.infCount; ++li) {
+                    int L = s.infinite[li];
+                    if (s.lights[L].kind == prgpu::LightExEnvironment)
+                        add_emitted(s, r, L, spec_sample(s.spec, s.lights[L].spectrum, r.lam) * s.lights[L].scale, r.rd);
+                }
                 alive = false;
             } else {
                 const prgpu::Primitive &pr = s.prims[prim];
                 Surf si;
                 interaction(pr, r.ro, r.rd, tHit, hv.z, hv.w, &si);
                 if (pr.light >= 0) add_emitted(s, r, pr.light, area_L(s, s.lights[pr.light], si.n, -r.rd, r.lam), r.rd);
-                if (is_interface(s, pr)) {
+                if (kInterfaces && is_interface(s, pr)) {
                     if (++r.crossings > 1000) {
                         alive = false;
                     } else {
@@ -798,59 +848,128 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, Params p, Bufs b, 
                 } else if (!has_bsdf(s, pr)) {
                     alive = false;
                 } else {
-                    BSDF bsdf = eval_surface(s, s.mats[pr.material], si, r.lam);
+                    const prgpu::Material &m = s.mats[pr.material];
+                    // A dispersive dielectric keeps only the hero wavelength (material.cpp:106-110),
+                    // also when the depth limit ends the path here.
+                    if (m.type == prgpu::MatDielectric && !spec_is_constant(s.spec, m.eta)) r.lam.terminate_secondary();
                     if (r.depth++ >= p.maxDepth) {
                         alive = false;
                     } else {
-                        if (bsdf.non_specular())
-                            sample_ld(s, b, slot, r, si.p, si.n, si.wo, &bsdf, 0, si.ns, true, pr.mediumInside,
-                                      pr.mediumOutside);
-                        float uc = r.rng.uniform();
-                        V2 u2 = get2(r.rng);
-                        BSample bs;
-                        if (!bsdf.sample(si.wo, uc, u2, &bs)) {
-                            alive = false;
-                        } else {
-                            r.beta = r.beta * (bs.f * (absdot(bs.wi, si.ns) / bs.pdf));
-                            r.prevPdf = bs.pdf;
-                            r.flags = (bs.flags & kSpecular) ? (r.flags | kSpecularFlag) : (r.flags & ~kSpecularFlag);
-                            if (bs.flags & kTrans) r.etaScale *= sq(bs.eta);
-                            r.prevP = si.p;
-                            r.prevN = si.n;
-                            r.ro = offset_origin(si.p, si.n, bs.wi);
-                            r.rd = bs.wi;
-                            r.medium = dot(bs.wi, si.n) > 0 ? pr.mediumOutside : pr.mediumInside;
-                            if (!r.beta.nonzero() || r.beta.has_nan()) alive = false;
-                            else if (roulette(p, r)) alive = false;
-                        }
+                        toMaterial = true;
+                        mq = m.type == prgpu::MatDiffuse ? QDiffuse : (m.type == prgpu::MatConductor ? QConductor : QDielectric);
+                        uint32_t media = uint32_t(pr.mediumInside & 0xffff) | (uint32_t(pr.mediumOutside & 0xffff) << 16);
+                        b.h0[slot] = make_float4(si.p.x, si.p.y, si.p.z, __int_as_float(pr.material));
+                        b.h1[slot] = make_float4(si.n.x, si.n.y, si.n.z, si.u);
+                        b.h2[slot] = make_float4(si.ns.x, si.ns.y, si.ns.z, si.v);
+                        b.h3[slot] = make_float4(si.dpdus.x, si.dpdus.y, si.dpdus.z, __uint_as_float(media));
                     }
                 }
             }
         }
 
-        // Store the state this kernel changed.
-        b.o[slot] = make_float4(r.ro.x, r.ro.y, r.ro.z, r.prevPdf);
-        b.d[slot] = make_float4(r.rd.x, r.rd.y, r.rd.z, r.etaScale);
-        b.rng[slot] = make_ulonglong2(r.rng.state, r.rng.inc);
         b.lpdf[slot] = make_float4(r.lam.pdf[0], r.lam.pdf[1], r.lam.pdf[2], r.lam.pdf[3]);
-        st4(b.beta, slot, r.beta);
-        st4(b.rpath, slot, r.rpath);
         st4(b.L, slot, r.L);
         st4(b.Lh, slot, r.Lh);
-        b.prevP[slot] = make_float4(r.prevP.x, r.prevP.y, r.prevP.z, 0.f);
-        b.prevN[slot] = make_float4(r.prevN.x, r.prevN.y, r.prevN.z, 0.f);
         b.st[slot] = make_int4(r.depth, r.flags, r.medium, r.crossings);
-        if (alive) nextQueue[atomicAdd(nextCount, 1)] = slot;
+        if (kMedia || kInterfaces) {
+            b.o[slot] = make_float4(r.ro.x, r.ro.y, r.ro.z, r.prevPdf);
+            b.d[slot] = make_float4(r.rd.x, r.rd.y, r.rd.z, r.etaScale);
+            st4(b.beta, slot, r.beta);
+            st4(b.rpath, slot, r.rpath);
+            b.prevP[slot] = make_float4(r.prevP.x, r.prevP.y, r.prevP.z, 0.f);
+            b.prevN[slot] = make_float4(r.prevN.x, r.prevN.y, r.prevN.z, 0.f);
+        }
+        const int target = toMaterial ? mq : (alive ? QCount : -1);
+        if (target == QDiffuse) b.qm[QDiffuse * b.qcap + wave_append(&b.counts[3 + QDiffuse])] = slot;
+        if (target == QConductor) b.qm[QConductor * b.qcap + wave_append(&b.counts[3 + QConductor])] = slot;
+        if (target == QDielectric) b.qm[QDielectric * b.qcap + wave_append(&b.counts[3 + QDielectric])] = slot;
+        if (target == QCount) nextQueue[wave_append(nextCount)] = slot;
+    }
+}
+
+// ---- stage 2: one BSDF family (path.cpp:208-235) ----------------------------------------------------
+// Builds the BSDF, next-event estimation when it has a non-specular lobe, BSDF sampling,
+// throughput update, spawn, Russian roulette.
+template <int kType, bool kMedia>
+__global__ __launch_bounds__(kBlock) void k_material(const DevScene *__restrict__ sp, Params p, Bufs b, int *nextQueue,
+                                                     int *nextCount) {
+    const DevScene &s = *sp;
+    const int n = b.counts[3 + kType];
+    const int *queue = b.qm + kType * b.qcap;
+    const int stride = int(gridDim.x * blockDim.x);
+    for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += stride) {
+        const int slot = queue[i];
+        PathRegs r;
+        float4 d = b.d[slot];
+        r.rd = V3(d.x, d.y, d.z);
+        r.etaScale = d.w;
+        r.rng = load_rng(b, slot);
+        float4 l = b.lam[slot], lp = b.lpdf[slot];
+        r.lam = {{l.x, l.y, l.z, l.w}, {lp.x, lp.y, lp.z, lp.w}};
+        r.beta = ld4(b.beta, slot);
+        r.rpath = ld4(b.rpath, slot);
+        int4 st = b.st[slot];
+        r.depth = st.x;
+        r.flags = st.y;
+        r.medium = st.z;
+        float4 h0 = b.h0[slot], h1 = b.h1[slot], h2 = b.h2[slot], h3 = b.h3[slot];
+        Surf si;
+        si.p = V3(h0.x, h0.y, h0.z);
+        si.n = V3(h1.x, h1.y, h1.z);
+        si.ns = V3(h2.x, h2.y, h2.z);
+        si.dpdus = V3(h3.x, h3.y, h3.z);
+        si.u = h1.w;
+        si.v = h2.w;
+        si.wo = -normalize(r.rd);
+        const prgpu::Material &m = s.mats[__float_as_int(h0.w)];
+        uint32_t media = __float_as_uint(h3.w);
+        int mIn = int(int16_t(media & 0xffff)), mOut = int(int16_t(media >> 16));
+        BSDF bsdf = eval_surface<kType>(s, m, si, r.lam);
+        bool alive = true;
+        if (bsdf.non_specular())
+            sample_ld<kMedia>(s, b, slot, r, si.p, si.n, si.wo, &bsdf, 0, si.ns, true, mIn, mOut);
+        float uc = r.rng.uniform();
+        V2 u2 = get2(r.rng);
+        BSample bs;
+        if (!bsdf.sample(si.wo, uc, u2, &bs)) {
+            alive = false;
+        } else {
+            r.beta = r.beta * (bs.f * (absdot(bs.wi, si.ns) / bs.pdf));
+            r.prevPdf = bs.pdf;
+            r.flags = (bs.flags & kSpecular) ? (r.flags | kSpecularFlag) : (r.flags & ~kSpecularFlag);
+            if (bs.flags & kTrans) r.etaScale *= sq(bs.eta);
+            r.ro = offset_origin(si.p, si.n, bs.wi);
+            r.rd = bs.wi;
+            if (kMedia) r.medium = dot(bs.wi, si.n) > 0 ? mOut : mIn;
+            if (!r.beta.nonzero() || r.beta.has_nan()) alive = false;
+            else if (roulette(p.rr, p.rrStart, r)) alive = false;
+        }
+        b.rng[slot] = make_ulonglong2(r.rng.state, r.rng.inc);
+        if (alive) {
+            b.o[slot] = make_float4(r.ro.x, r.ro.y, r.ro.z, r.prevPdf);
+            b.d[slot] = make_float4(r.rd.x, r.rd.y, r.rd.z, r.etaScale);
+            st4(b.beta, slot, r.beta);
+            b.prevP[slot] = make_float4(si.p.x, si.p.y, si.p.z, 0.f);
+            b.prevN[slot] = make_float4(si.n.x, si.n.y, si.n.z, 0.f);
+            b.st[slot] = make_int4(r.depth, r.flags, r.medium, st.w);
+            nextQueue[wave_append(nextCount)] = slot;
+        }
     }
 }
 
 // Transmittance of the shadow records (src/scene/scene.cpp:41-59) and the deferred add().
-__global__ __launch_bounds__(kTraceBlock) void k_shadow(DevScene s, Bufs b, const int *count) {
+template <bool kStats, bool kMedia, bool kInterfaces>
+__global__ __launch_bounds__(kTraceBlock) void k_shadow(const DevScene *__restrict__ sp, Bufs b, const int *count,
+                                                        unsigned long long *counters) {
+    const DevScene &s = *sp;
     __shared__ uint32_t lds[kStackLds * kTraceBlock];
     const int n = *count;
     const int tid = int(blockIdx.x * blockDim.x + threadIdx.x);
     const int stride = int(gridDim.x * blockDim.x);
+    TraceStats ts;
+    uint32_t rays = 0;
     for (int i = tid; i < n; i += stride) {
+        ++rays;
         float4 so = b.so[i], sd = b.sd[i], sp = b.sp[i], sn = b.sn[i];
         const int slot = __float_as_int(sd.w);
         V3 ro(so.x, so.y, so.z), rd(sd.x, sd.y, sd.z), pl(sp.x, sp.y, sp.z), nl(sn.x, sn.y, sn.z);
@@ -869,23 +988,22 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(DevScene s, Bufs b, cons
             }
             DRay r{ro.x, ro.y, ro.z, rd.x, rd.y, rd.z};
             DHit h;
-            TraceStats ts;
-            if (!s.hasInterfaces) {
-                if (dev_trace<true, false>(s.trace, r, tMax, &h, lds, tid, &ts)) {
+            if (!kInterfaces) {
+                if (dev_trace<true, kStats>(s.trace, r, tMax, &h, lds, tid, &ts)) {
                     T = SS(0.f);
-                } else if (medium >= 0) {
+                } else if (kMedia && medium >= 0) {
                     float l = len(rd);
                     T = T * medium_tr(s, medium, ro, rd / l, 1.f * l, lam);
                 }
                 break;
             }
-            bool hit = dev_trace<false, false>(s.trace, r, tMax, &h, lds, tid, &ts);
+            bool hit = dev_trace<false, kStats>(s.trace, r, tMax, &h, lds, tid, &ts);
             const prgpu::Primitive *pr = hit ? &s.prims[h.prim] : nullptr;
             if (hit && !is_interface(s, *pr)) {
                 T = SS(0.f);
                 break;
             }
-            if (medium >= 0) {
+            if (kMedia && medium >= 0) {
                 float l = len(rd);
                 T = T * medium_tr(s, medium, ro, rd / l, (hit ? h.t : 1.f) * l, lam);
             }
@@ -905,10 +1023,17 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(DevScene s, Bufs b, cons
         st4(b.Lh, slot, Lh);
         if (a > 0 && is_finite(a)) st4(b.L, slot, ld4(b.L, slot) + c / a);
     }
+    if (kStats && rays) {
+        atomicAdd(&counters[3], (unsigned long long)rays);
+        atomicAdd(&counters[4], (unsigned long long)ts.nodes);
+        atomicAdd(&counters[5], (unsigned long long)ts.recs);
+    }
 }
 
 // Final radiance per sample: film XYZ (path.cpp:71-73, film.cpp:133-139) or a parity record.
-__global__ __launch_bounds__(kBlock) void k_finish(DevScene s, Params p, Bufs b, int n, prgpu::SampleRecordEx *rec) {
+__global__ __launch_bounds__(kBlock) void k_finish(const DevScene *__restrict__ sp, Params p, Bufs b, int n,
+                                                   prgpu::SampleRecordEx *rec) {
+    const DevScene &s = *sp;
     int slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= n) return;
     float4 lp = b.lpdf[slot], l = b.lam[slot];
@@ -966,8 +1091,9 @@ __device__ __forceinline__ float filter_1d(const DevScene &s, float x) {
 // Film: each output pixel gathers the samples whose filter footprint covers it (the CPU film's
 // AddXYZ splat, film.cpp:141-157, evaluated per pixel so no atomics are needed), one pass at a
 // time, and adds the pass value and its square to the FP64 accumulators.
-__global__ __launch_bounds__(kBlock) void k_gather(DevScene s, Bufs b, int passes, int npix, int sw, int reach,
-                                                   double *sum, double *sumsq) {
+__global__ __launch_bounds__(kBlock) void k_gather(const DevScene *__restrict__ sp, Bufs b, int passes, int npix, int sw,
+                                                   int reach, double *sum, double *sumsq) {
+    const DevScene &s = *sp;
     int pix = blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= s.width * s.height) return;
     int y = pix / s.width, x = pix - y * s.width;
@@ -1020,6 +1146,8 @@ void EnsureBatch(PTScene &pt, size_t n) {
     auto f4 = [&](DevBuf &b) { b.ensure(n * sizeof(float4)); };
     f4(pt.o); f4(pt.d); f4(pt.lam); f4(pt.lpdf); f4(pt.beta); f4(pt.rpath); f4(pt.L); f4(pt.Lh);
     f4(pt.prevP); f4(pt.prevN); f4(pt.hit); f4(pt.xyz);
+    f4(pt.h0); f4(pt.h1); f4(pt.h2); f4(pt.h3);
+    pt.qm.ensure(n * 3 * sizeof(int));
     f4(pt.so); f4(pt.sd); f4(pt.sp); f4(pt.sn); f4(pt.sf); f4(pt.sL); f4(pt.sb);
     pt.rng.ensure(n * sizeof(ulonglong2));
     pt.st.ensure(n * sizeof(int4));
@@ -1036,6 +1164,9 @@ Bufs MakeBufs(PTScene &pt) {
     b.rpath = pt.rpath.as<float4>(); b.L = pt.L.as<float4>(); b.Lh = pt.Lh.as<float4>();
     b.prevP = pt.prevP.as<float4>(); b.prevN = pt.prevN.as<float4>(); b.st = pt.st.as<int4>();
     b.hit = pt.hit.as<float4>(); b.q0 = pt.q0.as<int>(); b.q1 = pt.q1.as<int>(); b.counts = pt.counts.as<int>();
+    b.h0 = pt.h0.as<float4>(); b.h1 = pt.h1.as<float4>(); b.h2 = pt.h2.as<float4>(); b.h3 = pt.h3.as<float4>();
+    b.qm = pt.qm.as<int>();
+    b.qcap = int(pt.cap);
     b.pfilm = pt.pfilm.as<float2>(); b.xyz = pt.xyz.as<float4>();
     b.so = pt.so.as<float4>(); b.sd = pt.sd.as<float4>(); b.sp = pt.sp.as<float4>(); b.sn = pt.sn.as<float4>();
     b.sf = pt.sf.as<float4>(); b.sL = pt.sL.as<float4>(); b.sb = pt.sb.as<float4>();
@@ -1053,14 +1184,68 @@ Params MakeParams(const prgpu::RenderParamsEx &rp) {
     return p;
 }
 
+// Event-timed launches when profiling is on: each launch is bracketed by two events on the
+// plugin's stream; durations are collected after the batch's synchronisation.
+struct Timer {
+    Device &dev;
+    struct Pending {
+        int kind;
+        hipEvent_t a, b;
+        int64_t items;
+        const int *itemsHost;  // items read after the sync (queue counts), or null
+    };
+    std::vector<Pending> pending;
+    explicit Timer(Device &d) : dev(d) {}
+    hipEvent_t Get() {
+        if (dev.eventPool.empty()) {
+            hipEvent_t e;
+            PG_CHECK(hipEventCreate(&e));
+            return e;
+        }
+        hipEvent_t e = dev.eventPool.back();
+        dev.eventPool.pop_back();
+        return e;
+    }
+    template <class F> void Run(int kind, int64_t items, const int *itemsHost, F launch) {
+        if (!dev.profileMode) {
+            launch();
+            return;
+        }
+        Pending p{kind, Get(), Get(), items, itemsHost};
+        PG_CHECK(hipEventRecord(p.a, dev.stream));
+        launch();
+        PG_CHECK(hipEventRecord(p.b, dev.stream));
+        pending.push_back(p);
+    }
+    // Call after a stream synchronisation; itemsHost values are read at this point.
+    void Collect() {
+        for (Pending &p : pending) {
+            float ms = 0;
+            PG_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
+            dev.prof.ms[p.kind] += ms;
+            dev.prof.launches[p.kind] += 1;
+            dev.prof.items[p.kind] += p.itemsHost ? *p.itemsHost : p.items;
+            dev.eventPool.push_back(p.a);
+            dev.eventPool.push_back(p.b);
+        }
+        pending.clear();
+    }
+};
+
 // Traces one batch of `passes` passes (samples firstSample .. firstSample + passes - 1) to
 // completion; the per-sample results are left in the xyz / record buffers.
-void TraceBatch(Device &dev, const DevScene &s, Params p, int passes, int npix, int sw) {
+void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, int passes, int npix, int sw) {
     PTScene &pt = *dev.pt;
     hipStream_t st = dev.stream;
     const int n = passes * npix;
     Bufs b = MakeBufs(pt);
-    hipLaunchKernelGGL(k_camera, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s, p, b, n, npix, sw);
+    Timer timer(dev);
+    const bool stats = dev.profileMode >= 2;
+    const bool media = pt.hasMedia != 0, interfaces = pt.hasInterfaces != 0;
+    unsigned long long *ctr = dev.travCounters.as<unsigned long long>();
+    timer.Run(prgpu::KernCamera, n, nullptr, [&] {
+        hipLaunchKernelGGL(k_camera, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, ds, p, b, n, npix, sw);
+    });
     PG_CHECK(hipGetLastError());
     int active = n;
     pt.countsHost[0] = n;
@@ -1068,19 +1253,44 @@ void TraceBatch(Device &dev, const DevScene &s, Params p, int passes, int npix, 
     int *cur = b.q0, *nxt = b.q1;
     int curIdx = 0;
     const int shadeGrid = std::max(1, std::min((active + kBlock - 1) / kBlock, dev.cus * 16));
-    for (int bounce = 0; active > 0; ++bounce) {
+    // Queue sizes per bounce, read back after each synchronisation for the profile.
+    std::vector<int> shadowCounts;
+    shadowCounts.reserve(64);
+    while (active > 0) {
         const int nxtIdx = 1 - curIdx;
         PG_CHECK(hipMemsetAsync(b.counts + nxtIdx, 0, sizeof(int), st));
-        PG_CHECK(hipMemsetAsync(b.counts + 2, 0, sizeof(int), st));
-        hipLaunchKernelGGL(k_intersect, dim3(TraceGrid(dev, active)), dim3(kTraceBlock), 0, st, s, b, cur,
-                           b.counts + curIdx);
+        PG_CHECK(hipMemsetAsync(b.counts + 2, 0, 4 * sizeof(int), st));
+        const int tgrid = TraceGrid(dev, active);
+        timer.Run(prgpu::KernIntersect, active, nullptr, [&] {
+            if (stats)
+                hipLaunchKernelGGL(k_intersect<true>, dim3(tgrid), dim3(kTraceBlock), 0, st, ds, b, cur, b.counts + curIdx, ctr);
+            else
+                hipLaunchKernelGGL(k_intersect<false>, dim3(tgrid), dim3(kTraceBlock), 0, st, ds, b, cur, b.counts + curIdx, ctr);
+        });
         int grid = std::max(1, std::min((active + kBlock - 1) / kBlock, shadeGrid));
-        hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, st, s, p, b, cur, b.counts + curIdx, nxt,
-                           b.counts + nxtIdx);
-        hipLaunchKernelGGL(k_shadow, dim3(TraceGrid(dev, active)), dim3(kTraceBlock), 0, st, s, b, b.counts + 2);
+        timer.Run(prgpu::KernShade, active, nullptr, [&] {
+            auto *k = media ? (interfaces ? k_surface<true, true> : k_surface<true, false>)
+                            : (interfaces ? k_surface<false, true> : k_surface<false, false>);
+            hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, st, ds, p, b, cur, b.counts + curIdx, nxt,
+                               b.counts + nxtIdx);
+            auto *kd = media ? k_material<QDiffuse, true> : k_material<QDiffuse, false>;
+            auto *kc = media ? k_material<QConductor, true> : k_material<QConductor, false>;
+            auto *ke = media ? k_material<QDielectric, true> : k_material<QDielectric, false>;
+            hipLaunchKernelGGL(kd, dim3(grid), dim3(kBlock), 0, st, ds, p, b, nxt, b.counts + nxtIdx);
+            hipLaunchKernelGGL(kc, dim3(grid), dim3(kBlock), 0, st, ds, p, b, nxt, b.counts + nxtIdx);
+            hipLaunchKernelGGL(ke, dim3(grid), dim3(kBlock), 0, st, ds, p, b, nxt, b.counts + nxtIdx);
+        });
+        timer.Run(prgpu::KernShadow, 0, &pt.countsHost[2], [&] {
+            auto *k = stats ? (media ? (interfaces ? k_shadow<true, true, true> : k_shadow<true, true, false>)
+                                     : (interfaces ? k_shadow<true, false, true> : k_shadow<true, false, false>))
+                            : (media ? (interfaces ? k_shadow<false, true, true> : k_shadow<false, true, false>)
+                                     : (interfaces ? k_shadow<false, false, true> : k_shadow<false, false, false>));
+            hipLaunchKernelGGL(k, dim3(tgrid), dim3(kTraceBlock), 0, st, ds, b, b.counts + 2, ctr);
+        });
         PG_CHECK(hipGetLastError());
         PG_CHECK(hipMemcpyAsync(pt.countsHost, b.counts, 3 * sizeof(int), hipMemcpyDeviceToHost, st));
         PG_CHECK(hipStreamSynchronize(st));
+        timer.Collect();
         active = pt.countsHost[nxtIdx];
         std::swap(cur, nxt);
         curIdx = nxtIdx;
@@ -1101,31 +1311,54 @@ void RenderExImpl(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::Render
     if (rp.spp <= 0) return;
     EnsurePTScene(dev, sd);
     DevScene s = MakeDevScene(dev, sd, rp);
+    const DevScene *ds = UploadDevScene(dev, s);
     const int sw = s.width + 2 * s.margin, sh = s.height + 2 * s.margin, npix = sw * sh;
     const int per = PassesPerBatch(npix, rp.spp);
     PTScene &pt = *dev.pt;
     EnsureBatch(pt, size_t(per) * size_t(npix));
     const size_t filmN = size_t(s.width) * s.height * 3;
-    DevBuf dsum, dsq;
-    dsum.upload(sum, filmN, dev.stream);
-    if (sumsq) dsq.upload(sumsq, filmN, dev.stream);
+    // Host film: stage it in HBM for the call. Resident film: accumulate in place.
+    DevBuf hsum, hsq;
+    double *dsum, *dsq;
+    if (sum) {
+        hsum.upload(sum, filmN, dev.stream);
+        if (sumsq) hsq.upload(sumsq, filmN, dev.stream);
+        dsum = hsum.as<double>();
+        dsq = sumsq ? hsq.as<double>() : nullptr;
+    } else {
+        if (!dev.filmSum.p || dev.filmW != s.width || dev.filmH != s.height)
+            throw std::runtime_error("resident film not reset for this resolution");
+        dsum = dev.filmSum.as<double>();
+        dsq = dev.filmSq.as<double>();
+    }
     const int reach = int(std::ceil(s.filterRadius + 0.5f));
     Params p = MakeParams(rp);
     for (int done = 0; done < rp.spp; done += per) {
         int passes = std::min(per, rp.spp - done);
         p.firstSample = rp.firstSample + done;
-        TraceBatch(dev, s, p, passes, npix, sw);
+        TraceBatch(dev, s, ds, p, passes, npix, sw);
         Bufs b = MakeBufs(pt);
         int n = passes * npix;
-        hipLaunchKernelGGL(k_finish, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, dev.stream, s, p, b, n,
-                           static_cast<prgpu::SampleRecordEx *>(nullptr));
+        Timer timer(dev);
+        timer.Run(prgpu::KernFinish, n, nullptr, [&] {
+            hipLaunchKernelGGL(k_finish, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, dev.stream, ds, p, b, n,
+                               static_cast<prgpu::SampleRecordEx *>(nullptr));
+        });
         int pixels = s.width * s.height;
-        hipLaunchKernelGGL(k_gather, dim3((pixels + kBlock - 1) / kBlock), dim3(kBlock), 0, dev.stream, s, b, passes,
-                           npix, sw, reach, dsum.as<double>(), sumsq ? dsq.as<double>() : static_cast<double *>(nullptr));
+        timer.Run(prgpu::KernGather, int64_t(pixels) * passes, nullptr, [&] {
+            hipLaunchKernelGGL(k_gather, dim3((pixels + kBlock - 1) / kBlock), dim3(kBlock), 0, dev.stream, ds, b,
+                               passes, npix, sw, reach, dsum, dsq);
+        });
         PG_CHECK(hipGetLastError());
+        if (dev.profileMode) {
+            PG_CHECK(hipStreamSynchronize(dev.stream));
+            timer.Collect();
+        }
     }
-    PG_CHECK(hipMemcpyAsync(sum, dsum.p, filmN * sizeof(double), hipMemcpyDeviceToHost, dev.stream));
-    if (sumsq) PG_CHECK(hipMemcpyAsync(sumsq, dsq.p, filmN * sizeof(double), hipMemcpyDeviceToHost, dev.stream));
+    if (sum) {
+        PG_CHECK(hipMemcpyAsync(sum, dsum, filmN * sizeof(double), hipMemcpyDeviceToHost, dev.stream));
+        if (sumsq) PG_CHECK(hipMemcpyAsync(sumsq, dsq, filmN * sizeof(double), hipMemcpyDeviceToHost, dev.stream));
+    }
     PG_CHECK(hipStreamSynchronize(dev.stream));
 }
 
@@ -1133,15 +1366,16 @@ void SampleRadianceImpl(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::
                         prgpu::SampleRecordEx *out) {
     EnsurePTScene(dev, sd);
     DevScene s = MakeDevScene(dev, sd, rp);
+    const DevScene *ds = UploadDevScene(dev, s);
     const int sw = s.width + 2 * s.margin, sh = s.height + 2 * s.margin, npix = sw * sh;
     PTScene &pt = *dev.pt;
     EnsureBatch(pt, size_t(npix));
     Params p = MakeParams(rp);
-    TraceBatch(dev, s, p, 1, npix, sw);
+    TraceBatch(dev, s, ds, p, 1, npix, sw);
     DevBuf rec;
     rec.ensure(size_t(npix) * sizeof(prgpu::SampleRecordEx));
     Bufs b = MakeBufs(pt);
-    hipLaunchKernelGGL(k_finish, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, dev.stream, s, p, b, npix,
+    hipLaunchKernelGGL(k_finish, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, dev.stream, ds, p, b, npix,
                        rec.as<prgpu::SampleRecordEx>());
     PG_CHECK(hipGetLastError());
     PG_CHECK(hipMemcpyAsync(out, rec.p, size_t(npix) * sizeof(prgpu::SampleRecordEx), hipMemcpyDeviceToHost, dev.stream));

@@ -558,17 +558,20 @@ PG_HD int checker_pick(const prgpu::Material &m, float u, float v, int a) {
     return a;
 }
 
+// kFamily: -1 = any material, else the material family the caller guarantees (0 diffuse,
+// 1 conductor, 2 dielectric), which removes the other families' code from the caller.
+template <int kFamily = -1>
 PG_HD BSDF eval_surface(const DevScene &s, const prgpu::Material &m, const Surf &si, Lambda &lam) {
     BSDF b;
     BxDF &bx = b.bx;
     bx.etaD = 1;
     bx.dist.ax = bx.dist.ay = 0;
-    if (m.type == prgpu::MatDiffuse) {
+    if (kFamily == 0 || (kFamily < 0 && m.type == prgpu::MatDiffuse)) {
         SS r = spec_sample(s.spec, checker_pick(m, si.u, si.v, m.albedo), lam);
         for (int i = 0; i < 4; ++i) r.v[i] = clampf(r.v[i], 0.f, 1.f);
         bx.type = BxDiffuse;
         bx.a = r;
-    } else if (m.type == prgpu::MatConductor) {
+    } else if (kFamily == 1 || (kFamily < 0 && m.type == prgpu::MatConductor)) {
         bx.type = BxConductor;
         bx.dist.init(m.alphaX, m.alphaY);
         if (m.eta >= 0 && m.k >= 0) {

@@ -96,6 +96,16 @@ class DeviceInfo(C.Structure):
                 ("isGpu", C.c_int32), ("backend", C.c_char * 32)]
 
 
+class ProfileStatsEx(C.Structure):
+    _fields_ = [("ms", C.c_double * 8), ("launches", C.c_int64 * 8), ("items", C.c_int64 * 8),
+                ("closestRays", C.c_int64), ("closestNodes", C.c_int64), ("closestRecs", C.c_int64),
+                ("shadowRays", C.c_int64), ("shadowNodes", C.c_int64), ("shadowRecs", C.c_int64),
+                ("nodeBytes", C.c_int64), ("recBytes", C.c_int64)]
+
+
+KERNEL_NAMES = ["camera", "intersect", "shade", "shadow", "finish", "gather"]
+
+assert C.sizeof(ProfileStatsEx) == 256
 assert C.sizeof(SceneDescEx) == 208 and C.sizeof(RenderParamsEx) == 40 and C.sizeof(DeviceInfo) == 304
 
 
@@ -274,6 +284,10 @@ class Plugin:
                                             C.c_char_p, C.c_int]
         L.prgpu_render.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_int]
         L.prgpu_release.argtypes = [C.c_int]
+        L.prgpu_film_reset.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.prgpu_film_read.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
+        L.prgpu_profile.argtypes = [C.c_int, C.c_int]
+        L.prgpu_profile_read.argtypes = [C.c_int, C.POINTER(ProfileStatsEx), C.c_int]
 
     def version(self):
         return self.lib.prgpu_api_version(), self.lib.prgpu_api_ext_version()
@@ -318,3 +332,36 @@ class Plugin:
 
     def release(self, device=0):
         return self.lib.prgpu_release(device)
+
+    # ---- device-resident film and profiling --------------------------------------------------
+    def film_reset(self, width, height, device=0):
+        if self.lib.prgpu_film_reset(device, width, height) != 0:
+            raise RuntimeError("prgpu_film_reset failed")
+
+    def render_resident(self, scene, params, device=0):
+        """Renders into the device-resident film (no host transfer)."""
+        err = C.create_string_buffer(1024)
+        self._check(self.lib.prgpu_render_ex(device, scene.desc_ptr(), C.byref(params), None, None, err, 1024), err)
+
+    def film_read(self, width, height, device=0, variance=True):
+        s = np.zeros(width * height * 3, np.float64)
+        sq = np.zeros_like(s) if variance else None
+        if self.lib.prgpu_film_read(device, s.ctypes.data, sq.ctypes.data if sq is not None else None) != 0:
+            raise RuntimeError("prgpu_film_read failed")
+        return s.reshape(height, width, 3), (sq.reshape(height, width, 3) if sq is not None else None)
+
+    def profile(self, mode, device=0):
+        if self.lib.prgpu_profile(device, mode) != 0:
+            raise RuntimeError("prgpu_profile failed")
+
+    def profile_read(self, reset=True, device=0):
+        st = ProfileStatsEx()
+        if self.lib.prgpu_profile_read(device, C.byref(st), 1 if reset else 0) != 0:
+            raise RuntimeError("prgpu_profile_read failed")
+        out = {"kernels": {}}
+        for i, name in enumerate(KERNEL_NAMES):
+            out["kernels"][name] = {"ms": st.ms[i], "launches": st.launches[i], "items": st.items[i]}
+        for f in ("closestRays", "closestNodes", "closestRecs", "shadowRays", "shadowNodes", "shadowRecs", "nodeBytes",
+                  "recBytes"):
+            out[f] = getattr(st, f)
+        return out

@@ -286,6 +286,20 @@ struct SampleRecordEx {
     uint32_t flags;      // bit 0: L holds NaN/Inf (the film drops the sample)
 };
 
+// Profiling counters of one kind of kernel launch (prgpu_profile_read).
+enum KernelKindEx : int {
+    KernCamera = 0, KernIntersect = 1, KernShade = 2, KernShadow = 3, KernFinish = 4, KernGather = 5, KernCount = 6
+};
+struct ProfileStatsEx {
+    double ms[8];          // summed HIP-event durations per kernel kind
+    int64_t launches[8];
+    int64_t items[8];      // rays / paths / pixels processed
+    // Traversal counters (profile mode 2): rays traced, wide nodes fetched, leaf records tested.
+    int64_t closestRays, closestNodes, closestRecs;
+    int64_t shadowRays, shadowNodes, shadowRecs;
+    int64_t nodeBytes, recBytes;  // record sizes of the layout in HBM
+};
+
 } // namespace prgpu
 
 // ---- v4 entry points (reference gpu_api.h:166-173) ------------------------------------------
@@ -313,6 +327,16 @@ PRGPU_API int prgpu_trace_rays(int deviceIndex, const prgpu::SceneDescEx *scene,
 // (params->spp is ignored): out has (width + 2m) * (height + 2m) records in raster order.
 PRGPU_API int prgpu_sample_radiance(int deviceIndex, const prgpu::SceneDescEx *scene, const prgpu::RenderParamsEx *params,
                                     prgpu::SampleRecordEx *out, char *err, int errLen);
+// Device-resident film: prgpu_render_ex with sumXYZ == null adds into an FP64 sum / sum-of-squares
+// film held in HBM (no per-call PCIe transfer); reset zeroes it for a width x height image and
+// read copies it out (sumSqXYZ may be null).
+PRGPU_API int prgpu_film_reset(int deviceIndex, int width, int height);
+PRGPU_API int prgpu_film_read(int deviceIndex, double *sumXYZ, double *sumSqXYZ);
+// Profiling: mode 0 off, 1 HIP-event timing of every kernel launch on the plugin's stream, 2 timing
+// plus traversal counters (a counting variant of the traversal kernels). read() returns the
+// totals since the last reset.
+PRGPU_API int prgpu_profile(int deviceIndex, int mode);
+PRGPU_API int prgpu_profile_read(int deviceIndex, prgpu::ProfileStatsEx *out, int reset);
 // Frees every device buffer the plugin holds for the device.
 PRGPU_API int prgpu_release(int deviceIndex);
 
@@ -342,3 +366,4 @@ static_assert(sizeof(prgpu::SpectrumEx) == 48, "SpectrumEx");
 static_assert(sizeof(prgpu::LightEx) == 48, "LightEx");
 static_assert(sizeof(prgpu::MediumEx) == 120, "MediumEx");
 static_assert(sizeof(prgpu::SampleRecordEx) == 64, "SampleRecordEx");
+static_assert(sizeof(prgpu::ProfileStatsEx) == 256, "ProfileStatsEx");
