@@ -439,6 +439,12 @@ int SceneBuilder::RGBUnbounded(float r, float g, float b) {  // rgb2spec.cpp:309
     return Sigmoid(c[0], c[1], c[2], scale);
 }
 
+int SceneBuilder::Scaled(int id, float f) {
+    prgpu::SpectrumEx s = spectra[size_t(id)];
+    s.scale *= f;
+    return AddSpectrum(s);
+}
+
 int SceneBuilder::BlackbodyUnitY(float T) {
     // BlackbodySpectrum(T) normalised to peak 1, then NormalizedToY (scene_loader.cpp:197-201).
     prgpu::SpectrumEx s{};
@@ -1021,10 +1027,46 @@ static void GenTerrain(const GenOptions &o, SceneBuilder *sb) {
     sb->seed = 1;
 }
 
+static void GenMedia(const GenOptions &o, SceneBuilder *sb) {
+    // A 6 x 4 x 6 box of heterogeneous medium (a 256^3 grid of max(0, fBm) densities; sigma_t =
+    // d * RGBUnbounded(1, 0.8, 0.5) * 4, 90% scattering, Henyey-Greenstein g = 0.6) beside a
+    // subsurface sphere (smooth IOR 1.33 interface around a homogeneous medium with
+    // sigma_s ~ U[0.5, 2]^3), on a grey ground, lit by one 5800 K distant light.
+    SceneRng rng(3);
+    int W = o.width > 0 ? o.width : 1280, H = o.height > 0 ? o.height : 720;
+    sb->SetCamera(f3(0, 4, -14), f3(0.5f, 2.f, 0), 40.f, W, H);
+    sb->AddMesh(MakeQuad(f3(-20, 0, -20), f3(0, 0, 40), f3(40, 0, 0)), sb->Diffuse(sb->Constant(0.5f)));
+
+    const int N = std::max(8, int(std::lround(256 * o.detail)));
+    const F3 bmin = f3(-5, 0.5f, -3), bmax = f3(1, 4.5f, 3);
+    F3 off = f3(rng.uniform(0, 100), rng.uniform(0, 100), rng.uniform(0, 100));
+    std::vector<float> density(size_t(N) * N * N);
+    for (int z = 0; z < N; ++z)
+        for (int y = 0; y < N; ++y)
+            for (int x = 0; x < N; ++x) {
+                F3 p = f3((x + 0.5f) / N, (y + 0.5f) / N, (z + 0.5f) / N);
+                density[(size_t(z) * N + y) * N + x] = std::max(0.f, FBm(p * 4.f + off, 6));
+            }
+    int st = sb->RGBUnbounded(1.0f, 0.8f, 0.5f);
+    int grid = sb->AddGrid(sb->Scaled(st, 0.1f), sb->Scaled(st, 0.9f), 4.f, 0.6f, N, N, N, density, bmin, bmax);
+    sb->AddMesh(MakeBox((bmin + bmax) * 0.5f, bmax - bmin), sb->Interface(), grid, -1);
+
+    float sr = rng.uniform(0.5f, 2.f), sg = rng.uniform(0.5f, 2.f), sbl = rng.uniform(0.5f, 2.f);
+    int sss = sb->AddHomogeneous(sb->Constant(0.05f), sb->RGBUnbounded(sr, sg, sbl), 1.f, 0.f);
+    sb->AddSphere(f3(4, 1.5f, 0), 1.5f, sb->Dielectric(sb->Constant(1.33f), 0, 0), sss, -1);
+
+    sb->AddDistantLight(f3(-0.5f, 1.f, -0.6f), sb->BlackbodyUnitY(5800.f), 3.f);
+    sb->spp = 512;
+    sb->maxDepth = 256;
+    sb->rrStartDepth = 1;
+    sb->seed = 1;
+}
+
 bool GenerateConfig(int config, const GenOptions &o, SceneBuilder *sb, std::string *err) {
     switch (config) {
     case 1: GenPyramid(o, sb); return true;
     case 2: GenTerrain(o, sb); return true;
+    case 3: GenMedia(o, sb); return true;
     default:
         if (err) *err = "configuration " + std::to_string(config) + " is not available";
         return false;

@@ -71,6 +71,7 @@ class SceneBuilder {
     int Sigmoid(float c0, float c1, float c2, float scale = 1.f);
     int RGBAlbedo(float r, float g, float b);     // smooth reflectance spectrum for an sRGB albedo
     int RGBUnbounded(float r, float g, float b);  // unbounded positive spectrum (scattering coefficients)
+    int Scaled(int spectrum, float s);            // a copy of `spectrum` multiplied by s
     int BlackbodyUnitY(float T);                  // Planck at T normalised to luminance Y = 1
     int Metal(const std::string &name, bool k);   // Au, Ag, Cu, Al: n (k = false) or k
     int Glass(const std::string &name);           // Sellmeier IOR: BK7, SF11, fused_silica, diamond, sapphire

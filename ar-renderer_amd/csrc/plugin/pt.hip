@@ -1382,8 +1382,4 @@ void SampleRadianceImpl(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::
     PG_CHECK(hipStreamSynchronize(dev.stream));
 }
 
-void RenderV4Impl(Device &, const prgpu::SceneDesc &, const prgpu::RenderParams &, float *) {
-    throw std::runtime_error("v4 render not implemented");
-}
-
 }  // namespace pgpu

@@ -23,6 +23,7 @@ EXT_VERSION = 1
 # ---- enums (include/prender_gpu.h) ---------------------------------------------------------------
 PRIM_TRIANGLE, PRIM_SPHERE = 0, 1
 MAT_DIFFUSE, MAT_CONDUCTOR, MAT_DIELECTRIC, MAT_BLACK, MAT_INTERFACE = 0, 1, 2, 3, 4
+MAT_COATED_DIFFUSE, MAT_COATED_CONDUCTOR = 5, 6
 SPEC_CONSTANT, SPEC_SIGMOID, SPEC_BLACKBODY, SPEC_PIECEWISE, SPEC_SELLMEIER, SPEC_CAUCHY, SPEC_TABLE95 = range(7)
 LIGHT_AREA, LIGHT_POINT, LIGHT_DISTANT, LIGHT_ENV = 0, 1, 2, 3
 MEDIUM_HOMOGENEOUS, MEDIUM_GRID = 0, 1
@@ -102,6 +103,50 @@ class ProfileStatsEx(C.Structure):
                 ("shadowRays", C.c_int64), ("shadowNodes", C.c_int64), ("shadowRecs", C.c_int64),
                 ("nodeBytes", C.c_int64), ("recBytes", C.c_int64)]
 
+
+# ---- v4 structures (reference src/gpu/gpu_api.h:31-154) ------------------------------------
+SPECTRUM95_DTYPE = np.dtype([("v", "<f4", 95)])
+LIGHT_V4_DTYPE = np.dtype({"names": ["type", "primitive", "spectrum", "scale", "position", "twoSided", "cosPower",
+                                     "area", "pmf", "pad"],
+                           "formats": ["<u4", "<i4", "<i4", "<f4", ("<f4", 3), "<u4", "<f4", "<f4", "<f4", "<u4"],
+                           "itemsize": 48})
+MEDIUM_V4_DTYPE = np.dtype([("sigmaA", "<i4"), ("sigmaS", "<i4"), ("scale", "<f4"), ("g", "<f4")])
+BVHNODE_V4_DTYPE = np.dtype({"names": ["bmin", "bmax", "offset", "nPrims", "axis", "pad"],
+                             "formats": [("<f4", 3), ("<f4", 3), "<i4", "<u2", "u1", "u1"], "itemsize": 32})
+V4_LIGHT_AREA, V4_LIGHT_POINT = 0, 1
+
+
+class CameraV4(C.Structure):
+    _fields_ = [("cameraToWorld", C.c_float * 12), ("tanHalfFov", C.c_float), ("aspect", C.c_float),
+                ("lensRadius", C.c_float), ("focalDistance", C.c_float), ("width", C.c_int32),
+                ("height", C.c_int32), ("margin", C.c_int32), ("pad", C.c_uint32)]
+
+
+class FilterV4(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("radius", C.c_float), ("sigma", C.c_float), ("norm", C.c_float),
+                ("cdf", C.c_float * 65)]
+
+
+class SceneDescV4(C.Structure):
+    _fields_ = [("apiVersion", C.c_int32), ("nodes", C.c_void_p), ("nodeCount", C.c_int32),
+                ("primOrder", C.c_void_p), ("prims", C.c_void_p), ("primCount", C.c_int32),
+                ("materials", C.c_void_p), ("materialCount", C.c_int32),
+                ("lights", C.c_void_p), ("lightCount", C.c_int32),
+                ("spectra", C.c_void_p), ("spectrumCount", C.c_int32),
+                ("media", C.c_void_p), ("mediumCount", C.c_int32),
+                ("cameraMedium", C.c_int32), ("envSpectrum", C.c_int32), ("envScale", C.c_float),
+                ("envPmf", C.c_float), ("worldCenter", Float3), ("worldRadius", C.c_float),
+                ("camera", CameraV4), ("filter", FilterV4), ("sheenAlbedo", C.c_void_p), ("cie", C.c_void_p),
+                ("cieYIntegral", C.c_float)]
+
+
+class RenderParamsV4(C.Structure):
+    _fields_ = [("spp", C.c_int32), ("firstSample", C.c_int32), ("seed", C.c_uint64), ("maxDepth", C.c_int32),
+                ("clampLuminance", C.c_float)]
+
+
+assert C.sizeof(SceneDescV4) == 520 and C.sizeof(RenderParamsV4) == 24 and C.sizeof(CameraV4) == 80
+assert C.sizeof(FilterV4) == 276 and LIGHT_V4_DTYPE.itemsize == 48 and SPECTRUM95_DTYPE.itemsize == 380
 
 KERNEL_NAMES = ["camera", "intersect", "shade", "shadow", "finish", "gather"]
 
@@ -328,6 +373,15 @@ class Plugin:
         err = C.create_string_buffer(1024)
         self._check(self.lib.prgpu_sample_radiance(device, scene.desc_ptr(), C.byref(params), out.ctypes.data, err,
                                                    1024), err)
+        return out
+
+    def render_v4(self, desc, params, width, height, out=None, device=0):
+        """The reference's v4 entry point: ADDS this call's XYZ sums into `out` (float32, h x w x 3)."""
+        if out is None:
+            out = np.zeros((height, width, 3), np.float32)
+        assert out.dtype == np.float32 and out.flags.c_contiguous and out.size == width * height * 3
+        err = C.create_string_buffer(1024)
+        self._check(self.lib.prgpu_render(device, C.byref(desc), C.byref(params), out.ctypes.data, err, 1024), err)
         return out
 
     def release(self, device=0):

@@ -305,8 +305,10 @@ struct ProfileStatsEx {
 // ---- v4 entry points (reference gpu_api.h:166-173) ------------------------------------------
 // Number of devices written to out (at most max).
 PRGPU_API int prgpu_list_devices(prgpu::DeviceInfo *out, int max);
-// Renders params->spp samples per pixel and ADDS the unnormalised XYZ sums (filter importance
-// sampled, margin 0) into outXYZ[3 * (y * width + x)]. 0 on success, else a message in err.
+// Renders params->spp samples per pixel and ADDS, per pixel, the sum over those samples of the
+// filter-weighted XYZ (normalised filter, norm 1 / (4 * CIE_Y_Integral)) into
+// outXYZ[3 * (y * width + x)]; the caller divides by the total spp. 0 on success, else a
+// message in err (coated and sheen materials are refused).
 PRGPU_API int prgpu_render(int deviceIndex, const prgpu::SceneDesc *scene, const prgpu::RenderParams *params,
                            float *outXYZ, char *err, int errLen);
 PRGPU_API int prgpu_api_version();
