@@ -52,7 +52,13 @@ Device &Get(int index) {
     return d;
 }
 
-constexpr int kTraceBlocksPerCU = 6;  // LDS: kStackLds * 256 * 4 B = 24 KiB per block
+// Resident traversal blocks per CU (the persistent grid): LDS holds kStackLds * 256 * 4 B per
+// block, 160 KiB per CU.
+#ifndef PG_TRACE_BLOCKS_PER_CU
+#define PG_TRACE_BLOCKS_PER_CU 8
+#endif
+constexpr int kTraceBlocksPerCU = PG_TRACE_BLOCKS_PER_CU;
+static_assert(kTraceBlocksPerCU * kStackLds * kTraceBlock * 4 <= 160 * 1024, "traversal blocks exceed the LDS of a CU");
 
 __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(TraceCtx c, const prgpu::RayEx *rays, prgpu::HitEx *hits,
                                                             int64_t n, int anyHit) {

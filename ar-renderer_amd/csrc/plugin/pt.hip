@@ -405,7 +405,8 @@ void EnsurePTScene(Device &dev, const prgpu::SceneDescEx &sd) {
     }
     pt->hostNodes = nodes;
     pt->lnodeCount = int(nodes.size());
-    pt->lnodes.upload(nodes.data(), std::max<size_t>(1, nodes.size()), s);
+    if (nodes.empty()) nodes.push_back(LNode{});  // no bounded lights: a placeholder the kernels never read
+    pt->lnodes.upload(nodes.data(), nodes.size(), s);
     pt->leafOf.upload(leafOf.data(), leafOf.size(), s);
     std::vector<int> inf = tb.infinite;
     pt->infCount = int(inf.size());
@@ -548,7 +549,7 @@ struct Bufs {
     int *q0, *q1;
     int *qm;     // material queues, QCount x qcap
     int qcap;
-    int *counts;  // [0] queue A, [1] queue B, [2] shadow queue, [3..5] material queues
+    int *counts;  // [0] queue A, [1] queue B, [2] shadow queue, [3..5] material queues, [6..7] ray fetch
     float2 *pfilm;
     float4 *xyz;
     // shadow records
@@ -632,24 +633,38 @@ __global__ __launch_bounds__(kBlock) void k_camera(const DevScene *__restrict__ 
     b.q0[slot] = slot;
 }
 
+// Closest hits of the queued paths. Persistent: each lane takes a new ray from the fetch counter
+// as soon as its previous one completes (fetch_rays), so a wave's lanes stay busy while ray
+// lengths differ.
 template <bool kStats>
 __global__ __launch_bounds__(kTraceBlock) void k_intersect(const DevScene *__restrict__ sp, Bufs b, const int *queue,
-                                                           const int *count, unsigned long long *counters) {
+                                                           const int *count, int *fetch, unsigned long long *counters) {
     const DevScene &s = *sp;
     __shared__ uint32_t lds[kStackLds * kTraceBlock];
     const int n = *count;
     const int tid = int(blockIdx.x * blockDim.x + threadIdx.x);
-    const int stride = int(gridDim.x * blockDim.x);
     TraceStats st;
     uint32_t rays = 0;
-    for (int i = tid; i < n; i += stride) {
-        int slot = queue[i];
-        float4 o = b.o[slot], d = b.d[slot];
-        DRay r{o.x, o.y, o.z, d.x, d.y, d.z};
-        DHit h;
-        bool hit = dev_trace<false, kStats>(s.trace, r, kInfF, &h, lds, tid, &st);
-        b.hit[slot] = hit ? make_float4(h.t, __int_as_float(h.prim), h.b1, h.b2) : make_float4(0, __int_as_float(-1), 0, 0);
-        ++rays;
+    int idx = -1, slot = 0;
+    bool exhausted = false, started = false;
+    DRay r{};
+    TravState t{};
+    while (fetch_rays(fetch, n, idx, exhausted)) {
+        if (idx < 0) continue;
+        if (!started) {
+            slot = queue[idx];
+            float4 o = b.o[slot], d = b.d[slot];
+            r = DRay{o.x, o.y, o.z, d.x, d.y, d.z};
+            trace_begin(t, r, kInfF);
+            started = true;
+        }
+        if (trace_step<false, kStats>(s.trace, r, t, lds, tid, &st)) {
+            b.hit[slot] = t.found ? make_float4(t.hit.t, __int_as_float(t.hit.prim), t.hit.b1, t.hit.b2)
+                                  : make_float4(0, __int_as_float(-1), 0, 0);
+            ++rays;
+            idx = -1;
+            started = false;
+        }
     }
     if (kStats && rays) {
         atomicAdd(&counters[0], (unsigned long long)rays);
@@ -958,70 +973,103 @@ __global__ __launch_bounds__(kBlock) void k_material(const DevScene *__restrict_
 }
 
 // Transmittance of the shadow records (src/scene/scene.cpp:41-59) and the deferred add().
+// The deferred add() of a visible shadow record: beta * (f * Tr * L * k), and the rPath
+// average at sampling time for the spectral-MIS estimate.
+__device__ __forceinline__ void shadow_add(const Bufs &b, int i, int slot, const SS &T, float k, float a) {
+    SS c = ld4(b.sb, i) * (ld4(b.sf, i) * T * ld4(b.sL, i) * k);
+    SS Lh = ld4(b.Lh, slot) + c;
+    st4(b.Lh, slot, Lh);
+    if (a > 0 && is_finite(a)) st4(b.L, slot, ld4(b.L, slot) + c / a);
+}
+
+// Transmittance of the shadow records (src/scene/scene.cpp:41-59) and the deferred add().
+// Without media or interfaces a shadow ray is one any-hit query, traversed persistently as in
+// k_intersect; otherwise each ray walks its interface crossings to completion.
 template <bool kStats, bool kMedia, bool kInterfaces>
 __global__ __launch_bounds__(kTraceBlock) void k_shadow(const DevScene *__restrict__ sp, Bufs b, const int *count,
-                                                        unsigned long long *counters) {
+                                                        int *fetch, unsigned long long *counters) {
     const DevScene &s = *sp;
     __shared__ uint32_t lds[kStackLds * kTraceBlock];
     const int n = *count;
     const int tid = int(blockIdx.x * blockDim.x + threadIdx.x);
-    const int stride = int(gridDim.x * blockDim.x);
     TraceStats ts;
     uint32_t rays = 0;
-    for (int i = tid; i < n; i += stride) {
-        ++rays;
-        float4 so = b.so[i], sd = b.sd[i], sp = b.sp[i], sn = b.sn[i];
-        const int slot = __float_as_int(sd.w);
-        V3 ro(so.x, so.y, so.z), rd(sd.x, sd.y, sd.z), pl(sp.x, sp.y, sp.z), nl(sn.x, sn.y, sn.z);
-        int medium = __float_as_int(so.w);
-        Lambda lam;
-        {
-            float4 l = b.lam[slot], lp = b.lpdf[slot];
-            lam = {{l.x, l.y, l.z, l.w}, {lp.x, lp.y, lp.z, lp.w}};
-        }
-        SS T(1.f);
-        const float tMax = 1 - 1e-4f;
-        for (int iter = 0;; ++iter) {
-            if (iter == 256) {
-                T = SS(0.f);
-                break;
+    const float tMax = 1 - 1e-4f;
+    if constexpr (!kMedia && !kInterfaces) {
+        int idx = -1;
+        bool exhausted = false, started = false;
+        DRay r{};
+        TravState t{};
+        while (fetch_rays(fetch, n, idx, exhausted)) {
+            if (idx < 0) continue;
+            if (!started) {
+                float4 so = b.so[idx], sd = b.sd[idx];
+                r = DRay{so.x, so.y, so.z, sd.x, sd.y, sd.z};
+                trace_begin(t, r, tMax);
+                started = true;
             }
-            DRay r{ro.x, ro.y, ro.z, rd.x, rd.y, rd.z};
-            DHit h;
-            if (!kInterfaces) {
-                if (dev_trace<true, kStats>(s.trace, r, tMax, &h, lds, tid, &ts)) {
-                    T = SS(0.f);
-                } else if (kMedia && medium >= 0) {
-                    float l = len(rd);
-                    T = T * medium_tr(s, medium, ro, rd / l, 1.f * l, lam);
+            if (trace_step<true, kStats>(s.trace, r, t, lds, tid, &ts)) {
+                ++rays;
+                if (!t.found) {
+                    const int slot = __float_as_int(b.sd[idx].w);
+                    float4 spk = b.sp[idx], snk = b.sn[idx];
+                    shadow_add(b, idx, slot, SS(1.f), spk.w, snk.w);
                 }
-                break;
+                idx = -1;
+                started = false;
             }
-            bool hit = dev_trace<false, kStats>(s.trace, r, tMax, &h, lds, tid, &ts);
-            const prgpu::Primitive *pr = hit ? &s.prims[h.prim] : nullptr;
-            if (hit && !is_interface(s, *pr)) {
-                T = SS(0.f);
-                break;
-            }
-            if (kMedia && medium >= 0) {
-                float l = len(rd);
-                T = T * medium_tr(s, medium, ro, rd / l, (hit ? h.t : 1.f) * l, lam);
-            }
-            if (!T.nonzero() || !hit) break;
-            Surf si;
-            interaction(*pr, ro, rd, h.t, h.b1, h.b2, &si);
-            medium = dot(rd, si.n) > 0 ? pr->mediumOutside : pr->mediumInside;
-            ro = offset_origin(si.p, si.n, pl - si.p);
-            V3 target = offset_origin(pl, nl, ro - pl);
-            rd = target - ro;
         }
-        if (!T.nonzero()) continue;
-        // beta * (f * Tr * L * k), then add() with the rPath average at sampling time.
-        SS c = ld4(b.sb, i) * (ld4(b.sf, i) * T * ld4(b.sL, i) * sp.w);
-        float a = sn.w;
-        SS Lh = ld4(b.Lh, slot) + c;
-        st4(b.Lh, slot, Lh);
-        if (a > 0 && is_finite(a)) st4(b.L, slot, ld4(b.L, slot) + c / a);
+    } else {
+        const int stride = int(gridDim.x * blockDim.x);
+        for (int i = tid; i < n; i += stride) {
+            ++rays;
+            float4 so = b.so[i], sd = b.sd[i], spt = b.sp[i], sn = b.sn[i];
+            const int slot = __float_as_int(sd.w);
+            V3 ro(so.x, so.y, so.z), rd(sd.x, sd.y, sd.z), pl(spt.x, spt.y, spt.z), nl(sn.x, sn.y, sn.z);
+            int medium = __float_as_int(so.w);
+            Lambda lam;
+            {
+                float4 l = b.lam[slot], lp = b.lpdf[slot];
+                lam = {{l.x, l.y, l.z, l.w}, {lp.x, lp.y, lp.z, lp.w}};
+            }
+            SS T(1.f);
+            for (int iter = 0;; ++iter) {
+                if (iter == 256) {
+                    T = SS(0.f);
+                    break;
+                }
+                DRay r{ro.x, ro.y, ro.z, rd.x, rd.y, rd.z};
+                DHit h;
+                if (!kInterfaces) {
+                    if (dev_trace<true, kStats>(s.trace, r, tMax, &h, lds, tid, &ts)) {
+                        T = SS(0.f);
+                    } else if (kMedia && medium >= 0) {
+                        float l = len(rd);
+                        T = T * medium_tr(s, medium, ro, rd / l, 1.f * l, lam);
+                    }
+                    break;
+                }
+                bool hit = dev_trace<false, kStats>(s.trace, r, tMax, &h, lds, tid, &ts);
+                const prgpu::Primitive *pr = hit ? &s.prims[h.prim] : nullptr;
+                if (hit && !is_interface(s, *pr)) {
+                    T = SS(0.f);
+                    break;
+                }
+                if (kMedia && medium >= 0) {
+                    float l = len(rd);
+                    T = T * medium_tr(s, medium, ro, rd / l, (hit ? h.t : 1.f) * l, lam);
+                }
+                if (!T.nonzero() || !hit) break;
+                Surf si;
+                interaction(*pr, ro, rd, h.t, h.b1, h.b2, &si);
+                medium = dot(rd, si.n) > 0 ? pr->mediumOutside : pr->mediumInside;
+                ro = offset_origin(si.p, si.n, pl - si.p);
+                V3 target = offset_origin(pl, nl, ro - pl);
+                rd = target - ro;
+            }
+            if (!T.nonzero()) continue;
+            shadow_add(b, i, slot, T, spt.w, sn.w);
+        }
     }
     if (kStats && rays) {
         atomicAdd(&counters[3], (unsigned long long)rays);
@@ -1259,13 +1307,15 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
     while (active > 0) {
         const int nxtIdx = 1 - curIdx;
         PG_CHECK(hipMemsetAsync(b.counts + nxtIdx, 0, sizeof(int), st));
-        PG_CHECK(hipMemsetAsync(b.counts + 2, 0, 4 * sizeof(int), st));
+        PG_CHECK(hipMemsetAsync(b.counts + 2, 0, 6 * sizeof(int), st));  // shadow, material queues, fetch counters
         const int tgrid = TraceGrid(dev, active);
         timer.Run(prgpu::KernIntersect, active, nullptr, [&] {
             if (stats)
-                hipLaunchKernelGGL(k_intersect<true>, dim3(tgrid), dim3(kTraceBlock), 0, st, ds, b, cur, b.counts + curIdx, ctr);
+                hipLaunchKernelGGL(k_intersect<true>, dim3(tgrid), dim3(kTraceBlock), 0, st, ds, b, cur, b.counts + curIdx,
+                                   b.counts + 6, ctr);
             else
-                hipLaunchKernelGGL(k_intersect<false>, dim3(tgrid), dim3(kTraceBlock), 0, st, ds, b, cur, b.counts + curIdx, ctr);
+                hipLaunchKernelGGL(k_intersect<false>, dim3(tgrid), dim3(kTraceBlock), 0, st, ds, b, cur, b.counts + curIdx,
+                                   b.counts + 6, ctr);
         });
         int grid = std::max(1, std::min((active + kBlock - 1) / kBlock, shadeGrid));
         timer.Run(prgpu::KernShade, active, nullptr, [&] {
@@ -1285,7 +1335,7 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
                                      : (interfaces ? k_shadow<true, false, true> : k_shadow<true, false, false>))
                             : (media ? (interfaces ? k_shadow<false, true, true> : k_shadow<false, true, false>)
                                      : (interfaces ? k_shadow<false, false, true> : k_shadow<false, false, false>));
-            hipLaunchKernelGGL(k, dim3(tgrid), dim3(kTraceBlock), 0, st, ds, b, b.counts + 2, ctr);
+            hipLaunchKernelGGL(k, dim3(tgrid), dim3(kTraceBlock), 0, st, ds, b, b.counts + 2, b.counts + 7, ctr);
         });
         PG_CHECK(hipGetLastError());
         PG_CHECK(hipMemcpyAsync(pt.countsHost, b.counts, 3 * sizeof(int), hipMemcpyDeviceToHost, st));

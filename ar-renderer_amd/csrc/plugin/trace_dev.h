@@ -17,8 +17,16 @@
 
 namespace pgpu {
 
+// Tuning knobs (compile-time): LDS stack entries per lane and the idle-lane threshold of the
+// per-lane ray fetch.
+#ifndef PG_STACK_LDS
+#define PG_STACK_LDS 16
+#endif
+#ifndef PG_REFILL
+#define PG_REFILL 32
+#endif
 constexpr int kTraceBlock = 256;
-constexpr int kStackLds = 24;  // LDS entries per lane
+constexpr int kStackLds = PG_STACK_LDS;  // LDS entries per lane
 
 struct DRay {
     float ox, oy, oz, dx, dy, dz;
@@ -122,100 +130,156 @@ struct TraceStats {
     uint32_t nodes = 0, recs = 0;
 };
 
-// Closest (kAny = false) or any (kAny = true) hit along r in (0, tMax). `lds` is this block's
-// stack area (kStackLds * blockDim.x entries), `tid` the global thread index for the spill slab.
+// Traversal state of one ray: the inverse direction, the current node or leaf reference, the
+// stack depth and the best hit so far. The stack itself lives in LDS (+ the HBM spill slab).
+struct TravState {
+    float ix, iy, iz;
+    float tMax;
+    uint32_t cur;
+    int sp;
+    bool found;
+    DHit hit;
+};
+
+__device__ __forceinline__ void trace_begin(TravState &t, const DRay &r, float tMax) {
+    t.ix = dev_inv(r.dx);
+    t.iy = dev_inv(r.dy);
+    t.iz = dev_inv(r.dz);
+    t.tMax = tMax;
+    t.cur = 0;  // root node
+    t.sp = 0;
+    t.found = false;
+    t.hit.prim = -1;
+}
+
+__device__ __forceinline__ uint32_t trace_pop(const TraceCtx &c, TravState &t, const uint32_t *lds, uint32_t lane,
+                                              uint32_t bd, int tid) {
+    --t.sp;
+    return t.sp < kStackLds ? lds[t.sp * bd + lane] : c.spill[size_t(t.sp - kStackLds) * c.spillStride + tid];
+}
+
+// One traversal step: one wide node (slab-test its children, push the hit ones far to near and
+// continue with the nearest) or one leaf (test its records). Returns true when the traversal of
+// this ray is complete. `lds` is the block's stack area (kStackLds * blockDim.x entries), `tid`
+// the global thread index for the spill slab.
+template <bool kAny, bool kStats>
+__device__ __forceinline__ bool trace_step(const TraceCtx &c, const DRay &r, TravState &t, uint32_t *lds, int tid,
+                                           TraceStats *st) {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t bd = blockDim.x;
+    if (!(t.cur & kLeafBit)) {
+        const uint4 *np = reinterpret_cast<const uint4 *>(c.nodes + t.cur);
+        uint4 h0 = np[0], c0 = np[1], c1 = np[2], q0 = np[3], q1 = np[4], q2 = np[5];
+        if (kStats) st->nodes++;
+        float ox = __uint_as_float(h0.x), oy = __uint_as_float(h0.y), oz = __uint_as_float(h0.z);
+        int ex = int(int8_t(h0.w & 0xff)), ey = int(int8_t((h0.w >> 8) & 0xff)), ez = int(int8_t((h0.w >> 16) & 0xff));
+        int nch = int(h0.w >> 24);
+        float sx = pow2f(ex), sy = pow2f(ey), sz = pow2f(ez);
+        // Per-axis origin terms shared by all children.
+        float bx = (ox - r.ox), by = (oy - r.oy), bz = (oz - r.oz);
+        uint32_t child[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+        uint32_t qlx[2] = {q0.x, q0.y}, qly[2] = {q0.z, q0.w}, qlz[2] = {q1.x, q1.y};
+        uint32_t qhx[2] = {q1.z, q1.w}, qhy[2] = {q2.x, q2.y}, qhz[2] = {q2.z, q2.w};
+        uint32_t key[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int w = i >> 2, sh = (i & 3) * 8;
+            float lx = float((qlx[w] >> sh) & 0xff) * sx + bx, hx = float((qhx[w] >> sh) & 0xff) * sx + bx;
+            float ly = float((qly[w] >> sh) & 0xff) * sy + by, hy = float((qhy[w] >> sh) & 0xff) * sy + by;
+            float lz = float((qlz[w] >> sh) & 0xff) * sz + bz, hz = float((qhz[w] >> sh) & 0xff) * sz + bz;
+            float t0x = lx * t.ix, t1x = hx * t.ix, t0y = ly * t.iy, t1y = hy * t.iy, t0z = lz * t.iz, t1z = hz * t.iz;
+            float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+            float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+            // Conservative widening as the reference slab test (bvh.cpp:137-156), on both ends.
+            tf = tf * (1.f + 2.f * 1.2e-7f * 3.f);
+            tn = tn - fabsf(tn) * (2.f * 1.2e-7f * 3.f);
+            bool h = (i < nch) && tn <= tf && tn < t.tMax && tf >= 0.f;
+            key[i] = h ? ((__float_as_uint(fmaxf(tn, 0.f)) & ~7u) | uint32_t(i)) : 0xffffffffu;
+        }
+        sort8(key);
+        if (key[0] == 0xffffffffu) {
+            if (t.sp == 0) return true;
+            t.cur = trace_pop(c, t, lds, lane, bd, tid);
+            return false;
+        }
+        // Push hit children far to near; continue with the nearest.
+#pragma unroll
+        for (int i = 7; i >= 1; --i) {
+            if (key[i] != 0xffffffffu) {
+                uint32_t ref = child[key[i] & 7u];
+                if (t.sp < kStackLds) {
+                    lds[t.sp * bd + lane] = ref;
+                } else if (t.sp - kStackLds < c.spillCap) {
+                    c.spill[size_t(t.sp - kStackLds) * c.spillStride + tid] = ref;
+                } else {
+                    *c.errorFlag = 1;
+                    --t.sp;  // keep the stack in bounds; the error flag reports the lost subtree
+                }
+                ++t.sp;
+            }
+        }
+        t.cur = child[key[0] & 7u];
+        return false;
+    }
+    // Leaf: test its records.
+    const uint32_t first = t.cur & 0x07ffffffu, cnt = ((t.cur >> 27) & 7u) + 1u;
+    for (uint32_t k = 0; k < cnt; ++k) {
+        const float4 *rp = reinterpret_cast<const float4 *>(c.recs + first + k);
+        float4 a = rp[0], b = rp[1], d = rp[2];
+        if (kStats) st->recs++;
+        uint32_t prim = __float_as_uint(d.y);
+        float th, u = 0.f, v = 0.f;
+        bool h;
+        if (prim & kSphereBit) h = dev_hit_sphere(r, a.x, a.y, a.z, a.w, t.tMax, &th);
+        else h = dev_hit_triangle(r, a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, d.x, t.tMax, &th, &u, &v);
+        if (h) {
+            t.found = true;
+            t.tMax = th;
+            t.hit.t = th;
+            t.hit.prim = int(prim & ~kSphereBit);
+            t.hit.b1 = u;
+            t.hit.b2 = v;
+            if (kAny) return true;
+        }
+    }
+    if (t.sp == 0) return true;
+    t.cur = trace_pop(c, t, lds, lane, bd, tid);
+    return false;
+}
+
+// Closest (kAny = false) or any (kAny = true) hit along r in (0, tMax), traversed to completion.
 template <bool kAny, bool kStats>
 __device__ __forceinline__ bool dev_trace(const TraceCtx &c, const DRay &r, float tMax, DHit *hit, uint32_t *lds,
                                           int tid, TraceStats *st) {
-    const float ix = dev_inv(r.dx), iy = dev_inv(r.dy), iz = dev_inv(r.dz);
-    const uint32_t lane = threadIdx.x;
-    const uint32_t bd = blockDim.x;
-    int sp = 0;
-    uint32_t cur = 0;  // root node
-    bool found = false;
-    hit->prim = -1;
-    while (true) {
-        if (!(cur & kLeafBit)) {
-            const uint4 *np = reinterpret_cast<const uint4 *>(c.nodes + cur);
-            uint4 h0 = np[0], c0 = np[1], c1 = np[2], q0 = np[3], q1 = np[4], q2 = np[5];
-            if (kStats) st->nodes++;
-            float ox = __uint_as_float(h0.x), oy = __uint_as_float(h0.y), oz = __uint_as_float(h0.z);
-            int ex = int(int8_t(h0.w & 0xff)), ey = int(int8_t((h0.w >> 8) & 0xff)), ez = int(int8_t((h0.w >> 16) & 0xff));
-            int nch = int(h0.w >> 24);
-            float sx = pow2f(ex), sy = pow2f(ey), sz = pow2f(ez);
-            // Per-axis origin terms shared by all children.
-            float bx = (ox - r.ox), by = (oy - r.oy), bz = (oz - r.oz);
-            uint32_t child[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
-            uint32_t qlx[2] = {q0.x, q0.y}, qly[2] = {q0.z, q0.w}, qlz[2] = {q1.x, q1.y};
-            uint32_t qhx[2] = {q1.z, q1.w}, qhy[2] = {q2.x, q2.y}, qhz[2] = {q2.z, q2.w};
-            uint32_t key[8];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int w = i >> 2, sh = (i & 3) * 8;
-                float lx = float((qlx[w] >> sh) & 0xff) * sx + bx, hx = float((qhx[w] >> sh) & 0xff) * sx + bx;
-                float ly = float((qly[w] >> sh) & 0xff) * sy + by, hy = float((qhy[w] >> sh) & 0xff) * sy + by;
-                float lz = float((qlz[w] >> sh) & 0xff) * sz + bz, hz = float((qhz[w] >> sh) & 0xff) * sz + bz;
-                float t0x = lx * ix, t1x = hx * ix, t0y = ly * iy, t1y = hy * iy, t0z = lz * iz, t1z = hz * iz;
-                float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
-                float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
-                // Conservative widening as the reference slab test (bvh.cpp:137-156), on both ends.
-                tf = tf * (1.f + 2.f * 1.2e-7f * 3.f);
-                tn = tn - fabsf(tn) * (2.f * 1.2e-7f * 3.f);
-                bool h = (i < nch) && tn <= tf && tn < tMax && tf >= 0.f;
-                key[i] = h ? ((__float_as_uint(fmaxf(tn, 0.f)) & ~7u) | uint32_t(i)) : 0xffffffffu;
-            }
-            sort8(key);
-            if (key[0] == 0xffffffffu) {
-                if (sp == 0) break;
-                --sp;
-                cur = sp < kStackLds ? lds[sp * bd + lane] : c.spill[size_t(sp - kStackLds) * c.spillStride + tid];
-                continue;
-            }
-            // Push hit children far to near; continue with the nearest.
-#pragma unroll
-            for (int i = 7; i >= 1; --i) {
-                if (key[i] != 0xffffffffu) {
-                    uint32_t ref = child[key[i] & 7u];
-                    if (sp < kStackLds) {
-                        lds[sp * bd + lane] = ref;
-                    } else if (sp - kStackLds < c.spillCap) {
-                        c.spill[size_t(sp - kStackLds) * c.spillStride + tid] = ref;
-                    } else {
-                        *c.errorFlag = 1;
-                        --sp;  // keep the stack in bounds; the error flag reports the lost subtree
-                    }
-                    ++sp;
-                }
-            }
-            cur = child[key[0] & 7u];
-            continue;
-        }
-        // Leaf: test its records.
-        const uint32_t first = cur & 0x07ffffffu, cnt = ((cur >> 27) & 7u) + 1u;
-        for (uint32_t k = 0; k < cnt; ++k) {
-            const float4 *rp = reinterpret_cast<const float4 *>(c.recs + first + k);
-            float4 a = rp[0], b = rp[1], d = rp[2];
-            if (kStats) st->recs++;
-            uint32_t prim = __float_as_uint(d.y);
-            float t, u = 0.f, v = 0.f;
-            bool h;
-            if (prim & kSphereBit) h = dev_hit_sphere(r, a.x, a.y, a.z, a.w, tMax, &t);
-            else h = dev_hit_triangle(r, a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, d.x, tMax, &t, &u, &v);
-            if (h) {
-                found = true;
-                tMax = t;
-                hit->t = t;
-                hit->prim = int(prim & ~kSphereBit);
-                hit->b1 = u;
-                hit->b2 = v;
-                if (kAny) return true;
-            }
-        }
-        if (sp == 0) break;
-        --sp;
-        cur = sp < kStackLds ? lds[sp * bd + lane] : c.spill[size_t(sp - kStackLds) * c.spillStride + tid];
+    TravState t;
+    trace_begin(t, r, tMax);
+    while (!trace_step<kAny, kStats>(c, r, t, lds, tid, st)) {
     }
-    return found;
+    *hit = t.hit;
+    return t.found;
+}
+
+// Per-lane ray fetch for persistent traversal kernels: lanes without a ray take consecutive
+// indices from a global counter, one atomic per wave, once at least `kRefill` lanes of the wave
+// are idle (or no lane has work left). Sets `idx` (< n) or marks the lane exhausted.
+constexpr int kRefill = PG_REFILL;
+__device__ __forceinline__ bool fetch_rays(int *counter, int n, int &idx, bool &exhausted) {
+    const bool idle = idx < 0 && !exhausted;
+    const unsigned long long idleMask = __ballot(idle);
+    const unsigned long long busyMask = __ballot(idx >= 0);
+    if (idleMask != 0 && (__popcll(idleMask) >= kRefill || busyMask == 0)) {
+        const int lane = __lane_id();
+        const int leader = __ffsll(idleMask) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(counter, __popcll(idleMask));
+        base = __shfl(base, leader);
+        if (idle) {
+            const int k = base + __popcll(idleMask & ((1ull << lane) - 1ull));
+            if (k < n) idx = k;
+            else exhausted = true;
+        }
+    }
+    return __ballot(idx >= 0) != 0;
 }
 
 }  // namespace pgpu

