@@ -883,6 +883,134 @@ Mesh MakeBox(F3 center, F3 size) {  // outward normals (shape.cpp:331-353)
 }
 
 // ---------------------------------------------------------------------------------------------
+// Convex hull (incremental, in double precision; the point sets here have ~150 points)
+
+Mesh ConvexHull(const std::vector<F3> &pts) {
+    struct D3 {
+        double x, y, z;
+    };
+    auto sub = [](D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; };
+    auto cross = [](D3 a, D3 b) { return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; };
+    auto dot = [](D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; };
+    std::vector<D3> P(pts.size());
+    double scale = 0;
+    for (size_t i = 0; i < pts.size(); ++i) {
+        P[i] = D3{pts[i].x, pts[i].y, pts[i].z};
+        scale = std::max(scale, std::max(std::abs(P[i].x), std::max(std::abs(P[i].y), std::abs(P[i].z))));
+    }
+    const double eps = 1e-9 * std::max(1.0, scale);
+    struct Face {
+        int a, b, c;
+        D3 n;
+        double d;
+        bool alive;
+    };
+    std::vector<Face> faces;
+    auto make = [&](int a, int b, int c) {
+        D3 n = cross(sub(P[size_t(b)], P[size_t(a)]), sub(P[size_t(c)], P[size_t(a)]));
+        double l = std::sqrt(dot(n, n));
+        n = D3{n.x / l, n.y / l, n.z / l};
+        faces.push_back(Face{a, b, c, n, dot(n, P[size_t(a)]), true});
+    };
+    Mesh m;
+    const int N = int(P.size());
+    if (N < 4) return m;
+    // Initial tetrahedron: two far-apart points, the point farthest from their line, and the
+    // point farthest from that plane.
+    int i0 = 0, i1 = 0;
+    for (int i = 1; i < N; ++i)
+        if (P[size_t(i)].x < P[size_t(i0)].x) i0 = i;
+    double best = -1;
+    for (int i = 0; i < N; ++i) {
+        D3 d = sub(P[size_t(i)], P[size_t(i0)]);
+        if (dot(d, d) > best) best = dot(d, d), i1 = i;
+    }
+    int i2 = -1;
+    best = -1;
+    for (int i = 0; i < N; ++i) {
+        D3 c = cross(sub(P[size_t(i1)], P[size_t(i0)]), sub(P[size_t(i)], P[size_t(i0)]));
+        if (dot(c, c) > best) best = dot(c, c), i2 = i;
+    }
+    int i3 = -1;
+    best = -1;
+    D3 n012 = cross(sub(P[size_t(i1)], P[size_t(i0)]), sub(P[size_t(i2)], P[size_t(i0)]));
+    for (int i = 0; i < N; ++i) {
+        double dd = std::abs(dot(n012, sub(P[size_t(i)], P[size_t(i0)])));
+        if (dd > best) best = dd, i3 = i;
+    }
+    if (best <= eps) return m;  // degenerate (planar) set
+    if (dot(n012, sub(P[size_t(i3)], P[size_t(i0)])) > 0) std::swap(i1, i2);
+    make(i0, i1, i2);
+    make(i0, i3, i1);
+    make(i1, i3, i2);
+    make(i2, i3, i0);
+    for (int p = 0; p < N; ++p) {
+        if (p == i0 || p == i1 || p == i2 || p == i3) continue;
+        std::vector<int> visible;
+        for (int f = 0; f < int(faces.size()); ++f)
+            if (faces[size_t(f)].alive && dot(faces[size_t(f)].n, P[size_t(p)]) - faces[size_t(f)].d > eps) visible.push_back(f);
+        if (visible.empty()) continue;
+        // Horizon: directed edges of visible faces whose twin belongs to no visible face.
+        std::vector<std::pair<int, int>> edges;
+        for (int f : visible) {
+            const Face &F = faces[size_t(f)];
+            int v[3] = {F.a, F.b, F.c};
+            for (int e = 0; e < 3; ++e) edges.push_back({v[e], v[(e + 1) % 3]});
+        }
+        for (int f : visible) faces[size_t(f)].alive = false;
+        for (const auto &e : edges) {
+            bool twin = false;
+            for (const auto &o : edges)
+                if (o.first == e.second && o.second == e.first) {
+                    twin = true;
+                    break;
+                }
+            if (!twin) make(e.first, e.second, p);
+        }
+    }
+    std::vector<int> remap(size_t(N), -1);
+    for (const Face &F : faces) {
+        if (!F.alive) continue;
+        for (int v : {F.a, F.b, F.c}) {
+            if (remap[size_t(v)] < 0) {
+                remap[size_t(v)] = int(m.p.size());
+                m.p.push_back(pts[size_t(v)]);
+            }
+            m.idx.push_back(remap[size_t(v)]);
+        }
+    }
+    m.uv.assign(m.p.size() * 2, 0.f);
+    return m;
+}
+
+// A round brilliant with ideal proportions (table 57%, crown 34.5 deg, pavilion 40.75 deg, girdle
+// 3%, star 50%, lower-girdle 77%): the corners of the table, the star points and the lower-girdle
+// points sit where neighbouring bezel / pavilion-main planes meet, so each kite is planar, and the
+// girdle is a ring of segments (the cut of the reference's diamonds generator,
+// scenes/diamonds/generate.py:30-58).
+std::vector<F3> RoundBrilliantPoints(int ring) {
+    const double kDeg = kPi / 180.0;
+    const double table = 0.57, star = 0.5, lower = 0.77, g = 0.03 * 2;
+    const double tc = std::tan(34.5 * kDeg), tp = std::tan(40.75 * kDeg), c22 = std::cos(22.5 * kDeg);
+    std::vector<F3> pts;
+    for (int k = 0; k < 8; ++k) {
+        double a = 45.0 * k * kDeg, b = a + 22.5 * kDeg;
+        pts.push_back(f3(float(table * std::cos(a)), float(g / 2 + (1 - table) * tc), float(table * std::sin(a))));
+        double rs = table * c22 + star * (1 - table * c22);
+        pts.push_back(f3(float(rs * std::cos(b)), float(g / 2 + (1 - rs * c22) * tc), float(rs * std::sin(b))));
+        double rl = 1 - lower;
+        pts.push_back(f3(float(rl * std::cos(b)), float(-g / 2 - (1 - rl * c22) * tp), float(rl * std::sin(b))));
+    }
+    for (int i = 0; i < ring; ++i) {
+        double a = 2 * kPi * i / ring;
+        pts.push_back(f3(float(std::cos(a)), float(g / 2), float(std::sin(a))));
+        pts.push_back(f3(float(std::cos(a)), float(-g / 2), float(std::sin(a))));
+    }
+    pts.push_back(f3(0, float(-g / 2 - tp), 0));  // culet
+    return pts;
+}
+
+// ---------------------------------------------------------------------------------------------
 // Noise: Perlin's improved noise with his reference permutation (src/core/noise.cpp:7-60).
 
 static const int kPermBase[256] = {
@@ -1062,11 +1190,74 @@ static void GenMedia(const GenOptions &o, SceneBuilder *sb) {
     sb->seed = 1;
 }
 
+static void GenGems(const GenOptions &o, SceneBuilder *sb) {
+    // 150 round-brilliant diamonds (Sellmeier dispersion) at random scales U[0.5, 2] and uniformly
+    // random orientations above a rough silver plane (GGX alpha 0.1), lit by a 0.2 x 0.2 quad at
+    // 5000 K and a dim constant environment; thin-lens camera at f/2.8.
+    SceneRng rng(5);
+    int W = o.width > 0 ? o.width : 3840, H = o.height > 0 ? o.height : 2160;
+    const float fov = 30.f;
+    const F3 eye = f3(0, 14, -32), at = f3(0, 0, 0);
+    // f-number to lens radius as the reference loader converts it (src/scene/scene_loader.cpp:1156-1159):
+    // focal length for a 24 mm sensor height, 100 scene units per metre.
+    const float focalMm = 24.f / (2 * std::tan((kPi / 180.f) * fov / 2)), fStop = 2.8f, unitsPerMetre = 100.f;
+    const float lensRadius = (focalMm / 1000.f) / (2 * fStop) * unitsPerMetre;
+    sb->SetCamera(eye, at, fov, W, H, lensRadius, 0.f);
+
+    int ag = sb->Conductor(sb->Metal("Ag", false), sb->Metal("Ag", true), 0.1f, 0.1f);
+    sb->AddMesh(MakeQuad(f3(-40, 0, -25), f3(0, 0, 50), f3(80, 0, 0)), ag);
+
+    const float B[3] = {0.3306f, 4.3356f, 0.f}, C[3] = {0.030625f, 0.011236f, 0.f};
+    int diamond = sb->Dielectric(sb->Sellmeier(B, C, 2), 0, 0);
+    const Mesh cut = ConvexHull(RoundBrilliantPoints(std::max(16, int(std::lround(64 * std::min(1.f, o.detail * 4))))));
+    const int nGems = std::max(4, int(std::lround(150 * std::min(1.f, o.detail * 4))));
+    std::vector<F3> centres;
+    std::vector<float> radii;
+    for (int g = 0; g < nGems; ++g) {
+        float sc = rng.uniform(0.5f, 2.f), bound = 1.15f * sc;
+        F3 c;
+        bool ok = false;
+        for (int attempt = 0; attempt < 1000 && !ok; ++attempt) {
+            c = f3(rng.uniform(-32, 32), bound, rng.uniform(-20, 20));
+            ok = true;
+            for (size_t k = 0; k < centres.size() && ok; ++k) {
+                F3 d = c - centres[k];
+                ok = dot3(d, d) > (bound + radii[k]) * (bound + radii[k]);
+            }
+        }
+        if (!ok) continue;
+        centres.push_back(c);
+        radii.push_back(bound);
+        // Uniform random rotation (Shoemake's quaternion from three uniforms).
+        float u1 = rng.uniform(), u2 = rng.uniform(), u3 = rng.uniform();
+        float qa = std::sqrt(1 - u1) * std::sin(2 * kPi * u2), qb = std::sqrt(1 - u1) * std::cos(2 * kPi * u2);
+        float qc = std::sqrt(u1) * std::sin(2 * kPi * u3), qd = std::sqrt(u1) * std::cos(2 * kPi * u3);
+        float x = qa, y = qb, z = qc, w = qd;
+        float R[9] = {1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w),
+                      2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w),
+                      2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)};
+        Mesh gem = cut;
+        for (F3 &p : gem.p) {
+            F3 q = f3(R[0] * p.x + R[1] * p.y + R[2] * p.z, R[3] * p.x + R[4] * p.y + R[5] * p.z,
+                      R[6] * p.x + R[7] * p.y + R[8] * p.z);
+            p = c + q * sc;
+        }
+        sb->AddMesh(gem, diamond, -1, -1, false);
+    }
+    sb->AddEmitter(MakeQuad(f3(-0.1f, 15, -0.1f), f3(0.2f, 0, 0), f3(0, 0, 0.2f)), sb->BlackbodyUnitY(5000.f), 5000.f);
+    sb->AddEnvironment(sb->Constant(1.f), 0.05f);
+    sb->spp = 4096;
+    sb->maxDepth = 32;
+    sb->rrStartDepth = 1;
+    sb->seed = 1;
+}
+
 bool GenerateConfig(int config, const GenOptions &o, SceneBuilder *sb, std::string *err) {
     switch (config) {
     case 1: GenPyramid(o, sb); return true;
     case 2: GenTerrain(o, sb); return true;
     case 3: GenMedia(o, sb); return true;
+    case 5: GenGems(o, sb); return true;
     default:
         if (err) *err = "configuration " + std::to_string(config) + " is not available";
         return false;

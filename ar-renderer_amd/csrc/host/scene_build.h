@@ -56,6 +56,10 @@ Mesh MakeUVSphere(F3 c, float r, int segments, int rings);
 Mesh MakeIcosphere(F3 c, float r, int subdivisions);
 Mesh MakeQuad(F3 corner, F3 edgeU, F3 edgeV);  // normal = normalize(edgeU x edgeV) ... see .cpp
 Mesh MakeBox(F3 center, F3 size);
+// Convex hull of a point set as an outward-wound triangle mesh (flat facets, no normals).
+Mesh ConvexHull(const std::vector<F3> &points);
+// Vertices of a 57-facet round brilliant with girdle radius 1, table up (+y).
+std::vector<F3> RoundBrilliantPoints(int ringSegments = 64);
 
 // Improved Perlin noise and fBm (src/core/noise.cpp:36-60).
 float PerlinNoise(F3 p);

@@ -650,14 +650,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_intersect(const DevScene *__res
     DRay r{};
     TravState t{};
     while (fetch_rays(fetch, n, idx, exhausted)) {
-        if (idx < 0) continue;
-        if (!started) {
+        if (idx >= 0 && !started) {
             slot = queue[idx];
             float4 o = b.o[slot], d = b.d[slot];
             r = DRay{o.x, o.y, o.z, d.x, d.y, d.z};
             trace_begin(t, r, kInfF);
             started = true;
         }
+        if (!trace_turn(idx >= 0, t.cur)) continue;
         if (trace_step<false, kStats>(s.trace, r, t, lds, tid, &st)) {
             b.hit[slot] = t.found ? make_float4(t.hit.t, __int_as_float(t.hit.prim), t.hit.b1, t.hit.b2)
                                   : make_float4(0, __int_as_float(-1), 0, 0);
@@ -784,20 +784,27 @@ __global__ __launch_bounds__(kBlock) void k_surface(const DevScene *__restrict__
         r.etaScale = d.w;
         float4 l = b.lam[slot], lp = b.lpdf[slot];
         r.lam = {{l.x, l.y, l.z, l.w}, {lp.x, lp.y, lp.z, lp.w}};
-        r.beta = ld4(b.beta, slot);
-        r.rpath = ld4(b.rpath, slot);
-        r.L = ld4(b.L, slot);
-        r.Lh = ld4(b.Lh, slot);
         int4 st = b.st[slot];
         r.depth = st.x;
         r.flags = st.y;
         r.medium = st.z;
         r.crossings = st.w;
-        {
+        // Throughput, MIS ratios, radiance and the previous vertex are read only by paths that
+        // meet a medium or emission (and written back only then): most surface hits route to a
+        // material queue without touching them.
+        bool loaded = false;
+        auto load = [&] {
+            if (loaded) return;
+            loaded = true;
+            r.beta = ld4(b.beta, slot);
+            r.rpath = ld4(b.rpath, slot);
+            r.L = ld4(b.L, slot);
+            r.Lh = ld4(b.Lh, slot);
             float4 pp = b.prevP[slot], pn = b.prevN[slot];
             r.prevP = V3(pp.x, pp.y, pp.z);
             r.prevN = V3(pn.x, pn.y, pn.z);
-        }
+        };
+        const bool lam0 = r.lam.terminated();
         const float4 hv = b.hit[slot];
         const int prim = __float_as_int(hv.y);
         const bool hit = prim >= 0;
@@ -806,6 +813,7 @@ __global__ __launch_bounds__(kBlock) void k_surface(const DevScene *__restrict__
         int mq = 0;
 
         if (kMedia && r.medium >= 0) {
+            load();
             r.rng = load_rng(b, slot);
             DistSample ds = medium_sample(s, r.medium, r.ro, r.rd, hit ? tHit : kInfF, r.rng.uniform(), r.lam);
             r.beta = r.beta * ds.weight;
@@ -844,15 +852,20 @@ __global__ __launch_bounds__(kBlock) void k_surface(const DevScene *__restrict__
             if (!hit) {
                 for (int li = 0; li < s.infCount; ++li) {
                     int L = s.infinite[li];
-                    if (s.lights[L].kind == prgpu::LightExEnvironment)
+                    if (s.lights[L].kind == prgpu::LightExEnvironment) {
+                        load();
                         add_emitted(s, r, L, spec_sample(s.spec, s.lights[L].spectrum, r.lam) * s.lights[L].scale, r.rd);
+                    }
                 }
                 alive = false;
             } else {
                 const prgpu::Primitive &pr = s.prims[prim];
                 Surf si;
                 interaction(pr, r.ro, r.rd, tHit, hv.z, hv.w, &si);
-                if (pr.light >= 0) add_emitted(s, r, pr.light, area_L(s, s.lights[pr.light], si.n, -r.rd, r.lam), r.rd);
+                if (pr.light >= 0) {
+                    load();
+                    add_emitted(s, r, pr.light, area_L(s, s.lights[pr.light], si.n, -r.rd, r.lam), r.rd);
+                }
                 if (kInterfaces && is_interface(s, pr)) {
                     if (++r.crossings > 1000) {
                         alive = false;
@@ -882,17 +895,22 @@ __global__ __launch_bounds__(kBlock) void k_surface(const DevScene *__restrict__
             }
         }
 
-        b.lpdf[slot] = make_float4(r.lam.pdf[0], r.lam.pdf[1], r.lam.pdf[2], r.lam.pdf[3]);
-        st4(b.L, slot, r.L);
-        st4(b.Lh, slot, r.Lh);
+        if (r.lam.terminated() != lam0)
+            b.lpdf[slot] = make_float4(r.lam.pdf[0], r.lam.pdf[1], r.lam.pdf[2], r.lam.pdf[3]);
+        if (loaded) {
+            st4(b.L, slot, r.L);
+            st4(b.Lh, slot, r.Lh);
+        }
         b.st[slot] = make_int4(r.depth, r.flags, r.medium, r.crossings);
         if (kMedia || kInterfaces) {
             b.o[slot] = make_float4(r.ro.x, r.ro.y, r.ro.z, r.prevPdf);
             b.d[slot] = make_float4(r.rd.x, r.rd.y, r.rd.z, r.etaScale);
-            st4(b.beta, slot, r.beta);
-            st4(b.rpath, slot, r.rpath);
-            b.prevP[slot] = make_float4(r.prevP.x, r.prevP.y, r.prevP.z, 0.f);
-            b.prevN[slot] = make_float4(r.prevN.x, r.prevN.y, r.prevN.z, 0.f);
+            if (loaded) {
+                st4(b.beta, slot, r.beta);
+                st4(b.rpath, slot, r.rpath);
+                b.prevP[slot] = make_float4(r.prevP.x, r.prevP.y, r.prevP.z, 0.f);
+                b.prevN[slot] = make_float4(r.prevN.x, r.prevN.y, r.prevN.z, 0.f);
+            }
         }
         const int target = toMaterial ? mq : (alive ? QCount : -1);
         if (target == QDiffuse) b.qm[QDiffuse * b.qcap + wave_append(&b.counts[3 + QDiffuse])] = slot;
@@ -972,7 +990,6 @@ __global__ __launch_bounds__(kBlock) void k_material(const DevScene *__restrict_
     }
 }
 
-// Transmittance of the shadow records (src/scene/scene.cpp:41-59) and the deferred add().
 // The deferred add() of a visible shadow record: beta * (f * Tr * L * k), and the rPath
 // average at sampling time for the spectral-MIS estimate.
 __device__ __forceinline__ void shadow_add(const Bufs &b, int i, int slot, const SS &T, float k, float a) {
@@ -1001,13 +1018,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(const DevScene *__restri
         DRay r{};
         TravState t{};
         while (fetch_rays(fetch, n, idx, exhausted)) {
-            if (idx < 0) continue;
-            if (!started) {
+            if (idx >= 0 && !started) {
                 float4 so = b.so[idx], sd = b.sd[idx];
                 r = DRay{so.x, so.y, so.z, sd.x, sd.y, sd.z};
                 trace_begin(t, r, tMax);
                 started = true;
             }
+            if (!trace_turn(idx >= 0, t.cur)) continue;
             if (trace_step<true, kStats>(s.trace, r, t, lds, tid, &ts)) {
                 ++rays;
                 if (!t.found) {

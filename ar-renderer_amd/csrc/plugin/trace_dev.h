@@ -25,6 +25,9 @@ namespace pgpu {
 #ifndef PG_REFILL
 #define PG_REFILL 32
 #endif
+#ifndef PG_LEAF_DEFER
+#define PG_LEAF_DEFER 0
+#endif
 constexpr int kTraceBlock = 256;
 constexpr int kStackLds = PG_STACK_LDS;  // LDS entries per lane
 
@@ -263,6 +266,19 @@ __device__ __forceinline__ bool dev_trace(const TraceCtx &c, const DRay &r, floa
 // indices from a global counter, one atomic per wave, once at least `kRefill` lanes of the wave
 // are idle (or no lane has work left). Sets `idx` (< n) or marks the lane exhausted.
 constexpr int kRefill = PG_REFILL;
+
+// Leaf deferral for persistent traversal: when > 0, a wave's lanes that sit at a leaf wait while
+// other lanes still descend inner nodes, until at least kLeafDefer of them have gathered (or no
+// lane is at an inner node); then only the leaf lanes test their records. Returns whether this
+// lane advances in the current iteration.
+constexpr int kLeafDefer = PG_LEAF_DEFER;
+__device__ __forceinline__ bool trace_turn(bool busy, uint32_t cur) {
+    if (kLeafDefer <= 0) return busy;
+    const bool atLeaf = busy && (cur & kLeafBit);
+    const unsigned long long leafMask = __ballot(atLeaf), innerMask = __ballot(busy && !atLeaf);
+    const bool leafTurn = innerMask == 0 || __popcll(leafMask) >= kLeafDefer;
+    return busy && (atLeaf == leafTurn);
+}
 __device__ __forceinline__ bool fetch_rays(int *counter, int n, int &idx, bool &exhausted) {
     const bool idle = idx < 0 && !exhausted;
     const unsigned long long idleMask = __ballot(idle);

@@ -104,3 +104,29 @@ def test_unavailable_configurations_fail_loudly(host):
     for cfg in (0, 6):
         with pytest.raises(RuntimeError):
             host.generate(cfg, 16, 16)
+
+
+def test_configuration_5_gems_are_closed_convex_solids(host):
+    s = host.generate(5, 64, 36)
+    assert s.camera.lensRadius > 0 and s.defaults["spp"] == 4096
+    assert {pa.LIGHT_AREA, pa.LIGHT_ENV} <= set(s.lights["kind"].tolist())
+    diel = np.flatnonzero(s.materials["type"] == pa.MAT_DIELECTRIC)
+    assert len(diel) == 1 and s.spectra[s.materials["eta"][diel[0]]]["kind"] == pa.SPEC_SELLMEIER
+    tris = s.prims[s.prims["material"] == diel[0]]
+    per_gem = len(tris) // 150
+    assert per_gem * 150 == len(tris)
+    P = np.stack([tris["p0"], tris["p1"], tris["p2"]], 1).astype(np.float64)
+    # every directed edge has exactly one reverse twin (closed, consistently wound)
+    edges = {}
+    for t in P:
+        for a, b in ((0, 1), (1, 2), (2, 0)):
+            key = (tuple(t[a]), tuple(t[b]))
+            edges[key] = edges.get(key, 0) + 1
+    assert all(c == 1 and edges.get((k[1], k[0]), 0) == 1 for k, c in edges.items())
+    # each gem is convex with outward normals: all its vertices lie behind every facet plane
+    for g in range(0, 150, 37):
+        G = P[g * per_gem:(g + 1) * per_gem]
+        n = np.cross(G[:, 1] - G[:, 0], G[:, 2] - G[:, 0])
+        verts = G.reshape(-1, 3)
+        dist = np.einsum("fk,fvk->fv", n, verts[None, :, :] - G[:, 0, None, :])
+        assert dist.max() <= 1e-4 * np.abs(n).max()
