@@ -248,6 +248,12 @@ class Host:
         L.ph_fit_sigmoid.argtypes = [C.c_void_p, C.c_void_p]
         L.ph_spectrum_value.restype = C.c_float
         L.ph_spectrum_value.argtypes = [C.c_void_p, C.c_void_p, C.c_float]
+        L.ph_scene_fingerprint.restype = C.c_uint64
+        L.ph_scene_fingerprint.argtypes = [C.POINTER(SceneDescEx)]
+        L.ph_checkpoint_write.argtypes = [C.c_char_p, C.POINTER(SceneDescEx), C.c_int32, C.c_uint64, C.c_void_p,
+                                          C.c_void_p, C.c_int32, C.c_char_p, C.c_int]
+        L.ph_checkpoint_read.argtypes = [C.c_char_p, C.POINTER(SceneDescEx), C.c_int32, C.c_uint64, C.c_void_p,
+                                         C.c_void_p, C.POINTER(C.c_int32), C.c_char_p, C.c_int]
 
     def generate(self, config, width=0, height=0, detail=1.0):
         err = C.create_string_buffer(512)
@@ -290,6 +296,28 @@ class Host:
         s = np.ascontiguousarray(np.array([spec_row], dtype=SPECTRUM_DTYPE))
         k = np.ascontiguousarray(knots, dtype=np.float32)
         return self.lib.ph_spectrum_value(s.ctypes.data, k.ctypes.data if len(k) else None, lam)
+
+    def fingerprint(self, scene):
+        return self.lib.ph_scene_fingerprint(scene.desc_ptr())
+
+    def checkpoint_write(self, path, scene, max_depth, seed, film_sum, film_sq, passes):
+        s = np.ascontiguousarray(film_sum, dtype=np.float64)
+        q = np.ascontiguousarray(film_sq, dtype=np.float64) if film_sq is not None else None
+        err = C.create_string_buffer(1024)
+        if self.lib.ph_checkpoint_write(str(path).encode(), scene.desc_ptr(), max_depth, seed, s.ctypes.data,
+                                        q.ctypes.data if q is not None else None, passes, err, 1024) != 0:
+            raise RuntimeError(err.value.decode())
+
+    def checkpoint_read(self, path, scene, max_depth, seed, variance=True):
+        shape = (scene.height, scene.width, 3)
+        s = np.zeros(shape, np.float64)
+        q = np.zeros(shape, np.float64) if variance else None
+        passes = C.c_int32()
+        err = C.create_string_buffer(1024)
+        if self.lib.ph_checkpoint_read(str(path).encode(), scene.desc_ptr(), max_depth, seed, s.ctypes.data,
+                                       q.ctypes.data if q is not None else None, C.byref(passes), err, 1024) != 0:
+            raise RuntimeError(err.value.decode())
+        return s, q, passes.value
 
     def open_plugin(self, path=PLUGIN_PATH):
         err = C.create_string_buffer(512)

@@ -32,6 +32,19 @@ PH_API int ph_render(void *plugin, int device, const prgpu::SceneDescEx *scene, 
                      int targetSpp, double timeLimit, double *sum, double *sumsq, int32_t *sppDone, double *seconds,
                      char *err, int errLen);
 
+// Checkpoints (reference src/integrators/integrator.cpp:16-90) ---------------------------------------
+// Scene fingerprint stored in checkpoint headers (the quantities of src/scene/scene.cpp:61-70).
+PH_API uint64_t ph_scene_fingerprint(const prgpu::SceneDescEx *scene);
+// Writes header + FP64 film sums + body {i32 passes done, optional FP64 sum-of-squares film}
+// atomically (via path.tmp). sumsq may be null. 0 on success.
+PH_API int ph_checkpoint_write(const char *path, const prgpu::SceneDescEx *scene, int32_t maxDepth, uint64_t seed,
+                               const double *sum, const double *sumsq, int32_t passes, char *err, int errLen);
+// Validates the header against (scene, maxDepth, seed) and reads the films and the pass count;
+// errors as the reference: "is not a prender checkpoint", "was made for a different scene,
+// integrator, device, resolution, seed or depth", "is truncated or corrupt". sumsq may be null.
+PH_API int ph_checkpoint_read(const char *path, const prgpu::SceneDescEx *scene, int32_t maxDepth, uint64_t seed,
+                              double *sum, double *sumsq, int32_t *passes, char *err, int errLen);
+
 // Film helpers -------------------------------------------------------------------------------------
 // Linear sRGB (the film's output colour space) of an FP64 XYZ film scaled by `scale` (src/core/film.cpp:170-175).
 PH_API void ph_resolve_srgb(const double *xyz, int64_t pixels, double scale, float *rgb);

@@ -130,3 +130,39 @@ def test_configuration_5_gems_are_closed_convex_solids(host):
         verts = G.reshape(-1, 3)
         dist = np.einsum("fk,fvk->fv", n, verts[None, :, :] - G[:, 0, None, :])
         assert dist.max() <= 1e-4 * np.abs(n).max()
+
+
+def test_checkpoint_round_trip_and_errors(host, tmp_path):
+    """The reference checkpoint format (src/integrators/integrator.cpp:16-90) and its three error
+    messages; the film and the pass count survive a write/read round trip exactly."""
+    s = host.generate(1, 24, 16)
+    rng = np.random.default_rng(3)
+    film, sq = rng.random((16, 24, 3)), rng.random((16, 24, 3))
+    path = tmp_path / "render.ckpt"
+    host.checkpoint_write(path, s, 8, 1, film, sq, 37)
+    raw = path.read_bytes()
+    assert raw[:8] == b"PRCKPT02"
+    assert int.from_bytes(raw[8:12], "little") == 0x100  # path integrator on a GPU device
+    assert int.from_bytes(raw[44:52], "little") == 24 * 16 * 3  # 44-byte header, then n
+    f2, q2, passes = host.checkpoint_read(path, s, 8, 1)
+    assert passes == 37 and np.array_equal(f2, film) and np.array_equal(q2, sq)
+    # a reader that does not want the variance film still gets the sums and passes
+    f3, _, p3 = host.checkpoint_read(path, s, 8, 1, variance=False)
+    assert p3 == 37 and np.array_equal(f3, film)
+    with pytest.raises(RuntimeError, match="different scene"):
+        host.checkpoint_read(path, s, 9, 1)
+    with pytest.raises(RuntimeError, match="different scene"):
+        host.checkpoint_read(path, s, 8, 2)
+    other = host.generate(1, 24, 16, 0.5)
+    assert host.fingerprint(other) != host.fingerprint(s)
+    with pytest.raises(RuntimeError, match="different scene"):
+        host.checkpoint_read(path, other, 8, 1)
+    bad = tmp_path / "bad.ckpt"
+    bad.write_bytes(b"not a checkpoint at all, just bytes" * 3)
+    with pytest.raises(RuntimeError, match="not a prender checkpoint"):
+        host.checkpoint_read(bad, s, 8, 1)
+    cut = tmp_path / "cut.ckpt"
+    cut.write_bytes(raw[:len(raw) // 2])
+    with pytest.raises(RuntimeError, match="truncated or corrupt"):
+        host.checkpoint_read(cut, s, 8, 1)
+    assert not (tmp_path / "render.ckpt.tmp").exists()

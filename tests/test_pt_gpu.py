@@ -103,3 +103,22 @@ def test_film_matches_oracle_film_same_samples(plugin, host):
     print(f"film same-samples: fraction within 1e-3 = {close:.4f}", flush=True)
     assert close >= 0.97
     np.testing.assert_allclose(g_sum.sum(), c_sum.sum(), rtol=5e-3)
+
+
+def test_checkpoint_resume_equals_uninterrupted(plugin, host, tmp_path):
+    """A render stopped at a checkpoint and resumed at its pass count equals the uninterrupted
+    render: sample streams are indexed by sample number (reference gpu_path.cpp:462-474)."""
+    s = host.generate(1, 48, 32)
+    handle = host.open_plugin()
+    p = pa.render_params(spp=1, first_sample=0, seed=1, max_depth=8, rr_start=3)
+    full, full_sq, done, _ = host.render(handle, s, p, 12)
+    assert done == 12
+    part, part_sq, done1, _ = host.render(handle, s, p, 5)
+    path = tmp_path / "c1.ckpt"
+    host.checkpoint_write(path, s, 8, 1, part, part_sq, done1)
+    film, sq, passes = host.checkpoint_read(path, s, 8, 1)
+    p2 = pa.render_params(spp=1, first_sample=passes, seed=1, max_depth=8, rr_start=3)
+    rest, rest_sq, done2, _ = host.render(handle, s, p2, 12 - passes)
+    np.testing.assert_allclose(film + rest, full, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(sq + rest_sq, full_sq, rtol=1e-9, atol=1e-12)
+    host.lib.ph_plugin_close(handle)
