@@ -1190,6 +1190,73 @@ static void GenMedia(const GenOptions &o, SceneBuilder *sb) {
     sb->seed = 1;
 }
 
+// A box with each face split into n x n quads (two triangles each), rotated about +y.
+static Mesh MakeTessellatedBox(F3 c, F3 size, int n, float rotY) {
+    Mesh m;
+    F3 h = size * 0.5f;
+    const float cs = std::cos(rotY), sn = std::sin(rotY);
+    auto place = [&](F3 p) { return c + f3(cs * p.x + sn * p.z, p.y, -sn * p.x + cs * p.z); };
+    struct Face {
+        F3 n, u, v;
+    };
+    const Face faces[6] = {{f3(1, 0, 0), f3(0, 0, -1), f3(0, 1, 0)}, {f3(-1, 0, 0), f3(0, 0, 1), f3(0, 1, 0)},
+                           {f3(0, 1, 0), f3(1, 0, 0), f3(0, 0, -1)}, {f3(0, -1, 0), f3(1, 0, 0), f3(0, 0, 1)},
+                           {f3(0, 0, 1), f3(1, 0, 0), f3(0, 1, 0)},  {f3(0, 0, -1), f3(-1, 0, 0), f3(0, 1, 0)}};
+    for (const Face &fc : faces) {
+        F3 cc = f3(fc.n.x * h.x, fc.n.y * h.y, fc.n.z * h.z);
+        F3 U = f3(fc.u.x * h.x, fc.u.y * h.y, fc.u.z * h.z), V = f3(fc.v.x * h.x, fc.v.y * h.y, fc.v.z * h.z);
+        int base = int(m.p.size());
+        for (int j = 0; j <= n; ++j)
+            for (int i = 0; i <= n; ++i) {
+                float a = 2.f * float(i) / float(n) - 1.f, b = 2.f * float(j) / float(n) - 1.f;
+                m.p.push_back(place(cc + U * a + V * b));
+                m.uv.insert(m.uv.end(), {float(i) / float(n), float(j) / float(n)});
+            }
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                int a = base + j * (n + 1) + i, b = a + 1, c2 = a + n + 2, d = a + n + 1;
+                m.idx.insert(m.idx.end(), {a, b, c2, a, c2, d});
+            }
+    }
+    return m;
+}
+
+static void GenRoom(const GenOptions &o, SceneBuilder *sb) {
+    // A closed 10 x 10 x 4 room lit only by a 5800 K sun through a 1 x 1 window, holding 120
+    // finely tessellated boxes (19 x 19 quads per face, albedo U[0.2, 0.8]^3): almost all light
+    // arrives indirectly, the case learned path guiding is meant for.
+    SceneRng rng(4);
+    int W = o.width > 0 ? o.width : 1920, H = o.height > 0 ? o.height : 1080;
+    sb->SetCamera(f3(-4.2f, 2.2f, -4.6f), f3(1.5f, 1.0f, 3.f), 60.f, W, H);
+    int wall = sb->Diffuse(sb->Constant(0.7f)), floorM = sb->Diffuse(sb->Constant(0.5f));
+    const float X = 5, Y = 4, Z = 5;
+    sb->AddMesh(MakeQuad(f3(-X, 0, -Z), f3(0, 0, 2 * Z), f3(2 * X, 0, 0)), floorM);       // floor, +y
+    sb->AddMesh(MakeQuad(f3(-X, Y, -Z), f3(2 * X, 0, 0), f3(0, 0, 2 * Z)), wall);         // ceiling, -y
+    sb->AddMesh(MakeQuad(f3(-X, 0, -Z), f3(0, Y, 0), f3(0, 0, 2 * Z)), wall);             // x = -5, +x
+    sb->AddMesh(MakeQuad(f3(X, 0, -Z), f3(0, 0, 2 * Z), f3(0, Y, 0)), wall);              // x = +5, -x
+    sb->AddMesh(MakeQuad(f3(-X, 0, -Z), f3(2 * X, 0, 0), f3(0, Y, 0)), wall);             // z = -5, +z
+    // z = +5 wall with a window over x in [1, 2], y in [2, 3], facing -z.
+    const float wx0 = 1, wx1 = 2, wy0 = 2, wy1 = 3;
+    sb->AddMesh(MakeQuad(f3(-X, 0, Z), f3(0, Y, 0), f3(wx0 + X, 0, 0)), wall);
+    sb->AddMesh(MakeQuad(f3(wx1, 0, Z), f3(0, Y, 0), f3(X - wx1, 0, 0)), wall);
+    sb->AddMesh(MakeQuad(f3(wx0, 0, Z), f3(0, wy0, 0), f3(wx1 - wx0, 0, 0)), wall);
+    sb->AddMesh(MakeQuad(f3(wx0, wy1, Z), f3(0, Y - wy1, 0), f3(wx1 - wx0, 0, 0)), wall);
+    const int n = std::max(1, int(std::lround(19 * o.detail)));
+    const int boxes = std::max(4, int(std::lround(120 * std::min(1.f, o.detail * 4))));
+    for (int b = 0; b < boxes; ++b) {
+        F3 size = f3(rng.uniform(0.3f, 1.0f), rng.uniform(0.3f, 1.0f), rng.uniform(0.3f, 1.0f));
+        F3 c = f3(rng.uniform(-4.4f, 4.4f), 0.5f * size.y, rng.uniform(-4.4f, 4.4f));
+        float rot = rng.uniform(0.f, kPi);
+        int mat = sb->Diffuse(sb->RGBAlbedo(rng.uniform(0.2f, 0.8f), rng.uniform(0.2f, 0.8f), rng.uniform(0.2f, 0.8f)));
+        sb->AddMesh(MakeTessellatedBox(c, size, n, rot), mat, -1, -1, false);
+    }
+    sb->AddDistantLight(f3(-0.35f, 0.55f, 0.76f), sb->BlackbodyUnitY(5800.f), 600.f);
+    sb->spp = 256;
+    sb->maxDepth = 32;
+    sb->rrStartDepth = 1;
+    sb->seed = 1;
+}
+
 static void GenGems(const GenOptions &o, SceneBuilder *sb) {
     // 150 round-brilliant diamonds (Sellmeier dispersion) at random scales U[0.5, 2] and uniformly
     // random orientations above a rough silver plane (GGX alpha 0.1), lit by a 0.2 x 0.2 quad at
@@ -1257,6 +1324,7 @@ bool GenerateConfig(int config, const GenOptions &o, SceneBuilder *sb, std::stri
     case 1: GenPyramid(o, sb); return true;
     case 2: GenTerrain(o, sb); return true;
     case 3: GenMedia(o, sb); return true;
+    case 4: GenRoom(o, sb); return true;
     case 5: GenGems(o, sb); return true;
     default:
         if (err) *err = "configuration " + std::to_string(config) + " is not available";

@@ -45,4 +45,11 @@ void or_medium_tr(void *h, int medium, const float *o, const float *d, float tMa
                   float *out);
 void or_medium_sample(void *h, int medium, const float *o, const float *d, float tMax, float u, const float *lam,
                       const float *lpdf, float *out);
+
+// Path-guiding model in FP64 (guide_oracle.cpp).
+int or_guide_param_count();
+void or_guide_forward(const float *params, const float *recs9, int n, float *out41);
+double or_guide_mix_pdf(const float *out41, const float *w, double *sel);
+void or_guide_mix_sample(const float *out41, float uLobe, float u1, float u2, float *w);
+double or_guide_loss_grad(const float *params, const float *recs16, int n, float *grad);
 }

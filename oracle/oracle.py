@@ -72,8 +72,51 @@ def lib():
         L.or_light_power.argtypes = [vp, C.c_int]
         L.or_medium_tr.argtypes = [vp, C.c_int, vp, vp, C.c_float, vp, vp, vp]
         L.or_medium_sample.argtypes = [vp, C.c_int, vp, vp, C.c_float, C.c_float, vp, vp, vp]
+        L.or_guide_param_count.restype = C.c_int
+        L.or_guide_forward.argtypes = [vp, vp, C.c_int, vp]
+        L.or_guide_mix_pdf.restype = C.c_double
+        L.or_guide_mix_pdf.argtypes = [vp, vp, C.POINTER(C.c_double)]
+        L.or_guide_mix_sample.argtypes = [vp, C.c_float, C.c_float, C.c_float, vp]
+        L.or_guide_loss_grad.restype = C.c_double
+        L.or_guide_loss_grad.argtypes = [vp, vp, C.c_int, vp]
         _lib = L
     return _lib
+
+
+# ---- path-guiding model (FP64) -------------------------------------------------------------------
+def guide_param_count():
+    return lib().or_guide_param_count()
+
+
+def guide_forward(params, recs9):
+    P = np.ascontiguousarray(params, dtype=np.float32)
+    R = np.ascontiguousarray(recs9, dtype=np.float32).reshape(-1, 9)
+    out = np.zeros((len(R), 41), np.float32)
+    lib().or_guide_forward(P.ctypes.data, R.ctypes.data, len(R), out.ctypes.data)
+    return out
+
+
+def guide_mix_pdf(out41, w):
+    o = np.ascontiguousarray(out41, dtype=np.float32)
+    wv = np.ascontiguousarray(w, dtype=np.float32)
+    sel = C.c_double()
+    p = lib().or_guide_mix_pdf(o.ctypes.data, wv.ctypes.data, C.byref(sel))
+    return p, sel.value
+
+
+def guide_mix_sample(out41, ul, u1, u2):
+    o = np.ascontiguousarray(out41, dtype=np.float32)
+    w = np.zeros(3, np.float32)
+    lib().or_guide_mix_sample(o.ctypes.data, ul, u1, u2, w.ctypes.data)
+    return w
+
+
+def guide_loss_grad(params, recs16):
+    P = np.ascontiguousarray(params, dtype=np.float32)
+    R = np.ascontiguousarray(recs16, dtype=np.float32).reshape(-1, 16)
+    g = np.zeros(len(P), np.float32)
+    loss = lib().or_guide_loss_grad(P.ctypes.data, R.ctypes.data, len(R), g.ctypes.data)
+    return loss, g
 
 
 def f32(*v):
