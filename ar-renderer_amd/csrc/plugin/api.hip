@@ -10,6 +10,7 @@
 #include <mutex>
 
 #include "device.h"
+#include "guide_host.h"
 #include "pt_host.h"
 #include "trace_dev.h"
 
@@ -367,4 +368,78 @@ PRGPU_API int prgpu_release(int deviceIndex) {
     d.eventPool.clear();
     d.geomKey = d.ptKey = 0;
     return 0;
+}
+
+// ---- path-guiding model probes -----------------------------------------------------------------
+
+namespace {
+pgpu::GuideState &EnsureGuide(pgpu::Device &d) {
+    if (!d.guide) {
+        d.guide = std::make_unique<pgpu::GuideState>();
+        d.guide->Init(d.stream);
+    }
+    return *d.guide;
+}
+}  // namespace
+
+PRGPU_API int prgpu_guide_param_count() { return pgpu::kGParams; }
+
+PRGPU_API int prgpu_guide_params(int deviceIndex, float *params, int count, int write) {
+    std::lock_guard<std::mutex> lock(gMutex);
+    try {
+        Enumerate();
+        if (!params || count != pgpu::kGParams) throw std::runtime_error("invalid arguments");
+        Device &d = Get(deviceIndex);
+        pgpu::GuideState &g = EnsureGuide(d);
+        if (write) g.SetParams(params, d.stream);
+        else PG_CHECK(hipMemcpyAsync(params, g.params.p, size_t(count) * sizeof(float), hipMemcpyDeviceToHost, d.stream));
+        PG_CHECK(hipStreamSynchronize(d.stream));
+        return 0;
+    } catch (const std::exception &) {
+        return 1;
+    }
+}
+
+PRGPU_API int prgpu_guide_eval(int deviceIndex, const float *records, int n, float *out41) {
+    std::lock_guard<std::mutex> lock(gMutex);
+    try {
+        Enumerate();
+        if (n < 0 || (n > 0 && (!records || !out41))) throw std::runtime_error("invalid arguments");
+        Device &d = Get(deviceIndex);
+        pgpu::GuideState &g = EnsureGuide(d);
+        if (n == 0) return 0;
+        DevBuf in, out;
+        in.upload(records, size_t(n) * 9, d.stream);
+        out.ensure(size_t(n) * pgpu::kGOut * sizeof(float));
+        g.Eval(in.as<float>(), n, out.as<float>(), d.stream);
+        PG_CHECK(hipMemcpyAsync(out41, out.p, size_t(n) * pgpu::kGOut * sizeof(float), hipMemcpyDeviceToHost, d.stream));
+        PG_CHECK(hipStreamSynchronize(d.stream));
+        return 0;
+    } catch (const std::exception &) {
+        return 1;
+    }
+}
+
+PRGPU_API int prgpu_guide_train(int deviceIndex, const float *records, int n, int steps, float lr, float *grad,
+                                float *stats) {
+    std::lock_guard<std::mutex> lock(gMutex);
+    try {
+        Enumerate();
+        if (n <= 0 || !records || steps < 0 || (steps == 0 && !grad)) throw std::runtime_error("invalid arguments");
+        Device &d = Get(deviceIndex);
+        pgpu::GuideState &g = EnsureGuide(d);
+        DevBuf in;
+        in.upload(records, size_t(n) * 16, d.stream);
+        if (steps == 0) {
+            g.Gradient(in.as<float>(), nullptr, 0, n, d.stream);
+            PG_CHECK(hipMemcpyAsync(grad, g.grad.p, size_t(pgpu::kGParams) * sizeof(float), hipMemcpyDeviceToHost, d.stream));
+        } else {
+            for (int k = 0; k < steps; ++k) g.Step(in.as<float>(), nullptr, 0, n, lr, d.stream);
+        }
+        if (stats) PG_CHECK(hipMemcpyAsync(stats, g.stats.p, 2 * sizeof(float), hipMemcpyDeviceToHost, d.stream));
+        PG_CHECK(hipStreamSynchronize(d.stream));
+        return 0;
+    } catch (const std::exception &) {
+        return 1;
+    }
 }

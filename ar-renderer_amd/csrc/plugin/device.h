@@ -47,7 +47,8 @@ struct DevBuf {
     }
 };
 
-struct PTScene;  // path-tracing data (pt_scene.h)
+struct PTScene;     // path-tracing data (pt.hip)
+struct GuideState;  // learned path-guiding model (guide_host.h)
 
 struct Device {
     int id = 0;
@@ -66,6 +67,7 @@ struct Device {
     // Path tracing
     uint64_t ptKey = 0;
     std::unique_ptr<PTScene> pt;
+    std::unique_ptr<GuideState> guide;
     DevBuf rays, hits;
     // Device-resident FP64 film (prgpu_film_reset / prgpu_film_read)
     DevBuf filmSum, filmSq;

@@ -1,0 +1,110 @@
+// Learned path guiding: the model's layout and its device-side mixture (decode, pdf, sampling).
+//
+// The model maps a surface vertex (position in the scene box, normal, outgoing direction) to a
+// mixture of 8 von Mises-Fisher lobes over world directions plus the probability s of sampling
+// that mixture instead of the BSDF. A guided vertex samples the one-sample MIS proposal
+//     q(w) = s * sum_k pi_k vMF(w; mu_k, kappa_k) + (1 - s) * p_bsdf(w)
+// and uses q as its scattering pdf (in the throughput and in both MIS weights), which keeps the
+// estimator unbiased whatever the network predicts. Training minimises the KL divergence of q
+// from the incident radiance (Mueller et al. 2019, "Neural Importance Sampling").
+//
+// Network: 64 features -> 4 x 64 ReLU -> 41 outputs (row-padded to 48), BF16 weights stored
+// transposed (Wt[out][in]) with FP32 biases and FP32 accumulation. Parameters live in one FP32
+// vector: per layer the weights Wt (rows x 64) then the biases (rows), rows = 64, 64, 64, 64, 48.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pt_common.h"
+
+namespace pgpu {
+
+constexpr int kGIn = 64, kGHidden = 64, kGOutRows = 48, kGOut = 41, kGLobes = 8, kGLayers = 5;
+constexpr float kGSelMin = 0.1f, kGSelMax = 0.9f, kGKappaMin = 0.01f, kGKappaMax = 1000.f;
+
+PG_HD constexpr int guide_rows(int l) { return l == kGLayers - 1 ? kGOutRows : kGHidden; }
+PG_HD constexpr int guide_woff(int l) {
+    int o = 0;
+    for (int i = 0; i < l; ++i) o += guide_rows(i) * kGIn + guide_rows(i);
+    return o;
+}
+PG_HD constexpr int guide_boff(int l) { return guide_woff(l) + guide_rows(l) * kGIn; }
+constexpr int kGParams = guide_woff(kGLayers);       // 19760 (19305 used)
+constexpr int kGWeights = (4 * kGHidden + kGOutRows) * kGIn;  // bf16 weights, all layers
+
+// Feature f (0..63) of a vertex: sin / cos of 2^j pi x on each normalised axis (48), the normal
+// and the outgoing direction (6), zeros (10).
+__device__ __forceinline__ float guide_feature(int f, const float *pos, const float *nrm, const float *wo) {
+    if (f < 48) {
+        const int axis = f >> 4, j = (f & 15) >> 1;
+        const float arg = pos[axis] * float(1 << j);
+        return (f & 1) ? cospif(arg) : sinpif(arg);
+    }
+    if (f < 51) return nrm[f - 48];
+    if (f < 54) return wo[f - 51];
+    return 0.f;
+}
+
+__device__ __forceinline__ float softplusf(float x) { return x > 20.f ? x : log1pf(expf(x)); }
+
+// vMF density with the stable normaliser kappa / (2 pi (1 - e^{-2 kappa})).
+__device__ __forceinline__ float vmf_pdf(V3 mu, float kappa, V3 w) {
+    return kappa / (2.f * kPi * -expm1f(-2.f * kappa)) * expf(kappa * (dot(mu, w) - 1.f));
+}
+
+// Lobe k of the raw outputs o[48]: mean direction (normalised; +z when degenerate), concentration.
+__device__ __forceinline__ void guide_lobe(const float *o, int k, V3 *mu, float *kappa) {
+    V3 v(o[5 * k], o[5 * k + 1], o[5 * k + 2]);
+    float l = sqrtf(dot(v, v));
+    *mu = l > 1e-6f ? v / l : V3(0.f, 0.f, 1.f);
+    *kappa = fminf(kGKappaMin + softplusf(o[5 * k + 3]), kGKappaMax);
+}
+
+struct GuideMix {
+    const float *o;  // the 48 raw outputs of this vertex
+    float amax, z;   // softmax shift and normaliser of the lobe logits
+    float sel;       // probability of sampling the mixture
+};
+
+__device__ __forceinline__ GuideMix guide_mix(const float *o) {
+    GuideMix m;
+    m.o = o;
+    m.amax = o[4];
+    for (int k = 1; k < kGLobes; ++k) m.amax = fmaxf(m.amax, o[5 * k + 4]);
+    m.z = 0.f;
+    for (int k = 0; k < kGLobes; ++k) m.z += expf(o[5 * k + 4] - m.amax);
+    m.sel = kGSelMin + (kGSelMax - kGSelMin) / (1.f + expf(-o[40]));
+    return m;
+}
+
+__device__ __forceinline__ float guide_pdf(const GuideMix &m, V3 w) {
+    float p = 0.f;
+    for (int k = 0; k < kGLobes; ++k) {
+        V3 mu;
+        float kappa;
+        guide_lobe(m.o, k, &mu, &kappa);
+        p += expf(m.o[5 * k + 4] - m.amax) * vmf_pdf(mu, kappa, w);
+    }
+    return p / m.z;
+}
+
+// Lobe by its weight (ul), then the vMF lobe (Jakob 2012) in the frame around its mean.
+__device__ __forceinline__ V3 guide_sample(const GuideMix &m, float ul, float u1, float u2) {
+    int k = 0;
+    float c = 0.f, t = ul * m.z;
+    for (; k < kGLobes - 1; ++k) {
+        c += expf(m.o[5 * k + 4] - m.amax);
+        if (t < c) break;
+    }
+    V3 mu;
+    float kappa;
+    guide_lobe(m.o, k, &mu, &kappa);
+    float cosT = 1.f + log1pf(u1 * expm1f(-2.f * kappa)) / kappa;
+    cosT = fminf(1.f, fmaxf(-1.f, cosT));
+    float sinT = sqrtf(fmaxf(0.f, 1.f - cosT * cosT)), phi = 2.f * kPi * u2;
+    float sign = copysignf(1.f, mu.z), a = -1.f / (sign + mu.z), b = mu.x * mu.y * a;
+    V3 tt(1.f + sign * mu.x * mu.x * a, sign * b, -sign * mu.x), ss(b, sign + mu.y * mu.y * a, -mu.y);
+    return tt * (sinT * cosf(phi)) + ss * (sinT * sinf(phi)) + mu * cosT;
+}
+
+}  // namespace pgpu

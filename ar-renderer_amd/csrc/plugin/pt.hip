@@ -21,6 +21,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "guide_host.h"
 #include "pt_dev.h"
 #include "pt_host.h"
 
@@ -216,7 +217,10 @@ Device::~Device() {
     pt.reset();
     if (stream) (void)hipStreamDestroy(stream);
 }
-void ReleasePT(Device &d) { d.pt.reset(); }
+void ReleasePT(Device &d) {
+    d.pt.reset();
+    d.guide.reset();
+}
 
 namespace {
 
@@ -305,6 +309,7 @@ void EnsurePTScene(Device &dev, const prgpu::SceneDescEx &sd) {
     uint64_t key = PTKey(sd);
     if (dev.pt && dev.ptKey == key) return;
     Validate(sd);
+    dev.guide.reset();  // a model learned for another scene
     auto pt = std::make_unique<PTScene>();
     hipStream_t s = dev.stream;
     UploadOrDummy(pt->prims, sd.prims, sd.primCount, s);
@@ -537,6 +542,8 @@ struct Params {
     float clampY;
     uint64_t seed;
     int firstSample;
+    int guide, train;  // learned path guiding: sample with the model / collect training records
+    GuideBox box;      // the box vertex positions are normalised to for the model
 };
 
 struct Bufs {
@@ -554,6 +561,14 @@ struct Bufs {
     float4 *xyz;
     // shadow records
     float4 *so, *sd, *sp, *sn, *sf, *sL, *sb;
+    // guiding: raw model outputs per diffuse-queue entry (12 float4), training record per path
+    // (6 float4 planes of qcap: {position, state}, {normal, p_bsdf}, wo, w, throughput, radiance
+    // snapshot), compacted training records (16 floats) and their counter
+    const float4 *gout;
+    float4 *trec;
+    float *train;
+    int *trainCount;
+    int trainCap;
 };
 
 constexpr int kBlock = 256;
@@ -631,6 +646,7 @@ __global__ __launch_bounds__(kBlock) void k_camera(const DevScene *__restrict__ 
     b.st[slot] = make_int4(0, 0, s.cameraMedium, 0);
     b.pfilm[slot] = make_float2(fx, fy);
     b.q0[slot] = slot;
+    if (p.train) b.trec[slot] = make_float4(0, 0, 0, 0);  // no guided vertex yet
 }
 
 // Closest hits of the queued paths. Persistent: each lane takes a new ray from the fetch counter
@@ -713,7 +729,7 @@ __device__ __forceinline__ bool roulette(int rr, int rrStart, PathRegs &r) {
 // the unoccluded contribution is nonzero, appends a shadow record that k_shadow completes.
 template <bool kMedia>
 __device__ __forceinline__ void sample_ld(const DevScene &s, const Bufs &b, int slot, PathRegs &r, V3 p, V3 n, V3 wo, const BSDF *bsdf,
-                          float g, V3 ns, bool surface, int mIn, int mOut) {
+                          float g, V3 ns, bool surface, int mIn, int mOut, const GuideMix *gm = nullptr) {
     float uLight = r.rng.uniform();
     V2 uL = get2(r.rng);
     float pmf;
@@ -728,6 +744,8 @@ __device__ __forceinline__ void sample_ld(const DevScene &s, const Bufs &b, int 
         f = bsdf->f(wo, wi) * absdot(wi, ns);
         if (!f.nonzero()) return;
         scatterPdf = bsdf->pdf(wo, wi);
+        // A guided vertex scatters with the one-sample MIS proposal: that is the pdf to weigh against.
+        if (gm) scatterPdf = gm->sel * guide_pdf(*gm, wi) + (1.f - gm->sel) * scatterPdf;
     } else {
         float ph = hg_phase(dot(wo, wi), g);
         f = SS(ph);
@@ -805,6 +823,16 @@ __global__ __launch_bounds__(kBlock) void k_surface(const DevScene *__restrict__
             r.prevN = V3(pn.x, pn.y, pn.z);
         };
         const bool lam0 = r.lam.terminated();
+        // Training: the radiance gathered before this vertex closes the record of the guided
+        // vertex the path just left (its own next-event estimate included).
+        if (p.train) {
+            float4 t0 = b.trec[slot];
+            if (t0.w == 1.f) {
+                b.trec[5 * size_t(b.qcap) + slot] = b.Lh[slot];
+                t0.w = 2.f;
+                b.trec[slot] = t0;
+            }
+        }
         const float4 hv = b.hit[slot];
         const int prim = __float_as_int(hv.y);
         const bool hit = prim >= 0;
@@ -923,7 +951,7 @@ __global__ __launch_bounds__(kBlock) void k_surface(const DevScene *__restrict__
 // ---- stage 2: one BSDF family (path.cpp:208-235) ----------------------------------------------------
 // Builds the BSDF, next-event estimation when it has a non-specular lobe, BSDF sampling,
 // throughput update, spawn, Russian roulette.
-template <int kType, bool kMedia>
+template <int kType, bool kMedia, bool kGuided = false>
 __global__ __launch_bounds__(kBlock) void k_material(const DevScene *__restrict__ sp, Params p, Bufs b, int *nextQueue,
                                                      int *nextCount) {
     const DevScene &s = *sp;
@@ -959,23 +987,72 @@ __global__ __launch_bounds__(kBlock) void k_material(const DevScene *__restrict_
         int mIn = int(int16_t(media & 0xffff)), mOut = int(int16_t(media >> 16));
         BSDF bsdf = eval_surface<kType>(s, m, si, r.lam);
         bool alive = true;
-        if (bsdf.non_specular())
-            sample_ld<kMedia>(s, b, slot, r, si.p, si.n, si.wo, &bsdf, 0, si.ns, true, mIn, mOut);
-        float uc = r.rng.uniform();
-        V2 u2 = get2(r.rng);
-        BSample bs;
-        if (!bsdf.sample(si.wo, uc, u2, &bs)) {
-            alive = false;
+        if (kGuided) {
+            // One-sample MIS between the learned mixture and the BSDF (q = s p_guide + (1 - s) p_bsdf).
+            const GuideMix gm = guide_mix(reinterpret_cast<const float *>(b.gout + size_t(i) * 12));
+            const SS beta0 = r.beta;
+            sample_ld<kMedia>(s, b, slot, r, si.p, si.n, si.wo, &bsdf, 0, si.ns, true, mIn, mOut, &gm);
+            const float usel = r.rng.uniform(), uc = r.rng.uniform();
+            const V2 u2 = get2(r.rng);
+            V3 wi;
+            SS f;
+            float pb;
+            if (usel < gm.sel) {
+                wi = guide_sample(gm, uc, u2.x, u2.y);
+                f = bsdf.f(si.wo, wi);
+                pb = bsdf.pdf(si.wo, wi);
+            } else {
+                BSample bs;
+                if (bsdf.sample(si.wo, uc, u2, &bs)) {
+                    wi = bs.wi;
+                    f = bs.f;
+                    pb = bs.pdf;
+                } else {
+                    f = SS(0.f);
+                    pb = 0.f;
+                }
+            }
+            const float q = gm.sel * guide_pdf(gm, wi) + (1.f - gm.sel) * pb;
+            if (!(q > 0.f) || !f.nonzero()) {
+                alive = false;
+            } else {
+                r.beta = r.beta * (f * (absdot(wi, si.ns) / q));
+                r.prevPdf = q;
+                r.flags &= ~kSpecularFlag;
+                r.ro = offset_origin(si.p, si.n, wi);
+                r.rd = wi;
+                if (kMedia) r.medium = dot(wi, si.n) > 0 ? mOut : mIn;
+                if (p.train && b.trec[slot].w == 0.f) {  // first guided vertex of the path
+                    const size_t c = size_t(b.qcap);
+                    b.trec[slot] = make_float4((si.p.x - p.box.lo[0]) * p.box.inv[0], (si.p.y - p.box.lo[1]) * p.box.inv[1],
+                                               (si.p.z - p.box.lo[2]) * p.box.inv[2], 1.f);
+                    b.trec[c + slot] = make_float4(si.n.x, si.n.y, si.n.z, pb);
+                    b.trec[2 * c + slot] = make_float4(si.wo.x, si.wo.y, si.wo.z, 0.f);
+                    b.trec[3 * c + slot] = make_float4(wi.x, wi.y, wi.z, 0.f);
+                    st4(b.trec, 4 * c + slot, beta0);
+                }
+                if (!r.beta.nonzero() || r.beta.has_nan()) alive = false;
+                else if (roulette(p.rr, p.rrStart, r)) alive = false;
+            }
         } else {
-            r.beta = r.beta * (bs.f * (absdot(bs.wi, si.ns) / bs.pdf));
-            r.prevPdf = bs.pdf;
-            r.flags = (bs.flags & kSpecular) ? (r.flags | kSpecularFlag) : (r.flags & ~kSpecularFlag);
-            if (bs.flags & kTrans) r.etaScale *= sq(bs.eta);
-            r.ro = offset_origin(si.p, si.n, bs.wi);
-            r.rd = bs.wi;
-            if (kMedia) r.medium = dot(bs.wi, si.n) > 0 ? mOut : mIn;
-            if (!r.beta.nonzero() || r.beta.has_nan()) alive = false;
-            else if (roulette(p.rr, p.rrStart, r)) alive = false;
+            if (bsdf.non_specular())
+                sample_ld<kMedia>(s, b, slot, r, si.p, si.n, si.wo, &bsdf, 0, si.ns, true, mIn, mOut);
+            float uc = r.rng.uniform();
+            V2 u2 = get2(r.rng);
+            BSample bs;
+            if (!bsdf.sample(si.wo, uc, u2, &bs)) {
+                alive = false;
+            } else {
+                r.beta = r.beta * (bs.f * (absdot(bs.wi, si.ns) / bs.pdf));
+                r.prevPdf = bs.pdf;
+                r.flags = (bs.flags & kSpecular) ? (r.flags | kSpecularFlag) : (r.flags & ~kSpecularFlag);
+                if (bs.flags & kTrans) r.etaScale *= sq(bs.eta);
+                r.ro = offset_origin(si.p, si.n, bs.wi);
+                r.rd = bs.wi;
+                if (kMedia) r.medium = dot(bs.wi, si.n) > 0 ? mOut : mIn;
+                if (!r.beta.nonzero() || r.beta.has_nan()) alive = false;
+                else if (roulette(p.rr, p.rrStart, r)) alive = false;
+            }
         }
         b.rng[slot] = make_ulonglong2(r.rng.state, r.rng.inc);
         if (alive) {
@@ -1105,6 +1182,32 @@ __global__ __launch_bounds__(kBlock) void k_finish(const DevScene *__restrict__ 
     Lambda lam = {{l.x, l.y, l.z, l.w}, {lp.x, lp.y, lp.z, lp.w}};
     SS L = lam.terminated() ? ld4(b.Lh, slot) : ld4(b.L, slot);
     L = L * 1.f;  // camera ray weight
+    if (p.train && b.trec[slot].w == 2.f) {
+        // Training weight of the path's guided vertex: the radiance gathered beyond it divided by
+        // the throughput that reached it, f L_i / q in the KL gradient (averaged over the hero
+        // wavelengths).
+        const size_t c = size_t(b.qcap);
+        const SS Lh = ld4(b.Lh, slot), snap = ld4(b.trec, 5 * c + slot), beta0 = ld4(b.trec, 4 * c + slot);
+        float w = 0.f;
+        int k = 0;
+        for (int j = 0; j < 4; ++j)
+            if (beta0.v[j] > 0.f && lam.pdf[j] > 0.f) {
+                w += (Lh.v[j] - snap.v[j]) / beta0.v[j];
+                ++k;
+            }
+        w = k ? w / float(k) : 0.f;
+        if (w > 0.f && is_finite(w)) {
+            const int at = wave_append(b.trainCount);
+            if (at < b.trainCap) {
+                float4 a = b.trec[slot], n1 = b.trec[c + slot], wo = b.trec[2 * c + slot], wi = b.trec[3 * c + slot];
+                float4 *dst = reinterpret_cast<float4 *>(b.train + size_t(at) * 16);
+                dst[0] = make_float4(a.x, a.y, a.z, n1.x);
+                dst[1] = make_float4(n1.y, n1.z, wo.x, wo.y);
+                dst[2] = make_float4(wo.z, wi.x, wi.y, wi.z);
+                dst[3] = make_float4(n1.w, w, 0.f, 0.f);
+            }
+        }
+    }
     float2 pf = b.pfilm[slot];
     if (rec) {
         prgpu::SampleRecordEx &o = rec[slot];
@@ -1235,7 +1338,23 @@ Bufs MakeBufs(PTScene &pt) {
     b.pfilm = pt.pfilm.as<float2>(); b.xyz = pt.xyz.as<float4>();
     b.so = pt.so.as<float4>(); b.sd = pt.sd.as<float4>(); b.sp = pt.sp.as<float4>(); b.sn = pt.sn.as<float4>();
     b.sf = pt.sf.as<float4>(); b.sL = pt.sL.as<float4>(); b.sb = pt.sb.as<float4>();
+    b.gout = nullptr;
+    b.trec = nullptr;
+    b.train = nullptr;
+    b.trainCount = nullptr;
+    b.trainCap = 0;
     return b;
+}
+
+// Guiding buffers of the device's model (sized for the batch), when a render uses them.
+void AttachGuide(Device &dev, Bufs &b) {
+    if (!dev.guide) return;
+    GuideState &g = *dev.guide;
+    b.gout = g.out.as<float4>();
+    b.trec = g.rec.as<float4>();
+    b.train = g.train.as<float>();
+    b.trainCount = g.trainCount.as<int>();
+    b.trainCap = int(g.trainCap);
 }
 
 Params MakeParams(const prgpu::RenderParamsEx &rp) {
@@ -1246,6 +1365,9 @@ Params MakeParams(const prgpu::RenderParamsEx &rp) {
     p.clampY = rp.clampLuminance;
     p.seed = rp.seed;
     p.firstSample = rp.firstSample;
+    p.guide = (rp.flags & prgpu::RenderGuide) ? 1 : 0;
+    p.train = p.guide && (rp.flags & prgpu::RenderGuideTrain) ? 1 : 0;
+    p.box = GuideBox{};
     return p;
 }
 
@@ -1304,6 +1426,10 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
     hipStream_t st = dev.stream;
     const int n = passes * npix;
     Bufs b = MakeBufs(pt);
+    if (p.guide) {
+        if (!dev.guide || dev.guide->cap < pt.cap) throw std::runtime_error("guiding state not prepared");
+        AttachGuide(dev, b);
+    }
     Timer timer(dev);
     const bool stats = dev.profileMode >= 2;
     const bool media = pt.hasMedia != 0, interfaces = pt.hasInterfaces != 0;
@@ -1340,7 +1466,11 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
                             : (interfaces ? k_surface<false, true> : k_surface<false, false>);
             hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, st, ds, p, b, cur, b.counts + curIdx, nxt,
                                b.counts + nxtIdx);
-            auto *kd = media ? k_material<QDiffuse, true> : k_material<QDiffuse, false>;
+            if (p.guide)
+                dev.guide->Infer(b.qm + size_t(QDiffuse) * b.qcap, b.counts + 3 + QDiffuse, active, b.h0, b.h1, b.d, p.box,
+                                 dev.cus, st);
+            auto *kd = p.guide ? (media ? k_material<QDiffuse, true, true> : k_material<QDiffuse, false, true>)
+                               : (media ? k_material<QDiffuse, true> : k_material<QDiffuse, false>);
             auto *kc = media ? k_material<QConductor, true> : k_material<QConductor, false>;
             auto *ke = media ? k_material<QDielectric, true> : k_material<QDielectric, false>;
             hipLaunchKernelGGL(kd, dim3(grid), dim3(kBlock), 0, st, ds, p, b, nxt, b.counts + nxtIdx);
@@ -1364,6 +1494,11 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
     }
     CheckTraceFlags(dev);
 }
+
+// Guided rendering trains the model after every batch: kGuideSteps Adam steps of kGuideBatch
+// records each (the first records of the batch's paths; a slice with no records is a no-op).
+constexpr int kGuideBatch = 1 << 18, kGuideSteps = 4, kGuideTrainCap = kGuideBatch * kGuideSteps;
+constexpr float kGuideLearningRate = 1e-3f;
 
 // Passes per batch: bounded by a path budget so that the state fits comfortably in HBM.
 int PassesPerBatch(int npix, int spp) {
@@ -1400,11 +1535,26 @@ void RenderExImpl(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::Render
     }
     const int reach = int(std::ceil(s.filterRadius + 0.5f));
     Params p = MakeParams(rp);
+    if (p.guide) {
+        if (!dev.guide || (rp.flags & prgpu::RenderGuideReset)) {
+            dev.guide = std::make_unique<GuideState>();
+            dev.guide->Init(dev.stream);
+        }
+        dev.guide->EnsureBatch(pt.cap);  // per-path planes are indexed with the batch capacity
+        if (p.train) dev.guide->EnsureTrain(size_t(kGuideTrainCap));
+        for (int a = 0; a < 3; ++a) {
+            p.box.lo[a] = dev.sceneLo[a];
+            const float ext = dev.sceneHi[a] - dev.sceneLo[a];
+            p.box.inv[a] = ext > 0 ? 1.f / ext : 0.f;
+        }
+    }
     for (int done = 0; done < rp.spp; done += per) {
         int passes = std::min(per, rp.spp - done);
         p.firstSample = rp.firstSample + done;
+        if (p.train) PG_CHECK(hipMemsetAsync(dev.guide->trainCount.p, 0, sizeof(int), dev.stream));
         TraceBatch(dev, s, ds, p, passes, npix, sw);
         Bufs b = MakeBufs(pt);
+        if (p.guide) AttachGuide(dev, b);
         int n = passes * npix;
         Timer timer(dev);
         timer.Run(prgpu::KernFinish, n, nullptr, [&] {
@@ -1417,6 +1567,19 @@ void RenderExImpl(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::Render
                                passes, npix, sw, reach, dsum, dsq);
         });
         PG_CHECK(hipGetLastError());
+        if (p.train) {
+            // Up to kGuideSteps Adam steps per batch, each on its own slice of the batch's records.
+            int records = 0;
+            PG_CHECK(hipMemcpyAsync(&records, dev.guide->trainCount.p, sizeof(int), hipMemcpyDeviceToHost, dev.stream));
+            PG_CHECK(hipStreamSynchronize(dev.stream));
+            const int steps = std::min(kGuideSteps, (std::min(records, kGuideTrainCap) + kGuideBatch - 1) / kGuideBatch);
+            if (steps > 0)
+                timer.Run(prgpu::KernGuide, steps, nullptr, [&] {
+                    for (int k = 0; k < steps; ++k)
+                        dev.guide->Step(dev.guide->train.as<float>(), dev.guide->trainCount.as<int>(), k * kGuideBatch,
+                                        kGuideBatch, kGuideLearningRate, dev.stream);
+                });
+        }
         if (dev.profileMode) {
             PG_CHECK(hipStreamSynchronize(dev.stream));
             timer.Collect();
@@ -1438,6 +1601,7 @@ void SampleRadianceImpl(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::
     PTScene &pt = *dev.pt;
     EnsureBatch(pt, size_t(npix));
     Params p = MakeParams(rp);
+    p.guide = p.train = 0;  // per-sample records are the plain path tracer's
     TraceBatch(dev, s, ds, p, 1, npix, sw);
     DevBuf rec;
     rec.ensure(size_t(npix) * sizeof(prgpu::SampleRecordEx));

@@ -29,6 +29,7 @@ LIGHT_AREA, LIGHT_POINT, LIGHT_DISTANT, LIGHT_ENV = 0, 1, 2, 3
 MEDIUM_HOMOGENEOUS, MEDIUM_GRID = 0, 1
 FILTER_BOX, FILTER_GAUSSIAN, FILTER_BH = 0, 1, 2
 LIGHT_SAMPLER_BVH, LIGHT_SAMPLER_POWER = 0, 1
+RENDER_GUIDE, RENDER_GUIDE_TRAIN, RENDER_GUIDE_RESET = 1, 2, 4
 
 # ---- numpy layouts of the POD arrays -------------------------------------------------------------
 PRIM_DTYPE = np.dtype({
@@ -148,15 +149,15 @@ class RenderParamsV4(C.Structure):
 assert C.sizeof(SceneDescV4) == 520 and C.sizeof(RenderParamsV4) == 24 and C.sizeof(CameraV4) == 80
 assert C.sizeof(FilterV4) == 276 and LIGHT_V4_DTYPE.itemsize == 48 and SPECTRUM95_DTYPE.itemsize == 380
 
-KERNEL_NAMES = ["camera", "intersect", "shade", "shadow", "finish", "gather"]
+KERNEL_NAMES = ["camera", "intersect", "shade", "shadow", "finish", "gather", "guide"]
 
 assert C.sizeof(ProfileStatsEx) == 256
 assert C.sizeof(SceneDescEx) == 208 and C.sizeof(RenderParamsEx) == 40 and C.sizeof(DeviceInfo) == 304
 
 
 def render_params(spp=1, first_sample=0, seed=1, max_depth=-1, rr_start=1, rr=True, light_sampler=LIGHT_SAMPLER_BVH,
-                  clamp=0.0):
-    return RenderParamsEx(spp, first_sample, seed, max_depth, clamp, rr_start, 1 if rr else 0, light_sampler, 0)
+                  clamp=0.0, flags=0):
+    return RenderParamsEx(spp, first_sample, seed, max_depth, clamp, rr_start, 1 if rr else 0, light_sampler, flags)
 
 
 def _ptr(a):
@@ -361,6 +362,10 @@ class Plugin:
         L.prgpu_film_read.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
         L.prgpu_profile.argtypes = [C.c_int, C.c_int]
         L.prgpu_profile_read.argtypes = [C.c_int, C.POINTER(ProfileStatsEx), C.c_int]
+        L.prgpu_guide_param_count.restype = C.c_int
+        L.prgpu_guide_params.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_int]
+        L.prgpu_guide_eval.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p]
+        L.prgpu_guide_train.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_void_p, C.c_void_p]
 
     def version(self):
         return self.lib.prgpu_api_version(), self.lib.prgpu_api_ext_version()
@@ -414,6 +419,42 @@ class Plugin:
 
     def release(self, device=0):
         return self.lib.prgpu_release(device)
+
+    # ---- path-guiding model ------------------------------------------------------------------
+    def guide_params(self, params=None, device=0):
+        """Reads the device's model parameters, or replaces them (and resets the optimiser)."""
+        n = self.lib.prgpu_guide_param_count()
+        if params is None:
+            out = np.zeros(n, np.float32)
+            if self.lib.prgpu_guide_params(device, out.ctypes.data, n, 0) != 0:
+                raise RuntimeError("prgpu_guide_params failed")
+            return out
+        P = np.ascontiguousarray(params, dtype=np.float32)
+        if self.lib.prgpu_guide_params(device, P.ctypes.data, len(P), 1) != 0:
+            raise RuntimeError("prgpu_guide_params failed")
+        return P
+
+    def guide_eval(self, recs9, device=0):
+        R = np.ascontiguousarray(recs9, dtype=np.float32).reshape(-1, 9)
+        out = np.zeros((len(R), 41), np.float32)
+        if self.lib.prgpu_guide_eval(device, R.ctypes.data, len(R), out.ctypes.data) != 0:
+            raise RuntimeError("prgpu_guide_eval failed")
+        return out
+
+    def guide_grad(self, recs16, device=0):
+        R = np.ascontiguousarray(recs16, dtype=np.float32).reshape(-1, 16)
+        g = np.zeros(self.lib.prgpu_guide_param_count(), np.float32)
+        st = np.zeros(2, np.float32)
+        if self.lib.prgpu_guide_train(device, R.ctypes.data, len(R), 0, 0.0, g.ctypes.data, st.ctypes.data) != 0:
+            raise RuntimeError("prgpu_guide_train failed")
+        return g, st
+
+    def guide_train(self, recs16, steps, lr=1e-3, device=0):
+        R = np.ascontiguousarray(recs16, dtype=np.float32).reshape(-1, 16)
+        st = np.zeros(2, np.float32)
+        if self.lib.prgpu_guide_train(device, R.ctypes.data, len(R), steps, lr, None, st.ctypes.data) != 0:
+            raise RuntimeError("prgpu_guide_train failed")
+        return st
 
     # ---- device-resident film and profiling --------------------------------------------------
     def film_reset(self, width, height, device=0):

@@ -252,6 +252,13 @@ struct SceneDescEx {
 
 enum LightSamplerKind : int32_t { LightSamplerBVH = 0, LightSamplerPower = 1 };
 
+// RenderParamsEx::flags. Learned path guiding (no counterpart in the reference): diffuse
+// vertices sample a one-sample MIS mixture of a learned 8-lobe vMF distribution and the BSDF;
+// with RenderGuideTrain the model is trained after every batch of the call (Adam on the KL
+// divergence from the incident radiance); RenderGuideReset starts from the initial model. The
+// model persists on the device across calls for the same scene.
+enum RenderFlagsEx : int32_t { RenderGuide = 1, RenderGuideTrain = 2, RenderGuideReset = 4 };
+
 struct RenderParamsEx {
     int32_t spp;               // passes rendered by this call
     int32_t firstSample;       // sample index of the first pass
@@ -288,7 +295,8 @@ struct SampleRecordEx {
 
 // Profiling counters of one kind of kernel launch (prgpu_profile_read).
 enum KernelKindEx : int {
-    KernCamera = 0, KernIntersect = 1, KernShade = 2, KernShadow = 3, KernFinish = 4, KernGather = 5, KernCount = 6
+    KernCamera = 0, KernIntersect = 1, KernShade = 2, KernShadow = 3, KernFinish = 4, KernGather = 5, KernGuide = 6,
+    KernCount = 7
 };
 struct ProfileStatsEx {
     double ms[8];          // summed HIP-event durations per kernel kind
@@ -341,6 +349,22 @@ PRGPU_API int prgpu_profile(int deviceIndex, int mode);
 PRGPU_API int prgpu_profile_read(int deviceIndex, prgpu::ProfileStatsEx *out, int reset);
 // Frees every device buffer the plugin holds for the device.
 PRGPU_API int prgpu_release(int deviceIndex);
+
+// ---- path-guiding model probes (no reference counterpart) ---------------------------------------
+// The model's parameters are one FP32 vector of prgpu_guide_param_count() values: per layer the
+// weights Wt[out][in] (rows 64, 64, 64, 64, 48; 64 columns) then the biases.
+PRGPU_API int prgpu_guide_param_count();
+// Reads (write = 0) or replaces (write = 1; also resets the optimiser) the device's model.
+PRGPU_API int prgpu_guide_params(int deviceIndex, float *params, int count, int write);
+// The network's 41 raw outputs for n records of 9 floats (position normalised to the scene box,
+// normal, outgoing direction), evaluated with the BF16 matrix-core kernel.
+PRGPU_API int prgpu_guide_eval(int deviceIndex, const float *records, int n, float *out41);
+// Gradient of the normalised training loss over n records of 16 floats (position, normal, wo,
+// sampled direction, p_bsdf, weight, 2 unused) into grad (param_count floats); stats = {summed
+// weight, summed weighted loss}. With steps > 0, instead runs that many Adam steps (learning rate
+// lr) on the records and returns the stats of the last step.
+PRGPU_API int prgpu_guide_train(int deviceIndex, const float *records, int n, int steps, float lr, float *grad,
+                                float *stats);
 
 // ---- layout pins (measured on the reference's LP64 build) -----------------------------------
 static_assert(sizeof(prgpu::Float3) == 12, "Float3");

@@ -75,10 +75,13 @@ def three_sigma(name, g_sum, g_sq, n_g, c_sum, c_sq, n_c):
     # Expected outlier fraction ~0.27%; allow a binomial 5-sigma margin and heavy-tail slack.
     n = valid.sum()
     bound = 0.0027 + 5 * np.sqrt(0.0027 * (1 - 0.0027) / max(1, n)) + 0.01
+    # The image means, against the standard error of their difference (pixels are independent).
+    z = abs(mg.mean() - mc.mean()) / max(np.sqrt((vg + vc).sum()) / mg.size, 1e-30)
     rel_mean = abs(mg.mean() - mc.mean()) / mc.mean()
-    print(f"{name}: pixels*channels={n} outside 3 sigma={frac:.4%} (bound {bound:.2%}) mean ratio diff={rel_mean:.3e}",
-          flush=True)
+    print(f"{name}: pixels*channels={n} outside 3 sigma={frac:.4%} (bound {bound:.2%}) mean ratio diff={rel_mean:.3e} "
+          f"(z = {z:.2f})", flush=True)
     assert frac <= bound, f"{name}: {frac:.4%} of pixel values outside 3 sigma"
+    assert z < 5, f"{name}: image means differ by {z:.1f} standard errors"
     return frac
 
 
