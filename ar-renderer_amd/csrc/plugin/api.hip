@@ -443,3 +443,18 @@ PRGPU_API int prgpu_guide_train(int deviceIndex, const float *records, int n, in
         return 1;
     }
 }
+
+PRGPU_API int prgpu_guide_stats(int deviceIndex, float *stats, int *steps) {
+    std::lock_guard<std::mutex> lock(gMutex);
+    try {
+        Enumerate();
+        Device &d = Get(deviceIndex);
+        if (!d.guide) return 1;
+        if (stats) PG_CHECK(hipMemcpyAsync(stats, d.guide->stats.p, 2 * sizeof(float), hipMemcpyDeviceToHost, d.stream));
+        if (steps) *steps = d.guide->steps;
+        PG_CHECK(hipStreamSynchronize(d.stream));
+        return 0;
+    } catch (const std::exception &) {
+        return 1;
+    }
+}

@@ -60,14 +60,85 @@ __device__ __forceinline__ void guide_lobe(const float *o, int k, V3 *mu, float 
     *kappa = fminf(kGKappaMin + softplusf(o[5 * k + 3]), kGKappaMax);
 }
 
+// A decoded mixture, as the inference kernel stores it per vertex (12 float4): lobes
+// (mu.x, mu.y, mu.z, kappa) in [0, 8), the pdf coefficients c_k = pi_k kappa_k / (2 pi (1 -
+// e^{-2 kappa_k})) in [8, 10), the lobe CDF pi_0 .. pi_0 + .. + pi_6 and s in [10, 12).
 struct GuideMix {
-    const float *o;  // the 48 raw outputs of this vertex
-    float amax, z;   // softmax shift and normaliser of the lobe logits
-    float sel;       // probability of sampling the mixture
+    const float4 *d;
+    float sel;
 };
 
-__device__ __forceinline__ GuideMix guide_mix(const float *o) {
+__device__ __forceinline__ GuideMix guide_mix(const float4 *d) {
     GuideMix m;
+    m.d = d;
+    m.sel = d[11].w;
+    return m;
+}
+
+// Decodes the 48 raw outputs o into the stored form (used by the inference kernel).
+__device__ __forceinline__ void guide_decode(const float *o, float4 *d) {
+    float amax = o[4];
+    for (int k = 1; k < kGLobes; ++k) amax = fmaxf(amax, o[5 * k + 4]);
+    float w[kGLobes], z = 0.f;
+    for (int k = 0; k < kGLobes; ++k) {
+        w[k] = expf(o[5 * k + 4] - amax);
+        z += w[k];
+    }
+    float c[kGLobes], cdf[kGLobes];
+    float acc = 0.f;
+    for (int k = 0; k < kGLobes; ++k) {
+        V3 mu;
+        float kappa;
+        guide_lobe(o, k, &mu, &kappa);
+        const float pi = w[k] / z;
+        d[k] = make_float4(mu.x, mu.y, mu.z, kappa);
+        c[k] = pi * kappa / (2.f * kPi * -expm1f(-2.f * kappa));
+        acc += pi;
+        cdf[k] = acc;
+    }
+    d[8] = make_float4(c[0], c[1], c[2], c[3]);
+    d[9] = make_float4(c[4], c[5], c[6], c[7]);
+    d[10] = make_float4(cdf[0], cdf[1], cdf[2], cdf[3]);
+    d[11] = make_float4(cdf[4], cdf[5], cdf[6], kGSelMin + (kGSelMax - kGSelMin) / (1.f + expf(-o[40])));
+}
+
+__device__ __forceinline__ float guide_pdf(const GuideMix &m, V3 w) {
+    const float4 c0 = m.d[8], c1 = m.d[9];
+    const float c[kGLobes] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+    float p = 0.f;
+#pragma unroll
+    for (int k = 0; k < kGLobes; ++k) {
+        const float4 l = m.d[k];
+        p += c[k] * expf(l.w * (l.x * w.x + l.y * w.y + l.z * w.z - 1.f));
+    }
+    return p;
+}
+
+// Lobe by the CDF (ul), then the vMF lobe (Jakob 2012) in the frame around its mean.
+__device__ __forceinline__ V3 guide_sample(const GuideMix &m, float ul, float u1, float u2) {
+    const float4 f0 = m.d[10], f1 = m.d[11];
+    const float cdf[kGLobes - 1] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z};
+    int k = 0;
+    while (k < kGLobes - 1 && !(ul < cdf[k])) ++k;
+    const float4 l = m.d[k];
+    const V3 mu(l.x, l.y, l.z);
+    const float kappa = l.w;
+    float cosT = 1.f + log1pf(u1 * expm1f(-2.f * kappa)) / kappa;
+    cosT = fminf(1.f, fmaxf(-1.f, cosT));
+    float sinT = sqrtf(fmaxf(0.f, 1.f - cosT * cosT)), phi = 2.f * kPi * u2;
+    float sign = copysignf(1.f, mu.z), a = -1.f / (sign + mu.z), b = mu.x * mu.y * a;
+    V3 tt(1.f + sign * mu.x * mu.x * a, sign * b, -sign * mu.x), ss(b, sign + mu.y * mu.y * a, -mu.y);
+    return tt * (sinT * cosf(phi)) + ss * (sinT * sinf(phi)) + mu * cosT;
+}
+
+// Raw-output helpers for the training kernel (the loss is differentiated through the raw outputs).
+struct GuideRaw {
+    const float *o;
+    float amax, z, sel;
+};
+
+__device__ __forceinline__ GuideRaw guide_raw(const float *o) {
+    GuideRaw m;
     m.o = o;
     m.amax = o[4];
     for (int k = 1; k < kGLobes; ++k) m.amax = fmaxf(m.amax, o[5 * k + 4]);
@@ -75,36 +146,6 @@ __device__ __forceinline__ GuideMix guide_mix(const float *o) {
     for (int k = 0; k < kGLobes; ++k) m.z += expf(o[5 * k + 4] - m.amax);
     m.sel = kGSelMin + (kGSelMax - kGSelMin) / (1.f + expf(-o[40]));
     return m;
-}
-
-__device__ __forceinline__ float guide_pdf(const GuideMix &m, V3 w) {
-    float p = 0.f;
-    for (int k = 0; k < kGLobes; ++k) {
-        V3 mu;
-        float kappa;
-        guide_lobe(m.o, k, &mu, &kappa);
-        p += expf(m.o[5 * k + 4] - m.amax) * vmf_pdf(mu, kappa, w);
-    }
-    return p / m.z;
-}
-
-// Lobe by its weight (ul), then the vMF lobe (Jakob 2012) in the frame around its mean.
-__device__ __forceinline__ V3 guide_sample(const GuideMix &m, float ul, float u1, float u2) {
-    int k = 0;
-    float c = 0.f, t = ul * m.z;
-    for (; k < kGLobes - 1; ++k) {
-        c += expf(m.o[5 * k + 4] - m.amax);
-        if (t < c) break;
-    }
-    V3 mu;
-    float kappa;
-    guide_lobe(m.o, k, &mu, &kappa);
-    float cosT = 1.f + log1pf(u1 * expm1f(-2.f * kappa)) / kappa;
-    cosT = fminf(1.f, fmaxf(-1.f, cosT));
-    float sinT = sqrtf(fmaxf(0.f, 1.f - cosT * cosT)), phi = 2.f * kPi * u2;
-    float sign = copysignf(1.f, mu.z), a = -1.f / (sign + mu.z), b = mu.x * mu.y * a;
-    V3 tt(1.f + sign * mu.x * mu.x * a, sign * b, -sign * mu.x), ss(b, sign + mu.y * mu.y * a, -mu.y);
-    return tt * (sinT * cosf(phi)) + ss * (sinT * sinf(phi)) + mu * cosT;
 }
 
 }  // namespace pgpu

@@ -102,20 +102,26 @@ __device__ __forceinline__ void output(const GuideLds &L, const bf16x4 xin[4], f
     }
 }
 
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // ---- inference -------------------------------------------------------------------------------
-// Raw outputs for the queued vertices: queue[i] is a path slot whose hit record (position h0,
-// geometric normal h1) and incoming direction d describe the vertex; outputs go to
-// out[48 * i ..]. Vertex positions are normalised to the box (lo, inv extent).
+// The decoded mixture (guide.h) for the queued vertices: queue[i] is a path slot whose hit record
+// (position h0, geometric normal h1) and incoming direction d describe the vertex; the mixture
+// goes to out[12 i ..]. Vertex positions are normalised to the box (lo, inv extent).
 __global__ __launch_bounds__(kGBlock) void k_guide_infer(const short *wt, const float *bias, const int *queue,
                                                          const int *count, const float4 *h0, const float4 *h1,
                                                          const float4 *dir, GuideBox box, float4 *out) {
     __shared__ GuideLds L;
+    __shared__ float tile[kGBlock / 64][16][kGOutRows];
     load_model(L, wt, bias);
     __syncthreads();
-    const int n = *count, lane = threadIdx.x & 63;
+    const int n = *count, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int waves = int(gridDim.x) * (kGBlock / 64);
-    for (int tile = int(blockIdx.x) * (kGBlock / 64) + int(threadIdx.x >> 6); tile * 16 < n; tile += waves) {
-        const int idx = tile * 16 + (lane & 15);
+    for (int tile_i = int(blockIdx.x) * (kGBlock / 64) + wave; tile_i * 16 < n; tile_i += waves) {
+        const int idx = tile_i * 16 + (lane & 15);
         const bool valid = idx < n;
         float pos[3] = {0.f, 0.f, 0.f}, nrm[3] = {0.f, 0.f, 0.f}, wo[3] = {0.f, 0.f, 0.f};
         if (valid) {
@@ -136,10 +142,43 @@ __global__ __launch_bounds__(kGBlock) void k_guide_infer(const short *wt, const 
         hidden(L, 3, xb, xa, lane);
         f32x4 o[3];
         output(L, xa, o, lane);
-        if (valid)
+        // Decode the mixture with all 64 lanes: the four lanes of a vertex (lane % 16) take two lobes
+        // each and share the softmax normaliser and the lobe CDF through shuffles.
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
-                out[size_t(idx) * 12 + j * 4 + (lane >> 4)] = make_float4(o[j][0], o[j][1], o[j][2], o[j][3]);
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) tile[wave][lane & 15][j * 16 + 4 * (lane >> 4) + i] = o[j][i];
+        wave_sync();
+        {
+            const int sidx = lane & 15, q = lane >> 4, k0 = 2 * q;
+            const float *t = tile[wave][sidx];
+            const float a0 = t[5 * k0 + 4], a1 = t[5 * k0 + 9];
+            float amax = fmaxf(a0, a1);
+            amax = fmaxf(amax, __shfl_xor(amax, 16));
+            amax = fmaxf(amax, __shfl_xor(amax, 32));
+            const float w0 = expf(a0 - amax), w1 = expf(a1 - amax), pair = w0 + w1;
+            const float p0 = __shfl(pair, sidx), p1 = __shfl(pair, sidx + 16), p2 = __shfl(pair, sidx + 32),
+                        p3 = __shfl(pair, sidx + 48);
+            const float z = p0 + p1 + p2 + p3;
+            const float excl = (q > 0 ? p0 : 0.f) + (q > 1 ? p1 : 0.f) + (q > 2 ? p2 : 0.f);
+            const int idx = tile_i * 16 + sidx;
+            if (idx < n) {
+                float *of = reinterpret_cast<float *>(out + size_t(idx) * 12);
+                const float w[2] = {w0, w1};
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    V3 mu;
+                    float kappa;
+                    guide_lobe(t, k0 + h, &mu, &kappa);
+                    const float pi = w[h] / z;
+                    reinterpret_cast<float4 *>(of)[k0 + h] = make_float4(mu.x, mu.y, mu.z, kappa);
+                    of[32 + k0 + h] = pi * kappa / (2.f * kPi * -expm1f(-2.f * kappa));
+                    if (k0 + h < kGLobes - 1) of[40 + k0 + h] = (excl + w0 + (h ? w1 : 0.f)) / z;
+                }
+                if (q == 3) of[47] = kGSelMin + (kGSelMax - kGSelMin) / (1.f + expf(-t[40]));
+            }
+        }
+        wave_sync();
     }
 }
 
@@ -185,7 +224,7 @@ __device__ void output_grad(const float *o, const float *rec, float *dout, float
     *lossOut = 0.f;
     if (!(wt > 0.f)) return;
     const V3 w(rec[9], rec[10], rec[11]);
-    GuideMix m = guide_mix(o);
+    GuideRaw m = guide_raw(o);
     float pk[kGLobes], pm = 0.f, pi[kGLobes];
     for (int k = 0; k < kGLobes; ++k) {
         V3 mu;
@@ -228,11 +267,6 @@ struct TrainLds {
     float tile[4][16][kGOutRows];  // per wave: raw outputs, then their gradients
     float wsum, loss;
 };
-
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
 
 // dW_l += dY_l X_l^T over this wave's 16 samples and db_l += sum_s dY_l (LDS atomics), with the
 // layer's input X (B layout, bf16) and output gradient dY (accumulator layout, fp32, nb blocks).
@@ -410,8 +444,10 @@ constexpr int kTrainBlocks = 256;
 
 size_t GuideTrainLdsBytes() { return sizeof(TrainLds); }
 
-// Initial parameters: He-uniform hidden layers, a small output layer, zero biases, so the first
-// proposal is close to uniform lobes with s = 1/2. Deterministic (PCG32 stream 54).
+// Initial parameters: He-uniform hidden layers, a small output layer, zero biases except the
+// selection output, so the first proposal is a set of broad lobes sampled with s = 0.2 (most
+// samples still follow the BSDF until training sharpens the mixture). Deterministic (PCG32
+// stream 54).
 std::vector<float> GuideInitialParams() {
     std::vector<float> P(size_t(kGParams), 0.f);
     uint64_t state = 0x853c49e6748fea9bull, inc = (54ull << 1u) | 1u;
@@ -428,6 +464,8 @@ std::vector<float> GuideInitialParams() {
         for (int r = 0; r < rows; ++r)
             for (int c = 0; c < kGIn; ++c) P[size_t(guide_woff(l) + r * kGIn + c)] = (2.f * next() - 1.f) * a;
     }
+    // s = kGSelMin + (kGSelMax - kGSelMin) sigmoid(b) = 0.2
+    P[size_t(guide_boff(kGLayers - 1) + 40)] = std::log((0.2f - kGSelMin) / (kGSelMax - 0.2f));
     return P;
 }
 

@@ -202,7 +202,7 @@ struct PTScene {
     // batch buffers
     size_t cap = 0;
     DevBuf o, d, rng, lam, lpdf, beta, rpath, L, Lh, prevP, prevN, st, hit, q0, q1, qm, pfilm, xyz;
-    DevBuf h0, h1, h2, h3;
+    DevBuf h0, h1, h2, h3, qe, qeLpdf;
     DevBuf sceneBuf;  // the DevScene the kernels read through a pointer
     DevBuf so, sd, sp, sn, sf, sL, sb, sq;
     int *countsHost = nullptr;
@@ -488,6 +488,12 @@ DevScene MakeDevScene(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::Re
     s.leafOf = pt.leafOf.as<int>();
     s.infinite = pt.infinite.as<int>();
     s.infCount = pt.infCount;
+    s.envCount = 0;
+    s.emitters = 0;
+    for (int i = 0; i < sd.lightCount; ++i) {
+        s.envCount += sd.lights[i].kind == prgpu::LightExEnvironment ? 1 : 0;
+        s.emitters |= (sd.lights[i].kind == prgpu::LightExArea || sd.lights[i].kind == prgpu::LightExEnvironment) ? 1 : 0;
+    }
     s.powerCdf = pt.powerCdf.as<float>();
     s.powerFunc = pt.powerFunc.as<float>();
     s.powerInt = pt.powerInt;
@@ -556,14 +562,17 @@ struct Bufs {
     int *q0, *q1;
     int *qm;     // material queues, QCount x qcap
     int qcap;
-    int *counts;  // [0] queue A, [1] queue B, [2] shadow queue, [3..5] material queues, [6..7] ray fetch
+    int *counts;  // [0] queue A, [1] queue B, [2] shadow queue, [3..5] material queues, [6..7] ray fetch,
+                  // [8] unused, [9] emission queue
     float2 *pfilm;
     float4 *xyz;
     // shadow records
     float4 *so, *sd, *sp, *sn, *sf, *sL, *sb;
-    // guiding: raw model outputs per diffuse-queue entry (12 float4), training record per path
+    // guiding: decoded mixture per diffuse-queue entry (12 float4), training record per path
     // (6 float4 planes of qcap: {position, state}, {normal, p_bsdf}, wo, w, throughput, radiance
     // snapshot), compacted training records (16 floats) and their counter
+    int4 *qe;       // emission queue: slot, depth and flags on arrival, primitive (-1: escaped)
+    float4 *qeLpdf; // and the wavelength pdfs on arrival
     const float4 *gout;
     float4 *trec;
     float *train;
@@ -782,8 +791,10 @@ __device__ __forceinline__ void add_emitted(const DevScene &s, PathRegs &r, int 
 
 // ---- stage 1 of a bounce: everything up to the BSDF (path.cpp:155-210) ----------------------------
 // Medium sampling (and, for scattering events, the whole medium vertex), escape and area
-// emission with MIS, interfaces, the depth limit. Paths that continue at a surface are
-// compacted into one queue per BSDF family together with their hit record.
+// emission, interfaces, the depth limit. Paths that continue at a surface are compacted into one
+// queue per BSDF family together with their hit record; paths that meet emission (an emitting
+// primitive, or escape with an environment light) are appended to the emission queue, whose
+// MIS-weighted contributions k_emit adds before the BSDF stage changes the throughput.
 enum MatQueue : int { QDiffuse = 0, QConductor = 1, QDielectric = 2, QCount = 3 };
 
 template <bool kMedia, bool kInterfaces>
@@ -800,28 +811,14 @@ __global__ __launch_bounds__(kBlock) void k_surface(const DevScene *__restrict__
         r.prevPdf = o.w;
         r.rd = V3(d.x, d.y, d.z);
         r.etaScale = d.w;
-        float4 l = b.lam[slot], lp = b.lpdf[slot];
-        r.lam = {{l.x, l.y, l.z, l.w}, {lp.x, lp.y, lp.z, lp.w}};
+        float4 lp = b.lpdf[slot];
+        const float4 lpIn = lp;
+        r.lam.pdf[0] = lp.x, r.lam.pdf[1] = lp.y, r.lam.pdf[2] = lp.z, r.lam.pdf[3] = lp.w;
         int4 st = b.st[slot];
         r.depth = st.x;
         r.flags = st.y;
         r.medium = st.z;
         r.crossings = st.w;
-        // Throughput, MIS ratios, radiance and the previous vertex are read only by paths that
-        // meet a medium or emission (and written back only then): most surface hits route to a
-        // material queue without touching them.
-        bool loaded = false;
-        auto load = [&] {
-            if (loaded) return;
-            loaded = true;
-            r.beta = ld4(b.beta, slot);
-            r.rpath = ld4(b.rpath, slot);
-            r.L = ld4(b.L, slot);
-            r.Lh = ld4(b.Lh, slot);
-            float4 pp = b.prevP[slot], pn = b.prevN[slot];
-            r.prevP = V3(pp.x, pp.y, pp.z);
-            r.prevN = V3(pn.x, pn.y, pn.z);
-        };
         const bool lam0 = r.lam.terminated();
         // Training: the radiance gathered before this vertex closes the record of the guided
         // vertex the path just left (its own next-event estimate included).
@@ -837,11 +834,20 @@ __global__ __launch_bounds__(kBlock) void k_surface(const DevScene *__restrict__
         const int prim = __float_as_int(hv.y);
         const bool hit = prim >= 0;
         const float tHit = hv.x;
-        bool alive = true, toMaterial = false, done = false;
+        bool alive = true, toMaterial = false, done = false, emits = false;
         int mq = 0;
 
         if (kMedia && r.medium >= 0) {
-            load();
+            // The medium vertex needs the whole path state (throughput, MIS ratios, radiance).
+            float4 l = b.lam[slot];
+            r.lam.l[0] = l.x, r.lam.l[1] = l.y, r.lam.l[2] = l.z, r.lam.l[3] = l.w;
+            r.beta = ld4(b.beta, slot);
+            r.rpath = ld4(b.rpath, slot);
+            r.L = ld4(b.L, slot);
+            r.Lh = ld4(b.Lh, slot);
+            float4 pp = b.prevP[slot], pn = b.prevN[slot];
+            r.prevP = V3(pp.x, pp.y, pp.z);
+            r.prevN = V3(pn.x, pn.y, pn.z);
             r.rng = load_rng(b, slot);
             DistSample ds = medium_sample(s, r.medium, r.ro, r.rd, hit ? tHit : kInfF, r.rng.uniform(), r.lam);
             r.beta = r.beta * ds.weight;
@@ -874,30 +880,28 @@ __global__ __launch_bounds__(kBlock) void k_surface(const DevScene *__restrict__
                 }
             }
             b.rng[slot] = make_ulonglong2(r.rng.state, r.rng.inc);
+            st4(b.beta, slot, r.beta);
+            st4(b.rpath, slot, r.rpath);
+            st4(b.L, slot, r.L);
+            st4(b.Lh, slot, r.Lh);
+            b.prevP[slot] = make_float4(r.prevP.x, r.prevP.y, r.prevP.z, 0.f);
+            b.prevN[slot] = make_float4(r.prevN.x, r.prevN.y, r.prevN.z, 0.f);
         }
 
+        const int depthIn = r.depth, flagsIn = r.flags;
         if (!done) {
             if (!hit) {
-                for (int li = 0; li < s.infCount; ++li) {
-                    int L = s.infinite[li];
-                    if (s.lights[L].kind == prgpu::LightExEnvironment) {
-                        load();
-                        add_emitted(s, r, L, spec_sample(s.spec, s.lights[L].spectrum, r.lam) * s.lights[L].scale, r.rd);
-                    }
-                }
+                emits = s.envCount > 0;
                 alive = false;
             } else {
                 const prgpu::Primitive &pr = s.prims[prim];
-                Surf si;
-                interaction(pr, r.ro, r.rd, tHit, hv.z, hv.w, &si);
-                if (pr.light >= 0) {
-                    load();
-                    add_emitted(s, r, pr.light, area_L(s, s.lights[pr.light], si.n, -r.rd, r.lam), r.rd);
-                }
+                emits = pr.light >= 0;
                 if (kInterfaces && is_interface(s, pr)) {
                     if (++r.crossings > 1000) {
                         alive = false;
                     } else {
+                        Surf si;
+                        interaction(pr, r.ro, r.rd, tHit, hv.z, hv.w, &si);
                         r.medium = dot(r.rd, si.n) > 0 ? pr.mediumOutside : pr.mediumInside;
                         r.ro = offset_origin(si.p, si.n, r.rd);
                     }
@@ -911,6 +915,8 @@ __global__ __launch_bounds__(kBlock) void k_surface(const DevScene *__restrict__
                     if (r.depth++ >= p.maxDepth) {
                         alive = false;
                     } else {
+                        Surf si;
+                        interaction(pr, r.ro, r.rd, tHit, hv.z, hv.w, &si);
                         toMaterial = true;
                         mq = m.type == prgpu::MatDiffuse ? QDiffuse : (m.type == prgpu::MatConductor ? QConductor : QDielectric);
                         uint32_t media = uint32_t(pr.mediumInside & 0xffff) | (uint32_t(pr.mediumOutside & 0xffff) << 16);
@@ -922,29 +928,67 @@ __global__ __launch_bounds__(kBlock) void k_surface(const DevScene *__restrict__
                 }
             }
         }
+        if (emits) {
+            // The emission is evaluated with the wavelength pdfs and depth the path arrived with.
+            const int e = wave_append(&b.counts[9]);
+            b.qe[e] = make_int4(slot, depthIn, flagsIn, prim);
+            b.qeLpdf[e] = lpIn;
+        }
 
         if (r.lam.terminated() != lam0)
             b.lpdf[slot] = make_float4(r.lam.pdf[0], r.lam.pdf[1], r.lam.pdf[2], r.lam.pdf[3]);
-        if (loaded) {
-            st4(b.L, slot, r.L);
-            st4(b.Lh, slot, r.Lh);
-        }
         b.st[slot] = make_int4(r.depth, r.flags, r.medium, r.crossings);
         if (kMedia || kInterfaces) {
             b.o[slot] = make_float4(r.ro.x, r.ro.y, r.ro.z, r.prevPdf);
             b.d[slot] = make_float4(r.rd.x, r.rd.y, r.rd.z, r.etaScale);
-            if (loaded) {
-                st4(b.beta, slot, r.beta);
-                st4(b.rpath, slot, r.rpath);
-                b.prevP[slot] = make_float4(r.prevP.x, r.prevP.y, r.prevP.z, 0.f);
-                b.prevN[slot] = make_float4(r.prevN.x, r.prevN.y, r.prevN.z, 0.f);
-            }
         }
         const int target = toMaterial ? mq : (alive ? QCount : -1);
         if (target == QDiffuse) b.qm[QDiffuse * b.qcap + wave_append(&b.counts[3 + QDiffuse])] = slot;
         if (target == QConductor) b.qm[QConductor * b.qcap + wave_append(&b.counts[3 + QConductor])] = slot;
         if (target == QDielectric) b.qm[QDielectric * b.qcap + wave_append(&b.counts[3 + QDielectric])] = slot;
         if (target == QCount) nextQueue[wave_append(nextCount)] = slot;
+    }
+}
+
+// Emission met by the paths of the emission queue, with its MIS weight (path.cpp:143-153,
+// 159-167): area light of the hit primitive, or the environment for an escaped path.
+__global__ __launch_bounds__(kBlock) void k_emit(const DevScene *__restrict__ sp, Bufs b) {
+    const DevScene &s = *sp;
+    const int n = b.counts[9];
+    const int stride = int(gridDim.x * blockDim.x);
+    for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += stride) {
+        const int4 e = b.qe[i];
+        const int slot = e.x, prim = e.w;
+        PathRegs r;
+        float4 o = b.o[slot], d = b.d[slot], l = b.lam[slot], lp = b.qeLpdf[i];
+        r.ro = V3(o.x, o.y, o.z);
+        r.prevPdf = o.w;
+        r.rd = V3(d.x, d.y, d.z);
+        r.lam = {{l.x, l.y, l.z, l.w}, {lp.x, lp.y, lp.z, lp.w}};
+        r.depth = e.y;
+        r.flags = e.z;
+        r.beta = ld4(b.beta, slot);
+        r.rpath = ld4(b.rpath, slot);
+        r.L = ld4(b.L, slot);
+        r.Lh = ld4(b.Lh, slot);
+        float4 pp = b.prevP[slot], pn = b.prevN[slot];
+        r.prevP = V3(pp.x, pp.y, pp.z);
+        r.prevN = V3(pn.x, pn.y, pn.z);
+        if (prim < 0) {
+            for (int li = 0; li < s.infCount; ++li) {
+                int L = s.infinite[li];
+                if (s.lights[L].kind == prgpu::LightExEnvironment)
+                    add_emitted(s, r, L, spec_sample(s.spec, s.lights[L].spectrum, r.lam) * s.lights[L].scale, r.rd);
+            }
+        } else {
+            const prgpu::Primitive &pr = s.prims[prim];
+            const float4 hv = b.hit[slot];
+            Surf si;
+            interaction(pr, r.ro, r.rd, hv.x, hv.z, hv.w, &si);
+            add_emitted(s, r, pr.light, area_L(s, s.lights[pr.light], si.n, -r.rd, r.lam), r.rd);
+        }
+        st4(b.L, slot, r.L);
+        st4(b.Lh, slot, r.Lh);
     }
 }
 
@@ -989,7 +1033,7 @@ __global__ __launch_bounds__(kBlock) void k_material(const DevScene *__restrict_
         bool alive = true;
         if (kGuided) {
             // One-sample MIS between the learned mixture and the BSDF (q = s p_guide + (1 - s) p_bsdf).
-            const GuideMix gm = guide_mix(reinterpret_cast<const float *>(b.gout + size_t(i) * 12));
+            const GuideMix gm = guide_mix(b.gout + size_t(i) * 12);
             const SS beta0 = r.beta;
             sample_ld<kMedia>(s, b, slot, r, si.p, si.n, si.wo, &bsdf, 0, si.ns, true, mIn, mOut, &gm);
             const float usel = r.rng.uniform(), uc = r.rng.uniform();
@@ -1314,7 +1358,7 @@ void EnsureBatch(PTScene &pt, size_t n) {
     auto f4 = [&](DevBuf &b) { b.ensure(n * sizeof(float4)); };
     f4(pt.o); f4(pt.d); f4(pt.lam); f4(pt.lpdf); f4(pt.beta); f4(pt.rpath); f4(pt.L); f4(pt.Lh);
     f4(pt.prevP); f4(pt.prevN); f4(pt.hit); f4(pt.xyz);
-    f4(pt.h0); f4(pt.h1); f4(pt.h2); f4(pt.h3);
+    f4(pt.h0); f4(pt.h1); f4(pt.h2); f4(pt.h3); f4(pt.qe); f4(pt.qeLpdf);
     pt.qm.ensure(n * 3 * sizeof(int));
     f4(pt.so); f4(pt.sd); f4(pt.sp); f4(pt.sn); f4(pt.sf); f4(pt.sL); f4(pt.sb);
     pt.rng.ensure(n * sizeof(ulonglong2));
@@ -1334,6 +1378,8 @@ Bufs MakeBufs(PTScene &pt) {
     b.hit = pt.hit.as<float4>(); b.q0 = pt.q0.as<int>(); b.q1 = pt.q1.as<int>(); b.counts = pt.counts.as<int>();
     b.h0 = pt.h0.as<float4>(); b.h1 = pt.h1.as<float4>(); b.h2 = pt.h2.as<float4>(); b.h3 = pt.h3.as<float4>();
     b.qm = pt.qm.as<int>();
+    b.qe = pt.qe.as<int4>();
+    b.qeLpdf = pt.qeLpdf.as<float4>();
     b.qcap = int(pt.cap);
     b.pfilm = pt.pfilm.as<float2>(); b.xyz = pt.xyz.as<float4>();
     b.so = pt.so.as<float4>(); b.sd = pt.sd.as<float4>(); b.sp = pt.sp.as<float4>(); b.sn = pt.sn.as<float4>();
@@ -1421,6 +1467,11 @@ struct Timer {
 
 // Traces one batch of `passes` passes (samples firstSample .. firstSample + passes - 1) to
 // completion; the per-sample results are left in the xyz / record buffers.
+#ifndef PG_GUIDE_BOUNCES
+#define PG_GUIDE_BOUNCES 2
+#endif
+constexpr int kGuideBounces = PG_GUIDE_BOUNCES;
+
 void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, int passes, int npix, int sw) {
     PTScene &pt = *dev.pt;
     hipStream_t st = dev.stream;
@@ -1442,7 +1493,7 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
     pt.countsHost[0] = n;
     PG_CHECK(hipMemcpyAsync(b.counts, pt.countsHost, sizeof(int), hipMemcpyHostToDevice, st));
     int *cur = b.q0, *nxt = b.q1;
-    int curIdx = 0;
+    int curIdx = 0, bounce = 0;
     const int shadeGrid = std::max(1, std::min((active + kBlock - 1) / kBlock, dev.cus * 16));
     // Queue sizes per bounce, read back after each synchronisation for the profile.
     std::vector<int> shadowCounts;
@@ -1450,7 +1501,7 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
     while (active > 0) {
         const int nxtIdx = 1 - curIdx;
         PG_CHECK(hipMemsetAsync(b.counts + nxtIdx, 0, sizeof(int), st));
-        PG_CHECK(hipMemsetAsync(b.counts + 2, 0, 6 * sizeof(int), st));  // shadow, material queues, fetch counters
+        PG_CHECK(hipMemsetAsync(b.counts + 2, 0, 8 * sizeof(int), st));  // shadow, material, fetch, emission
         const int tgrid = TraceGrid(dev, active);
         timer.Run(prgpu::KernIntersect, active, nullptr, [&] {
             if (stats)
@@ -1461,16 +1512,23 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
                                    b.counts + 6, ctr);
         });
         int grid = std::max(1, std::min((active + kBlock - 1) / kBlock, shadeGrid));
-        timer.Run(prgpu::KernShade, active, nullptr, [&] {
+        // Guiding applies to the vertices of the first kGuideBounces bounces (0: all).
+        const bool guided = p.guide && (kGuideBounces == 0 || bounce < kGuideBounces);
+        timer.Run(prgpu::KernSurface, active, nullptr, [&] {
             auto *k = media ? (interfaces ? k_surface<true, true> : k_surface<true, false>)
                             : (interfaces ? k_surface<false, true> : k_surface<false, false>);
             hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, st, ds, p, b, cur, b.counts + curIdx, nxt,
                                b.counts + nxtIdx);
-            if (p.guide)
+            if (s.emitters) hipLaunchKernelGGL(k_emit, dim3(grid), dim3(kBlock), 0, st, ds, b);
+        });
+        if (guided)
+            timer.Run(prgpu::KernGuide, 0, nullptr, [&] {
                 dev.guide->Infer(b.qm + size_t(QDiffuse) * b.qcap, b.counts + 3 + QDiffuse, active, b.h0, b.h1, b.d, p.box,
                                  dev.cus, st);
-            auto *kd = p.guide ? (media ? k_material<QDiffuse, true, true> : k_material<QDiffuse, false, true>)
-                               : (media ? k_material<QDiffuse, true> : k_material<QDiffuse, false>);
+            });
+        timer.Run(prgpu::KernMaterial, active, nullptr, [&] {
+            auto *kd = guided ? (media ? k_material<QDiffuse, true, true> : k_material<QDiffuse, false, true>)
+                              : (media ? k_material<QDiffuse, true> : k_material<QDiffuse, false>);
             auto *kc = media ? k_material<QConductor, true> : k_material<QConductor, false>;
             auto *ke = media ? k_material<QDielectric, true> : k_material<QDielectric, false>;
             hipLaunchKernelGGL(kd, dim3(grid), dim3(kBlock), 0, st, ds, p, b, nxt, b.counts + nxtIdx);
@@ -1491,14 +1549,24 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
         active = pt.countsHost[nxtIdx];
         std::swap(cur, nxt);
         curIdx = nxtIdx;
+        ++bounce;
     }
     CheckTraceFlags(dev);
 }
 
-// Guided rendering trains the model after every batch: kGuideSteps Adam steps of kGuideBatch
-// records each (the first records of the batch's paths; a slice with no records is a no-op).
-constexpr int kGuideBatch = 1 << 18, kGuideSteps = 4, kGuideTrainCap = kGuideBatch * kGuideSteps;
-constexpr float kGuideLearningRate = 1e-3f;
+// Guided rendering trains the model after every batch: up to kGuideSteps Adam steps of
+// kGuideBatch records each (the records of the batch's paths, in the order they were written).
+#ifndef PG_GUIDE_BATCH
+#define PG_GUIDE_BATCH (1 << 16)
+#endif
+#ifndef PG_GUIDE_STEPS
+#define PG_GUIDE_STEPS 8
+#endif
+#ifndef PG_GUIDE_LR
+#define PG_GUIDE_LR 1e-2f
+#endif
+constexpr int kGuideBatch = PG_GUIDE_BATCH, kGuideSteps = PG_GUIDE_STEPS, kGuideTrainCap = kGuideBatch * kGuideSteps;
+constexpr float kGuideLearningRate = PG_GUIDE_LR;
 
 // Passes per batch: bounded by a path budget so that the state fits comfortably in HBM.
 int PassesPerBatch(int npix, int spp) {

@@ -117,6 +117,8 @@ struct DevScene {
     const int *leafOf;     // light -> leaf node, -1 when not in the tree
     const int *infinite;   // infinite (unbounded) lights
     int infCount;
+    int envCount;          // environment lights (emission seen by escaping paths)
+    int emitters;          // any area or environment light: the emission stage runs
     const float *powerCdf;  // power sampler: cdf[count + 1], func[count]
     const float *powerFunc;
     float powerInt;

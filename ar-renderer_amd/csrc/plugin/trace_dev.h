@@ -23,7 +23,7 @@ namespace pgpu {
 #define PG_STACK_LDS 16
 #endif
 #ifndef PG_REFILL
-#define PG_REFILL 32
+#define PG_REFILL 48
 #endif
 #ifndef PG_LEAF_DEFER
 #define PG_LEAF_DEFER 0

@@ -149,7 +149,7 @@ class RenderParamsV4(C.Structure):
 assert C.sizeof(SceneDescV4) == 520 and C.sizeof(RenderParamsV4) == 24 and C.sizeof(CameraV4) == 80
 assert C.sizeof(FilterV4) == 276 and LIGHT_V4_DTYPE.itemsize == 48 and SPECTRUM95_DTYPE.itemsize == 380
 
-KERNEL_NAMES = ["camera", "intersect", "shade", "shadow", "finish", "gather", "guide"]
+KERNEL_NAMES = ["camera", "intersect", "surface", "shadow", "finish", "gather", "guide", "material"]
 
 assert C.sizeof(ProfileStatsEx) == 256
 assert C.sizeof(SceneDescEx) == 208 and C.sizeof(RenderParamsEx) == 40 and C.sizeof(DeviceInfo) == 304
@@ -366,6 +366,7 @@ class Plugin:
         L.prgpu_guide_params.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_int]
         L.prgpu_guide_eval.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p]
         L.prgpu_guide_train.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_void_p, C.c_void_p]
+        L.prgpu_guide_stats.argtypes = [C.c_int, C.c_void_p, C.POINTER(C.c_int)]
 
     def version(self):
         return self.lib.prgpu_api_version(), self.lib.prgpu_api_ext_version()
@@ -448,6 +449,14 @@ class Plugin:
         if self.lib.prgpu_guide_train(device, R.ctypes.data, len(R), 0, 0.0, g.ctypes.data, st.ctypes.data) != 0:
             raise RuntimeError("prgpu_guide_train failed")
         return g, st
+
+    def guide_stats(self, device=0):
+        """(summed weight, summed weighted loss) of the model's last step, and its step count."""
+        st = np.zeros(2, np.float32)
+        steps = C.c_int()
+        if self.lib.prgpu_guide_stats(device, st.ctypes.data, C.byref(steps)) != 0:
+            return None
+        return float(st[0]), float(st[1]), steps.value
 
     def guide_train(self, recs16, steps, lr=1e-3, device=0):
         R = np.ascontiguousarray(recs16, dtype=np.float32).reshape(-1, 16)

@@ -293,10 +293,11 @@ struct SampleRecordEx {
     uint32_t flags;      // bit 0: L holds NaN/Inf (the film drops the sample)
 };
 
-// Profiling counters of one kind of kernel launch (prgpu_profile_read).
+// Profiling counters of one kind of kernel launch (prgpu_profile_read). Surface: the surface and
+// emission stages; material: the three BSDF-family kernels; guide: guiding inference and training.
 enum KernelKindEx : int {
-    KernCamera = 0, KernIntersect = 1, KernShade = 2, KernShadow = 3, KernFinish = 4, KernGather = 5, KernGuide = 6,
-    KernCount = 7
+    KernCamera = 0, KernIntersect = 1, KernSurface = 2, KernShadow = 3, KernFinish = 4, KernGather = 5, KernGuide = 6,
+    KernMaterial = 7, KernCount = 8
 };
 struct ProfileStatsEx {
     double ms[8];          // summed HIP-event durations per kernel kind
@@ -365,6 +366,9 @@ PRGPU_API int prgpu_guide_eval(int deviceIndex, const float *records, int n, flo
 // lr) on the records and returns the stats of the last step.
 PRGPU_API int prgpu_guide_train(int deviceIndex, const float *records, int n, int steps, float lr, float *grad,
                                 float *stats);
+// State of the device's model: stats = {summed weight, summed weighted loss} of its last training
+// step, *steps = Adam steps taken since it was created or reset. 1 when no model exists.
+PRGPU_API int prgpu_guide_stats(int deviceIndex, float *stats, int *steps);
 
 // ---- layout pins (measured on the reference's LP64 build) -----------------------------------
 static_assert(sizeof(prgpu::Float3) == 12, "Float3");

@@ -10,7 +10,7 @@ import pytest
 
 import oracle as orc
 import prender_amd as pa
-from test_pt_gpu import check_samples, three_sigma
+from test_pt_gpu import check_samples, splat_correlation, three_sigma
 
 pytestmark = pytest.mark.gpu
 
@@ -29,4 +29,4 @@ def test_image_3sigma_gems(plugin, c5):
     spp = 64
     g_sum, g_sq = plugin.render_ex(c5, pa.render_params(spp=spp, first_sample=0, seed=1, max_depth=32))
     c_sum, c_sq, _ = orc.OracleScene(c5).render(pa.render_params(spp=spp, first_sample=spp, seed=2, max_depth=32))
-    three_sigma("c5 image", g_sum, g_sq, spp, c_sum, c_sq, spp)
+    three_sigma("c5 image", g_sum, g_sq, spp, c_sum, c_sq, spp, splat_correlation(c5))

@@ -13,7 +13,7 @@ import pytest
 import oracle as orc
 import prender_amd as pa
 from test_guide_oracle import random_records
-from test_pt_gpu import check_samples, three_sigma
+from test_pt_gpu import check_samples, splat_correlation, three_sigma
 
 pytestmark = pytest.mark.gpu
 
@@ -103,4 +103,4 @@ def test_guided_render_is_unbiased(plugin, c4):
         s, q = plugin.render_ex(c4, p)
         g_sum, g_sq = g_sum + s, g_sq + q
     c_sum, c_sq, _ = orc.OracleScene(c4).render(pa.render_params(spp=spp, first_sample=spp, seed=2, max_depth=32))
-    three_sigma("c4 guided vs plain oracle", g_sum, g_sq, spp, c_sum, c_sq, spp)
+    three_sigma("c4 guided vs plain oracle", g_sum, g_sq, spp, c_sum, c_sq, spp, splat_correlation(c4))

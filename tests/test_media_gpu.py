@@ -15,7 +15,7 @@ import pytest
 
 import oracle as orc
 import prender_amd as pa
-from test_pt_gpu import check_samples, three_sigma
+from test_pt_gpu import check_samples, splat_correlation, three_sigma
 
 pytestmark = pytest.mark.gpu
 
@@ -52,7 +52,7 @@ def test_image_3sigma_media(plugin, c3):
     spp = 128
     g_sum, g_sq = plugin.render_ex(c3, pa.render_params(spp=spp, first_sample=0, seed=1, max_depth=256))
     c_sum, c_sq, _ = orc.OracleScene(c3).render(pa.render_params(spp=spp, first_sample=spp, seed=2, max_depth=256))
-    three_sigma("c3 image", g_sum, g_sq, spp, c_sum, c_sq, spp)
+    three_sigma("c3 image", g_sum, g_sq, spp, c_sum, c_sq, spp, splat_correlation(c3))
 
 
 def test_media_power_light_sampler_and_depth_cap(plugin, c3):

@@ -63,7 +63,34 @@ def test_sample_radiance_power_light_sampler(plugin, host):
     check_samples("c2-power", s, p, plugin, 0.99)
 
 
-def three_sigma(name, g_sum, g_sq, n_g, c_sum, c_sq, n_c):
+def splat_correlation(scene, n=4096, seed=0):
+    """Variance inflation of an image mean over independent pixels when every sample is splatted
+    into its filter footprint: E[(sum_k w_k)^2] / E[sum_k w_k^2] over sample positions."""
+    cam = scene.camera
+    r = 0.5 if cam.filterType == pa.FILTER_BOX else cam.filterRadius
+
+    def w(x):
+        x = np.abs(x)
+        if cam.filterType == pa.FILTER_GAUSSIAN:
+            sg = r / 3
+            v = np.maximum(0, np.exp(-x * x / (2 * sg * sg)) - np.exp(-r * r / (2 * sg * sg)))
+        elif cam.filterType == pa.FILTER_BH:
+            t = 0.5 + x / (2 * r)
+            v = 0.35875 - 0.48829 * np.cos(2 * np.pi * t) + 0.14128 * np.cos(4 * np.pi * t) - 0.01168 * np.cos(6 * np.pi * t)
+        else:
+            v = np.ones_like(x)
+        return np.where(x > r, 0, v)
+
+    u = np.random.default_rng(seed).random((n, 2))
+    k = np.arange(-3, 4)
+    sums = []
+    for axis in range(2):
+        wx = w(u[:, axis:axis + 1] - (k[None, :] + 0.5))
+        sums.append((wx.sum(1), (wx * wx).sum(1)))
+    return float(np.mean((sums[0][0] * sums[1][0]) ** 2) / np.mean(sums[0][1] * sums[1][1]))
+
+
+def three_sigma(name, g_sum, g_sq, n_g, c_sum, c_sq, n_c, corr=1.0):
     mg, mc = g_sum / n_g, c_sum / n_c
     vg = np.maximum(g_sq / n_g - mg * mg, 0) / n_g
     vc = np.maximum(c_sq / n_c - mc * mc, 0) / n_c
@@ -75,8 +102,9 @@ def three_sigma(name, g_sum, g_sq, n_g, c_sum, c_sq, n_c):
     # Expected outlier fraction ~0.27%; allow a binomial 5-sigma margin and heavy-tail slack.
     n = valid.sum()
     bound = 0.0027 + 5 * np.sqrt(0.0027 * (1 - 0.0027) / max(1, n)) + 0.01
-    # The image means, against the standard error of their difference (pixels are independent).
-    z = abs(mg.mean() - mc.mean()) / max(np.sqrt((vg + vc).sum()) / mg.size, 1e-30)
+    # The image means, against the standard error of their difference; `corr` inflates the
+    # per-pixel variance sum for the correlation the film's filter splat puts between neighbours.
+    z = abs(mg.mean() - mc.mean()) / max(np.sqrt(corr * (vg + vc).sum()) / mg.size, 1e-30)
     rel_mean = abs(mg.mean() - mc.mean()) / mc.mean()
     print(f"{name}: pixels*channels={n} outside 3 sigma={frac:.4%} (bound {bound:.2%}) mean ratio diff={rel_mean:.3e} "
           f"(z = {z:.2f})", flush=True)
@@ -92,7 +120,7 @@ def test_image_3sigma_c1(plugin, host):
     g_sum, g_sq = plugin.render_ex(s, pg)
     pc = pa.render_params(spp=spp, first_sample=spp, seed=2, max_depth=8, rr_start=3)
     c_sum, c_sq, _ = orc.OracleScene(s).render(pc)
-    three_sigma("c1 image", g_sum, g_sq, spp, c_sum, c_sq, spp)
+    three_sigma("c1 image", g_sum, g_sq, spp, c_sum, c_sq, spp, splat_correlation(s))
 
 
 def test_film_matches_oracle_film_same_samples(plugin, host):
