@@ -32,6 +32,12 @@ float or_spectrum_eval(const prgpu::SpectrumEx *s, const float *knots, int nknot
 float or_spectrum_max(const prgpu::SpectrumEx *s, const float *knots, int nknots);
 float or_spectrum_y(const prgpu::SpectrumEx *s, const float *knots, int nknots);
 void or_bxdf(const float *desc, int op, const float *wo, const float *wi, float uc, const float *u, float *out);
+// Plain, coated (layered) and sheen closures with transport mode (0 radiance, 1 importance) and
+// sample flags (1 reflection, 2 transmission, 3 all); descriptor layout in pt_oracle.cpp.
+void or_closure(const float *desc, int op, const float *wo, const float *wi, float uc, const float *u, int mode, int rt,
+                float *out);
+float or_sheen_albedo(float cosTheta, float alpha);
+void or_sheen_table(float *out);  // 16 alphas x 32 cosines
 float or_fr_dielectric(float c, float eta);
 float or_fr_complex(float c, float eta, float k);
 float or_hg(float c, float g);

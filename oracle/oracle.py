@@ -58,6 +58,10 @@ def lib():
             getattr(L, f).restype = C.c_float
             getattr(L, f).argtypes = [vp, vp, C.c_int]
         L.or_bxdf.argtypes = [vp, C.c_int, vp, vp, C.c_float, vp, vp]
+        L.or_closure.argtypes = [vp, C.c_int, vp, vp, C.c_float, vp, C.c_int, C.c_int, vp]
+        L.or_sheen_albedo.restype = C.c_float
+        L.or_sheen_albedo.argtypes = [C.c_float, C.c_float]
+        L.or_sheen_table.argtypes = [vp]
         for f in ("or_fr_dielectric", "or_fr_complex", "or_hg"):
             getattr(L, f).restype = C.c_float
         L.or_fr_dielectric.argtypes = [C.c_float, C.c_float]
@@ -81,6 +85,37 @@ def lib():
         L.or_guide_loss_grad.argtypes = [vp, vp, C.c_int, vp]
         _lib = L
     return _lib
+
+
+# ---- closures (plain, coated, sheen) ---------------------------------------------------------------
+CLOSURE_DESC = 32  # floats per closure descriptor (layout in pt_oracle.cpp, or_closure)
+
+
+def closure_desc(kind, alpha=(0.0, 0.0), eta=1.5, R=(0.5, 0.5, 0.5, 0.5), ceta=(0.2, 0.9, 1.1, 1.3),
+                 k=(3.9, 2.4, 2.2, 1.9), coat_alpha=0.0, thickness=0.01, g=0.0, albedo=(0, 0, 0, 0), max_depth=64,
+                 n_samples=1, sheen=0.0, sheen_alpha=0.3, sheen_color=(1, 1, 1, 1)):
+    """Closure descriptor: kind 1 diffuse, 2 conductor, 3 dielectric, 4 thin, 5 coated diffuse,
+    6 coated conductor; eta is the dielectric's or the coat's IOR."""
+    return [kind, alpha[0], alpha[1], eta, *R, *ceta, *k, coat_alpha, thickness, g, *albedo, max_depth, n_samples, 1,
+            sheen, sheen_alpha, *sheen_color]
+
+
+def closure(desc, op, wo, wi=(0, 0, 1), uc=0.0, u=(0, 0), mode=0, rt=3):
+    """One call of the closure probe: op 0 f, 1 pdf, 2 sample, 3 (flags, exact)."""
+    d = np.ascontiguousarray(desc, dtype=np.float32)
+    a = np.ascontiguousarray(wo, dtype=np.float32)
+    b = np.ascontiguousarray(wi, dtype=np.float32)
+    uu = np.ascontiguousarray(u, dtype=np.float32)
+    out = np.zeros(12, np.float32)
+    lib().or_closure(d.ctypes.data, op, a.ctypes.data, b.ctypes.data, float(uc), uu.ctypes.data, mode, rt,
+                     out.ctypes.data)
+    return out
+
+
+def sheen_table():
+    t = np.zeros(16 * 32, np.float32)
+    lib().or_sheen_table(t.ctypes.data)
+    return t
 
 
 # ---- path-guiding model (FP64) -------------------------------------------------------------------

@@ -756,17 +756,25 @@ inline V3 sample_hg(V3 wo, float g, V2 u, float *pdf) {  // bxdf.cpp:69-80
     return wi;
 }
 
+// Sample flags (BxDFReflTransFlags, bxdf.h:39-40) and transport mode (bxdf.h:35-38). The mode only
+// changes the dielectric's transmission (the 1/eta^2 radiance scaling); the layered walk samples
+// the exit interface in the adjoint mode.
+enum : uint32_t { kRtRefl = 1, kRtTrans = 2, kRtAll = 3 };
+enum class Mode { Radiance, Importance };
+inline Mode adjoint(Mode m) { return m == Mode::Radiance ? Mode::Importance : Mode::Radiance; }
+
 struct BSample {
     SS f;
     V3 wi;
     float pdf = 0;
     uint32_t flags = 0;
     float eta = 1;
+    bool proportional = false;  // pdf only proportional to the sampling density (bxdf.h:44-47)
 };
 
 enum BxType { BxNone = 0, BxDiffuse, BxConductor, BxDielectric, BxThin };
 
-struct BxDF {  // one closure per surface hit (TransportMode::Radiance throughout)
+struct BxDF {  // the plain closures (bxdf.cpp:82-256)
     BxType type = BxNone;
     SS R;          // diffuse reflectance
     SS eta, k;     // conductor
@@ -786,7 +794,7 @@ struct BxDF {  // one closure per surface hit (TransportMode::Radiance throughou
         }
     }
 
-    SS f(V3 wo, V3 wi) const {
+    SS f(V3 wo, V3 wi, Mode mode = Mode::Radiance) const {
         switch (type) {
         case BxDiffuse:  // bxdf.cpp:85-88
             if (!same_hemi(wo, wi)) return SS(0.f);
@@ -815,19 +823,20 @@ struct BxDF {  // one closure per surface hit (TransportMode::Radiance throughou
             if (refl) return SS(dist.D(wm) * dist.G(wo, wi) * F / std::abs(4 * ci * co));
             float denom = sq(dot(wi, wm) + dot(wo, wm) / etap) * ci * co;
             float ft = dist.D(wm) * (1 - F) * dist.G(wo, wi) * std::abs(dot(wi, wm) * dot(wo, wm) / denom);
-            ft /= sq(etap);
+            if (mode == Mode::Radiance) ft /= sq(etap);
             return SS(ft);
         }
         default: return SS(0.f);
         }
     }
 
-    float pdf(V3 wo, V3 wi) const {
+    float pdf(V3 wo, V3 wi, Mode = Mode::Radiance, uint32_t rt = kRtAll) const {
         switch (type) {
         case BxDiffuse:  // bxdf.cpp:99-102
-            if (!same_hemi(wo, wi)) return 0;
+            if (!(rt & kRtRefl) || !same_hemi(wo, wi)) return 0;
             return abscos(wi) * kInvPi;
         case BxConductor: {  // bxdf.cpp:138-145
+            if (!(rt & kRtRefl)) return 0;
             if (!same_hemi(wo, wi) || dist.smooth()) return 0;
             V3 wm = wo + wi;
             if (len2(wm) == 0) return 0;
@@ -845,7 +854,7 @@ struct BxDF {  // one closure per surface hit (TransportMode::Radiance throughou
             wm = face_forward(normalize(wm), V3(0, 0, 1));
             if (dot(wm, wi) * ci < 0 || dot(wm, wo) * co < 0) return 0;
             float R = fr_dielectric(dot(wo, wm), etaD), T = 1 - R;
-            float pr = R, pt = T;
+            float pr = (rt & kRtRefl) ? R : 0.f, pt = (rt & kRtTrans) ? T : 0.f;
             if (pr == 0 && pt == 0) return 0;
             if (refl) return dist.pdf(wo, wm) / (4 * absdot(wo, wm)) * pr / (pr + pt);
             float denom = sq(dot(wi, wm) + dot(wo, wm) / etap);
@@ -856,15 +865,17 @@ struct BxDF {  // one closure per surface hit (TransportMode::Radiance throughou
         }
     }
 
-    std::optional<BSample> sample(V3 wo, float uc, V2 u) const {
+    std::optional<BSample> sample(V3 wo, float uc, V2 u, Mode mode = Mode::Radiance, uint32_t rt = kRtAll) const {
         switch (type) {
         case BxDiffuse: {  // bxdf.cpp:90-97
+            if (!(rt & kRtRefl)) return std::nullopt;
             V3 wi = sample_cosine_hemisphere(u);
             if (wo.z < 0) wi.z *= -1;
             float p = abscos(wi) * kInvPi;
             return BSample{R * kInvPi, wi, p, kDiffuse | kRefl, 1};
         }
         case BxConductor: {  // bxdf.cpp:118-136
+            if (!(rt & kRtRefl)) return std::nullopt;
             if (dist.smooth()) {
                 V3 wi(-wo.x, -wo.y, wo.z);
                 SS f = fr_complex(abscos(wi), eta, k) / abscos(wi);
@@ -884,7 +895,7 @@ struct BxDF {  // one closure per surface hit (TransportMode::Radiance throughou
         case BxDielectric: {  // bxdf.cpp:150-194
             if (etaD == 1 || dist.smooth()) {
                 float R = fr_dielectric(wo.z, etaD), T = 1 - R;
-                float pr = R, pt = T;
+                float pr = (rt & kRtRefl) ? R : 0.f, pt = (rt & kRtTrans) ? T : 0.f;
                 if (pr == 0 && pt == 0) return std::nullopt;
                 if (uc < pr / (pr + pt)) {
                     V3 wi(-wo.x, -wo.y, wo.z);
@@ -895,12 +906,12 @@ struct BxDF {  // one closure per surface hit (TransportMode::Radiance throughou
                 float etap;
                 if (!refract(wo, V3(0, 0, 1), etaD, &etap, &wi)) return std::nullopt;
                 SS ft(T / abscos(wi));
-                ft = ft / sq(etap);
+                if (mode == Mode::Radiance) ft = ft / sq(etap);
                 return BSample{ft, wi, pt / (pr + pt), kSpecular | kTrans, etap};
             }
             V3 wm = dist.sample_wm(wo, u);
             float R = fr_dielectric(dot(wo, wm), etaD), T = 1 - R;
-            float pr = R, pt = T;
+            float pr = (rt & kRtRefl) ? R : 0.f, pt = (rt & kRtTrans) ? T : 0.f;
             if (pr == 0 && pt == 0) return std::nullopt;
             if (uc < pr / (pr + pt)) {
                 V3 wi = reflect(wo, wm);
@@ -917,7 +928,7 @@ struct BxDF {  // one closure per surface hit (TransportMode::Radiance throughou
             float dwm_dwi = absdot(wi, wm) / denom;
             float p = dist.pdf(wo, wm) * dwm_dwi * pt / (pr + pt);
             SS ft(T * dist.D(wm) * dist.G(wo, wi) * std::abs(dot(wi, wm) * dot(wo, wm) / (wi.z * wo.z * denom)));
-            ft = ft / sq(etap);
+            if (mode == Mode::Radiance) ft = ft / sq(etap);
             return BSample{ft, wi, p, kGlossy | kTrans, etap};
         }
         case BxThin: {  // bxdf.cpp:239-256
@@ -926,7 +937,7 @@ struct BxDF {  // one closure per surface hit (TransportMode::Radiance throughou
                 R += sq(T) * R / (1 - sq(R));
                 T = 1 - R;
             }
-            float pr = R, pt = T;
+            float pr = (rt & kRtRefl) ? R : 0.f, pt = (rt & kRtTrans) ? T : 0.f;
             if (pr == 0 && pt == 0) return std::nullopt;
             if (uc < pr / (pr + pt)) {
                 V3 wi(-wo.x, -wo.y, wo.z);
@@ -940,30 +951,414 @@ struct BxDF {  // one closure per surface hit (TransportMode::Radiance throughou
     }
 };
 
+inline bool usable(const std::optional<BSample> &s) { return s && s->f.nonzero() && s->pdf != 0 && s->wi.z != 0; }
+
+// Stochastic layered closure (reference bxdf.cpp:260-517, the position-free random walk of
+// pbrt-v4's LayeredBxDF): a dielectric coat over a base, optionally with a homogeneous slab of
+// unit extinction and the given albedo between them. The walk's random stream is hashed from
+// the directions (f: Hash(0, wo), Hash(wi); pdf: Hash(0, wi), Hash(wo); sample: Hash(0, wo),
+// Hash(uc, u)), so all three are deterministic functions of their arguments. Each 2D draw takes
+// its second coordinate first: the reference's Vec2f(r(), r()) evaluates its arguments right to
+// left (MSVC and g++ alike; pinned against the compiled reference).
+struct Layered {
+    BxDF top, bottom;
+    float thick = 0.01f, g = 0;
+    SS albedo;
+    bool twoSided = true;
+    int maxDepth = 64, nSamples = 1;
+
+    struct Walk {
+        Pcg32 rng;
+        Walk(uint64_t seq, uint64_t offset) { rng.set_sequence(seq, offset); }
+        float u() { return std::min(rng.uniform(), kOneMinusEps); }
+        V2 u2() {
+            float y = u();
+            float x = u();
+            return V2{x, y};
+        }
+    };
+    static float tr(float dz, V3 w) {
+        if (std::abs(dz) <= std::numeric_limits<float>::min()) return 1;
+        return std::exp(-std::abs(dz / w.z));
+    }
+    static float free_flight(float u, V3 w) { return -std::log(1 - u) / (1.f / std::abs(w.z)); }  // sampling.h:88
+
+    uint32_t flags() const {  // bxdf.cpp:260-268
+        uint32_t t = top.flags(), b = bottom.flags(), f = kRefl;
+        if (t & kSpecular) f |= kSpecular;
+        if ((t & kDiffuse) || (b & kDiffuse) || albedo.nonzero()) f |= kDiffuse;
+        else if ((t & kGlossy) || (b & kGlossy)) f |= kGlossy;
+        if ((t & kTrans) && (b & kTrans)) f |= kTrans;
+        return f;
+    }
+
+    SS f(V3 wo, V3 wi, Mode mode) const {  // bxdf.cpp:270-373
+        if (twoSided && wo.z < 0) {
+            wo = -wo;
+            wi = -wi;
+        }
+        const bool fromTop = twoSided || wo.z > 0;
+        const bool same = same_hemi(wo, wi);
+        const bool exitBottom = same != fromTop;  // the walk leaves through the interface on wi's side
+        const BxDF &enter = fromTop ? top : bottom;
+        const BxDF &exitI = exitBottom ? bottom : top, &inner = exitBottom ? top : bottom;
+        const float exitZ = exitBottom ? 0.f : thick;
+        const bool exitSpec = (exitI.flags() & kSpecular) != 0, innerSpec = (inner.flags() & kSpecular) != 0;
+        SS f(0.f);
+        if (same) f = enter.f(wo, wi, mode) * float(nSamples);
+        Walk w(hash_args(uint64_t(0), wo), hash_args(wi));
+        for (int s = 0; s < nSamples; ++s) {
+            float uc = w.u();
+            V2 u0 = w.u2();
+            auto wos = enter.sample(wo, uc, u0, mode, kRtTrans);
+            if (!usable(wos)) continue;
+            uc = w.u();
+            V2 u1 = w.u2();
+            auto wis = exitI.sample(wi, uc, u1, adjoint(mode), kRtTrans);
+            if (!usable(wis)) continue;
+            SS beta = wos->f * abscos(wos->wi) / wos->pdf;
+            float z = fromTop ? thick : 0;
+            V3 d = wos->wi;
+            for (int depth = 0; depth < maxDepth; ++depth) {
+                if (depth > 3 && beta.maxc() < 0.25f) {
+                    float q = std::max(0.f, 1 - beta.maxc());
+                    if (w.u() < q) break;
+                    beta = beta / (1 - q);
+                }
+                if (!albedo.nonzero()) {
+                    z = (z == thick) ? 0 : thick;
+                    beta = beta * tr(thick, d);
+                } else {
+                    float zp = d.z > 0 ? (z + free_flight(w.u(), d)) : (z - free_flight(w.u(), d));
+                    if (0 < zp && zp < thick) {
+                        // Scattering in the slab: next-event estimate through the exit interface's
+                        // sampled direction, then continue with a phase-function sample.
+                        float wt = 1;
+                        float phaseExit = hg_phase(dot(-d, -wis->wi), g);
+                        if (!exitSpec) wt = power_heuristic(wis->pdf, phaseExit);
+                        f = f + beta * albedo * phaseExit * wt * tr(zp - exitZ, wis->wi) * wis->f / wis->pdf;
+                        float phasePdf;
+                        V3 wp = sample_hg(-d, g, w.u2(), &phasePdf);
+                        if (phasePdf == 0 || wp.z == 0) continue;
+                        beta = beta * albedo;
+                        d = wp;
+                        z = zp;
+                        if (((z < exitZ && d.z > 0) || (z > exitZ && d.z < 0)) && !exitSpec) {
+                            SS fExit = exitI.f(-d, wi, mode);
+                            if (fExit.nonzero()) {
+                                float exitPdf = exitI.pdf(-d, wi, mode, kRtTrans);
+                                float wt2 = power_heuristic(phasePdf, exitPdf);
+                                f = f + beta * tr(zp - exitZ, wp) * fExit * wt2;
+                            }
+                        }
+                        continue;
+                    }
+                    z = clampv(zp, 0.f, thick);
+                }
+                if (z == exitZ) {
+                    float uc2 = w.u();
+                    V2 u2 = w.u2();
+                    auto bs = exitI.sample(-d, uc2, u2, mode, kRtRefl);
+                    if (!usable(bs)) break;
+                    beta = beta * (bs->f * abscos(bs->wi) / bs->pdf);
+                    d = bs->wi;
+                } else {
+                    if (!innerSpec) {
+                        float wt = 1;
+                        if (!exitSpec) wt = power_heuristic(wis->pdf, inner.pdf(-d, -wis->wi, mode));
+                        f = f + beta * inner.f(-d, -wis->wi, mode) * abscos(wis->wi) * wt * tr(thick, wis->wi) * wis->f /
+                                    wis->pdf;
+                    }
+                    float uc2 = w.u();
+                    V2 u2 = w.u2();
+                    auto bs = inner.sample(-d, uc2, u2, mode, kRtRefl);
+                    if (!usable(bs)) break;
+                    beta = beta * (bs->f * abscos(bs->wi) / bs->pdf);
+                    d = bs->wi;
+                    if (!exitSpec) {
+                        SS fExit = exitI.f(-d, wi, mode);
+                        if (fExit.nonzero()) {
+                            float wt = 1;
+                            if (!innerSpec) wt = power_heuristic(bs->pdf, exitI.pdf(-d, wi, mode, kRtTrans));
+                            f = f + beta * tr(thick, bs->wi) * fExit * wt;
+                        }
+                    }
+                }
+            }
+        }
+        return f / float(nSamples);
+    }
+
+    std::optional<BSample> sample(V3 wo, float uc, V2 u, Mode mode) const {  // bxdf.cpp:375-454
+        bool flip = false;
+        if (twoSided && wo.z < 0) {
+            wo = -wo;
+            flip = true;
+        }
+        const bool fromTop = twoSided || wo.z > 0;
+        auto bs = (fromTop ? top : bottom).sample(wo, uc, u, mode);
+        if (!usable(bs)) return std::nullopt;
+        if (bs->flags & kRefl) {
+            if (flip) bs->wi = -bs->wi;
+            bs->proportional = true;
+            return bs;
+        }
+        V3 d = bs->wi;
+        bool specPath = (bs->flags & kSpecular) != 0;
+        Walk w(hash_args(uint64_t(0), wo), hash_args(uc, u));
+        SS f = bs->f * abscos(bs->wi);
+        float pdf = bs->pdf;
+        float z = fromTop ? thick : 0;
+        for (int depth = 0; depth < maxDepth; ++depth) {
+            float rrBeta = f.maxc() / pdf;
+            if (depth > 3 && rrBeta < 0.25f) {
+                float q = std::max(0.f, 1 - rrBeta);
+                if (w.u() < q) return std::nullopt;
+                pdf *= 1 - q;
+            }
+            if (d.z == 0) return std::nullopt;
+            if (albedo.nonzero()) {
+                float zp = d.z > 0 ? (z + free_flight(w.u(), d)) : (z - free_flight(w.u(), d));
+                if (zp == z) return std::nullopt;
+                if (0 < zp && zp < thick) {
+                    float phasePdf;
+                    V3 wp = sample_hg(-d, g, w.u2(), &phasePdf);
+                    if (phasePdf == 0 || wp.z == 0) return std::nullopt;
+                    f = f * (albedo * phasePdf);
+                    pdf *= phasePdf;
+                    specPath = false;
+                    d = wp;
+                    z = zp;
+                    continue;
+                }
+                z = clampv(zp, 0.f, thick);
+            } else {
+                z = (z == thick) ? 0 : thick;
+                f = f * tr(thick, d);
+            }
+            const BxDF &iface = (z == 0) ? bottom : top;
+            float uc2 = w.u();
+            V2 u2 = w.u2();
+            auto bs2 = iface.sample(-d, uc2, u2, mode);
+            if (!usable(bs2)) return std::nullopt;
+            f = f * bs2->f;
+            pdf *= bs2->pdf;
+            specPath = specPath && (bs2->flags & kSpecular);
+            d = bs2->wi;
+            if (bs2->flags & kTrans) {
+                uint32_t fl = (same_hemi(wo, d) ? kRefl : kTrans) | (specPath ? kSpecular : kGlossy);
+                if (flip) d = -d;
+                return BSample{f, d, pdf, fl, 1.f, true};
+            }
+            f = f * abscos(bs2->wi);
+        }
+        return std::nullopt;
+    }
+
+    float pdf(V3 wo, V3 wi, Mode mode) const {  // bxdf.cpp:456-517
+        if (twoSided && wo.z < 0) {
+            wo = -wo;
+            wi = -wi;
+        }
+        Walk w(hash_args(uint64_t(0), wi), hash_args(wo));
+        const bool fromTop = twoSided || wo.z > 0;
+        const bool same = same_hemi(wo, wi);
+        float sum = 0;
+        if (same) sum += nSamples * (fromTop ? top.pdf(wo, wi, mode, kRtRefl) : bottom.pdf(wo, wi, mode, kRtRefl));
+        for (int s = 0; s < nSamples; ++s) {
+            if (same) {
+                // Transmission through the entry interface, one bounce off the other.
+                const BxDF &rI = fromTop ? bottom : top, &tI = fromTop ? top : bottom;
+                float a = w.u();
+                V2 ua = w.u2();
+                auto wos = tI.sample(wo, a, ua, mode, kRtTrans);
+                float b = w.u();
+                V2 ub = w.u2();
+                auto wis = tI.sample(wi, b, ub, adjoint(mode), kRtTrans);
+                if (wos && wos->f.nonzero() && wos->pdf > 0 && wis && wis->f.nonzero() && wis->pdf > 0) {
+                    if (!(tI.flags() & (kDiffuse | kGlossy))) {
+                        sum += rI.pdf(-wos->wi, -wis->wi, mode);
+                    } else {
+                        float c = w.u();
+                        V2 uc2 = w.u2();
+                        auto rs = rI.sample(-wos->wi, c, uc2, mode);
+                        if (rs && rs->f.nonzero() && rs->pdf > 0) {
+                            if (!(rI.flags() & (kDiffuse | kGlossy))) {
+                                sum += tI.pdf(-rs->wi, wi, mode);
+                            } else {
+                                float rPdf = rI.pdf(-wos->wi, -wis->wi, mode);
+                                sum += power_heuristic(wis->pdf, rPdf) * rPdf;
+                                float tPdf = tI.pdf(-rs->wi, wi, mode);
+                                sum += power_heuristic(rs->pdf, tPdf) * tPdf;
+                            }
+                        }
+                    }
+                }
+            } else {
+                // Transmission through both interfaces.
+                const BxDF &toI = fromTop ? top : bottom, &tiI = fromTop ? bottom : top;
+                float a = w.u();
+                V2 ua = w.u2();
+                auto wos = toI.sample(wo, a, ua, mode);
+                if (!usable(wos) || (wos->flags & kRefl)) continue;
+                float b = w.u();
+                V2 ub = w.u2();
+                auto wis = tiI.sample(wi, b, ub, adjoint(mode));
+                if (!usable(wis) || (wis->flags & kRefl)) continue;
+                if (toI.flags() & kSpecular) sum += tiI.pdf(-wos->wi, wi, mode);
+                else if (tiI.flags() & kSpecular) sum += toI.pdf(wo, -wis->wi, mode);
+                else sum += (toI.pdf(wo, -wis->wi, mode) + tiI.pdf(-wos->wi, wi, mode)) / 2;
+            }
+        }
+        return lerpf(0.9f, 1 / (4 * kPi), sum / nSamples);
+    }
+};
+
+// "Charlie" sheen (reference bxdf.cpp:519-631): the microfibre lobe D / (4 (ci + co - ci co))
+// and its directional albedo, tabulated over 16 alphas x 32 cosines by 32 x 32 stratified
+// cosine samples and interpolated bilinearly.
+constexpr int kSheenMuN = 32, kSheenAlphaN = 16;
+inline float sheen_value(V3 wo, V3 wi, float alpha) {
+    if (!same_hemi(wo, wi)) return 0;
+    float co = abscos(wo), ci = abscos(wi);
+    if (co == 0 || ci == 0) return 0;
+    V3 wh = normalize(wo + wi);
+    float c = abscos(wh), inv = 1 / alpha;
+    float sin2 = std::max(0.f, 1 - c * c);
+    float D = (2 + inv) * std::pow(sin2, 0.5f * inv) * kInv2Pi;
+    return D / (4 * (ci + co - ci * co));
+}
+inline const float *sheen_table() {
+    static const std::vector<float> table = [] {
+        std::vector<float> t(size_t(kSheenMuN) * kSheenAlphaN);
+        for (int ia = 0; ia < kSheenAlphaN; ++ia) {
+            float a = lerpf(float(ia) / (kSheenAlphaN - 1), 0.05f, 1.f);
+            for (int im = 0; im < kSheenMuN; ++im) {
+                float mu = std::max(1e-3f, float(im) / (kSheenMuN - 1));
+                V3 wo(safe_sqrt(1 - mu * mu), 0, mu);
+                double sum = 0;
+                for (int i = 0; i < 32; ++i)
+                    for (int j = 0; j < 32; ++j) {
+                        V3 wi = sample_cosine_hemisphere(V2{(i + 0.5f) / 32, (j + 0.5f) / 32});
+                        sum += sheen_value(wo, wi, a) * kPi;
+                    }
+                t[size_t(ia) * kSheenMuN + im] = float(sum / (32 * 32));
+            }
+        }
+        return t;
+    }();
+    return table.data();
+}
+inline float sheen_albedo(const float *t, float cosTheta, float alpha) {
+    float fa = (clampv(alpha, 0.05f, 1.f) - 0.05f) / 0.95f * (kSheenAlphaN - 1);
+    float fm = clampv(std::abs(cosTheta), 0.f, 1.f) * (kSheenMuN - 1);
+    int ia = std::min(int(fa), kSheenAlphaN - 2), im = std::min(int(fm), kSheenMuN - 2);
+    float ta = fa - ia, tm = fm - im;
+    auto at = [&](int a, int m) { return t[a * kSheenMuN + m]; };
+    return lerpf(ta, lerpf(tm, at(ia, im), at(ia, im + 1)), lerpf(tm, at(ia + 1, im), at(ia + 1, im + 1)));
+}
+
+struct SheenLobe {
+    SS color;
+    float alpha = 0.3f, weight = 0, colorMax = 0;
+    const float *table = nullptr;
+    SS f(V3 wo, V3 wi) const { return color * (weight * sheen_value(wo, wi, alpha)); }
+    float base_scale(V3 wo, V3 wi) const {
+        float e = std::max(sheen_albedo(table, wo.z, alpha), sheen_albedo(table, wi.z, alpha));
+        return std::max(0.f, 1 - weight * colorMax * e);
+    }
+    float prob(V3 wo, uint32_t baseFlags) const {
+        float e = weight * colorMax * sheen_albedo(table, wo.z, alpha);
+        if (!baseFlags) return 1;
+        return clampv(e, 0.f, 0.9f);
+    }
+};
+
+// The closure of one surface hit (Material::GetBSDF, material.cpp:5-14): a plain or layered
+// base, optionally under a sheen lobe. Local-frame directions.
+struct Closure {
+    BxDF bx;
+    bool layered = false, sheen = false;
+    Layered lay;
+    SheenLobe sh;
+
+    uint32_t base_flags() const { return layered ? lay.flags() : bx.flags(); }
+    uint32_t flags() const { return sheen ? (base_flags() | kGlossy | kRefl) : base_flags(); }
+    bool pdf_exact() const { return !layered; }
+    SS base_f(V3 wo, V3 wi, Mode m) const { return layered ? lay.f(wo, wi, m) : bx.f(wo, wi, m); }
+    float base_pdf(V3 wo, V3 wi, Mode m, uint32_t rt) const { return layered ? lay.pdf(wo, wi, m) : bx.pdf(wo, wi, m, rt); }
+    std::optional<BSample> base_sample(V3 wo, float uc, V2 u, Mode m, uint32_t rt) const {
+        return layered ? lay.sample(wo, uc, u, m) : bx.sample(wo, uc, u, m, rt);
+    }
+
+    SS f(V3 wo, V3 wi, Mode m = Mode::Radiance) const {  // bxdf.cpp:589-591
+        if (!sheen) return base_f(wo, wi, m);
+        return sh.f(wo, wi) + base_f(wo, wi, m) * sh.base_scale(wo, wi);
+    }
+    float pdf(V3 wo, V3 wi, Mode m = Mode::Radiance, uint32_t rt = kRtAll) const {  // bxdf.cpp:593-599
+        if (!sheen) return base_pdf(wo, wi, m, rt);
+        float ps = sh.prob(wo, base_flags());
+        float pSheen = (rt & kRtRefl) && same_hemi(wo, wi) ? abscos(wi) * kInvPi : 0.f;
+        return ps * pSheen + (1 - ps) * base_pdf(wo, wi, m, rt);
+    }
+    std::optional<BSample> sample(V3 wo, float uc, V2 u, Mode m = Mode::Radiance, uint32_t rt = kRtAll) const {
+        if (!sheen) return base_sample(wo, uc, u, m, rt);
+        // bxdf.cpp:601-629: pick the sheen lobe with probability ps, else the base.
+        float ps = sh.prob(wo, base_flags());
+        if (!(rt & kRtRefl)) ps = 0;
+        const bool exact = pdf_exact();
+        if (uc < ps) {
+            V3 wi = sample_cosine_hemisphere(u);
+            if (wo.z < 0) wi.z = -wi.z;
+            if (exact) {
+                float p = pdf(wo, wi, m, rt);
+                if (p == 0) return std::nullopt;
+                return BSample{f(wo, wi, m), wi, p, kGlossy | kRefl, 1};
+            }
+            float p = ps * (abscos(wi) * kInvPi);
+            return BSample{sh.f(wo, wi), wi, p, kGlossy | kRefl, 1, true};
+        }
+        float uc2 = ps < 1 ? std::min((uc - ps) / (1 - ps), kOneMinusEps) : 0.f;
+        auto bs = base_sample(wo, uc2, u, m, rt);
+        if (!bs || bs->pdf == 0) return std::nullopt;
+        if ((bs->flags & kSpecular) || !exact || bs->proportional) {
+            bs->f = bs->f * sh.base_scale(wo, bs->wi);
+            bs->pdf *= (1 - ps);
+            if (!(bs->flags & kSpecular)) bs->proportional = true;
+            return bs;
+        }
+        float p = pdf(wo, bs->wi, m, rt);
+        if (p == 0) return std::nullopt;
+        bs->f = f(wo, bs->wi, m);
+        bs->pdf = p;
+        return bs;
+    }
+};
+
 struct BSDF {  // src/materials/bsdf.h:8-51
     Frame frame;
-    BxDF bx;
+    Closure c;
     bool valid = false;
     SS f(V3 woW, V3 wiW) const {
         V3 wi = frame.to_local(wiW), wo = frame.to_local(woW);
         if (wo.z == 0) return SS(0.f);
-        return bx.f(wo, wi);
+        return c.f(wo, wi);
     }
     float pdf(V3 woW, V3 wiW) const {
         V3 wo = frame.to_local(woW), wi = frame.to_local(wiW);
         if (wo.z == 0) return 0;
-        return bx.pdf(wo, wi);
+        return c.pdf(wo, wi);
     }
     std::optional<BSample> sample(V3 woW, float u, V2 u2) const {
         V3 wo = frame.to_local(woW);
-        if (wo.z == 0 || !(bx.flags() & (kRefl | kTrans))) return std::nullopt;
-        auto bs = bx.sample(wo, u, u2);
+        if (wo.z == 0 || !(c.flags() & (kRefl | kTrans))) return std::nullopt;
+        auto bs = c.sample(wo, u, u2);
         if (!bs || !bs->f.nonzero() || bs->pdf == 0 || bs->wi.z == 0 || bs->f.has_nan() || !std::isfinite(bs->pdf))
             return std::nullopt;
         bs->wi = frame.from_local(bs->wi);
         return bs;
     }
-    bool has_non_specular() const { return (bx.flags() & (kDiffuse | kGlossy)) != 0; }
+    bool has_non_specular() const { return (c.flags() & (kDiffuse | kGlossy)) != 0; }
 };
 
 // =============================================================================================
@@ -1658,41 +2053,67 @@ struct Scene {
         return SS(0.f);
     }
 
-    // ---- materials (src/materials/material.cpp:64-117) ----
+    // ---- materials (src/materials/material.cpp:5-14, 64-166) ----
+    int checker(const prgpu::Material &m, const SurfHit &si, int sid) const {  // CheckerSpectrumTexture (texture.h:126-134)
+        if (m.albedoB >= 0 && m.checkerScale != 0) {
+            float sx = si.uv.x * m.checkerScale + 0.f, sy = si.uv.y * m.checkerScale + 0.f;
+            bool odd = (int(std::floor(sx)) + int(std::floor(sy))) & 1;
+            if (odd) return m.albedoB;
+        }
+        return sid;
+    }
+    void diffuse_bx(const prgpu::Material &m, const SurfHit &si, const Lambda &lam, BxDF &bx) const {
+        SS r = spec.sample(checker(m, si, m.albedo), lam);
+        for (int i = 0; i < 4; ++i) r.v[i] = clampv(r.v[i], 0.f, 1.f);
+        bx.type = BxDiffuse;
+        bx.R = r;
+    }
+    // Conductor closure; under a coat its complex IOR is relative to the coat (eta, k / coatEta).
+    void conductor_bx(const prgpu::Material &m, const SurfHit &si, const Lambda &lam, float coatEta, BxDF &bx) const {
+        bx.type = BxConductor;
+        bx.dist = TrDist(m.alphaX, m.alphaY);
+        if (m.eta >= 0 && m.k >= 0) {
+            bx.eta = spec.sample(m.eta, lam);
+            bx.k = spec.sample(m.k, lam);
+            if (coatEta != 0) {
+                bx.eta = bx.eta / coatEta;
+                bx.k = bx.k / coatEta;
+            }
+        } else {
+            SS r = spec.sample(checker(m, si, m.albedo), lam);
+            for (int i = 0; i < 4; ++i) r.v[i] = clampv(r.v[i], 0.f, 0.9999f);
+            bx.eta = SS(1.f);
+            bx.k = ss_sqrt(r) * 2.f / ss_sqrt(ss_clamp0(SS(1.f) - r));
+        }
+    }
     BSDF eval_surface(const SurfHit &si, Lambda &lam) const {
         BSDF b;
         const Primitive &pr = prims[size_t(si.prim)];
         const Material &m = mats[size_t(pr.material)];
-        BxDF &bx = b.bx;
+        Closure &c = b.c;
+        BxDF &bx = c.bx;
         if (m.type == prgpu::MatDiffuse) {
-            int sid = m.albedo;
-            if (m.albedoB >= 0 && m.checkerScale != 0) {  // CheckerSpectrumTexture (texture.h:126-134)
-                float sx = si.uv.x * m.checkerScale + 0.f, sy = si.uv.y * m.checkerScale + 0.f;
-                bool odd = (int(std::floor(sx)) + int(std::floor(sy))) & 1;
-                if (odd) sid = m.albedoB;
-            }
-            SS r = spec.sample(sid, lam);
-            for (int i = 0; i < 4; ++i) r.v[i] = clampv(r.v[i], 0.f, 1.f);
-            bx.type = BxDiffuse;
-            bx.R = r;
+            diffuse_bx(m, si, lam, bx);
         } else if (m.type == prgpu::MatConductor) {
-            bx.type = BxConductor;
-            bx.dist = TrDist(m.alphaX, m.alphaY);
-            if (m.eta >= 0 && m.k >= 0) {
-                bx.eta = spec.sample(m.eta, lam);
-                bx.k = spec.sample(m.k, lam);
-            } else {
-                int sid = m.albedo;
-                if (m.albedoB >= 0 && m.checkerScale != 0) {
-                    float sx = si.uv.x * m.checkerScale + 0.f, sy = si.uv.y * m.checkerScale + 0.f;
-                    bool odd = (int(std::floor(sx)) + int(std::floor(sy))) & 1;
-                    if (odd) sid = m.albedoB;
-                }
-                SS r = spec.sample(sid, lam);
-                for (int i = 0; i < 4; ++i) r.v[i] = clampv(r.v[i], 0.f, 0.9999f);
-                bx.eta = SS(1.f);
-                bx.k = ss_sqrt(r) * 2.f / ss_sqrt(ss_clamp0(SS(1.f) - r));
-            }
+            conductor_bx(m, si, lam, 0.f, bx);
+        } else if (m.type == prgpu::MatCoatedDiffuse || m.type == prgpu::MatCoatedConductor) {
+            // MakeCoatTop (material.cpp:120-131): a rough dielectric coat at the hero wavelength's IOR.
+            float ce = spec.eval(m.coatEta, lam.l[0]);
+            if (!spec.is_constant(m.coatEta)) lam.terminate_secondary();
+            if (ce == 0) ce = 1;
+            Layered &L = c.lay;
+            c.layered = true;
+            L.top.type = BxDielectric;
+            L.top.etaD = ce;
+            L.top.dist = TrDist(m.coatAlpha, m.coatAlpha);
+            L.thick = std::max(m.thickness, std::numeric_limits<float>::min());
+            L.g = m.coatG;
+            L.albedo = m.coatAlbedo >= 0 ? ss_clamp0(spec.sample(m.coatAlbedo, lam)) : SS(0.f);
+            L.twoSided = true;
+            L.maxDepth = m.coatMaxDepth;
+            L.nSamples = std::max(1, m.coatSamples);
+            if (m.type == prgpu::MatCoatedDiffuse) diffuse_bx(m, si, lam, L.bottom);
+            else conductor_bx(m, si, lam, ce, L.bottom);
         } else {  // dielectric
             float e = spec.eval(m.eta, lam.l[0]);
             if (!spec.is_constant(m.eta)) lam.terminate_secondary();
@@ -1704,6 +2125,15 @@ struct Scene {
                 bx.type = BxDielectric;
                 bx.dist = TrDist(m.alphaX, m.alphaY);
             }
+        }
+        if (m.sheenWeight > 0 && m.sheenColor >= 0) {  // material.cpp:8-12
+            c.sheen = true;
+            SS col = ss_clamp0(spec.sample(m.sheenColor, lam));
+            c.sh.color = col;
+            c.sh.alpha = clampv(m.sheenRoughness, 0.05f, 1.f);
+            c.sh.weight = clampv(m.sheenWeight, 0.f, 1.f);
+            c.sh.colorMax = clampv(col.maxc(), 0.f, 1.f);
+            c.sh.table = sheen_table();
         }
         b.frame = Frame::from_xz(si.dpdus, si.ns);
         b.valid = true;
@@ -1856,7 +2286,7 @@ static SS li(const Scene &sc, const Settings &st, Ray ray, Lambda &lam, Sampler 
         auto bs = bsdf.sample(wo, uc, u2);
         if (!bs) break;
         beta = beta * (bs->f * (absdot(bs->wi, si.ns) / bs->pdf));
-        prevPdf = bs->pdf;
+        prevPdf = bs->proportional ? bsdf.pdf(wo, bs->wi) : bs->pdf;  // path.cpp:220
         specularBounce = (bs->flags & kSpecular) != 0;
         if (bs->flags & kTrans) etaScale *= sq(bs->eta);
         prevP = si.p;
@@ -2296,6 +2726,75 @@ void or_bxdf(const float *desc, int op, const float *wo, const float *wi, float 
         }
     }
 }
+
+// Closure probe: desc = [type (1 diffuse, 2 conductor, 3 dielectric, 4 thin, 5 coated diffuse,
+// 6 coated conductor), alphaX, alphaY, etaD | coatEta, R[4], eta[4], k[4], coatAlpha, thickness,
+// g, albedo[4], maxDepth, nSamples, twoSided, sheenWeight, sheenAlpha, sheenColor[4]].
+// op 0: f (4), 1: pdf (1), 2: sample (f[4], wi[3], pdf, flags, eta, ok, proportional), 3: flags, exact.
+void or_closure(const float *desc, int op, const float *wo, const float *wi, float uc, const float *u, int mode, int rt,
+                float *out) {
+    const int type = int(desc[0]);
+    auto plain = [&](int t, float etaD, BxDF &b) {
+        b.type = BxType(t);
+        b.dist = TrDist(desc[1], desc[2]);
+        b.etaD = etaD;
+        for (int i = 0; i < 4; ++i) {
+            b.R.v[i] = desc[4 + i];
+            b.eta.v[i] = desc[8 + i];
+            b.k.v[i] = desc[12 + i];
+        }
+    };
+    Closure c;
+    if (type == 5 || type == 6) {
+        c.layered = true;
+        Layered &L = c.lay;
+        L.top.type = BxDielectric;
+        L.top.etaD = desc[3];
+        L.top.dist = TrDist(desc[16], desc[16]);
+        plain(type == 5 ? BxDiffuse : BxConductor, 1.f, L.bottom);
+        L.thick = std::max(desc[17], std::numeric_limits<float>::min());
+        L.g = desc[18];
+        for (int i = 0; i < 4; ++i) L.albedo.v[i] = desc[19 + i];
+        L.maxDepth = int(desc[23]);
+        L.nSamples = int(desc[24]);
+        L.twoSided = desc[25] != 0;
+    } else {
+        plain(type, desc[3], c.bx);
+    }
+    if (desc[26] > 0) {
+        c.sheen = true;
+        for (int i = 0; i < 4; ++i) c.sh.color.v[i] = desc[28 + i];
+        c.sh.alpha = clampv(desc[27], 0.05f, 1.f);
+        c.sh.weight = clampv(desc[26], 0.f, 1.f);
+        c.sh.colorMax = clampv(c.sh.color.maxc(), 0.f, 1.f);
+        c.sh.table = sheen_table();
+    }
+    const Mode m = mode ? Mode::Importance : Mode::Radiance;
+    V3 o(wo[0], wo[1], wo[2]);
+    if (op == 0) {
+        SS f = c.f(o, V3(wi[0], wi[1], wi[2]), m);
+        for (int i = 0; i < 4; ++i) out[i] = f.v[i];
+    } else if (op == 1) {
+        out[0] = c.pdf(o, V3(wi[0], wi[1], wi[2]), m, uint32_t(rt));
+    } else if (op == 2) {
+        auto s = c.sample(o, uc, V2{u[0], u[1]}, m, uint32_t(rt));
+        for (int i = 0; i < 12; ++i) out[i] = 0;
+        if (s) {
+            for (int i = 0; i < 4; ++i) out[i] = s->f.v[i];
+            out[4] = s->wi.x; out[5] = s->wi.y; out[6] = s->wi.z;
+            out[7] = s->pdf;
+            out[8] = float(s->flags);
+            out[9] = s->eta;
+            out[10] = 1;
+            out[11] = s->proportional ? 1.f : 0.f;
+        }
+    } else {
+        out[0] = float(c.flags());
+        out[1] = c.pdf_exact() ? 1.f : 0.f;
+    }
+}
+float or_sheen_albedo(float cosTheta, float alpha) { return sheen_albedo(sheen_table(), cosTheta, alpha); }
+void or_sheen_table(float *out) { std::memcpy(out, sheen_table(), sizeof(float) * kSheenMuN * kSheenAlphaN); }
 
 float or_fr_dielectric(float c, float eta) { return fr_dielectric(c, eta); }
 float or_fr_complex(float c, float eta, float k) { return fr_complex1(c, std::complex<float>(eta, k)); }

@@ -169,6 +169,82 @@ void rf_bxdf(const float *desc, int op, const float *wo, const float *wi, float 
     }
 }
 
+// Closure probe with the descriptor of or_closure (oracle.h): plain, coated (LayeredBxDF over a
+// DielectricBxDF coat) and sheen-wrapped BxDFs, with transport mode and sample flags.
+void rf_closure(const float *desc, int op, const float *wo, const float *wi, float uc, const float *u, int mode,
+                int rt, float *out) {
+    const int type = int(desc[0]);
+    SampledSpectrum R, eta, k, albedo, sheen;
+    for (int i = 0; i < 4; ++i) {
+        R[i] = desc[4 + i];
+        eta[i] = desc[8 + i];
+        k[i] = desc[12 + i];
+        albedo[i] = desc[19 + i];
+        sheen[i] = desc[28 + i];
+    }
+    TrowbridgeReitzDistribution d(desc[1], desc[2]);
+    std::vector<std::unique_ptr<BxDF>> keep;
+    auto plain = [&](int t, float etaD) -> const BxDF * {
+        if (t == 1) keep.push_back(std::make_unique<DiffuseBxDF>(R));
+        else if (t == 2) keep.push_back(std::make_unique<ConductorBxDF>(d, eta, k));
+        else if (t == 3) keep.push_back(std::make_unique<DielectricBxDF>(etaD, d));
+        else keep.push_back(std::make_unique<ThinDielectricBxDF>(etaD));
+        return keep.back().get();
+    };
+    const BxDF *b;
+    if (type == 5 || type == 6) {
+        keep.push_back(std::make_unique<DielectricBxDF>(desc[3], TrowbridgeReitzDistribution(desc[16], desc[16])));
+        const BxDF *top = keep.back().get();
+        const BxDF *bottom = plain(type == 5 ? 1 : 2, 1.f);
+        keep.push_back(std::make_unique<LayeredBxDF>(top, bottom, desc[17], albedo, desc[18], desc[25] != 0,
+                                                     int(desc[23]), int(desc[24])));
+        b = keep.back().get();
+    } else {
+        b = plain(type, desc[3]);
+    }
+    if (desc[26] > 0) {
+        keep.push_back(std::make_unique<SheenBxDF>(b, sheen, desc[27], desc[26]));
+        b = keep.back().get();
+    }
+    const TransportMode m = mode ? TransportMode::Importance : TransportMode::Radiance;
+    const BxDFReflTransFlags f = BxDFReflTransFlags(rt);
+    Vec3f o(wo[0], wo[1], wo[2]);
+    if (op == 0) {
+        SampledSpectrum v = b->f(o, Vec3f(wi[0], wi[1], wi[2]), m);
+        for (int i = 0; i < 4; ++i) out[i] = v[i];
+    } else if (op == 1) {
+        out[0] = b->PDF(o, Vec3f(wi[0], wi[1], wi[2]), m, f);
+    } else if (op == 2) {
+        auto s = b->Sample_f(o, uc, Vec2f(u[0], u[1]), m, f);
+        for (int i = 0; i < 12; ++i) out[i] = 0;
+        if (s) {
+            for (int i = 0; i < 4; ++i) out[i] = s->f[i];
+            out[4] = s->wi.x;
+            out[5] = s->wi.y;
+            out[6] = s->wi.z;
+            out[7] = s->pdf;
+            out[8] = float(uint32_t(s->flags));
+            out[9] = s->eta;
+            out[10] = 1;
+            out[11] = s->pdfIsProportional ? 1.f : 0.f;
+        }
+    } else {
+        out[0] = float(uint32_t(b->Flags()));
+        out[1] = b->PDFIsExact() ? 1.f : 0.f;
+    }
+}
+
+float rf_sheen_albedo(float cosTheta, float alpha) { return SheenBxDF::Albedo(cosTheta, alpha); }
+
+// Order in which the compiler evaluates the two arguments of Vec2f(r(), r()) (the reference's
+// layered random walk draws its 2D samples that way): 0 = left to right, 1 = right to left.
+int rf_pair_order() {
+    int n = 0;
+    auto r = [&n]() { return float(n++); };
+    Vec2f v(r(), r());
+    return v.x == 0 ? 0 : 1;
+}
+
 float rf_fr_dielectric(float c, float eta) { return FrDielectric(c, eta); }
 float rf_fr_complex(float c, float eta, float k) {
     return FrComplex(c, SampledSpectrum(eta), SampledSpectrum(k))[0];

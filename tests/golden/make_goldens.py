@@ -17,7 +17,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "ar-renderer_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import prender_amd as pa  # noqa: E402
+from oracle import closure_desc  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "libprref.so")
 
@@ -53,6 +55,8 @@ def load_ref():
         ("rf_light_sample_li", None, [vp, C.c_int, vp, vp, vp, C.c_float, vp]),
         ("rf_light_pdf_li", C.c_float, [vp, C.c_int, vp, vp, vp]),
         ("rf_world_sphere", None, [vp, vp]),
+        ("rf_closure", None, [vp, C.c_int, vp, vp, C.c_float, vp, C.c_int, C.c_int, vp]),
+        ("rf_sheen_albedo", C.c_float, [C.c_float, C.c_float]),
     ]:
         f = getattr(L, name)
         f.restype = res
@@ -302,9 +306,69 @@ def main():
     out.update(c1_lsu=LU, c1_lsl=LL, c1_lsi=lsi, c1_lpdf=lpdf)
     L.rf_scene_destroy(h)
 
+    closures(L, out)
+
     path = os.path.join(HERE, "ref_components.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")
+
+
+def closures(L, out):
+    """Coated (layered) and sheen closures: f, pdf and sample under both transport modes and all
+    sample flags, and the sheen albedo table."""
+    rng = np.random.default_rng(20261203)
+    R0 = (0.8, 0.4, 0.2, 0.6)
+    descs = [
+        closure_desc(5, R=R0),                                                    # clear smooth coat, diffuse
+        closure_desc(5, R=R0, coat_alpha=0.09, eta=1.33),                         # rough coat
+        closure_desc(5, R=R0, thickness=0.3, albedo=(0.9, 0.5, 0.3, 0.7), g=0.4),  # scattering slab
+        closure_desc(5, R=R0, coat_alpha=0.25, thickness=0.1, albedo=(0.6, 0.6, 0.6, 0.6), g=-0.3, n_samples=2),
+        closure_desc(6, alpha=(0.0, 0.0)),                                        # smooth conductor under a coat
+        closure_desc(6, alpha=(0.1, 0.1), coat_alpha=0.04),                       # rough conductor, rough coat
+        closure_desc(6, alpha=(0.2, 0.05), ceta=(1, 1, 1, 1), k=(2.0, 1.0, 0.5, 3.0), thickness=0.2,
+                     albedo=(0.3, 0.8, 0.5, 0.2), g=0.7, max_depth=8),
+        closure_desc(1, R=R0, sheen=0.8, sheen_alpha=0.4, sheen_color=(1.0, 0.7, 0.5, 0.9)),   # sheen on diffuse
+        closure_desc(2, alpha=(0.3, 0.3), sheen=1.0, sheen_alpha=0.05),                      # sheen on conductor
+        closure_desc(3, alpha=(0.2, 0.2), sheen=0.5, sheen_alpha=1.0, sheen_color=(0.4, 0.4, 0.4, 0.4)),
+        closure_desc(5, R=R0, coat_alpha=0.09, sheen=0.7, sheen_alpha=0.6, sheen_color=(0.9, 0.2, 0.2, 0.5)),
+        closure_desc(1, R=(0, 0, 0, 0), sheen=0.6),                               # sheen over a black base
+    ]
+    descs = f32(descs)
+    nd, per = len(descs), 24
+    wo = rng.normal(size=(nd, per, 3))
+    wo /= np.linalg.norm(wo, axis=2, keepdims=True)
+    wi = rng.normal(size=(nd, per, 3))
+    wi /= np.linalg.norm(wi, axis=2, keepdims=True)
+    wo, wi = f32(wo), f32(wi)
+    uc = f32(rng.random((nd, per)))
+    u2 = f32(rng.random((nd, per, 2)))
+    mode = (rng.random((nd, per)) < 0.25).astype(np.int32)
+    rt = rng.choice([3, 3, 1, 2], size=(nd, per)).astype(np.int32)
+    cf = np.zeros((nd, per, 4), np.float32)
+    cp = np.zeros((nd, per), np.float32)
+    cs = np.zeros((nd, per, 12), np.float32)
+    cfl = np.zeros((nd, 2), np.float32)
+    for i in range(nd):
+        d = np.ascontiguousarray(descs[i])
+        o = np.zeros(12, np.float32)
+        L.rf_closure(ptr(d), 3, ptr(wo[i, 0]), ptr(wi[i, 0]), 0.0, ptr(u2[i, 0]), 0, 3, ptr(o))
+        cfl[i] = o[:2]
+        for j in range(per):
+            a, b, cc, uu = f32(wo[i, j]), f32(wi[i, j]), float(uc[i, j]), f32(u2[i, j])
+            m, r = int(mode[i, j]), int(rt[i, j])
+            L.rf_closure(ptr(d), 0, ptr(a), ptr(b), cc, ptr(uu), m, r, ptr(o))
+            cf[i, j] = o[:4]
+            L.rf_closure(ptr(d), 1, ptr(a), ptr(b), cc, ptr(uu), m, r, ptr(o))
+            cp[i, j] = o[0]
+            L.rf_closure(ptr(d), 2, ptr(a), ptr(b), cc, ptr(uu), m, r, ptr(o))
+            cs[i, j] = o
+    out.update(cl_desc=descs, cl_wo=wo, cl_wi=wi, cl_uc=uc, cl_u=u2, cl_mode=mode, cl_rt=rt, cl_f=cf, cl_pdf=cp,
+               cl_sample=cs, cl_flags=cfl)
+    # Sheen directional albedo: on the table's nodes and between them.
+    sc = f32(np.concatenate([np.arange(32) / 31, rng.uniform(-1, 1, 40)]))
+    sa = f32(np.concatenate([0.05 + 0.95 * np.arange(16) / 15, [0.0, 1.2], rng.uniform(0, 1, 14)]))
+    out["sheen_cos"], out["sheen_alpha"] = sc, sa
+    out["sheen_albedo"] = f32([[L.rf_sheen_albedo(float(c), float(a)) for c in sc] for a in sa])
 
 
 if __name__ == "__main__":

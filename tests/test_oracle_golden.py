@@ -110,6 +110,35 @@ def test_bxdf_eval_pdf_sample():
     print(f"bxdf worst relative deviation {worst:.2e}")
 
 
+def test_coated_and_sheen_closures_bit_exact():
+    """Layered (coated diffuse / conductor, clear and scattering slabs) and sheen closures: f, pdf
+    and sample under both transport modes and all sample flags, bit for bit. The layered walk's
+    random stream is hashed from the directions, so equal bits also pin the walk's draw order."""
+    D = G["cl_desc"]
+    for i in range(len(D)):
+        fl = orc.closure(D[i], 3, G["cl_wo"][i, 0])
+        assert np.array_equal(fl[:2], G["cl_flags"][i]), (i, fl[:2], G["cl_flags"][i])
+        for j in range(G["cl_wo"].shape[1]):
+            args = (G["cl_wo"][i, j], G["cl_wi"][i, j], float(G["cl_uc"][i, j]), G["cl_u"][i, j],
+                    int(G["cl_mode"][i, j]), int(G["cl_rt"][i, j]))
+            f = orc.closure(D[i], 0, *args)[:4]
+            assert np.array_equal(f.view(np.uint32), G["cl_f"][i, j].view(np.uint32)), ("f", i, j, f, G["cl_f"][i, j])
+            pdf = orc.closure(D[i], 1, *args)[0]
+            assert np.float32(pdf).view(np.uint32) == G["cl_pdf"][i, j].view(np.uint32), ("pdf", i, j)
+            s = orc.closure(D[i], 2, *args)
+            assert np.array_equal(s.view(np.uint32), G["cl_sample"][i, j].view(np.uint32)), ("sample", i, j, s,
+                                                                                            G["cl_sample"][i, j])
+    # the fixture exercises both outcomes of sampling, proportional pdfs and nonzero layered values
+    assert 0 < G["cl_sample"][..., 10].mean() < 1 and G["cl_sample"][..., 11].sum() > 10
+    assert np.count_nonzero(G["cl_f"][:7].max(axis=2)) > 40
+
+
+def test_sheen_albedo_table():
+    for a_i, a in enumerate(G["sheen_alpha"]):
+        vals = np.array([L.or_sheen_albedo(float(c), float(a)) for c in G["sheen_cos"]], np.float32)
+        assert np.array_equal(vals, G["sheen_albedo"][a_i]), a
+
+
 def test_fresnel_and_phase_function():
     for c, e, k, rd, rc in zip(G["fr_cos"], G["fr_eta"], G["fr_k"], G["fr_dielectric"], G["fr_complex"]):
         np.testing.assert_allclose(L.or_fr_dielectric(float(c), float(e)), rd, rtol=1e-6, atol=1e-9)

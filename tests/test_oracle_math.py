@@ -291,3 +291,126 @@ def test_grid_spectral_tracking_scatter_probability():
     interact = 1 - np.exp(-0.8 * 0.5)
     for got, want in ((scat / n, interact * 0.5 / 0.8), (absorbed / n, interact * 0.3 / 0.8)):
         assert abs(got - want) < 4 * np.sqrt(want * (1 - want) / n), (got, want)
+
+
+# ---- coated (layered) and sheen closures -----------------------------------------------------------
+closure_desc = orc.closure_desc
+
+
+def sample_weights(desc, wo, n, seed):
+    """Sample_f weights f |cos| / pdf (zero for failed samples): their mean is the directional albedo.
+    A walk through many near-specular rough bounces can overflow f and pdf in FP32 (as in the
+    reference); such samples, about 1 in 10^4, are dropped as the film drops non-finite values."""
+    rng = np.random.default_rng(seed)
+    w = np.zeros((n, 4))
+    dropped = 0
+    for k in range(n):
+        s = orc.closure(desc, 2, wo, uc=rng.random(), u=rng.random(2))
+        if s[10]:
+            v = s[:4].astype(np.float64) * abs(s[6]) / s[7]
+            if np.all(np.isfinite(v)):
+                w[k] = v
+            else:
+                dropped += 1
+    assert dropped <= max(2, n // 2000), dropped
+    return w
+
+
+def test_coated_white_diffuse_conserves_energy():
+    """The slab between coat and base has unit extinction, so it absorbs exp(-thickness / |cos|)
+    per crossing. A white Lambertian base under a clear smooth coat over a vanishingly thin slab
+    absorbs nothing: light leaves after enough internal bounces and the albedo is 1 (radiance
+    scaling 1/eta^2 in and eta^2 out cancel). The default thickness 0.01 loses a few percent."""
+    wo = f32(0.3, -0.2, 0.932738)
+    wo /= np.linalg.norm(wo)
+    w = sample_weights(closure_desc(5, R=(1, 1, 1, 1), eta=1.5, thickness=1e-7), wo, 20000, 11)
+    mean, err = w.mean(axis=0), w.std(axis=0) / np.sqrt(len(w))
+    assert np.all(np.abs(mean - 1) < 4 * err + 0.005), (mean, err)
+    thick = sample_weights(closure_desc(5, R=(1, 1, 1, 1), eta=1.5, thickness=0.01), wo, 8000, 12).mean()
+    assert 0.85 < thick < 0.97
+    # a grey base absorbs on every internal bounce: strictly less than its own albedo comes back
+    w2 = sample_weights(closure_desc(5, R=(0.5, 0.5, 0.5, 0.5), eta=1.5, thickness=1e-7), wo, 8000, 12)
+    assert w2.mean() < 0.5
+
+
+@pytest.mark.parametrize("kind,kw", [
+    (5, dict(R=(0.8, 0.4, 0.2, 0.6), coat_alpha=0.09, eta=1.33)),
+    (5, dict(R=(0.8, 0.4, 0.2, 0.6), coat_alpha=0.25, thickness=0.3, albedo=(0.9, 0.5, 0.3, 0.7), g=0.4)),
+    (6, dict(alpha=(0.1, 0.1), coat_alpha=0.04)),
+])
+def test_layered_f_integrates_to_sampled_albedo(kind, kw):
+    """The stochastic f and the stochastic sampler estimate the same directional albedo:
+    4 pi E_uniform[f |cos|] = E_sample[f |cos| / pdf] (rough coats: no delta lobes)."""
+    desc = closure_desc(kind, **kw)
+    wo = f32(-0.4, 0.1, 0.91104)
+    wo /= np.linalg.norm(wo)
+    wi = uniform_sphere(30000, 13)
+    fv = np.array([orc.closure(desc, 0, wo, w)[:4] * abs(w[2]) for w in wi]) * 4 * np.pi
+    ws = sample_weights(desc, wo, 20000, 14)
+    a, ea = fv.mean(axis=0), fv.std(axis=0) / np.sqrt(len(fv))
+    b, eb = ws.mean(axis=0), ws.std(axis=0) / np.sqrt(len(ws))
+    assert np.all(np.abs(a - b) < 4 * np.hypot(ea, eb) + 0.01), (a, b, ea, eb)
+    assert b.max() > 0.05
+
+
+def concentric_cosine(u):
+    """Cosine-weighted hemisphere directions by the concentric disk map (Shirley-Chiu)."""
+    x, y = 2 * u[..., 0] - 1, 2 * u[..., 1] - 1
+    big = np.abs(x) > np.abs(y)
+    safe_x = np.where(x == 0, 1, x)
+    safe_y = np.where(y == 0, 1, y)
+    r = np.where(big, x, y)
+    th = np.where(big, np.pi / 4 * (y / safe_x), np.pi / 2 - np.pi / 4 * (x / safe_y))
+    dx, dy = r * np.cos(th), r * np.sin(th)
+    z = np.sqrt(np.maximum(0, 1 - dx * dx - dy * dy))
+    return np.stack([dx, dy, z], -1)
+
+
+def test_charlie_sheen_lobe_normalised_and_tabulated():
+    """On a black base the sheen closure is the Charlie lobe alone, f = D(wh) / (4 (ci + co - ci co)).
+    (1) D is a normalised microfacet distribution: with wo = wi = wh, D = 4 f (2c - c^2) and
+    the hemisphere integral of D(wh) cos(wh) is 1. (2) The albedo table's node (alpha_i, mu_j) is
+    the 32 x 32 stratified cosine-weighted quadrature of f |cos| pi. (3) Where the lobe is broad
+    that quadrature agrees with uniform Monte Carlo; at small alpha or grazing wo the peaked
+    integrand makes the 32 x 32 rule read several percent off, as the reference's table does."""
+    table = orc.sheen_table().reshape(16, 32)
+    wh = uniform_sphere(40000, 15)
+    wh[:, 2] = np.abs(wh[:, 2])
+    wh = wh[wh[:, 2] > 1e-3]
+    for alpha in (0.1, 0.3, 0.7, 1.0):
+        desc = closure_desc(1, R=(0, 0, 0, 0), sheen=1.0, sheen_alpha=alpha)
+        c = wh[:, 2].astype(np.float64)
+        D = np.array([orc.closure(desc, 0, w, w)[0] for w in wh]) * 4 * (2 * c - c * c)
+        est = 2 * np.pi * np.mean(D * c)
+        err = 2 * np.pi * np.std(D * c) / np.sqrt(len(c))
+        assert abs(est - 1) < 4 * err + 0.01, (alpha, est, err)
+    g = (np.arange(32) + 0.5) / 32
+    wi = concentric_cosine(np.stack(np.meshgrid(g, g, indexing="ij"), -1).reshape(-1, 2)).astype(np.float32)
+    for ia, im in ((2, 31), (7, 16), (15, 5), (0, 24), (11, 0)):
+        alpha = float(np.float32(0.05 + 0.95 * ia / 15))
+        mu = max(1e-3, im / 31)
+        wo = f32(np.sqrt(1 - mu * mu), 0, mu)
+        desc = closure_desc(1, R=(0, 0, 0, 0), sheen=1.0, sheen_alpha=alpha)
+        quad = np.mean([orc.closure(desc, 0, wo, w)[0] for w in wi]) * np.pi
+        np.testing.assert_allclose(table[ia, im], quad, rtol=2e-5, err_msg=f"node {ia} {im}")
+        if ia >= 7 and im >= 5:  # broad lobe, wo away from grazing
+            u = uniform_sphere(40000, 16 + ia)
+            fv = np.array([orc.closure(desc, 0, wo, w)[0] * abs(w[2]) for w in u]) * 4 * np.pi
+            est, err = fv.mean(), fv.std() / np.sqrt(len(fv))
+            assert abs(est - table[ia, im]) < 4 * err + 0.02 * table[ia, im], (ia, im, est, table[ia, im], err)
+
+
+def test_transport_mode_scales_dielectric_transmission():
+    """Radiance transport divides refracted radiance by eta'^2; importance transport does not."""
+    desc = closure_desc(3, alpha=(0.3, 0.3), eta=1.5)
+    wo, wi = f32(0.2, 0.1, 0.974679), f32(-0.1, 0.05, -0.993734)
+    fr = orc.closure(desc, 0, wo, wi, mode=0)[0]
+    fi = orc.closure(desc, 0, wo, wi, mode=1)[0]
+    assert fi > 0 and abs(fr / fi - 1 / 1.5 ** 2) < 1e-6
+    # reflection-only and transmission-only sampling never return the other kind
+    rng = np.random.default_rng(16)
+    for rt, kind in ((1, 1), (2, 2)):
+        for _ in range(200):
+            s = orc.closure(desc, 2, wo, uc=rng.random(), u=rng.random(2), rt=rt)
+            if s[10]:
+                assert int(s[8]) & 3 == kind
