@@ -547,6 +547,10 @@ int SceneBuilder::ConductorReflectance(int refl, float ax, float ay) {
     return id;
 }
 
+static bool ConstantSpectrum(const prgpu::SpectrumEx &s) {
+    return s.kind == prgpu::SpecConstant || (s.kind == prgpu::SpecCauchy && s.p[1] == 0);
+}
+
 int SceneBuilder::Dielectric(int eta, float ax, float ay, bool thin) {
     prgpu::Material m{};
     m.type = prgpu::MatDielectric;
@@ -556,9 +560,51 @@ int SceneBuilder::Dielectric(int eta, float ax, float ay, bool thin) {
     m.alphaY = ay;
     m.thin = thin ? 1 : 0;
     const prgpu::SpectrumEx &s = spectra[size_t(eta)];
-    m.dispersive = (s.kind == prgpu::SpecConstant || (s.kind == prgpu::SpecCauchy && s.p[1] == 0)) ? 0 : 1;
+    m.dispersive = ConstantSpectrum(s) ? 0 : 1;
     materials.push_back(m);
     return int(materials.size() - 1);
+}
+
+
+static void SetCoat(prgpu::Material &m, const SceneBuilder::Coat &c, const std::vector<prgpu::SpectrumEx> &spectra) {
+    m.coatEta = c.eta;
+    float r = std::min(std::max(c.roughness, 0.f), 1.f);
+    m.coatAlpha = r * r;
+    m.thickness = c.thickness;
+    m.coatAlbedo = c.albedo;
+    m.coatG = c.g;
+    m.coatMaxDepth = c.maxDepth;
+    m.coatSamples = c.nSamples;
+    m.coatDispersive = ConstantSpectrum(spectra[size_t(c.eta)]) ? 0 : 1;
+}
+
+int SceneBuilder::CoatedDiffuse(const Coat &coat, int reflectance) {
+    int id = Diffuse(reflectance);
+    prgpu::Material &m = materials[size_t(id)];
+    m.type = prgpu::MatCoatedDiffuse;
+    SetCoat(m, coat, spectra);
+    return id;
+}
+
+int SceneBuilder::CoatedConductor(const Coat &coat, int eta, int k, float ax, float ay) {
+    int id = Conductor(eta, k, ax, ay);
+    prgpu::Material &m = materials[size_t(id)];
+    m.type = prgpu::MatCoatedConductor;
+    SetCoat(m, coat, spectra);
+    return id;
+}
+
+int SceneBuilder::CoatedConductorReflectance(const Coat &coat, int refl, float ax, float ay) {
+    int id = CoatedConductor(coat, -1, -1, ax, ay);
+    materials[size_t(id)].albedo = refl;
+    return id;
+}
+
+void SceneBuilder::SetSheen(int material, int color, float roughness, float weight) {
+    prgpu::Material &m = materials[size_t(material)];
+    m.sheenColor = color;
+    m.sheenRoughness = roughness;
+    m.sheenWeight = weight;
 }
 
 int SceneBuilder::Interface() {
@@ -1319,6 +1365,73 @@ static void GenGems(const GenOptions &o, SceneBuilder *sb) {
     sb->seed = 1;
 }
 
+static void GenMaterials(const GenOptions &o, SceneBuilder *sb) {
+    // The pyramid of spheres with coated and sheen materials (no reference configuration uses
+    // them; they exercise the coated / sheen closures): clear, rough, scattering and dispersive
+    // coats over diffuse bases, coated gold and coated artist conductors, sheen over diffuse,
+    // coated and rough dielectric bases. Quad light plus a dim constant environment.
+    SceneRng rng(6);
+    int W = o.width > 0 ? o.width : 256, H = o.height > 0 ? o.height : 256;
+    sb->SetCamera(f3(0, 6, -18), f3(0, 0, 0), 40.f, W, H);
+    sb->AddMesh(MakeQuad(f3(-20, 0, -20), f3(0, 0, 40), f3(40, 0, 0)), sb->Diffuse(sb->Constant(0.5f)));
+    const float ys[4] = {1.f, 2.41f, 3.83f, 5.24f};
+    int segs = std::max(4, int(std::lround(32 * o.detail))), rings = std::max(3, int(std::lround(16 * o.detail)));
+    const int glass = sb->Constant(1.5f), water = sb->Constant(1.33f), bk7 = sb->Glass("BK7");
+    const int auN = sb->Metal("Au", false), auK = sb->Metal("Au", true);
+    int k = 0;
+    for (int level = 0; level < 4; ++level) {
+        int n = 4 - level;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j, ++k) {
+                float x = 2.f * float(i) - float(n - 1), z = 2.f * float(j) - float(n - 1);
+                float r = rng.uniform(0.1f, 0.9f), g = rng.uniform(0.1f, 0.9f), b = rng.uniform(0.1f, 0.9f);
+                int albedo = sb->RGBAlbedo(r, g, b);
+                SceneBuilder::Coat c;
+                c.eta = glass;
+                int mat;
+                switch (k % 8) {
+                case 0: mat = sb->CoatedDiffuse(c, albedo); break;
+                case 1:
+                    c.roughness = 0.3f;
+                    c.eta = water;
+                    mat = sb->CoatedDiffuse(c, albedo);
+                    break;
+                case 2:
+                    c.thickness = 0.1f;
+                    c.albedo = sb->RGBAlbedo(0.8f, 0.6f, 0.3f);
+                    c.g = 0.3f;
+                    mat = sb->CoatedDiffuse(c, albedo);
+                    break;
+                case 3:
+                    c.eta = bk7;
+                    mat = sb->CoatedDiffuse(c, albedo);
+                    break;
+                case 4: mat = sb->CoatedConductor(c, auN, auK, 0.04f, 0.04f); break;
+                case 5:
+                    c.roughness = 0.2f;
+                    mat = sb->CoatedConductorReflectance(c, albedo, 0.1f, 0.1f);
+                    break;
+                case 6:
+                    mat = sb->Diffuse(albedo);
+                    sb->SetSheen(mat, sb->RGBAlbedo(0.9f, 0.9f, 0.9f), 0.3f, 1.f);
+                    break;
+                default:
+                    c.roughness = 0.1f;
+                    mat = sb->CoatedDiffuse(c, albedo);
+                    sb->SetSheen(mat, sb->RGBAlbedo(1.f, 0.8f, 0.6f), 0.5f, 0.7f);
+                    break;
+                }
+                sb->AddMesh(MakeUVSphere(f3(x, ys[level], z), 1.f, segs, rings), mat);
+            }
+    }
+    sb->AddEmitter(MakeQuad(f3(-1, 12, -1), f3(2, 0, 0), f3(0, 0, 2)), sb->BlackbodyUnitY(6500.f), 60.f);
+    sb->AddEnvironment(sb->Constant(1.f), 0.05f);
+    sb->spp = 64;
+    sb->maxDepth = 8;
+    sb->rrStartDepth = 3;
+    sb->seed = 1;
+}
+
 bool GenerateConfig(int config, const GenOptions &o, SceneBuilder *sb, std::string *err) {
     switch (config) {
     case 1: GenPyramid(o, sb); return true;
@@ -1326,6 +1439,7 @@ bool GenerateConfig(int config, const GenOptions &o, SceneBuilder *sb, std::stri
     case 3: GenMedia(o, sb); return true;
     case 4: GenRoom(o, sb); return true;
     case 5: GenGems(o, sb); return true;
+    case 6: GenMaterials(o, sb); return true;
     default:
         if (err) *err = "configuration " + std::to_string(config) + " is not available";
         return false;

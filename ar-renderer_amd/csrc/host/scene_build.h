@@ -88,6 +88,22 @@ class SceneBuilder {
     int ConductorReflectance(int reflectance, float alphaX, float alphaY);
     int Dielectric(int eta, float alphaX, float alphaY, bool thin = false);
     int Interface();
+    // Coated materials (src/materials/material.h:124-170): a rough dielectric coat of IOR `eta`
+    // over a diffuse or conductor base, with a slab of unit extinction between them whose
+    // scattering albedo is `albedo` (-1: purely absorbing) and phase asymmetry g.
+    struct Coat {
+        int eta = -1;
+        float roughness = 0;     // artist roughness; alpha = roughness^2
+        float thickness = 0.01f;
+        int albedo = -1;
+        float g = 0;
+        int maxDepth = 64, nSamples = 1;
+    };
+    int CoatedDiffuse(const Coat &coat, int reflectance);
+    int CoatedConductor(const Coat &coat, int eta, int k, float alphaX, float alphaY);
+    int CoatedConductorReflectance(const Coat &coat, int reflectance, float alphaX, float alphaY);
+    // Charlie sheen over the material's closure (src/materials/material.cpp:8-12).
+    void SetSheen(int material, int color, float roughness, float weight);
 
     // ---- geometry ----
     int AddMesh(const Mesh &m, int material, int mediumInside = -1, int mediumOutside = -1, bool normals = true);

@@ -12,6 +12,7 @@
 #include "device.h"
 #include "guide_host.h"
 #include "pt_host.h"
+#include "sheen_table.h"
 #include "trace_dev.h"
 
 namespace pgpu {
@@ -383,6 +384,11 @@ pgpu::GuideState &EnsureGuide(pgpu::Device &d) {
 }  // namespace
 
 PRGPU_API int prgpu_guide_param_count() { return pgpu::kGParams; }
+
+PRGPU_API int prgpu_sheen_table(float *out) {
+    if (out) pgpu::SheenAlbedoTable(out);
+    return prgpu::kSheenAlpha * prgpu::kSheenMu;
+}
 
 PRGPU_API int prgpu_guide_params(int deviceIndex, float *params, int count, int write) {
     std::lock_guard<std::mutex> lock(gMutex);

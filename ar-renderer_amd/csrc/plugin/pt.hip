@@ -23,6 +23,7 @@
 
 #include "guide_host.h"
 #include "pt_dev.h"
+#include "sheen_table.h"
 #include "pt_host.h"
 
 namespace pgpu {
@@ -191,11 +192,11 @@ Box3 PrimBounds(const prgpu::Primitive &p) {
 // Device-resident path-tracing scene and batch buffers
 
 struct PTScene {
-    DevBuf prims, mats, specs, knots, lights, lightArea, lnodes, leafOf, infinite, powerCdf, powerFunc, media, grid;
+    DevBuf prims, mats, specs, knots, lights, lightArea, lnodes, leafOf, infinite, powerCdf, powerFunc, media, grid, sheen;
     int lightCount = 0, lnodeCount = 0, infCount = 0, powerCount = 0;
     float powerInt = 0;
     float worldR = 1;
-    int hasInterfaces = 0, hasMedia = 0;
+    int hasInterfaces = 0, hasMedia = 0, hasLayered = 0;  // hasLayered: coated or sheen materials
     float cieYInt = 1;
     // host copies for inspection
     std::vector<LNode> hostNodes;
@@ -224,7 +225,7 @@ void ReleasePT(Device &d) {
 
 namespace {
 
-uint64_t PTKey(const prgpu::SceneDescEx &sd) {
+uint64_t PTKey(const prgpu::SceneDescEx &sd, const float *sheenTable) {
     uint64_t h = 1469598103934665603ull;
     auto mix = [&](const void *p, size_t n) {
         const unsigned char *b = static_cast<const unsigned char *>(p);
@@ -243,6 +244,7 @@ uint64_t PTKey(const prgpu::SceneDescEx &sd) {
     }
     mix(&sd.cameraMedium, sizeof(int));
     mix(&sd.camera, sizeof(sd.camera));
+    if (sheenTable) mix(sheenTable, sizeof(float) * prgpu::kSheenAlpha * prgpu::kSheenMu);
     return h ^ GeometryKey(sd.prims, sd.primCount);
 }
 
@@ -264,8 +266,13 @@ void Validate(const prgpu::SceneDescEx &sd) {
         if (m.type == prgpu::MatConductor && !((specOK(m.eta, false) && specOK(m.k, false)) || specOK(m.albedo, false)))
             bad("conductor spectra");
         if (m.type == prgpu::MatDielectric && !specOK(m.eta, false)) bad("dielectric IOR");
-        if (m.type == prgpu::MatCoatedDiffuse || m.type == prgpu::MatCoatedConductor)
-            bad("coated materials are not supported by this plugin version");
+        if (m.type == prgpu::MatCoatedDiffuse && !(specOK(m.albedo, false) && m.albedoB < sd.spectrumCount))
+            bad("coated diffuse reflectance");
+        if (m.type == prgpu::MatCoatedConductor && !((specOK(m.eta, false) && specOK(m.k, false)) || specOK(m.albedo, false)))
+            bad("coated conductor spectra");
+        if (is_coated(m) && !(specOK(m.coatEta, false) && specOK(m.coatAlbedo, true))) bad("coat spectra");
+        if (m.sheenWeight > 0 && !specOK(m.sheenColor, true)) bad("sheen colour");
+        if (m.type > prgpu::MatCoatedConductor) bad("material type");
     }
     for (int i = 0; i < sd.spectrumCount; ++i) {
         const prgpu::SpectrumEx &s = sd.spectra[i];
@@ -305,8 +312,21 @@ template <class T> void UploadOrDummy(DevBuf &b, const T *src, int count, hipStr
     }
 }
 
-void EnsurePTScene(Device &dev, const prgpu::SceneDescEx &sd) {
-    uint64_t key = PTKey(sd);
+// The sheen albedo table of the CPU path tracer, built once per process.
+const std::vector<float> &HostSheenTable() {
+    static const std::vector<float> t = [] {
+        std::vector<float> v(size_t(prgpu::kSheenAlpha) * prgpu::kSheenMu);
+        SheenAlbedoTable(v.data());
+        return v;
+    }();
+    return t;
+}
+
+void EnsurePTScene(Device &dev, const prgpu::SceneDescEx &sd, const float *sheenTable) {
+    // The traversal reads the device-resident BVH: it must be this scene's.
+    if (!dev.nodes.p || dev.geomKey != GeometryKey(sd.prims, sd.primCount))
+        throw std::runtime_error("internal error: the scene's geometry is not on the device");
+    uint64_t key = PTKey(sd, sheenTable);
     if (dev.pt && dev.ptKey == key) return;
     Validate(sd);
     dev.guide.reset();  // a model learned for another scene
@@ -323,6 +343,10 @@ void EnsurePTScene(Device &dev, const prgpu::SceneDescEx &sd) {
     grid.push_back(0.f);
     pt->grid.upload(grid.data(), grid.size(), s);
     pt->cieYInt = host_cie_y_integral();
+    pt->sheen.upload(sheenTable ? sheenTable : HostSheenTable().data(), size_t(prgpu::kSheenAlpha) * prgpu::kSheenMu, s);
+    pt->hasLayered = 0;
+    for (int i = 0; i < sd.materialCount; ++i)
+        pt->hasLayered |= (is_coated(sd.materials[i]) || has_sheen(sd.materials[i])) ? 1 : 0;
 
     // World sphere: the primitives' bounds joined with the camera position (src/scene/scene.cpp:9-27).
     Box3 b(V3(dev.sceneLo[0], dev.sceneLo[1], dev.sceneLo[2]), V3(dev.sceneHi[0], dev.sceneHi[1], dev.sceneHi[2]));
@@ -504,6 +528,7 @@ DevScene MakeDevScene(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::Re
     s.cameraMedium = sd.cameraMedium;
     s.worldR = pt.worldR;
     s.hasInterfaces = pt.hasInterfaces;
+    s.sheenTable = pt.sheen.as<float>();
     std::memcpy(s.c2w, sd.camera.cameraToWorld, sizeof(s.c2w));
     s.tanHalf = sd.camera.tanHalfFov;
     s.aspect = float(sd.camera.width) / float(sd.camera.height);
@@ -562,8 +587,8 @@ struct Bufs {
     int *q0, *q1;
     int *qm;     // material queues, QCount x qcap
     int qcap;
-    int *counts;  // [0] queue A, [1] queue B, [2] shadow queue, [3..5] material queues, [6..7] ray fetch,
-                  // [8] unused, [9] emission queue
+    int *counts;  // [0] queue A, [1] queue B, [2] shadow queue, [3..5] plain material queues, [6..7] ray
+                  // fetch, [8] unused, [9] emission queue, [10] coated / sheen material queue
     float2 *pfilm;
     float4 *xyz;
     // shadow records
@@ -736,8 +761,8 @@ __device__ __forceinline__ bool roulette(int rr, int rrStart, PathRegs &r) {
 
 // Next-event estimation up to the transmittance (path.cpp:82-111): samples a light and, when
 // the unoccluded contribution is nonzero, appends a shadow record that k_shadow completes.
-template <bool kMedia>
-__device__ __forceinline__ void sample_ld(const DevScene &s, const Bufs &b, int slot, PathRegs &r, V3 p, V3 n, V3 wo, const BSDF *bsdf,
+template <bool kMedia, class B = BSDF>
+__device__ __forceinline__ void sample_ld(const DevScene &s, const Bufs &b, int slot, PathRegs &r, V3 p, V3 n, V3 wo, const B *bsdf,
                           float g, V3 ns, bool surface, int mIn, int mOut, const GuideMix *gm = nullptr) {
     float uLight = r.rng.uniform();
     V2 uL = get2(r.rng);
@@ -795,7 +820,9 @@ __device__ __forceinline__ void add_emitted(const DevScene &s, PathRegs &r, int 
 // queue per BSDF family together with their hit record; paths that meet emission (an emitting
 // primitive, or escape with an environment light) are appended to the emission queue, whose
 // MIS-weighted contributions k_emit adds before the BSDF stage changes the throughput.
-enum MatQueue : int { QDiffuse = 0, QConductor = 1, QDielectric = 2, QCount = 3 };
+enum MatQueue : int { QDiffuse = 0, QConductor = 1, QDielectric = 2, QLayered = 3, QCount = 4 };
+// Index in Bufs::counts of each material queue's length.
+__device__ __host__ constexpr int mat_count_index(int q) { return q == QLayered ? 10 : 3 + q; }
 
 template <bool kMedia, bool kInterfaces>
 __global__ __launch_bounds__(kBlock) void k_surface(const DevScene *__restrict__ sp, Params p, Bufs b, const int *queue,
@@ -863,7 +890,7 @@ __global__ __launch_bounds__(kBlock) void k_surface(const DevScene *__restrict__
                     V3 mp = r.ro + r.rd * ds.t;
                     V3 mwo = -r.rd;
                     float g = clampf(s.media[r.medium].g, -0.99f, 0.99f);
-                    sample_ld<kMedia>(s, b, slot, r, mp, V3(), mwo, nullptr, g, V3(), false, -1, -1);
+                    sample_ld<kMedia>(s, b, slot, r, mp, V3(), mwo, static_cast<const BSDF *>(nullptr), g, V3(), false, -1, -1);
                     float phasePdf;
                     V3 wi = sample_hg(mwo, g, get2(r.rng), &phasePdf);
                     if (phasePdf == 0) {
@@ -909,16 +936,19 @@ __global__ __launch_bounds__(kBlock) void k_surface(const DevScene *__restrict__
                     alive = false;
                 } else {
                     const prgpu::Material &m = s.mats[pr.material];
-                    // A dispersive dielectric keeps only the hero wavelength (material.cpp:106-110),
-                    // also when the depth limit ends the path here.
+                    // A dispersive dielectric or coat keeps only the hero wavelength
+                    // (material.cpp:106-110, 120-124), also when the depth limit ends the path here.
                     if (m.type == prgpu::MatDielectric && !spec_is_constant(s.spec, m.eta)) r.lam.terminate_secondary();
+                    if (is_coated(m) && !spec_is_constant(s.spec, m.coatEta)) r.lam.terminate_secondary();
                     if (r.depth++ >= p.maxDepth) {
                         alive = false;
                     } else {
                         Surf si;
                         interaction(pr, r.ro, r.rd, tHit, hv.z, hv.w, &si);
                         toMaterial = true;
-                        mq = m.type == prgpu::MatDiffuse ? QDiffuse : (m.type == prgpu::MatConductor ? QConductor : QDielectric);
+                        mq = (is_coated(m) || has_sheen(m)) ? QLayered
+                             : m.type == prgpu::MatDiffuse ? QDiffuse
+                             : (m.type == prgpu::MatConductor ? QConductor : QDielectric);
                         uint32_t media = uint32_t(pr.mediumInside & 0xffff) | (uint32_t(pr.mediumOutside & 0xffff) << 16);
                         b.h0[slot] = make_float4(si.p.x, si.p.y, si.p.z, __int_as_float(pr.material));
                         b.h1[slot] = make_float4(si.n.x, si.n.y, si.n.z, si.u);
@@ -942,10 +972,11 @@ __global__ __launch_bounds__(kBlock) void k_surface(const DevScene *__restrict__
             b.o[slot] = make_float4(r.ro.x, r.ro.y, r.ro.z, r.prevPdf);
             b.d[slot] = make_float4(r.rd.x, r.rd.y, r.rd.z, r.etaScale);
         }
-        const int target = toMaterial ? mq : (alive ? QCount : -1);
-        if (target == QDiffuse) b.qm[QDiffuse * b.qcap + wave_append(&b.counts[3 + QDiffuse])] = slot;
-        if (target == QConductor) b.qm[QConductor * b.qcap + wave_append(&b.counts[3 + QConductor])] = slot;
-        if (target == QDielectric) b.qm[QDielectric * b.qcap + wave_append(&b.counts[3 + QDielectric])] = slot;
+        const int target = toMaterial ? mq : (alive ? QCount : -1);  // QCount: straight to the next bounce
+        if (target == QDiffuse) b.qm[QDiffuse * b.qcap + wave_append(&b.counts[mat_count_index(QDiffuse)])] = slot;
+        if (target == QConductor) b.qm[QConductor * b.qcap + wave_append(&b.counts[mat_count_index(QConductor)])] = slot;
+        if (target == QDielectric) b.qm[QDielectric * b.qcap + wave_append(&b.counts[mat_count_index(QDielectric)])] = slot;
+        if (target == QLayered) b.qm[QLayered * b.qcap + wave_append(&b.counts[mat_count_index(QLayered)])] = slot;
         if (target == QCount) nextQueue[wave_append(nextCount)] = slot;
     }
 }
@@ -999,7 +1030,7 @@ template <int kType, bool kMedia, bool kGuided = false>
 __global__ __launch_bounds__(kBlock) void k_material(const DevScene *__restrict__ sp, Params p, Bufs b, int *nextQueue,
                                                      int *nextCount) {
     const DevScene &s = *sp;
-    const int n = b.counts[3 + kType];
+    const int n = b.counts[mat_count_index(kType)];
     const int *queue = b.qm + kType * b.qcap;
     const int stride = int(gridDim.x * blockDim.x);
     for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += stride) {
@@ -1029,9 +1060,12 @@ __global__ __launch_bounds__(kBlock) void k_material(const DevScene *__restrict_
         const prgpu::Material &m = s.mats[__float_as_int(h0.w)];
         uint32_t media = __float_as_uint(h3.w);
         int mIn = int(int16_t(media & 0xffff)), mOut = int(int16_t(media >> 16));
-        BSDF bsdf = eval_surface<kType>(s, m, si, r.lam);
+        auto bsdf = [&] {
+            if constexpr (kType == QLayered) return eval_closure(s, m, si, r.lam);
+            else return eval_surface<kType>(s, m, si, r.lam);
+        }();
         bool alive = true;
-        if (kGuided) {
+        if constexpr (kGuided) {
             // One-sample MIS between the learned mixture and the BSDF (q = s p_guide + (1 - s) p_bsdf).
             const GuideMix gm = guide_mix(b.gout + size_t(i) * 12);
             const SS beta0 = r.beta;
@@ -1089,6 +1123,9 @@ __global__ __launch_bounds__(kBlock) void k_material(const DevScene *__restrict_
             } else {
                 r.beta = r.beta * (bs.f * (absdot(bs.wi, si.ns) / bs.pdf));
                 r.prevPdf = bs.pdf;
+                // A stochastic closure's sampling pdf is only proportional: MIS uses its pdf (path.cpp:220).
+                if constexpr (kType == QLayered)
+                    if (bs.proportional) r.prevPdf = bsdf.pdf(si.wo, bs.wi);
                 r.flags = (bs.flags & kSpecular) ? (r.flags | kSpecularFlag) : (r.flags & ~kSpecularFlag);
                 if (bs.flags & kTrans) r.etaScale *= sq(bs.eta);
                 r.ro = offset_origin(si.p, si.n, bs.wi);
@@ -1359,7 +1396,7 @@ void EnsureBatch(PTScene &pt, size_t n) {
     f4(pt.o); f4(pt.d); f4(pt.lam); f4(pt.lpdf); f4(pt.beta); f4(pt.rpath); f4(pt.L); f4(pt.Lh);
     f4(pt.prevP); f4(pt.prevN); f4(pt.hit); f4(pt.xyz);
     f4(pt.h0); f4(pt.h1); f4(pt.h2); f4(pt.h3); f4(pt.qe); f4(pt.qeLpdf);
-    pt.qm.ensure(n * 3 * sizeof(int));
+    pt.qm.ensure(n * QCount * sizeof(int));
     f4(pt.so); f4(pt.sd); f4(pt.sp); f4(pt.sn); f4(pt.sf); f4(pt.sL); f4(pt.sb);
     pt.rng.ensure(n * sizeof(ulonglong2));
     pt.st.ensure(n * sizeof(int4));
@@ -1501,7 +1538,7 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
     while (active > 0) {
         const int nxtIdx = 1 - curIdx;
         PG_CHECK(hipMemsetAsync(b.counts + nxtIdx, 0, sizeof(int), st));
-        PG_CHECK(hipMemsetAsync(b.counts + 2, 0, 8 * sizeof(int), st));  // shadow, material, fetch, emission
+        PG_CHECK(hipMemsetAsync(b.counts + 2, 0, 9 * sizeof(int), st));  // shadow, material, fetch, emission
         const int tgrid = TraceGrid(dev, active);
         timer.Run(prgpu::KernIntersect, active, nullptr, [&] {
             if (stats)
@@ -1523,7 +1560,7 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
         });
         if (guided)
             timer.Run(prgpu::KernGuide, 0, nullptr, [&] {
-                dev.guide->Infer(b.qm + size_t(QDiffuse) * b.qcap, b.counts + 3 + QDiffuse, active, b.h0, b.h1, b.d, p.box,
+                dev.guide->Infer(b.qm + size_t(QDiffuse) * b.qcap, b.counts + mat_count_index(QDiffuse), active, b.h0, b.h1, b.d, p.box,
                                  dev.cus, st);
             });
         timer.Run(prgpu::KernMaterial, active, nullptr, [&] {
@@ -1534,6 +1571,10 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
             hipLaunchKernelGGL(kd, dim3(grid), dim3(kBlock), 0, st, ds, p, b, nxt, b.counts + nxtIdx);
             hipLaunchKernelGGL(kc, dim3(grid), dim3(kBlock), 0, st, ds, p, b, nxt, b.counts + nxtIdx);
             hipLaunchKernelGGL(ke, dim3(grid), dim3(kBlock), 0, st, ds, p, b, nxt, b.counts + nxtIdx);
+            if (pt.hasLayered) {
+                auto *kl = media ? k_material<QLayered, true> : k_material<QLayered, false>;
+                hipLaunchKernelGGL(kl, dim3(grid), dim3(kBlock), 0, st, ds, p, b, nxt, b.counts + nxtIdx);
+            }
         });
         timer.Run(prgpu::KernShadow, 0, &pt.countsHost[2], [&] {
             auto *k = stats ? (media ? (interfaces ? k_shadow<true, true, true> : k_shadow<true, true, false>)
@@ -1577,9 +1618,9 @@ int PassesPerBatch(int npix, int spp) {
 }  // namespace
 
 void RenderExImpl(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::RenderParamsEx &rp, double *sum,
-                  double *sumsq) {
+                  double *sumsq, const float *sheenTable) {
     if (rp.spp <= 0) return;
-    EnsurePTScene(dev, sd);
+    EnsurePTScene(dev, sd, sheenTable);
     DevScene s = MakeDevScene(dev, sd, rp);
     const DevScene *ds = UploadDevScene(dev, s);
     const int sw = s.width + 2 * s.margin, sh = s.height + 2 * s.margin, npix = sw * sh;
@@ -1662,7 +1703,7 @@ void RenderExImpl(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::Render
 
 void SampleRadianceImpl(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::RenderParamsEx &rp,
                         prgpu::SampleRecordEx *out) {
-    EnsurePTScene(dev, sd);
+    EnsurePTScene(dev, sd, nullptr);
     DevScene s = MakeDevScene(dev, sd, rp);
     const DevScene *ds = UploadDevScene(dev, s);
     const int sw = s.width + 2 * s.margin, sh = s.height + 2 * s.margin, npix = sw * sh;

@@ -129,6 +129,7 @@ struct DevScene {
     int cameraMedium;
     float worldR;
     int hasInterfaces;
+    const float *sheenTable;  // sheen directional albedo, 16 alphas x 32 cosines
     // camera and film
     float c2w[12];
     float tanHalf, aspect, lensR, focal;
@@ -353,12 +354,18 @@ PG_HD V3 sample_hg(V3 wo, float g, V2 u, float *pdf) {  // bxdf.cpp:69-80
     return wi;
 }
 
+// Sample flags (reflection / transmission) and transport mode (src/materials/bxdf.h:35-40). The
+// mode only changes dielectric transmission (the 1/eta'^2 radiance scaling); the layered walk
+// samples its exit interface in the adjoint (importance) mode.
+constexpr uint32_t kRtRefl = 1, kRtTrans = 2, kRtAll = 3;
+
 struct BSample {
     SS f;
     V3 wi;
     float pdf;
     uint32_t flags;
     float eta;
+    bool proportional = false;  // pdf only proportional to the sampling density (layered, sheen)
 };
 
 struct BxDF {
@@ -376,7 +383,7 @@ struct BxDF {
         default: return 0u;
         }
     }
-    PG_HD SS f(V3 wo, V3 wi) const {
+    PG_HD SS f(V3 wo, V3 wi, bool radiance = true) const {
         if (type == BxDiffuse) {
             if (!same_hemi(wo, wi)) return SS(0.f);
             return a * kInvPi;
@@ -405,18 +412,18 @@ struct BxDF {
             if (refl) return SS(dist.D(wm) * dist.G(wo, wi) * F / fabsf(4 * ci * co));
             float denom = sq(dot(wi, wm) + dot(wo, wm) / etap) * ci * co;
             float ft = dist.D(wm) * (1 - F) * dist.G(wo, wi) * fabsf(dot(wi, wm) * dot(wo, wm) / denom);
-            ft /= sq(etap);
+            if (radiance) ft /= sq(etap);
             return SS(ft);
         }
         return SS(0.f);
     }
-    PG_HD float pdf(V3 wo, V3 wi) const {
+    PG_HD float pdf(V3 wo, V3 wi, uint32_t rt = kRtAll) const {
         if (type == BxDiffuse) {
-            if (!same_hemi(wo, wi)) return 0;
+            if (!(rt & kRtRefl) || !same_hemi(wo, wi)) return 0;
             return abscos(wi) * kInvPi;
         }
         if (type == BxConductor) {
-            if (!same_hemi(wo, wi) || dist.smooth()) return 0;
+            if (!(rt & kRtRefl) || !same_hemi(wo, wi) || dist.smooth()) return 0;
             V3 wm = wo + wi;
             if (len2(wm) == 0) return 0;
             wm = face_forward(normalize(wm), V3(0, 0, 1));
@@ -433,7 +440,7 @@ struct BxDF {
             wm = face_forward(normalize(wm), V3(0, 0, 1));
             if (dot(wm, wi) * ci < 0 || dot(wm, wo) * co < 0) return 0;
             float R = fr_dielectric(dot(wo, wm), etaD), T = 1 - R;
-            float pr = R, pt = T;
+            float pr = (rt & kRtRefl) ? R : 0.f, pt = (rt & kRtTrans) ? T : 0.f;
             if (pr == 0 && pt == 0) return 0;
             if (refl) return dist.pdf(wo, wm) / (4 * absdot(wo, wm)) * pr / (pr + pt);
             float denom = sq(dot(wi, wm) + dot(wo, wm) / etap);
@@ -442,14 +449,16 @@ struct BxDF {
         }
         return 0;
     }
-    PG_HD bool sample(V3 wo, float uc, V2 u, BSample *s) const {
+    PG_HD bool sample(V3 wo, float uc, V2 u, BSample *s, bool radiance = true, uint32_t rt = kRtAll) const {
         if (type == BxDiffuse) {
+            if (!(rt & kRtRefl)) return false;
             V3 wi = cosine_hemisphere(u);
             if (wo.z < 0) wi.z *= -1;
             *s = {a * kInvPi, wi, abscos(wi) * kInvPi, kDiffuse | kRefl, 1};
             return true;
         }
         if (type == BxConductor) {
+            if (!(rt & kRtRefl)) return false;
             if (dist.smooth()) {
                 V3 wi(-wo.x, -wo.y, wo.z);
                 *s = {fr_complex(abscos(wi), a, b) / abscos(wi), wi, 1, kSpecular | kRefl, 1};
@@ -469,7 +478,7 @@ struct BxDF {
         if (type == BxDielectric) {
             if (etaD == 1 || dist.smooth()) {
                 float R = fr_dielectric(wo.z, etaD), T = 1 - R;
-                float pr = R, pt = T;
+                float pr = (rt & kRtRefl) ? R : 0.f, pt = (rt & kRtTrans) ? T : 0.f;
                 if (pr == 0 && pt == 0) return false;
                 if (uc < pr / (pr + pt)) {
                     V3 wi(-wo.x, -wo.y, wo.z);
@@ -480,12 +489,12 @@ struct BxDF {
                 float etap;
                 if (!refract(wo, V3(0, 0, 1), etaD, &etap, &wi)) return false;
                 SS ft(T / abscos(wi));
-                *s = {ft / sq(etap), wi, pt / (pr + pt), kSpecular | kTrans, etap};
+                *s = {radiance ? ft / sq(etap) : ft, wi, pt / (pr + pt), kSpecular | kTrans, etap};
                 return true;
             }
             V3 wm = dist.sample_wm(wo, u);
             float R = fr_dielectric(dot(wo, wm), etaD), T = 1 - R;
-            float pr = R, pt = T;
+            float pr = (rt & kRtRefl) ? R : 0.f, pt = (rt & kRtTrans) ? T : 0.f;
             if (pr == 0 && pt == 0) return false;
             if (uc < pr / (pr + pt)) {
                 V3 wi = reflect(wo, wm);
@@ -502,7 +511,7 @@ struct BxDF {
             float dwm_dwi = absdot(wi, wm) / denom;
             float p = dist.pdf(wo, wm) * dwm_dwi * pt / (pr + pt);
             SS ft(T * dist.D(wm) * dist.G(wo, wi) * fabsf(dot(wi, wm) * dot(wo, wm) / (wi.z * wo.z * denom)));
-            *s = {ft / sq(etap), wi, p, kGlossy | kTrans, etap};
+            *s = {radiance ? ft / sq(etap) : ft, wi, p, kGlossy | kTrans, etap};
             return true;
         }
         if (type == BxThin) {
@@ -511,7 +520,7 @@ struct BxDF {
                 R += sq(T) * R / (1 - sq(R));
                 T = 1 - R;
             }
-            float pr = R, pt = T;
+            float pr = (rt & kRtRefl) ? R : 0.f, pt = (rt & kRtTrans) ? T : 0.f;
             if (pr == 0 && pt == 0) return false;
             if (uc < pr / (pr + pt)) {
                 V3 wi(-wo.x, -wo.y, wo.z);
@@ -550,6 +559,391 @@ struct BSDF {  // src/materials/bsdf.h:8-51
     PG_HD bool non_specular() const { return (bx.flags() & (kDiffuse | kGlossy)) != 0; }
 };
 
+// ---- coated and sheen closures -------------------------------------------------------------------
+// FP32 results of exp and log rounded from double precision: these agree with the CPU's
+// correctly rounded expf / logf where the single-precision device routines can differ in the
+// last place, and the layered walk's random stream is hashed from the directions it produces.
+PG_HD float exp_f32(float x) { return float(exp(double(x))); }
+PG_HD float log_f32(float x) { return float(log(double(x))); }
+
+PG_HD bool usable(const BSample &s) { return s.f.nonzero() && s.pdf != 0 && s.wi.z != 0; }
+
+// Stochastic layered closure (reference src/materials/bxdf.cpp:260-517, pbrt-v4's position-free
+// random walk): a dielectric coat over a diffuse or conductor base, with a slab of unit
+// extinction and albedo `albedo` between them. Its random stream is hashed from the arguments
+// (f: (0, wo), wi; pdf: (0, wi), wo; sample: (0, wo), (uc, u)), so f, pdf and sample are
+// deterministic; every 2D draw takes its second coordinate first, the order the reference's
+// compiled Vec2f(r(), r()) evaluates in.
+struct Layered {
+    BxDF top, bottom;
+    SS albedo;
+    float thick, g;
+    int maxDepth, nSamples;
+
+    struct Walk {
+        Pcg rng;
+        PG_HD float u() { return minf(rng.uniform(), kOneMinusEps); }
+        PG_HD V2 u2() {
+            float y = u();
+            float x = u();
+            return {x, y};
+        }
+    };
+    PG_HD static Walk walk(V3 a, uint64_t offset) {  // RNG(Hash(0ull, a), offset)
+        const uint64_t w[2] = {0ull, fbits(a.x, a.y)};
+        Walk r;
+        r.rng.set_sequence(murmur_words(w, 2, true, __float_as_uint(a.z)), offset);
+        return r;
+    }
+    PG_HD static uint64_t hash_dir(V3 a) {
+        const uint64_t w[1] = {fbits(a.x, a.y)};
+        return murmur_words(w, 1, true, __float_as_uint(a.z));
+    }
+    PG_HD static float tr(float dz, V3 w) {
+        if (fabsf(dz) <= 1.17549435e-38f) return 1;
+        return exp_f32(-fabsf(dz / w.z));
+    }
+    PG_HD static float free_flight(float u, V3 w) { return -log_f32(1 - u) / (1.f / fabsf(w.z)); }
+
+    PG_HD uint32_t flags() const {
+        uint32_t t = top.flags(), b = bottom.flags(), f = kRefl;
+        if (t & kSpecular) f |= kSpecular;
+        if ((t & kDiffuse) || (b & kDiffuse) || albedo.nonzero()) f |= kDiffuse;
+        else if ((t & kGlossy) || (b & kGlossy)) f |= kGlossy;
+        if ((t & kTrans) && (b & kTrans)) f |= kTrans;
+        return f;
+    }
+
+    PG_HD SS f(V3 wo, V3 wi) const {  // radiance transport
+        if (wo.z < 0) {  // two-sided
+            wo = -wo;
+            wi = -wi;
+        }
+        const bool same = same_hemi(wo, wi);
+        // Entered through the coat; the walk leaves through the interface on wi's side.
+        const BxDF &exitI = same ? top : bottom, &inner = same ? bottom : top;
+        const float exitZ = same ? thick : 0.f;
+        const bool exitSpec = (exitI.flags() & kSpecular) != 0, innerSpec = (inner.flags() & kSpecular) != 0;
+        SS f(0.f);
+        if (same) f = top.f(wo, wi) * float(nSamples);
+        Walk w = walk(wo, hash_dir(wi));
+        for (int s = 0; s < nSamples; ++s) {
+            BSample wos, wis;
+            float uc = w.u();
+            V2 u0 = w.u2();
+            if (!top.sample(wo, uc, u0, &wos, true, kRtTrans) || !usable(wos)) continue;
+            uc = w.u();
+            V2 u1 = w.u2();
+            if (!exitI.sample(wi, uc, u1, &wis, false, kRtTrans) || !usable(wis)) continue;
+            SS beta = wos.f * abscos(wos.wi) / wos.pdf;
+            float z = thick;
+            V3 d = wos.wi;
+            for (int depth = 0; depth < maxDepth; ++depth) {
+                if (depth > 3 && beta.maxc() < 0.25f) {
+                    float q = maxf(0.f, 1 - beta.maxc());
+                    if (w.u() < q) break;
+                    beta = beta / (1 - q);
+                }
+                if (!albedo.nonzero()) {
+                    z = (z == thick) ? 0 : thick;
+                    beta = beta * tr(thick, d);
+                } else {
+                    const float dz = free_flight(w.u(), d);
+                    const float zp = d.z > 0 ? (z + dz) : (z - dz);
+                    if (0 < zp && zp < thick) {
+                        float wt = 1;
+                        float phaseExit = hg_phase(dot(-d, -wis.wi), g);
+                        if (!exitSpec) wt = power_heuristic(wis.pdf, phaseExit);
+                        f = f + beta * albedo * phaseExit * wt * tr(zp - exitZ, wis.wi) * wis.f / wis.pdf;
+                        float phasePdf;
+                        V3 wp = sample_hg(-d, g, w.u2(), &phasePdf);
+                        if (phasePdf == 0 || wp.z == 0) continue;
+                        beta = beta * albedo;
+                        d = wp;
+                        z = zp;
+                        if (((z < exitZ && d.z > 0) || (z > exitZ && d.z < 0)) && !exitSpec) {
+                            SS fExit = exitI.f(-d, wi);
+                            if (fExit.nonzero()) {
+                                float wt2 = power_heuristic(phasePdf, exitI.pdf(-d, wi, kRtTrans));
+                                f = f + beta * tr(zp - exitZ, wp) * fExit * wt2;
+                            }
+                        }
+                        continue;
+                    }
+                    z = clampf(zp, 0.f, thick);
+                }
+                BSample bs;
+                if (z == exitZ) {
+                    float uc2 = w.u();
+                    V2 u2 = w.u2();
+                    if (!exitI.sample(-d, uc2, u2, &bs, true, kRtRefl) || !usable(bs)) break;
+                    beta = beta * (bs.f * abscos(bs.wi) / bs.pdf);
+                    d = bs.wi;
+                } else {
+                    if (!innerSpec) {
+                        float wt = 1;
+                        if (!exitSpec) wt = power_heuristic(wis.pdf, inner.pdf(-d, -wis.wi));
+                        f = f + beta * inner.f(-d, -wis.wi) * abscos(wis.wi) * wt * tr(thick, wis.wi) * wis.f / wis.pdf;
+                    }
+                    float uc2 = w.u();
+                    V2 u2 = w.u2();
+                    if (!inner.sample(-d, uc2, u2, &bs, true, kRtRefl) || !usable(bs)) break;
+                    beta = beta * (bs.f * abscos(bs.wi) / bs.pdf);
+                    d = bs.wi;
+                    if (!exitSpec) {
+                        SS fExit = exitI.f(-d, wi);
+                        if (fExit.nonzero()) {
+                            float wt = 1;
+                            if (!innerSpec) wt = power_heuristic(bs.pdf, exitI.pdf(-d, wi, kRtTrans));
+                            f = f + beta * tr(thick, bs.wi) * fExit * wt;
+                        }
+                    }
+                }
+            }
+        }
+        return f / float(nSamples);
+    }
+
+    PG_HD bool sample(V3 wo, float uc, V2 u, BSample *out) const {
+        const bool flip = wo.z < 0;
+        if (flip) wo = -wo;
+        BSample bs;
+        if (!top.sample(wo, uc, u, &bs) || !usable(bs)) return false;
+        if (bs.flags & kRefl) {
+            if (flip) bs.wi = -bs.wi;
+            bs.proportional = true;
+            *out = bs;
+            return true;
+        }
+        V3 d = bs.wi;
+        bool specPath = (bs.flags & kSpecular) != 0;
+        const uint64_t uw[1] = {fbits(uc, u.x)};
+        Walk w = walk(wo, murmur_words(uw, 1, true, __float_as_uint(u.y)));
+        SS f = bs.f * abscos(bs.wi);
+        float pdf = bs.pdf;
+        float z = thick;
+        for (int depth = 0; depth < maxDepth; ++depth) {
+            float rrBeta = f.maxc() / pdf;
+            if (depth > 3 && rrBeta < 0.25f) {
+                float q = maxf(0.f, 1 - rrBeta);
+                if (w.u() < q) return false;
+                pdf *= 1 - q;
+            }
+            if (d.z == 0) return false;
+            if (albedo.nonzero()) {
+                const float dz = free_flight(w.u(), d);
+                const float zp = d.z > 0 ? (z + dz) : (z - dz);
+                if (zp == z) return false;
+                if (0 < zp && zp < thick) {
+                    float phasePdf;
+                    V3 wp = sample_hg(-d, g, w.u2(), &phasePdf);
+                    if (phasePdf == 0 || wp.z == 0) return false;
+                    f = f * (albedo * phasePdf);
+                    pdf *= phasePdf;
+                    specPath = false;
+                    d = wp;
+                    z = zp;
+                    continue;
+                }
+                z = clampf(zp, 0.f, thick);
+            } else {
+                z = (z == thick) ? 0 : thick;
+                f = f * tr(thick, d);
+            }
+            const BxDF &iface = (z == 0) ? bottom : top;
+            float uc2 = w.u();
+            V2 u2 = w.u2();
+            BSample b2;
+            if (!iface.sample(-d, uc2, u2, &b2) || !usable(b2)) return false;
+            f = f * b2.f;
+            pdf *= b2.pdf;
+            specPath = specPath && (b2.flags & kSpecular);
+            d = b2.wi;
+            if (b2.flags & kTrans) {
+                uint32_t fl = (same_hemi(wo, d) ? kRefl : kTrans) | (specPath ? kSpecular : kGlossy);
+                if (flip) d = -d;
+                *out = {f, d, pdf, fl, 1.f, true};
+                return true;
+            }
+            f = f * abscos(b2.wi);
+        }
+        return false;
+    }
+
+    PG_HD float pdf(V3 wo, V3 wi) const {
+        if (wo.z < 0) {
+            wo = -wo;
+            wi = -wi;
+        }
+        Walk w = walk(wi, hash_dir(wo));
+        const bool same = same_hemi(wo, wi);
+        float sum = 0;
+        if (same) sum += nSamples * top.pdf(wo, wi, kRtRefl);
+        for (int s = 0; s < nSamples; ++s) {
+            BSample wos, wis;
+            if (same) {
+                // Transmission through the coat, one bounce off the base.
+                float a = w.u();
+                V2 ua = w.u2();
+                const bool okO = top.sample(wo, a, ua, &wos, true, kRtTrans);
+                float b = w.u();
+                V2 ub = w.u2();
+                const bool okI = top.sample(wi, b, ub, &wis, false, kRtTrans);
+                if (okO && wos.f.nonzero() && wos.pdf > 0 && okI && wis.f.nonzero() && wis.pdf > 0) {
+                    if (!(top.flags() & (kDiffuse | kGlossy))) {
+                        sum += bottom.pdf(-wos.wi, -wis.wi);
+                    } else {
+                        float c = w.u();
+                        V2 uc2 = w.u2();
+                        BSample rs;
+                        if (bottom.sample(-wos.wi, c, uc2, &rs) && rs.f.nonzero() && rs.pdf > 0) {
+                            if (!(bottom.flags() & (kDiffuse | kGlossy))) {
+                                sum += top.pdf(-rs.wi, wi);
+                            } else {
+                                float rPdf = bottom.pdf(-wos.wi, -wis.wi);
+                                sum += power_heuristic(wis.pdf, rPdf) * rPdf;
+                                float tPdf = top.pdf(-rs.wi, wi);
+                                sum += power_heuristic(rs.pdf, tPdf) * tPdf;
+                            }
+                        }
+                    }
+                }
+            } else {
+                // Transmission through both interfaces.
+                float a = w.u();
+                V2 ua = w.u2();
+                if (!top.sample(wo, a, ua, &wos) || !usable(wos) || (wos.flags & kRefl)) continue;
+                float b = w.u();
+                V2 ub = w.u2();
+                if (!bottom.sample(wi, b, ub, &wis, false) || !usable(wis) || (wis.flags & kRefl)) continue;
+                if (top.flags() & kSpecular) sum += bottom.pdf(-wos.wi, wi);
+                else if (bottom.flags() & kSpecular) sum += top.pdf(wo, -wis.wi);
+                else sum += (top.pdf(wo, -wis.wi) + bottom.pdf(-wos.wi, wi)) / 2;
+            }
+        }
+        return lerp(0.9f, 1 / (4 * kPi), sum / nSamples);
+    }
+};
+
+// "Charlie" sheen (reference src/materials/bxdf.cpp:519-631) and its directional albedo table
+// (16 alphas x 32 cosines, bilinear).
+constexpr int kSheenMuN = 32, kSheenAlphaN = 16;
+PG_HD float sheen_value(V3 wo, V3 wi, float alpha) {
+    if (!same_hemi(wo, wi)) return 0;
+    float co = abscos(wo), ci = abscos(wi);
+    if (co == 0 || ci == 0) return 0;
+    V3 wh = normalize(wo + wi);
+    float c = abscos(wh), inv = 1 / alpha;
+    float sin2 = maxf(0.f, 1 - c * c);
+    float D = (2 + inv) * float(pow(double(sin2), double(0.5f * inv))) * kInv2Pi;
+    return D / (4 * (ci + co - ci * co));
+}
+PG_HD float sheen_albedo(const float *t, float cosTheta, float alpha) {
+    float fa = (clampf(alpha, 0.05f, 1.f) - 0.05f) / 0.95f * (kSheenAlphaN - 1);
+    float fm = clampf(fabsf(cosTheta), 0.f, 1.f) * (kSheenMuN - 1);
+    int ia = int(fa), im = int(fm);
+    ia = ia < kSheenAlphaN - 2 ? ia : kSheenAlphaN - 2;
+    im = im < kSheenMuN - 2 ? im : kSheenMuN - 2;
+    float ta = fa - ia, tm = fm - im;
+    const float *r0 = t + ia * kSheenMuN + im, *r1 = r0 + kSheenMuN;
+    return lerp(ta, lerp(tm, r0[0], r0[1]), lerp(tm, r1[0], r1[1]));
+}
+
+struct SheenLobe {
+    SS color;
+    float alpha, weight, colorMax;
+    const float *table;
+    PG_HD SS f(V3 wo, V3 wi) const { return color * (weight * sheen_value(wo, wi, alpha)); }
+    PG_HD float base_scale(V3 wo, V3 wi) const {
+        float e = maxf(sheen_albedo(table, wo.z, alpha), sheen_albedo(table, wi.z, alpha));
+        return maxf(0.f, 1 - weight * colorMax * e);
+    }
+    PG_HD float prob(V3 wo, uint32_t baseFlags) const {
+        float e = weight * colorMax * sheen_albedo(table, wo.z, alpha);
+        if (!baseFlags) return 1;
+        return clampf(e, 0.f, 0.9f);
+    }
+};
+
+// The closure of the coated / sheen family (src/materials/material.cpp:5-14): a plain or
+// layered base, optionally under a sheen lobe, in the shading frame. Radiance transport and all
+// sample flags, as the path tracer calls it.
+struct LBSDF {
+    Frame frame;
+    BxDF bx;
+    Layered lay;
+    SheenLobe sh;
+    bool layered, sheen;
+
+    PG_HD uint32_t base_flags() const { return layered ? lay.flags() : bx.flags(); }
+    PG_HD uint32_t flags() const { return sheen ? (base_flags() | kGlossy | kRefl) : base_flags(); }
+    PG_HD SS base_f(V3 wo, V3 wi) const { return layered ? lay.f(wo, wi) : bx.f(wo, wi); }
+    PG_HD float base_pdf(V3 wo, V3 wi) const { return layered ? lay.pdf(wo, wi) : bx.pdf(wo, wi); }
+    PG_HD bool base_sample(V3 wo, float uc, V2 u, BSample *s) const {
+        return layered ? lay.sample(wo, uc, u, s) : bx.sample(wo, uc, u, s);
+    }
+    PG_HD SS local_f(V3 wo, V3 wi) const {
+        if (!sheen) return base_f(wo, wi);
+        return sh.f(wo, wi) + base_f(wo, wi) * sh.base_scale(wo, wi);
+    }
+    PG_HD float local_pdf(V3 wo, V3 wi) const {
+        if (!sheen) return base_pdf(wo, wi);
+        float ps = sh.prob(wo, base_flags());
+        float pSheen = same_hemi(wo, wi) ? abscos(wi) * kInvPi : 0.f;
+        return ps * pSheen + (1 - ps) * base_pdf(wo, wi);
+    }
+    PG_HD bool local_sample(V3 wo, float uc, V2 u, BSample *bs) const {
+        if (!sheen) return base_sample(wo, uc, u, bs);
+        float ps = sh.prob(wo, base_flags());
+        const bool exact = !layered;
+        if (uc < ps) {
+            V3 wi = cosine_hemisphere(u);
+            if (wo.z < 0) wi.z = -wi.z;
+            if (exact) {
+                float p = local_pdf(wo, wi);
+                if (p == 0) return false;
+                *bs = {local_f(wo, wi), wi, p, kGlossy | kRefl, 1};
+                return true;
+            }
+            *bs = {sh.f(wo, wi), wi, ps * (abscos(wi) * kInvPi), kGlossy | kRefl, 1, true};
+            return true;
+        }
+        float uc2 = ps < 1 ? minf((uc - ps) / (1 - ps), kOneMinusEps) : 0.f;
+        if (!base_sample(wo, uc2, u, bs) || bs->pdf == 0) return false;
+        if ((bs->flags & kSpecular) || !exact || bs->proportional) {
+            bs->f = bs->f * sh.base_scale(wo, bs->wi);
+            bs->pdf *= (1 - ps);
+            if (!(bs->flags & kSpecular)) bs->proportional = true;
+            return true;
+        }
+        float p = local_pdf(wo, bs->wi);
+        if (p == 0) return false;
+        bs->f = local_f(wo, bs->wi);
+        bs->pdf = p;
+        return true;
+    }
+
+    PG_HD SS f(V3 woW, V3 wiW) const {
+        V3 wi = frame.to_local(wiW), wo = frame.to_local(woW);
+        if (wo.z == 0) return SS(0.f);
+        return local_f(wo, wi);
+    }
+    PG_HD float pdf(V3 woW, V3 wiW) const {
+        V3 wo = frame.to_local(woW), wi = frame.to_local(wiW);
+        if (wo.z == 0) return 0;
+        return local_pdf(wo, wi);
+    }
+    PG_HD bool sample(V3 woW, float u, V2 u2, BSample *bs) const {
+        V3 wo = frame.to_local(woW);
+        if (wo.z == 0 || !(flags() & (kRefl | kTrans))) return false;
+        if (!local_sample(wo, u, u2, bs)) return false;
+        if (!bs->f.nonzero() || bs->pdf == 0 || bs->wi.z == 0 || bs->f.has_nan() || !is_finite(bs->pdf)) return false;
+        bs->wi = frame.from_local(bs->wi);
+        return true;
+    }
+    PG_HD bool non_specular() const { return (flags() & (kDiffuse | kGlossy)) != 0; }
+};
+
 // src/materials/material.cpp:64-117 (textures: constant spectra and uv checkers).
 PG_HD int checker_pick(const prgpu::Material &m, float u, float v, int a) {
     if (m.albedoB >= 0 && m.checkerScale != 0) {
@@ -560,7 +954,46 @@ PG_HD int checker_pick(const prgpu::Material &m, float u, float v, int a) {
     return a;
 }
 
-// kFamily: -1 = any material, else the material family the caller guarantees (0 diffuse,
+PG_HD void diffuse_bx(const DevScene &s, const prgpu::Material &m, const Surf &si, const Lambda &lam, BxDF &bx) {
+    SS r = spec_sample(s.spec, checker_pick(m, si.u, si.v, m.albedo), lam);
+    for (int i = 0; i < 4; ++i) r.v[i] = clampf(r.v[i], 0.f, 1.f);
+    bx.type = BxDiffuse;
+    bx.a = r;
+}
+// Conductor closure; under a coat (coatEta != 0) its complex IOR is relative to the coat
+// (src/materials/material.cpp:140-166).
+PG_HD void conductor_bx(const DevScene &s, const prgpu::Material &m, const Surf &si, const Lambda &lam, float coatEta,
+                        BxDF &bx) {
+    bx.type = BxConductor;
+    bx.dist.init(m.alphaX, m.alphaY);
+    if (m.eta >= 0 && m.k >= 0) {
+        bx.a = spec_sample(s.spec, m.eta, lam);
+        bx.b = spec_sample(s.spec, m.k, lam);
+        if (coatEta != 0) {
+            bx.a = bx.a / coatEta;
+            bx.b = bx.b / coatEta;
+        }
+    } else {
+        SS r = spec_sample(s.spec, checker_pick(m, si.u, si.v, m.albedo), lam);
+        for (int i = 0; i < 4; ++i) r.v[i] = clampf(r.v[i], 0.f, 0.9999f);
+        bx.a = SS(1.f);
+        bx.b = ss_sqrt(r) * 2.f / ss_sqrt(ss_clamp0(SS(1.f) - r));
+    }
+}
+PG_HD void dielectric_bx(const DevScene &s, const prgpu::Material &m, Lambda &lam, BxDF &bx) {
+    float e = spec_eval(s.spec, m.eta, lam.l[0]);
+    if (!spec_is_constant(s.spec, m.eta)) lam.terminate_secondary();
+    if (e == 0) e = 1;
+    bx.etaD = e;
+    if (m.thin) {
+        bx.type = BxThin;
+    } else {
+        bx.type = BxDielectric;
+        bx.dist.init(m.alphaX, m.alphaY);
+    }
+}
+
+// kFamily: -1 = any plain material, else the material family the caller guarantees (0 diffuse,
 // 1 conductor, 2 dielectric), which removes the other families' code from the caller.
 template <int kFamily = -1>
 PG_HD BSDF eval_surface(const DevScene &s, const prgpu::Material &m, const Surf &si, Lambda &lam) {
@@ -568,34 +1001,58 @@ PG_HD BSDF eval_surface(const DevScene &s, const prgpu::Material &m, const Surf 
     BxDF &bx = b.bx;
     bx.etaD = 1;
     bx.dist.ax = bx.dist.ay = 0;
-    if (kFamily == 0 || (kFamily < 0 && m.type == prgpu::MatDiffuse)) {
-        SS r = spec_sample(s.spec, checker_pick(m, si.u, si.v, m.albedo), lam);
-        for (int i = 0; i < 4; ++i) r.v[i] = clampf(r.v[i], 0.f, 1.f);
-        bx.type = BxDiffuse;
-        bx.a = r;
-    } else if (kFamily == 1 || (kFamily < 0 && m.type == prgpu::MatConductor)) {
-        bx.type = BxConductor;
-        bx.dist.init(m.alphaX, m.alphaY);
-        if (m.eta >= 0 && m.k >= 0) {
-            bx.a = spec_sample(s.spec, m.eta, lam);
-            bx.b = spec_sample(s.spec, m.k, lam);
-        } else {
-            SS r = spec_sample(s.spec, checker_pick(m, si.u, si.v, m.albedo), lam);
-            for (int i = 0; i < 4; ++i) r.v[i] = clampf(r.v[i], 0.f, 0.9999f);
-            bx.a = SS(1.f);
-            bx.b = ss_sqrt(r) * 2.f / ss_sqrt(ss_clamp0(SS(1.f) - r));
-        }
+    if (kFamily == 0 || (kFamily < 0 && m.type == prgpu::MatDiffuse)) diffuse_bx(s, m, si, lam, bx);
+    else if (kFamily == 1 || (kFamily < 0 && m.type == prgpu::MatConductor)) conductor_bx(s, m, si, lam, 0.f, bx);
+    else dielectric_bx(s, m, lam, bx);
+    b.frame = Frame::from_xz(si.dpdus, si.ns);
+    return b;
+}
+
+PG_HD bool is_coated(const prgpu::Material &m) {
+    return m.type == prgpu::MatCoatedDiffuse || m.type == prgpu::MatCoatedConductor;
+}
+PG_HD bool has_sheen(const prgpu::Material &m) { return m.sheenWeight > 0 && m.sheenColor >= 0; }
+
+// The coated / sheen family (Material::GetBSDF, src/materials/material.cpp:5-14, 120-166): a
+// layered closure under a rough dielectric coat whose IOR is taken at the hero wavelength, or
+// a plain closure, optionally wrapped in the sheen lobe.
+PG_HD LBSDF eval_closure(const DevScene &s, const prgpu::Material &m, const Surf &si, Lambda &lam) {
+    LBSDF b;
+    b.layered = b.sheen = false;
+    b.bx.etaD = 1;
+    b.bx.dist.ax = b.bx.dist.ay = 0;
+    if (is_coated(m)) {
+        float ce = spec_eval(s.spec, m.coatEta, lam.l[0]);
+        if (!spec_is_constant(s.spec, m.coatEta)) lam.terminate_secondary();
+        if (ce == 0) ce = 1;
+        Layered &L = b.lay;
+        b.layered = true;
+        L.top.type = BxDielectric;
+        L.top.etaD = ce;
+        L.top.dist.init(m.coatAlpha, m.coatAlpha);
+        L.bottom.etaD = 1;
+        L.thick = maxf(m.thickness, 1.17549435e-38f);
+        L.g = m.coatG;
+        L.albedo = m.coatAlbedo >= 0 ? ss_clamp0(spec_sample(s.spec, m.coatAlbedo, lam)) : SS(0.f);
+        L.maxDepth = m.coatMaxDepth;
+        L.nSamples = m.coatSamples > 1 ? m.coatSamples : 1;
+        if (m.type == prgpu::MatCoatedDiffuse) diffuse_bx(s, m, si, lam, L.bottom);
+        else conductor_bx(s, m, si, lam, ce, L.bottom);
+    } else if (m.type == prgpu::MatDiffuse) {
+        diffuse_bx(s, m, si, lam, b.bx);
+    } else if (m.type == prgpu::MatConductor) {
+        conductor_bx(s, m, si, lam, 0.f, b.bx);
     } else {
-        float e = spec_eval(s.spec, m.eta, lam.l[0]);
-        if (!spec_is_constant(s.spec, m.eta)) lam.terminate_secondary();
-        if (e == 0) e = 1;
-        bx.etaD = e;
-        if (m.thin) {
-            bx.type = BxThin;
-        } else {
-            bx.type = BxDielectric;
-            bx.dist.init(m.alphaX, m.alphaY);
-        }
+        dielectric_bx(s, m, lam, b.bx);
+    }
+    if (has_sheen(m)) {
+        b.sheen = true;
+        SS c = ss_clamp0(spec_sample(s.spec, m.sheenColor, lam));
+        b.sh.color = c;
+        b.sh.alpha = clampf(m.sheenRoughness, 0.05f, 1.f);
+        b.sh.weight = clampf(m.sheenWeight, 0.f, 1.f);
+        b.sh.colorMax = clampf(c.maxc(), 0.f, 1.f);
+        b.sh.table = s.sheenTable;
     }
     b.frame = Frame::from_xz(si.dpdus, si.ns);
     return b;
@@ -857,7 +1314,6 @@ PG_HD float light_pmf(const DevScene &s, V3 p, V3 n, int li) {
 // more than 99% of arguments, so free-flight distances, and with them the ray origins that the
 // grid medium hashes into its random stream, match the CPU's far more often than with a 1-ulp
 // float log.
-PG_HD float log_f32(float x) { return float(log(double(x))); }
 
 struct DistSample {
     bool scattered;

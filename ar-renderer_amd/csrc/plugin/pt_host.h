@@ -14,8 +14,9 @@ int TraceGrid(const Device &d, int64_t n);
 void CheckTraceFlags(Device &d);
 
 void ReleasePT(Device &d);
+// sheenTable: the sheen albedo table (16 x 32) to use instead of the built-in one, or null.
 void RenderExImpl(Device &d, const prgpu::SceneDescEx &sd, const prgpu::RenderParamsEx &rp, double *sum,
-                  double *sumsq);
+                  double *sumsq, const float *sheenTable = nullptr);
 void SampleRadianceImpl(Device &d, const prgpu::SceneDescEx &sd, const prgpu::RenderParamsEx &rp,
                         prgpu::SampleRecordEx *out);
 void RenderV4Impl(Device &d, const prgpu::SceneDesc &sd, const prgpu::RenderParams &rp, float *outXYZ);

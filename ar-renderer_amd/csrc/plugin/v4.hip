@@ -105,11 +105,7 @@ void Export(const prgpu::SceneDesc &sd, V4Scene &out) {
     if (sd.spectrumCount > 0 && !sd.spectra) throw std::runtime_error("invalid scene: spectra");
     if (sd.lightCount > 0 && !sd.lights) throw std::runtime_error("invalid scene: lights");
     if (sd.mediumCount > 0 && !sd.media) throw std::runtime_error("invalid scene: media");
-    for (int i = 0; i < sd.materialCount; ++i) {
-        const prgpu::Material &m = sd.materials[i];
-        if (m.type == prgpu::MatCoatedDiffuse || m.type == prgpu::MatCoatedConductor || m.sheenWeight > 0)
-            throw std::runtime_error("coated and sheen materials are not supported by this plugin");
-    }
+    if (sd.materialCount > 0 && !sd.materials) throw std::runtime_error("invalid scene: materials");
     ExportSpectra(sd, out);
     ExportLights(sd, out);
     ExportMedia(sd, out);
@@ -148,6 +144,8 @@ void RenderV4Impl(Device &dev, const prgpu::SceneDesc &sd, const prgpu::RenderPa
     if (rp.spp <= 0) return;
     V4Scene s;
     Export(sd, s);
+    // The scene's own geometry: the traversal reads the device-resident BVH of these primitives.
+    EnsureGeometry(dev, sd.prims, sd.primCount);
     prgpu::RenderParamsEx p{};
     p.spp = rp.spp;
     p.firstSample = rp.firstSample;
@@ -159,7 +157,9 @@ void RenderV4Impl(Device &dev, const prgpu::SceneDesc &sd, const prgpu::RenderPa
     p.lightSampler = prgpu::LightSamplerBVH;
     const size_t n = size_t(sd.camera.width) * size_t(sd.camera.height) * 3;
     std::vector<double> sum(n, 0.0);
-    RenderExImpl(dev, s.desc, p, sum.data(), nullptr);
+    // Coated and sheen materials: the caller's sheen albedo table when it passes one (the
+    // reference host fills it from SheenBxDF::Albedo), else the same table built here.
+    RenderExImpl(dev, s.desc, p, sum.data(), nullptr, sd.sheenAlbedo);
     for (size_t i = 0; i < n; ++i) outXYZ[i] += float(sum[i]);
 }
 

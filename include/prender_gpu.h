@@ -317,7 +317,7 @@ PRGPU_API int prgpu_list_devices(prgpu::DeviceInfo *out, int max);
 // Renders params->spp samples per pixel and ADDS, per pixel, the sum over those samples of the
 // filter-weighted XYZ (normalised filter, norm 1 / (4 * CIE_Y_Integral)) into
 // outXYZ[3 * (y * width + x)]; the caller divides by the total spp. 0 on success, else a
-// message in err (coated and sheen materials are refused).
+// message in err. Coated and sheen materials use scene->sheenAlbedo when it is not null.
 PRGPU_API int prgpu_render(int deviceIndex, const prgpu::SceneDesc *scene, const prgpu::RenderParams *params,
                            float *outXYZ, char *err, int errLen);
 PRGPU_API int prgpu_api_version();
@@ -369,6 +369,9 @@ PRGPU_API int prgpu_guide_train(int deviceIndex, const float *records, int n, in
 // State of the device's model: stats = {summed weight, summed weighted loss} of its last training
 // step, *steps = Adam steps taken since it was created or reset. 1 when no model exists.
 PRGPU_API int prgpu_guide_stats(int deviceIndex, float *stats, int *steps);
+// Writes the sheen directional-albedo table (kSheenAlpha x kSheenMu, alpha-major) that coated and
+// sheen materials use when the scene description carries none. Host only; returns the count.
+PRGPU_API int prgpu_sheen_table(float *out);
 
 // ---- layout pins (measured on the reference's LP64 build) -----------------------------------
 static_assert(sizeof(prgpu::Float3) == 12, "Float3");

@@ -53,6 +53,15 @@ def test_versions_without_gpu():
     assert lib.prgpu_api_ext_version() == pa.EXT_VERSION
 
 
+def test_sheen_table_matches_oracle_without_gpu():
+    """The plugin's built-in sheen albedo table is the CPU path tracer's, bit for bit."""
+    import oracle as orc
+    lib = C.CDLL(pa.PLUGIN_PATH)
+    t = np.zeros(16 * 32, np.float32)
+    assert lib.prgpu_sheen_table(C.c_void_p(t.ctypes.data)) == 512
+    assert np.array_equal(t.view(np.uint32), orc.sheen_table().view(np.uint32))
+
+
 def test_struct_sizes_match_header():
     text = open(os.path.join(INCLUDE, "prender_gpu.h")).read()
     sizes = dict(re.findall(r"static_assert\(sizeof\(prgpu::(\w+)\) == (\d+)", text))

@@ -101,7 +101,7 @@ def test_resolve_srgb_of_d65_white_is_white(host):
 
 
 def test_unavailable_configurations_fail_loudly(host):
-    for cfg in (0, 6):
+    for cfg in (0, 7):
         with pytest.raises(RuntimeError):
             host.generate(cfg, 16, 16)
 
@@ -130,6 +130,19 @@ def test_configuration_5_gems_are_closed_convex_solids(host):
         verts = G.reshape(-1, 3)
         dist = np.einsum("fk,fvk->fv", n, verts[None, :, :] - G[:, 0, None, :])
         assert dist.max() <= 1e-4 * np.abs(n).max()
+
+
+def test_configuration_6_coated_and_sheen_materials(host):
+    s = host.generate(6, 64, 36, 0.25)
+    m = s.materials
+    coated = np.isin(m["type"], [pa.MAT_COATED_DIFFUSE, pa.MAT_COATED_CONDUCTOR])
+    assert coated.sum() >= 20 and (m["type"] == pa.MAT_COATED_CONDUCTOR).sum() >= 6
+    assert (m["sheenWeight"] > 0).sum() >= 6 and np.all(m["sheenColor"][m["sheenWeight"] > 0] >= 0)
+    # a scattering slab, a rough coat and a dispersive (Sellmeier) coat are all present
+    assert np.any(m["coatAlbedo"][coated] >= 0) and np.any(m["coatAlpha"][coated] > 0)
+    kinds = s.spectra["kind"][m["coatEta"][coated]]
+    assert pa.SPEC_SELLMEIER in kinds.tolist() and np.any(m["coatDispersive"][coated] == 1)
+    assert np.all(m["thickness"][coated] > 0) and np.all(m["coatMaxDepth"][coated] == 64)
 
 
 def test_checkpoint_round_trip_and_errors(host, tmp_path):

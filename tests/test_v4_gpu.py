@@ -28,6 +28,21 @@ def test_v4_render_matches_oracle_same_samples(plugin, host):
     np.testing.assert_allclose(out.sum(), ref.sum(), rtol=5e-3)
 
 
+def test_v4_render_uploads_its_own_geometry(plugin, host):
+    """prgpu_render after a render of other geometry traces its own scene's primitives: the same
+    film as after a render of its own geometry."""
+    s = host.generate(1, 48, 48)
+    v4 = V4Scene(host, s)
+    first = plugin.render_v4(v4.desc, params_v4(2, 0, seed=3, max_depth=8), v4.width, v4.height)
+    other = host.generate(6, 32, 32, 0.25)  # different triangles
+    plugin.render_ex(other, pa.render_params(spp=1, first_sample=0, seed=1, max_depth=4))
+    again = plugin.render_v4(v4.desc, params_v4(2, 0, seed=3, max_depth=8), v4.width, v4.height)
+    np.testing.assert_array_equal(first, again)
+    ref, _, _ = orc.OracleScene(v4.tabulated_scene(s)).render(
+        pa.render_params(spp=2, first_sample=0, seed=3, max_depth=8, rr_start=1), variance=False)
+    np.testing.assert_allclose(again.sum(), ref.sum(), rtol=5e-3)
+
+
 def test_v4_render_adds_into_output(plugin, host):
     s = host.generate(1, 32, 32)
     v4 = V4Scene(host, s)
@@ -48,8 +63,12 @@ def test_v4_render_errors(plugin, host):
     with pytest.raises(RuntimeError, match="version"):
         plugin.render_v4(v4.desc, params_v4(1, 0), v4.width, v4.height)
     v4.desc.apiVersion = 4
-    coated = s.materials.copy()
-    coated["type"][0] = pa.MAT_COATED_DIFFUSE
-    v4.desc.materials = coated.ctypes.data
-    with pytest.raises(RuntimeError, match="coated"):
+    bad = s.materials.copy()
+    bad["type"][0] = 9
+    v4.desc.materials = bad.ctypes.data
+    with pytest.raises(RuntimeError, match="material type"):
+        plugin.render_v4(v4.desc, params_v4(1, 0), v4.width, v4.height)
+    bad["type"][0] = pa.MAT_COATED_DIFFUSE  # a coat needs its IOR spectrum
+    bad["coatEta"][0] = -1
+    with pytest.raises(RuntimeError, match="coat spectra"):
         plugin.render_v4(v4.desc, params_v4(1, 0), v4.width, v4.height)

@@ -55,8 +55,9 @@ def filter_v4(cam):
 class V4Scene:
     """The v4 SceneDesc of `scene` and the arrays it points at."""
 
-    def __init__(self, host, scene):
+    def __init__(self, host, scene, sheen_table=None):
         self.spectra = tabulate(host, scene)
+        self.sheen = None if sheen_table is None else np.ascontiguousarray(sheen_table, np.float32)
         self.prims = scene.prims
         self.materials = scene.materials
         lights = [l for l in scene.lights if l["kind"] in (pa.LIGHT_AREA, pa.LIGHT_POINT)]
@@ -94,6 +95,7 @@ class V4Scene:
         c.lensRadius, c.focalDistance = cam.lensRadius, cam.focalDistance
         c.width, c.height, c.margin = cam.width, cam.height, 0
         d.filter = filter_v4(cam)
+        d.sheenAlbedo = None if self.sheen is None else self.sheen.ctypes.data
         self.width, self.height = cam.width, cam.height
 
     def tabulated_scene(self, scene):
