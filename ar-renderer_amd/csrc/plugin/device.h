@@ -77,6 +77,7 @@ struct Device {
     prgpu::ProfileStatsEx prof{};
     DevBuf travCounters;  // 6 x uint64: closest rays/nodes/recs, shadow rays/nodes/recs
     std::vector<hipEvent_t> eventPool;
+    hipEvent_t pipeEvents[2] = {nullptr, nullptr};  // read-back events of the pipelined bounce loop
     Device();
     ~Device();
 };

@@ -191,6 +191,10 @@ Box3 PrimBounds(const prgpu::Primitive &p) {
 // =============================================================================================
 // Device-resident path-tracing scene and batch buffers
 
+// Bufs::counts: queue lengths and the per-XCD ray fetch counters of the persistent traversals.
+constexpr int kFetchIntersect = 16, kFetchShadow = 32, kCountsInts = 48, kCountsBytes = kCountsInts * 4;
+static_assert(kFetchParts <= 16, "fetch counters overlap");
+
 struct PTScene {
     DevBuf prims, mats, specs, knots, lights, lightArea, lnodes, leafOf, infinite, powerCdf, powerFunc, media, grid, sheen;
     int lightCount = 0, lnodeCount = 0, infCount = 0, powerCount = 0;
@@ -208,7 +212,11 @@ struct PTScene {
     DevBuf so, sd, sp, sn, sf, sL, sb, sq;
     int *countsHost = nullptr;
     DevBuf counts;
+    // The captured bounce rounds (see TraceBatch) and the key of what they were captured with.
+    hipGraphExec_t bounceExec = nullptr;
+    uint64_t bounceKey = 0;
     ~PTScene() {
+        if (bounceExec) (void)hipGraphExecDestroy(bounceExec);
         if (countsHost) (void)hipHostFree(countsHost);
     }
 };
@@ -216,6 +224,8 @@ struct PTScene {
 Device::Device() = default;
 Device::~Device() {
     pt.reset();
+    for (hipEvent_t &e : pipeEvents)
+        if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
 }
 void ReleasePT(Device &d) {
@@ -479,9 +489,9 @@ void EnsurePTScene(Device &dev, const prgpu::SceneDescEx &sd, const float *sheen
     pt->hasMedia = (sd.cameraMedium >= 0) ? 1 : 0;
     for (int i = 0; i < sd.primCount && !pt->hasMedia; ++i)
         if (sd.prims[i].mediumInside >= 0 || sd.prims[i].mediumOutside >= 0) pt->hasMedia = 1;
-    PG_CHECK(hipHostMalloc(reinterpret_cast<void **>(&pt->countsHost), 64, hipHostMallocDefault));
-    pt->counts.ensure(64);
-    PG_CHECK(hipMemsetAsync(pt->counts.p, 0, 64, s));
+    PG_CHECK(hipHostMalloc(reinterpret_cast<void **>(&pt->countsHost), kCountsBytes, hipHostMallocDefault));
+    pt->counts.ensure(kCountsBytes);
+    PG_CHECK(hipMemsetAsync(pt->counts.p, 0, kCountsBytes, s));
     PG_CHECK(hipStreamSynchronize(s));
     dev.pt = std::move(pt);
     dev.ptKey = key;
@@ -587,8 +597,9 @@ struct Bufs {
     int *q0, *q1;
     int *qm;     // material queues, QCount x qcap
     int qcap;
-    int *counts;  // [0] queue A, [1] queue B, [2] shadow queue, [3..5] plain material queues, [6..7] ray
-                  // fetch, [8] unused, [9] emission queue, [10] coated / sheen material queue
+    int *counts;  // [0] queue A, [1] queue B, [2] shadow queue, [3..5] plain material queues, [9] emission
+                  // queue, [10] coated / sheen material queue, [16..16+P) closest-hit fetch counters,
+                  // [32..32+P) shadow fetch counters (P = kFetchParts)
     float2 *pfilm;
     float4 *xyz;
     // shadow records
@@ -631,6 +642,13 @@ __device__ __forceinline__ V2 get2(Pcg &r) {
     float a = r.uniform();
     float c = r.uniform();
     return {a, c};
+}
+
+// Start of a bounce: every queue length and fetch counter but the current path queue's is zeroed
+// (a kernel rather than a memset, so that the bounce can be captured in a graph).
+__global__ __launch_bounds__(64) void k_reset_counts(int *counts, int keep) {
+    for (int i = int(threadIdx.x); i < kCountsInts; i += 64)
+        if (i != keep) counts[i] = 0;
 }
 
 // Camera samples (src/integrators/path.cpp:61-70, src/cameras/camera.cpp:36-48).
@@ -683,9 +701,9 @@ __global__ __launch_bounds__(kBlock) void k_camera(const DevScene *__restrict__ 
     if (p.train) b.trec[slot] = make_float4(0, 0, 0, 0);  // no guided vertex yet
 }
 
-// Closest hits of the queued paths. Persistent: each lane takes a new ray from the fetch counter
-// as soon as its previous one completes (fetch_rays), so a wave's lanes stay busy while ray
-// lengths differ.
+// Closest hits of the queued paths. Persistent: each lane takes a new ray from its XCD's part of
+// the queue as soon as its previous one completes (fetch_rays_parts), so a wave's lanes stay busy
+// while ray lengths differ and neighbouring rays share one L2.
 template <bool kStats>
 __global__ __launch_bounds__(kTraceBlock) void k_intersect(const DevScene *__restrict__ sp, Bufs b, const int *queue,
                                                            const int *count, int *fetch, unsigned long long *counters) {
@@ -696,10 +714,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_intersect(const DevScene *__res
     TraceStats st;
     uint32_t rays = 0;
     int idx = -1, slot = 0;
-    bool exhausted = false, started = false;
+    bool started = false;
     DRay r{};
     TravState t{};
-    while (fetch_rays(fetch, n, idx, exhausted)) {
+    FetchState fs = fetch_begin();
+    while (fetch_rays_parts(fetch, n, idx, fs)) {
         if (idx >= 0 && !started) {
             slot = queue[idx];
             float4 o = b.o[slot], d = b.d[slot];
@@ -1172,10 +1191,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(const DevScene *__restri
     const float tMax = 1 - 1e-4f;
     if constexpr (!kMedia && !kInterfaces) {
         int idx = -1;
-        bool exhausted = false, started = false;
+        bool started = false;
         DRay r{};
         TravState t{};
-        while (fetch_rays(fetch, n, idx, exhausted)) {
+        FetchState fs = fetch_begin();
+        while (fetch_rays_parts(fetch, n, idx, fs)) {
             if (idx >= 0 && !started) {
                 float4 so = b.so[idx], sd = b.sd[idx];
                 r = DRay{so.x, so.y, so.z, sd.x, sd.y, sd.z};
@@ -1407,7 +1427,7 @@ void EnsureBatch(PTScene &pt, size_t n) {
 }
 
 Bufs MakeBufs(PTScene &pt) {
-    Bufs b;
+    Bufs b{};  // zeroed, padding included: the bounce-graph key hashes its bytes
     b.o = pt.o.as<float4>(); b.d = pt.d.as<float4>(); b.rng = pt.rng.as<ulonglong2>();
     b.lam = pt.lam.as<float4>(); b.lpdf = pt.lpdf.as<float4>(); b.beta = pt.beta.as<float4>();
     b.rpath = pt.rpath.as<float4>(); b.L = pt.L.as<float4>(); b.Lh = pt.Lh.as<float4>();
@@ -1509,6 +1529,35 @@ struct Timer {
 #endif
 constexpr int kGuideBounces = PG_GUIDE_BOUNCES;
 
+// Bounce rounds per captured graph (even: a replay ends on the queue it started from); 0 = no
+// graphs. Off by default: on configuration 2 at 1080p, replaying 4 captured rounds took 27.9 ms per
+// step against 25.4 ms for the pipelined eager loop (late, nearly empty bounces still launch the
+// whole-batch grids a graph bakes in, and a replay costs more than it saves in synchronisation).
+#ifndef PG_GRAPH_ROUNDS
+#define PG_GRAPH_ROUNDS 0
+#endif
+constexpr int kGraphRounds = PG_GRAPH_ROUNDS;
+static_assert(kGraphRounds % 2 == 0, "graph rounds must alternate back to the first queue");
+
+// What a captured bounce graph bakes in: the kernel variants, grids, buffer addresses and the
+// parameters the bounce kernels read (not the sample range, seed or clamp, which only the camera
+// and finish kernels use).
+uint64_t BounceKey(const Params &p, const Bufs &b, const DevScene *ds, int tgrid, int grid, bool media, bool interfaces,
+                   int emitters, int layered) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void *v, size_t n) {
+        const unsigned char *c = static_cast<const unsigned char *>(v);
+        for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ull;
+    };
+    const int ints[10] = {p.maxDepth, p.rrStart, p.rr, p.guide, p.train, tgrid, grid, media ? 1 : 0, interfaces ? 1 : 0,
+                          emitters * 2 + layered};
+    mix(ints, sizeof(ints));
+    mix(&p.box, sizeof(p.box));
+    mix(&b, sizeof(b));
+    mix(&ds, sizeof(ds));
+    return h;
+}
+
 void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, int passes, int npix, int sw) {
     PTScene &pt = *dev.pt;
     hipStream_t st = dev.stream;
@@ -1529,26 +1578,22 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
     int active = n;
     pt.countsHost[0] = n;
     PG_CHECK(hipMemcpyAsync(b.counts, pt.countsHost, sizeof(int), hipMemcpyHostToDevice, st));
-    int *cur = b.q0, *nxt = b.q1;
-    int curIdx = 0, bounce = 0;
     const int shadeGrid = std::max(1, std::min((active + kBlock - 1) / kBlock, dev.cus * 16));
-    // Queue sizes per bounce, read back after each synchronisation for the profile.
-    std::vector<int> shadowCounts;
-    shadowCounts.reserve(64);
-    while (active > 0) {
+    // One bounce: closest hits of the current queue, the surface and emission stages, the
+    // material families (which fill the next queue), the shadow rays. Every kernel reads its
+    // queue length from device memory, so rounds can be queued without reading them back.
+    auto round = [&](int curIdx, int bounce, int tgrid, int grid) {
         const int nxtIdx = 1 - curIdx;
-        PG_CHECK(hipMemsetAsync(b.counts + nxtIdx, 0, sizeof(int), st));
-        PG_CHECK(hipMemsetAsync(b.counts + 2, 0, 9 * sizeof(int), st));  // shadow, material, fetch, emission
-        const int tgrid = TraceGrid(dev, active);
+        int *cur = curIdx ? b.q1 : b.q0, *nxt = curIdx ? b.q0 : b.q1;
+        hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, st, b.counts, curIdx);
         timer.Run(prgpu::KernIntersect, active, nullptr, [&] {
             if (stats)
                 hipLaunchKernelGGL(k_intersect<true>, dim3(tgrid), dim3(kTraceBlock), 0, st, ds, b, cur, b.counts + curIdx,
-                                   b.counts + 6, ctr);
+                                   b.counts + kFetchIntersect, ctr);
             else
                 hipLaunchKernelGGL(k_intersect<false>, dim3(tgrid), dim3(kTraceBlock), 0, st, ds, b, cur, b.counts + curIdx,
-                                   b.counts + 6, ctr);
+                                   b.counts + kFetchIntersect, ctr);
         });
-        int grid = std::max(1, std::min((active + kBlock - 1) / kBlock, shadeGrid));
         // Guiding applies to the vertices of the first kGuideBounces bounces (0: all).
         const bool guided = p.guide && (kGuideBounces == 0 || bounce < kGuideBounces);
         timer.Run(prgpu::KernSurface, active, nullptr, [&] {
@@ -1560,8 +1605,8 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
         });
         if (guided)
             timer.Run(prgpu::KernGuide, 0, nullptr, [&] {
-                dev.guide->Infer(b.qm + size_t(QDiffuse) * b.qcap, b.counts + mat_count_index(QDiffuse), active, b.h0, b.h1, b.d, p.box,
-                                 dev.cus, st);
+                dev.guide->Infer(b.qm + size_t(QDiffuse) * b.qcap, b.counts + mat_count_index(QDiffuse), active, b.h0,
+                                 b.h1, b.d, p.box, dev.cus, st);
             });
         timer.Run(prgpu::KernMaterial, active, nullptr, [&] {
             auto *kd = guided ? (media ? k_material<QDiffuse, true, true> : k_material<QDiffuse, false, true>)
@@ -1581,16 +1626,82 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
                                      : (interfaces ? k_shadow<true, false, true> : k_shadow<true, false, false>))
                             : (media ? (interfaces ? k_shadow<false, true, true> : k_shadow<false, true, false>)
                                      : (interfaces ? k_shadow<false, false, true> : k_shadow<false, false, false>));
-            hipLaunchKernelGGL(k, dim3(tgrid), dim3(kTraceBlock), 0, st, ds, b, b.counts + 2, b.counts + 7, ctr);
+            hipLaunchKernelGGL(k, dim3(tgrid), dim3(kTraceBlock), 0, st, ds, b, b.counts + 2, b.counts + kFetchShadow, ctr);
         });
-        PG_CHECK(hipGetLastError());
-        PG_CHECK(hipMemcpyAsync(pt.countsHost, b.counts, 3 * sizeof(int), hipMemcpyDeviceToHost, st));
+    };
+
+    if (!dev.profileMode && !p.guide && kGraphRounds > 0) {
+        // Unprofiled, unguided renders replay kGraphRounds captured bounces per launch (an even
+        // count, so every replay starts from queue A) with grids sized for the whole batch, and
+        // read the path count back once per replay instead of once per bounce.
+        const int tgrid = TraceGrid(dev, n);
+        const uint64_t key = BounceKey(p, b, ds, tgrid, shadeGrid, media, interfaces, s.emitters, pt.hasLayered);
+        if (!pt.bounceExec || pt.bounceKey != key) {
+            if (pt.bounceExec) PG_CHECK(hipGraphExecDestroy(pt.bounceExec));
+            pt.bounceExec = nullptr;
+            hipGraph_t g = nullptr;
+            PG_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            try {
+                for (int k = 0; k < kGraphRounds; ++k) round(k & 1, k, tgrid, shadeGrid);
+            } catch (...) {
+                (void)hipStreamEndCapture(st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            PG_CHECK(hipStreamEndCapture(st, &g));
+            const hipError_t e = hipGraphInstantiate(&pt.bounceExec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            PG_CHECK(e);
+            pt.bounceKey = key;
+        }
+        while (active > 0) {
+            PG_CHECK(hipGraphLaunch(pt.bounceExec, st));
+            PG_CHECK(hipMemcpyAsync(pt.countsHost, b.counts, sizeof(int), hipMemcpyDeviceToHost, st));
+            PG_CHECK(hipStreamSynchronize(st));
+            active = pt.countsHost[0];
+        }
+    } else if (!dev.profileMode) {
+        // Pipelined: round k + 1 is queued before the host waits for the path count of round k,
+        // so the device never idles on a read-back. A round's grids are sized by the count the host
+        // last saw, an upper bound (paths only ever leave the queues); the kernels read the exact
+        // count on the device. The count of the paths entering round k + 1 is copied right after
+        // round k, before round k + 2 could clear it.
+        int *slot = pt.countsHost + 8;  // two read-back slots, alternating
+        for (hipEvent_t &e : dev.pipeEvents)
+            if (!e) PG_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        hipEvent_t ready[2] = {dev.pipeEvents[0], dev.pipeEvents[1]};
+        int bound = n, curIdx = 0, k = 0;
+        auto launch = [&](int r, int curI, int bnd) {
+            round(curI, r, TraceGrid(dev, bnd), std::max(1, std::min((bnd + kBlock - 1) / kBlock, shadeGrid)));
+            PG_CHECK(hipMemcpyAsync(slot + (r & 1), b.counts + (1 - curI), sizeof(int), hipMemcpyDeviceToHost, st));
+            PG_CHECK(hipEventRecord(ready[r & 1], st));
+        };
+        launch(0, 0, bound);
+        while (true) {
+            // Round k is queued and bounds round k + 1 (bound >= paths entering round k >= those entering k + 1).
+            launch(k + 1, 1 - curIdx, bound);
+            PG_CHECK(hipGetLastError());
+            PG_CHECK(hipEventSynchronize(ready[k & 1]));
+            bound = slot[k & 1];  // paths entering round k + 1
+            curIdx = 1 - curIdx;
+            ++k;
+            if (bound == 0) break;  // round k (queued, empty) is the last
+        }
         PG_CHECK(hipStreamSynchronize(st));
-        timer.Collect();
-        active = pt.countsHost[nxtIdx];
-        std::swap(cur, nxt);
-        curIdx = nxtIdx;
-        ++bounce;
+    } else {
+        int curIdx = 0, bounce = 0;
+        while (active > 0) {
+            const int tgrid = TraceGrid(dev, active);
+            const int grid = std::max(1, std::min((active + kBlock - 1) / kBlock, shadeGrid));
+            round(curIdx, bounce, tgrid, grid);
+            PG_CHECK(hipGetLastError());
+            PG_CHECK(hipMemcpyAsync(pt.countsHost, b.counts, 3 * sizeof(int), hipMemcpyDeviceToHost, st));
+            PG_CHECK(hipStreamSynchronize(st));
+            timer.Collect();
+            curIdx = 1 - curIdx;
+            active = pt.countsHost[curIdx];
+            ++bounce;
+        }
     }
     CheckTraceFlags(dev);
 }

@@ -28,6 +28,9 @@ namespace pgpu {
 #ifndef PG_LEAF_DEFER
 #define PG_LEAF_DEFER 0
 #endif
+#ifndef PG_FETCH_PARTS
+#define PG_FETCH_PARTS 1
+#endif
 constexpr int kTraceBlock = 256;
 constexpr int kStackLds = PG_STACK_LDS;  // LDS entries per lane
 
@@ -279,23 +282,47 @@ __device__ __forceinline__ bool trace_turn(bool busy, uint32_t cur) {
     const bool leafTurn = innerMask == 0 || __popcll(leafMask) >= kLeafDefer;
     return busy && (atLeaf == leafTurn);
 }
-__device__ __forceinline__ bool fetch_rays(int *counter, int n, int &idx, bool &exhausted) {
-    const bool idle = idx < 0 && !exhausted;
+// Ray fetch of the persistent traversals. With kFetchParts > 1 the queue is split into that many
+// contiguous parts, one per XCD (workgroups are dispatched to the 8 XCDs round-robin, so block b
+// runs on XCD b % 8): a wave drains its own XCD's part first, then the others in turn. Measured on
+// configuration 2 this is slower (8 parts: closest hit 8.9 ms per step against 6.7 ms with one
+// part): one queue front keeps every XCD on the same rays, whose nodes then stay in the shared
+// MALL. The default is one part. `part` and `left` are wave-uniform; the loop ends when every
+// part is drained and no lane holds a ray.
+constexpr int kFetchParts = PG_FETCH_PARTS;
+struct FetchState {
+    int part, left;
+    bool done;
+};
+__device__ __forceinline__ FetchState fetch_begin() {
+    FetchState f;
+    f.part = int(blockIdx.x % kFetchParts);
+    f.left = kFetchParts;
+    f.done = false;
+    return f;
+}
+__device__ __forceinline__ bool fetch_rays_parts(int *counters, int n, int &idx, FetchState &f) {
+    const bool idle = idx < 0 && !f.done;
     const unsigned long long idleMask = __ballot(idle);
     const unsigned long long busyMask = __ballot(idx >= 0);
     if (idleMask != 0 && (__popcll(idleMask) >= kRefill || busyMask == 0)) {
         const int lane = __lane_id();
         const int leader = __ffsll(idleMask) - 1;
+        const int want = __popcll(idleMask);
+        const int lo = int(int64_t(f.part) * n / kFetchParts), hi = int(int64_t(f.part + 1) * n / kFetchParts);
         int base = 0;
-        if (lane == leader) base = atomicAdd(counter, __popcll(idleMask));
+        if (lane == leader) base = atomicAdd(&counters[f.part], want);
         base = __shfl(base, leader);
         if (idle) {
-            const int k = base + __popcll(idleMask & ((1ull << lane) - 1ull));
-            if (k < n) idx = k;
-            else exhausted = true;
+            const int k = lo + base + __popcll(idleMask & ((1ull << lane) - 1ull));
+            if (k < hi) idx = k;
+        }
+        if (lo + base + want >= hi) {  // this part is drained: the next one
+            f.part = f.part + 1 == kFetchParts ? 0 : f.part + 1;
+            if (--f.left == 0) f.done = true;
         }
     }
-    return __ballot(idx >= 0) != 0;
+    return __ballot(idx >= 0) != 0 || !f.done;
 }
 
 }  // namespace pgpu

@@ -153,3 +153,23 @@ def test_checkpoint_resume_equals_uninterrupted(plugin, host, tmp_path):
     np.testing.assert_allclose(film + rest, full, rtol=1e-9, atol=1e-12)
     np.testing.assert_allclose(sq + rest_sq, full_sq, rtol=1e-9, atol=1e-12)
     host.lib.ph_plugin_close(handle)
+
+
+def test_unprofiled_bounce_loop_matches_profiled_rounds(plugin, host):
+    """Unprofiled renders queue each bounce one ahead of the path-count read-back (or replay
+    captured bounce graphs when those are compiled in); profiled renders synchronise after every
+    bounce. Both must give the same film bit for bit, on every repetition, with other work and
+    synchronisations in between (a plain scene, media with interfaces, coated materials)."""
+    scenes = [host.generate(2, 96, 54, 0.25), host.generate(3, 64, 36, 0.125), host.generate(6, 64, 36, 0.25)]
+    for k, scene in enumerate(scenes):
+        prm = pa.render_params(spp=3, first_sample=5, seed=2, max_depth=24, rr_start=1)
+        plugin.profile(1)
+        eager, eager_sq = plugin.render_ex(scene, prm)
+        plugin.profile_read(reset=True)
+        plugin.profile(0)
+        for rep in range(3):
+            g, g_sq = plugin.render_ex(scene, prm)
+            assert np.array_equal(g, eager) and np.array_equal(g_sq, eager_sq), f"scene {k} replay {rep}"
+            plugin.profile(1)  # eager work between the replays
+            plugin.render_ex(scene, pa.render_params(spp=1, first_sample=rep, seed=3, max_depth=4))
+            plugin.profile(0)
