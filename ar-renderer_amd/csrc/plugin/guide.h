@@ -20,7 +20,10 @@
 namespace pgpu {
 
 constexpr int kGIn = 64, kGHidden = 64, kGOutRows = 48, kGOut = 41, kGLobes = 8, kGLayers = 5;
-constexpr float kGSelMin = 0.1f, kGSelMax = 0.9f, kGKappaMin = 0.01f, kGKappaMax = 1000.f;
+#ifndef PG_GUIDE_SEL_MIN
+#define PG_GUIDE_SEL_MIN 0.1f
+#endif
+constexpr float kGSelMin = PG_GUIDE_SEL_MIN, kGSelMax = 0.9f, kGKappaMin = 0.01f, kGKappaMax = 1000.f;
 
 PG_HD constexpr int guide_rows(int l) { return l == kGLayers - 1 ? kGOutRows : kGHidden; }
 PG_HD constexpr int guide_woff(int l) {

@@ -9,6 +9,9 @@
 // 16 outputs x 16 samples with lane l owning sample l % 16, outputs 4 (l / 16) .. +3: exactly
 // the B-operand layout of the next layer, so activations stay in registers from layer to layer.
 // A wave evaluates 16 vertices at a time; the 19 456 BF16 weights and 304 biases sit in LDS.
+// LDS rows of weights (and of the training kernel's staging tiles) are padded by 8 bytes: the
+// 16 lanes of an A-operand read take 16 consecutive rows, which at a 128-byte stride would all
+// fall in one LDS bank (a 16-way conflict); at 136 bytes they cover the 32 banks once.
 #include "guide_host.h"
 
 #include <cmath>
@@ -22,16 +25,25 @@ typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kGBlock = 256;
+constexpr int kInferBlock = 256;       // inference: 4 waves share one LDS copy of the model (3 blocks per CU)
+constexpr int kGTileRow = kGOutRows + 1;  // floats per row of the raw-output tiles (padded)
 constexpr int kGBiases = 4 * kGHidden + kGOutRows;  // 304
 constexpr int kGRec = 16;                            // floats per training record
 
 __host__ __device__ constexpr int wl_off(int l) { return l * kGHidden * kGIn; }  // bf16 weight array
 __host__ __device__ constexpr int bl_off(int l) { return l * kGHidden; }         // bias array
 
+#ifndef PG_GUIDE_NATIVE_BF16
+#define PG_GUIDE_NATIVE_BF16 1
+#endif
 __device__ __forceinline__ short f2bf(float f) {  // round to nearest even
+#if PG_GUIDE_NATIVE_BF16
+    return __builtin_bit_cast(short, static_cast<__bf16>(f));  // gfx950: v_cvt_pk_bf16_f32
+#else
     uint32_t u = __float_as_uint(f);
     u += 0x7fffu + ((u >> 16) & 1u);
     return short(u >> 16);
+#endif
 }
 __device__ __forceinline__ float bf2f(short h) { return __uint_as_float(uint32_t(uint16_t(h)) << 16); }
 
@@ -39,19 +51,23 @@ __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+constexpr int kGLdsStride = kGIn + 4;                 // bf16 per LDS weight row
+constexpr int kGRows = kGWeights / kGIn;               // 304 weight rows, all layers
+__host__ __device__ constexpr int lds_row(int l, int r) { return (l * kGHidden + r) * kGLdsStride; }
+
 struct GuideLds {
-    short w[kGWeights];
+    short w[kGRows * kGLdsStride];
     float b[kGBiases];
 };
 
 __device__ __forceinline__ void load_model(GuideLds &L, const short *wt, const float *bias) {
     for (int i = threadIdx.x; i < kGWeights / 4; i += blockDim.x)
-        reinterpret_cast<uint2 *>(L.w)[i] = reinterpret_cast<const uint2 *>(wt)[i];
+        *reinterpret_cast<uint2 *>(&L.w[(i >> 4) * kGLdsStride + 4 * (i & 15)]) = reinterpret_cast<const uint2 *>(wt)[i];
     for (int i = threadIdx.x; i < kGBiases; i += blockDim.x) L.b[i] = bias[i];
 }
 
 __device__ __forceinline__ bf16x4 wblock(const GuideLds &L, int l, int j, int kb, int lane) {
-    return *reinterpret_cast<const bf16x4 *>(&L.w[wl_off(l) + (j * 16 + (lane & 15)) * kGIn + kb * 16 + 4 * (lane >> 4)]);
+    return *reinterpret_cast<const bf16x4 *>(&L.w[lds_row(l, j * 16 + (lane & 15)) + kb * 16 + 4 * (lane >> 4)]);
 }
 
 // Features of this lane's vertex (sample lane % 16) in B layout: lane group q = lane / 16 holds
@@ -111,16 +127,16 @@ __device__ __forceinline__ void wave_sync() {
 // The decoded mixture (guide.h) for the queued vertices: queue[i] is a path slot whose hit record
 // (position h0, geometric normal h1) and incoming direction d describe the vertex; the mixture
 // goes to out[12 i ..]. Vertex positions are normalised to the box (lo, inv extent).
-__global__ __launch_bounds__(kGBlock) void k_guide_infer(const short *wt, const float *bias, const int *queue,
-                                                         const int *count, const float4 *h0, const float4 *h1,
-                                                         const float4 *dir, GuideBox box, float4 *out) {
+__global__ __launch_bounds__(kInferBlock) void k_guide_infer(const short *wt, const float *bias, const int *queue,
+                                                             const int *count, const float4 *h0, const float4 *h1,
+                                                             const float4 *dir, GuideBox box, float4 *out) {
     __shared__ GuideLds L;
-    __shared__ float tile[kGBlock / 64][16][kGOutRows];
+    __shared__ float tile[kInferBlock / 64][16][kGTileRow];
     load_model(L, wt, bias);
     __syncthreads();
     const int n = *count, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int waves = int(gridDim.x) * (kGBlock / 64);
-    for (int tile_i = int(blockIdx.x) * (kGBlock / 64) + wave; tile_i * 16 < n; tile_i += waves) {
+    const int waves = int(gridDim.x) * (kInferBlock / 64);
+    for (int tile_i = int(blockIdx.x) * (kInferBlock / 64) + wave; tile_i * 16 < n; tile_i += waves) {
         const int idx = tile_i * 16 + (lane & 15);
         const bool valid = idx < n;
         float pos[3] = {0.f, 0.f, 0.f}, nrm[3] = {0.f, 0.f, 0.f}, wo[3] = {0.f, 0.f, 0.f};
@@ -260,13 +276,17 @@ __device__ void output_grad(const float *o, const float *rec, float *dout, float
     }
 }
 
+constexpr int kGStage = 16 + 4;  // bf16 per staging row (16 samples, padded)
+
 struct TrainLds {
     GuideLds m;
     float acc[kGParams];            // this block's gradient, in the parameter layout
-    short stage[4][2][kGIn * 16];  // per wave: layer input X[in][s] and output gradient dY[out][s]
-    float tile[4][16][kGOutRows];  // per wave: raw outputs, then their gradients
+    short stage[4][2][kGIn * kGStage];  // per wave: layer input X[in][s] and output gradient dY[out][s]
+    float tile[4][16][kGTileRow];  // per wave: raw outputs, then their gradients
     float wsum, loss;
 };
+
+static_assert(sizeof(TrainLds) <= 160 * 1024, "training LDS exceeds a CU");
 
 // dW_l += dY_l X_l^T over this wave's 16 samples and db_l += sum_s dY_l (LDS atomics), with the
 // layer's input X (B layout, bf16) and output gradient dY (accumulator layout, fp32, nb blocks).
@@ -277,17 +297,17 @@ __device__ __forceinline__ void weight_grad(TrainLds &T, int wave, int lane, int
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) sx[(kb * 16 + 4 * q + i) * 16 + s] = x[kb][i];
+        for (int i = 0; i < 4; ++i) sx[(kb * 16 + 4 * q + i) * kGStage + s] = x[kb][i];
     for (int j = 0; j < nb; ++j)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) sd[(j * 16 + 4 * q + i) * 16 + s] = f2bf(dy[j][i]);
+        for (int i = 0; i < 4; ++i) sd[(j * 16 + 4 * q + i) * kGStage + s] = f2bf(dy[j][i]);
     wave_sync();
     float *gw = T.acc + guide_woff(l), *gb = T.acc + guide_boff(l);
     for (int j = 0; j < nb; ++j) {
-        const bf16x4 a = *reinterpret_cast<const bf16x4 *>(&sd[(j * 16 + s) * 16 + 4 * q]);
+        const bf16x4 a = *reinterpret_cast<const bf16x4 *>(&sd[(j * 16 + s) * kGStage + 4 * q]);
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const bf16x4 b = *reinterpret_cast<const bf16x4 *>(&sx[(c * 16 + s) * 16 + 4 * q]);
+            const bf16x4 b = *reinterpret_cast<const bf16x4 *>(&sx[(c * 16 + s) * kGStage + 4 * q]);
             f32x4 d = mfma(a, b, f32x4{0.f, 0.f, 0.f, 0.f});
 #pragma unroll
             for (int i = 0; i < 4; ++i) atomicAdd(&gw[(j * 16 + 4 * q + i) * kGIn + c * 16 + s], d[i]);
@@ -295,7 +315,7 @@ __device__ __forceinline__ void weight_grad(TrainLds &T, int wave, int lane, int
     }
     if (lane < nb * 16) {
         float sum = 0.f;
-        for (int k = 0; k < 16; ++k) sum += bf2f(sd[lane * 16 + k]);
+        for (int k = 0; k < 16; ++k) sum += bf2f(sd[lane * kGStage + k]);
         atomicAdd(&gb[lane], sum);
     }
     wave_sync();
@@ -314,8 +334,7 @@ __device__ __forceinline__ void input_grad(const GuideLds &L, int l, const f32x4
         for (int kb = 0; kb < nb; ++kb) {
             bf16x4 a;
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                a[i] = L.w[wl_off(l) + (kb * 16 + 4 * (lane >> 4) + i) * kGIn + jb * 16 + (lane & 15)];
+            for (int i = 0; i < 4; ++i) a[i] = L.w[lds_row(l, kb * 16 + 4 * (lane >> 4) + i) + jb * 16 + (lane & 15)];
             acc = mfma(a, b[kb], acc);
         }
 #pragma unroll
@@ -351,7 +370,7 @@ __global__ __launch_bounds__(kGBlock) void k_guide_train(const short *wt, const 
         hidden(T.m, 3, x[3], x[4], lane);
         f32x4 o[3];
         output(T.m, x[4], o, lane);
-        float(*tl)[kGOutRows] = T.tile[wave];
+        float(*tl)[kGTileRow] = T.tile[wave];
 #pragma unroll
         for (int j = 0; j < 3; ++j)
 #pragma unroll
@@ -465,7 +484,8 @@ std::vector<float> GuideInitialParams() {
             for (int c = 0; c < kGIn; ++c) P[size_t(guide_woff(l) + r * kGIn + c)] = (2.f * next() - 1.f) * a;
     }
     // s = kGSelMin + (kGSelMax - kGSelMin) sigmoid(b) = 0.2
-    P[size_t(guide_boff(kGLayers - 1) + 40)] = std::log((0.2f - kGSelMin) / (kGSelMax - 0.2f));
+    // Initial selection probability: an eighth of the way from its floor to its ceiling.
+    P[size_t(guide_boff(kGLayers - 1) + 40)] = std::log(0.125f / 0.875f);
     return P;
 }
 
@@ -516,8 +536,8 @@ void GuideState::Infer(const int *queue, const int *count, int maxCount, const f
                        const float4 *dir, const GuideBox &box, int cus, hipStream_t s) {
     if (maxCount <= 0) return;
     const int tiles = (maxCount + 15) / 16;
-    const int blocks = std::max(1, std::min((tiles + 3) / 4, cus * 8));
-    hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(kGBlock), 0, s, wt.as<short>(), bias.as<float>(), queue,
+    const int blocks = std::max(1, std::min((tiles + kInferBlock / 64 - 1) / (kInferBlock / 64), cus * 8));
+    hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(kInferBlock), 0, s, wt.as<short>(), bias.as<float>(), queue,
                        count, h0, h1, dir, box, out.as<float4>());
     PG_CHECK(hipGetLastError());
 }

@@ -1717,7 +1717,11 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
 #ifndef PG_GUIDE_LR
 #define PG_GUIDE_LR 1e-2f
 #endif
+#ifndef PG_GUIDE_MIN_STEPS
+#define PG_GUIDE_MIN_STEPS 1
+#endif
 constexpr int kGuideBatch = PG_GUIDE_BATCH, kGuideSteps = PG_GUIDE_STEPS, kGuideTrainCap = kGuideBatch * kGuideSteps;
+constexpr int kGuideMinSteps = PG_GUIDE_MIN_STEPS;
 constexpr float kGuideLearningRate = PG_GUIDE_LR;
 
 // Passes per batch: bounded by a path budget so that the state fits comfortably in HBM.
@@ -1788,16 +1792,18 @@ void RenderExImpl(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::Render
         });
         PG_CHECK(hipGetLastError());
         if (p.train) {
-            // Up to kGuideSteps Adam steps per batch, each on its own slice of the batch's records.
+            // One Adam step per slice of kGuideBatch records of the batch (at most kGuideSteps), and at
+            // least kGuideMinSteps steps, cycling over the slices, when the batch has records.
             int records = 0;
             PG_CHECK(hipMemcpyAsync(&records, dev.guide->trainCount.p, sizeof(int), hipMemcpyDeviceToHost, dev.stream));
             PG_CHECK(hipStreamSynchronize(dev.stream));
-            const int steps = std::min(kGuideSteps, (std::min(records, kGuideTrainCap) + kGuideBatch - 1) / kGuideBatch);
+            const int slices = std::min(kGuideSteps, (std::min(records, kGuideTrainCap) + kGuideBatch - 1) / kGuideBatch);
+            const int steps = slices > 0 ? std::max(slices, kGuideMinSteps) : 0;
             if (steps > 0)
                 timer.Run(prgpu::KernGuide, steps, nullptr, [&] {
                     for (int k = 0; k < steps; ++k)
-                        dev.guide->Step(dev.guide->train.as<float>(), dev.guide->trainCount.as<int>(), k * kGuideBatch,
-                                        kGuideBatch, kGuideLearningRate, dev.stream);
+                        dev.guide->Step(dev.guide->train.as<float>(), dev.guide->trainCount.as<int>(),
+                                        (k % slices) * kGuideBatch, kGuideBatch, kGuideLearningRate, dev.stream);
                 });
         }
         if (dev.profileMode) {
