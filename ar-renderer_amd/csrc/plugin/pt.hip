@@ -191,6 +191,18 @@ Box3 PrimBounds(const prgpu::Primitive &p) {
 // =============================================================================================
 // Device-resident path-tracing scene and batch buffers
 
+// Minimum waves per SIMD a kernel is compiled for (its register budget: 8 waves = 64 VGPRs,
+// 6 = 80, 5 = 96, 4 = 128); 0 leaves the choice to the compiler. The traversal is latency-bound
+// and its LDS stacks allow 8 waves per SIMD, which 64 VGPRs reach without spilling.
+#define PG_WAVES(n) __attribute__((amdgpu_waves_per_eu((n) > 0 ? (n) : 1, 8)))
+#ifndef PG_SURFACE_WAVES
+#define PG_SURFACE_WAVES 0
+#endif
+#ifndef PG_MATERIAL_WAVES
+#define PG_MATERIAL_WAVES 0
+#endif
+constexpr int kSurfaceWaves = PG_SURFACE_WAVES, kMaterialWaves = PG_MATERIAL_WAVES;
+
 // Bufs::counts: queue lengths and the per-XCD ray fetch counters of the persistent traversals.
 constexpr int kFetchIntersect = 16, kFetchShadow = 32, kCountsInts = 48, kCountsBytes = kCountsInts * 4;
 static_assert(kFetchParts <= 16, "fetch counters overlap");
@@ -705,7 +717,7 @@ __global__ __launch_bounds__(kBlock) void k_camera(const DevScene *__restrict__ 
 // the queue as soon as its previous one completes (fetch_rays_parts), so a wave's lanes stay busy
 // while ray lengths differ and neighbouring rays share one L2.
 template <bool kStats>
-__global__ __launch_bounds__(kTraceBlock) void k_intersect(const DevScene *__restrict__ sp, Bufs b, const int *queue,
+__global__ __launch_bounds__(kTraceBlock) PG_WAVES(8) void k_intersect(const DevScene *__restrict__ sp, Bufs b, const int *queue,
                                                            const int *count, int *fetch, unsigned long long *counters) {
     const DevScene &s = *sp;
     __shared__ uint32_t lds[kStackLds * kTraceBlock];
@@ -844,7 +856,7 @@ enum MatQueue : int { QDiffuse = 0, QConductor = 1, QDielectric = 2, QLayered = 
 __device__ __host__ constexpr int mat_count_index(int q) { return q == QLayered ? 10 : 3 + q; }
 
 template <bool kMedia, bool kInterfaces>
-__global__ __launch_bounds__(kBlock) void k_surface(const DevScene *__restrict__ sp, Params p, Bufs b, const int *queue,
+__global__ __launch_bounds__(kBlock) PG_WAVES(kSurfaceWaves) void k_surface(const DevScene *__restrict__ sp, Params p, Bufs b, const int *queue,
                                                     const int *count, int *nextQueue, int *nextCount) {
     const DevScene &s = *sp;
     const int n = *count;
@@ -1046,7 +1058,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(const DevScene *__restrict__ sp
 // Builds the BSDF, next-event estimation when it has a non-specular lobe, BSDF sampling,
 // throughput update, spawn, Russian roulette.
 template <int kType, bool kMedia, bool kGuided = false>
-__global__ __launch_bounds__(kBlock) void k_material(const DevScene *__restrict__ sp, Params p, Bufs b, int *nextQueue,
+__global__ __launch_bounds__(kBlock) PG_WAVES(kType == QLayered ? 0 : kMaterialWaves) void k_material(const DevScene *__restrict__ sp, Params p, Bufs b, int *nextQueue,
                                                      int *nextCount) {
     const DevScene &s = *sp;
     const int n = b.counts[mat_count_index(kType)];

@@ -31,6 +31,12 @@ namespace pgpu {
 #ifndef PG_FETCH_PARTS
 #define PG_FETCH_PARTS 1
 #endif
+#ifndef PG_ORDERED_SLAB
+#define PG_ORDERED_SLAB 1
+#endif
+#ifndef PG_CHILD_RELOAD
+#define PG_CHILD_RELOAD 0
+#endif
 constexpr int kTraceBlock = 256;
 constexpr int kStackLds = PG_STACK_LDS;  // LDS entries per lane
 
@@ -147,10 +153,24 @@ struct TravState {
     DHit hit;
 };
 
+// Inverse direction for the ordered slab test: |1/d| clamped to 2^64, so that the per-node
+// products with the node scale (a power of two at most 2^60) and the origin offset stay finite
+// and an axis the ray is parallel to yields signed, very large but never infinite distances.
+__device__ __forceinline__ float slab_inv(float d) {
+    float r = 1.f / d;
+    return __builtin_copysignf(fminf(fabsf(r), 0x1p64f), d);
+}
+
 __device__ __forceinline__ void trace_begin(TravState &t, const DRay &r, float tMax) {
+#if PG_ORDERED_SLAB
+    t.ix = slab_inv(r.dx);
+    t.iy = slab_inv(r.dy);
+    t.iz = slab_inv(r.dz);
+#else
     t.ix = dev_inv(r.dx);
     t.iy = dev_inv(r.dy);
     t.iz = dev_inv(r.dz);
+#endif
     t.tMax = tMax;
     t.cur = 0;  // root node
     t.sp = 0;
@@ -175,7 +195,7 @@ __device__ __forceinline__ bool trace_step(const TraceCtx &c, const DRay &r, Tra
     const uint32_t bd = blockDim.x;
     if (!(t.cur & kLeafBit)) {
         const uint4 *np = reinterpret_cast<const uint4 *>(c.nodes + t.cur);
-        uint4 h0 = np[0], c0 = np[1], c1 = np[2], q0 = np[3], q1 = np[4], q2 = np[5];
+        uint4 h0 = np[0], q0 = np[3], q1 = np[4], q2 = np[5];
         if (kStats) st->nodes++;
         float ox = __uint_as_float(h0.x), oy = __uint_as_float(h0.y), oz = __uint_as_float(h0.z);
         int ex = int(int8_t(h0.w & 0xff)), ey = int(int8_t((h0.w >> 8) & 0xff)), ez = int(int8_t((h0.w >> 16) & 0xff));
@@ -183,10 +203,45 @@ __device__ __forceinline__ bool trace_step(const TraceCtx &c, const DRay &r, Tra
         float sx = pow2f(ex), sy = pow2f(ey), sz = pow2f(ez);
         // Per-axis origin terms shared by all children.
         float bx = (ox - r.ox), by = (oy - r.oy), bz = (oz - r.oz);
+#if !PG_CHILD_RELOAD
+        const uint4 c0 = np[1], c1 = np[2];
         uint32_t child[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+#endif
         uint32_t qlx[2] = {q0.x, q0.y}, qly[2] = {q0.z, q0.w}, qlz[2] = {q1.x, q1.y};
         uint32_t qhx[2] = {q1.z, q1.w}, qhy[2] = {q2.x, q2.y}, qhz[2] = {q2.z, q2.w};
         uint32_t key[8];
+#if PG_ORDERED_SLAB
+        // Ordered slab test: the ray's direction signs pick each axis's entry (near) and exit (far)
+        // planes once per node, and a plane distance is one FMA, q (scale / d) + (origin - o) / d.
+        // The quantised child boxes are rounded outwards by a grid step, far more than these
+        // roundings, and the widening below is kept.
+        // exact (power-of-two scales); clamped so that q * a stays finite for any node exponent
+        const float ax = __builtin_copysignf(fminf(fabsf(sx * t.ix), 0x1p118f), t.ix);
+        const float ay = __builtin_copysignf(fminf(fabsf(sy * t.iy), 0x1p118f), t.iy);
+        const float az = __builtin_copysignf(fminf(fabsf(sz * t.iz), 0x1p118f), t.iz);
+        const float cx = bx * t.ix, cy = by * t.iy, cz = bz * t.iz;
+        const bool px = t.ix >= 0.f, py = t.iy >= 0.f, pz = t.iz >= 0.f;
+        const uint32_t nx[2] = {px ? qlx[0] : qhx[0], px ? qlx[1] : qhx[1]};
+        const uint32_t fx[2] = {px ? qhx[0] : qlx[0], px ? qhx[1] : qlx[1]};
+        const uint32_t ny[2] = {py ? qly[0] : qhy[0], py ? qly[1] : qhy[1]};
+        const uint32_t fy[2] = {py ? qhy[0] : qly[0], py ? qhy[1] : qly[1]};
+        const uint32_t nz[2] = {pz ? qlz[0] : qhz[0], pz ? qlz[1] : qhz[1]};
+        const uint32_t fz[2] = {pz ? qhz[0] : qlz[0], pz ? qhz[1] : qlz[1]};
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int w = i >> 2, sh = (i & 3) * 8;
+            float tn = fmaxf(fmaxf(__builtin_fmaf(float((nx[w] >> sh) & 0xff), ax, cx),
+                                   __builtin_fmaf(float((ny[w] >> sh) & 0xff), ay, cy)),
+                             __builtin_fmaf(float((nz[w] >> sh) & 0xff), az, cz));
+            float tf = fminf(fminf(__builtin_fmaf(float((fx[w] >> sh) & 0xff), ax, cx),
+                                   __builtin_fmaf(float((fy[w] >> sh) & 0xff), ay, cy)),
+                             __builtin_fmaf(float((fz[w] >> sh) & 0xff), az, cz));
+            tf = tf * (1.f + 2.f * 1.2e-7f * 3.f);
+            tn = tn - fabsf(tn) * (2.f * 1.2e-7f * 3.f);
+            bool h = (i < nch) && tn <= tf && tn < t.tMax && tf >= 0.f;
+            key[i] = h ? ((__float_as_uint(fmaxf(tn, 0.f)) & ~7u) | uint32_t(i)) : 0xffffffffu;
+        }
+#else
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int w = i >> 2, sh = (i & 3) * 8;
@@ -202,17 +257,26 @@ __device__ __forceinline__ bool trace_step(const TraceCtx &c, const DRay &r, Tra
             bool h = (i < nch) && tn <= tf && tn < t.tMax && tf >= 0.f;
             key[i] = h ? ((__float_as_uint(fmaxf(tn, 0.f)) & ~7u) | uint32_t(i)) : 0xffffffffu;
         }
+#endif
         sort8(key);
         if (key[0] == 0xffffffffu) {
             if (t.sp == 0) return true;
             t.cur = trace_pop(c, t, lds, lane, bd, tid);
             return false;
         }
-        // Push hit children far to near; continue with the nearest.
+        // Push hit children far to near; continue with the nearest. A child's reference is read by
+        // its slot index (key & 7): from the node, just loaded and still in the vector L1, rather
+        // than from the register copy, whose per-lane indexing costs a compare-select chain.
+#if PG_CHILD_RELOAD
+        const uint32_t *cref = reinterpret_cast<const uint32_t *>(np) + 4;
+#define PG_CHILD(k) cref[(k) & 7u]
+#else
+#define PG_CHILD(k) child[(k) & 7u]
+#endif
 #pragma unroll
         for (int i = 7; i >= 1; --i) {
             if (key[i] != 0xffffffffu) {
-                uint32_t ref = child[key[i] & 7u];
+                uint32_t ref = PG_CHILD(key[i]);
                 if (t.sp < kStackLds) {
                     lds[t.sp * bd + lane] = ref;
                 } else if (t.sp - kStackLds < c.spillCap) {
@@ -224,7 +288,8 @@ __device__ __forceinline__ bool trace_step(const TraceCtx &c, const DRay &r, Tra
                 ++t.sp;
             }
         }
-        t.cur = child[key[0] & 7u];
+        t.cur = PG_CHILD(key[0]);
+#undef PG_CHILD
         return false;
     }
     // Leaf: test its records.
