@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <string>
 
@@ -126,6 +127,80 @@ PH_API int ph_render(void *pl, int device, const prgpu::SceneDescEx *scene, cons
     }
     if (sppDone) *sppDone = done;
     if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+    return 0;
+}
+
+PH_API int ph_render_session(void *pl, int device, const prgpu::SceneDescEx *scene, const prgpu::RenderParamsEx *base,
+                             int targetSpp, double timeLimit, const PhRenderControl *ctl, double *sum, double *sumsq,
+                             int32_t *sppDone, double *scale, char *err, int errLen) {
+    using Clock = std::chrono::steady_clock;
+    auto *p = static_cast<Plugin *>(pl);
+    if (!p || !p->renderEx) {
+        SetErr(err, errLen, "GPU plugin lacks the extension interface");
+        return 1;
+    }
+    if (!scene || !base || !sum) {
+        SetErr(err, errLen, "invalid arguments");
+        return 1;
+    }
+    const PhRenderControl none{};
+    const PhRenderControl &c = ctl ? *ctl : none;
+    const std::string ckptPath = c.checkpointPath ? c.checkpointPath : "";
+    const std::string resumePath = c.resumePath ? c.resumePath : "";
+    const auto start = Clock::now();
+    auto lastProgress = start, lastCheckpoint = start;
+    auto seconds = [](Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+
+    const int target = timeLimit > 0 ? std::numeric_limits<int>::max() : std::max(1, targetSpp);
+    int done = 0, batch = 1;
+    char msg[1024] = {0};
+    if (!resumePath.empty()) {
+        int32_t passes = 0;
+        if (ph_checkpoint_read(resumePath.c_str(), scene, base->maxDepth, base->seed, sum, sumsq, &passes, msg,
+                               sizeof(msg)) != 0) {
+            SetErr(err, errLen, msg);
+            return 1;
+        }
+        done = passes;
+    }
+    auto checkpoint = [&] {
+        if (ckptPath.empty()) return;
+        char e[1024] = {0};
+        if (ph_checkpoint_write(ckptPath.c_str(), scene, base->maxDepth, base->seed, sum, sumsq, done, e,
+                                sizeof(e)) != 0)
+            std::fprintf(stderr, "warning: %s\n", e);
+    };
+    bool cancel = c.cancel && *c.cancel;
+    while (done < target && !cancel) {
+        prgpu::RenderParamsEx rp = *base;
+        rp.spp = std::min(batch, target - done);
+        rp.firstSample = done;
+        const auto t0 = Clock::now();
+        if (p->renderEx(device, scene, &rp, sum, sumsq, msg, sizeof(msg)) != 0) {
+            SetErr(err, errLen, std::string("GPU render failed: ") + msg);
+            return 1;
+        }
+        const auto t1 = Clock::now();
+        const double dt = seconds(t0, t1);
+        done += rp.spp;
+        if (dt < 0.15) batch = std::min(batch * 2, 256);
+        else if (dt > 0.5 && batch > 1) batch /= 2;
+        const double elapsed = seconds(start, t1);
+        if (c.onProgress && (done == target || seconds(lastProgress, t1) >= c.progressInterval)) {
+            lastProgress = t1;
+            const double fraction = timeLimit > 0 ? std::min(1.0, elapsed / timeLimit) : double(done) / target;
+            if (c.onProgress(c.user, "render", fraction, elapsed, double(done)) != 0) cancel = true;
+        }
+        if (!ckptPath.empty() && seconds(lastCheckpoint, Clock::now()) >= c.checkpointInterval) {
+            lastCheckpoint = Clock::now();
+            checkpoint();
+        }
+        if (timeLimit > 0 && elapsed >= timeLimit) break;
+        if (c.cancel && *c.cancel) cancel = true;
+    }
+    checkpoint();
+    if (sppDone) *sppDone = done;
+    if (scale) *scale = done > 0 ? 1.0 / done : 0.0;
     return 0;
 }
 

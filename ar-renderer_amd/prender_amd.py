@@ -220,6 +220,15 @@ class Scene:
             self._handle = None
 
 
+PROGRESS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_char_p, C.c_double, C.c_double, C.c_double)
+
+
+class PhRenderControl(C.Structure):
+    _fields_ = [("checkpointPath", C.c_char_p), ("resumePath", C.c_char_p), ("checkpointInterval", C.c_double),
+                ("progressInterval", C.c_double), ("onProgress", PROGRESS_FN), ("user", C.c_void_p),
+                ("cancel", C.POINTER(C.c_int32))]
+
+
 def _load(path, what):
     if not os.path.exists(path):
         raise RuntimeError(f"{what} is not built: {path} is missing (run __graft_entry__.build())")
@@ -255,6 +264,14 @@ class Host:
                                           C.c_void_p, C.c_int32, C.c_char_p, C.c_int]
         L.ph_checkpoint_read.argtypes = [C.c_char_p, C.POINTER(SceneDescEx), C.c_int32, C.c_uint64, C.c_void_p,
                                          C.c_void_p, C.POINTER(C.c_int32), C.c_char_p, C.c_int]
+        L.ph_render_session.argtypes = [C.c_void_p, C.c_int, C.POINTER(SceneDescEx), C.POINTER(RenderParamsEx),
+                                        C.c_int, C.c_double, C.POINTER(PhRenderControl), C.c_void_p, C.c_void_p,
+                                        C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_char_p, C.c_int]
+        L.ph_list_devices_json.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_int]
+        L.ph_progress_json.argtypes = [C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_char_p,
+                                       C.c_int]
+        L.ph_stage_json.argtypes = [C.c_char_p, C.c_char_p, C.c_int]
+        L.ph_error_json.argtypes = [C.c_char_p, C.c_char_p, C.c_int]
 
     def generate(self, config, width=0, height=0, detail=1.0):
         err = C.create_string_buffer(512)
@@ -340,6 +357,54 @@ class Host:
             raise RuntimeError(err.value.decode())
         return s.reshape(scene.height, scene.width, 3), (sq.reshape(scene.height, scene.width, 3)
                                                           if sq is not None else None), done.value, secs.value
+
+    def render_session(self, plugin, scene, params, spp, device=0, time_limit=0.0, checkpoint=None, resume=None,
+                       checkpoint_interval=60.0, progress_interval=0.5, on_progress=None, cancel=None,
+                       variance=True):
+        """Progressive render with resume, checkpoints and progress (ph_render_session).
+
+        on_progress(stage, fraction, elapsed, spp) may return True to cancel; cancel is an optional
+        ctypes.c_int32 the caller may set from another thread. Returns (sum, sumsq, spp_done, scale).
+        """
+        shape = (scene.height, scene.width, 3)
+        s = np.zeros(shape, np.float64)
+        sq = np.zeros(shape, np.float64) if variance else None
+        cb = PROGRESS_FN(0)
+        if on_progress is not None:
+            def _cb(_user, stage, fraction, elapsed, spp_done):
+                return 1 if on_progress(stage.decode(), fraction, elapsed, spp_done) else 0
+            cb = PROGRESS_FN(_cb)
+        ctl = PhRenderControl(str(checkpoint).encode() if checkpoint else None,
+                              str(resume).encode() if resume else None, checkpoint_interval, progress_interval, cb,
+                              None, C.pointer(cancel) if cancel is not None else None)
+        done = C.c_int32()
+        scale = C.c_double()
+        err = C.create_string_buffer(1024)
+        rc = self.lib.ph_render_session(plugin, device, scene.desc_ptr(), C.byref(params), spp, time_limit,
+                                        C.byref(ctl), s.ctypes.data, sq.ctypes.data if sq is not None else None,
+                                        C.byref(done), C.byref(scale), err, 1024)
+        if rc != 0:
+            raise RuntimeError(err.value.decode())
+        return s, sq, done.value, scale.value
+
+    @staticmethod
+    def _line(fn, *args):
+        n = fn(*args, None, 0)
+        buf = C.create_string_buffer(n + 1)
+        fn(*args, buf, n + 1)
+        return buf.value.decode()
+
+    def list_devices_json(self, plugin_path=PLUGIN_PATH, cpu_threads=0):
+        return self._line(self.lib.ph_list_devices_json, plugin_path.encode() if plugin_path else None, cpu_threads)
+
+    def progress_json(self, stage, fraction, elapsed, spp, accept_rate=-1.0):
+        return self._line(self.lib.ph_progress_json, stage.encode(), fraction, elapsed, spp, accept_rate)
+
+    def stage_json(self, name):
+        return self._line(self.lib.ph_stage_json, name.encode())
+
+    def error_json(self, message):
+        return self._line(self.lib.ph_error_json, message.encode())
 
 
 class Plugin:

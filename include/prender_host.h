@@ -32,6 +32,49 @@ PH_API int ph_render(void *plugin, int device, const prgpu::SceneDescEx *scene, 
                      int targetSpp, double timeLimit, double *sum, double *sumsq, int32_t *sppDone, double *seconds,
                      char *err, int errLen);
 
+// Render session (reference GpuPathIntegrator::Render, src/integrators/gpu_path.cpp:440-513, with the
+// RenderControl of src/integrators/integrator.h:49-95) ------------------------------------------------
+// Called after every batch, at most once per progressInterval seconds, and always after the batch
+// that reaches targetSpp. stage is "render"; fraction = spp done / targetSpp, or elapsed / timeLimit
+// (at most 1) for a time-limited job. A nonzero return cancels the job after that batch.
+typedef int (*ph_progress_fn)(void *user, const char *stage, double fraction, double elapsed, double samplesPerPixel);
+
+struct PhRenderControl {
+    const char *checkpointPath;  // null or empty: no checkpoints
+    const char *resumePath;      // null or empty: start at sample 0
+    double checkpointInterval;   // seconds between periodic checkpoints
+    double progressInterval;     // seconds between progress calls (the reference uses 0.5)
+    ph_progress_fn onProgress;   // may be null
+    void *user;                  // passed to onProgress
+    volatile int32_t *cancel;    // may be null; nonzero stops the job before its next batch
+};
+
+// Renders until targetSpp samples per pixel (timeLimit <= 0) or until timeLimit seconds have passed,
+// in the adaptive batches of ph_render, adding into the caller's FP64 film sum / sumsq (w*h*3; sumsq
+// may be null). With a resume path the films and the sample count come from that checkpoint first
+// (its header must match scene, base->maxDepth and base->seed); sample s of pixel p is the same
+// function of (p, s, seed) either way, so a resumed job ends with the film of an uninterrupted one.
+// base->spp and base->firstSample are ignored: passes run from the samples already done. A checkpoint
+// is written every checkpointInterval seconds and once at the end (also after a cancel or a time
+// limit); a failed periodic or final write is reported on stderr and does not fail the job, as in the
+// reference. *sppDone = samples per pixel in the film, *scale = 1 / *sppDone (0 when none).
+// Returns 0 on success, 1 with a message in err (checkpoint rejected, render failed).
+PH_API int ph_render_session(void *plugin, int device, const prgpu::SceneDescEx *scene,
+                             const prgpu::RenderParamsEx *base, int targetSpp, double timeLimit,
+                             const PhRenderControl *control, double *sum, double *sumsq, int32_t *sppDone,
+                             double *scale, char *err, int errLen);
+
+// CLI events (reference src/cli/main.cpp:299-305, 453-468, 576-587) ----------------------------------
+// Each writes one JSON line, as the reference's nlohmann::json dump() prints it (sorted keys, no
+// spaces), into out (NUL-terminated, truncated to outLen - 1) and returns its full length.
+// The devices event: the CPU ("cpu", cpuThreads or the hardware thread count when <= 0), then the
+// devices of the plugin at pluginPath ("gpu:<i>"), and gpu_status "ok" or the loader's message.
+PH_API int ph_list_devices_json(const char *pluginPath, int cpuThreads, char *out, int outLen);
+PH_API int ph_progress_json(const char *stage, double fraction, double elapsed, double samplesPerPixel,
+                            double acceptRate, char *out, int outLen);  // accept_rate written only when >= 0
+PH_API int ph_stage_json(const char *name, char *out, int outLen);
+PH_API int ph_error_json(const char *message, char *out, int outLen);
+
 // Checkpoints (reference src/integrators/integrator.cpp:16-90) ---------------------------------------
 // Scene fingerprint stored in checkpoint headers (the quantities of src/scene/scene.cpp:61-70).
 PH_API uint64_t ph_scene_fingerprint(const prgpu::SceneDescEx *scene);
