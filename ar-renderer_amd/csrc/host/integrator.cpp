@@ -12,6 +12,7 @@
 #include <limits>
 #include <memory>
 #include <string>
+#include <vector>
 
 namespace {
 
@@ -31,6 +32,8 @@ struct Plugin {
 
 struct GeneratedScene {
     phost::SceneBuilder sb;
+    phost::SceneFileSettings settings;
+    std::vector<std::string> warnings;
 };
 
 }  // namespace
@@ -48,6 +51,48 @@ PH_API void *ph_generate(int config, int width, int height, float detail, char *
     }
     gs->sb.Desc();
     return gs.release();
+}
+
+PH_API void *ph_load_scene(const char *path, const char *rigs, int width, int height, char *err, int errLen) {
+    if (!path) {
+        SetErr(err, errLen, "invalid arguments");
+        return nullptr;
+    }
+    auto gs = std::make_unique<GeneratedScene>();
+    std::vector<std::string> rigList;
+    if (rigs) {
+        std::string r(rigs), cur;
+        for (char c : r + ",") {
+            if (c == ',') {
+                if (!cur.empty()) rigList.push_back(cur);
+                cur.clear();
+            } else if (c != ' ') {
+                cur += c;
+            }
+        }
+    }
+    std::string e;
+    if (!phost::LoadSceneFile(path, rigs ? &rigList : nullptr, width, height, &gs->sb, &gs->settings, &gs->warnings,
+                              &e)) {
+        SetErr(err, errLen, e);
+        return nullptr;
+    }
+    gs->sb.Desc();
+    return gs.release();
+}
+
+PH_API int ph_scene_warnings(void *s, char *out, int outLen) {
+    std::string all;
+    for (const std::string &w : static_cast<GeneratedScene *>(s)->warnings) all += w + "\n";
+    if (out && outLen > 0) std::snprintf(out, size_t(outLen), "%s", all.c_str());
+    return int(all.size());
+}
+
+PH_API void ph_scene_file_settings(void *s, float *clampLuminance, double *timeLimit, int32_t *flags) {
+    const phost::SceneFileSettings &f = static_cast<GeneratedScene *>(s)->settings;
+    if (clampLuminance) *clampLuminance = f.clampLuminance;
+    if (timeLimit) *timeLimit = f.timeLimit;
+    if (flags) *flags = (f.russianRoulette ? 1 : 0) | (f.lightBVH ? 2 : 0);
 }
 
 PH_API void ph_scene_free(void *s) { delete static_cast<GeneratedScene *>(s); }

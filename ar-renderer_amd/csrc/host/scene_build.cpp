@@ -470,6 +470,30 @@ int SceneBuilder::Piecewise(const std::vector<float> &l, const std::vector<float
     return AddSpectrum(s);
 }
 
+int SceneBuilder::Table95(const std::vector<float> &values) {
+    prgpu::SpectrumEx s{};
+    s.kind = prgpu::SpecTable95;
+    s.scale = 1;
+    s.offset = int(knots.size());
+    s.count = 95;
+    knots.insert(knots.end(), values.begin(), values.begin() + 95);
+    return AddSpectrum(s);
+}
+
+int SceneBuilder::D65Illuminant() {
+    if (d65_ < 0) d65_ = Piecewise(D65().l, D65().v);
+    return d65_;
+}
+
+int SceneBuilder::Cauchy(float A, float B) {
+    prgpu::SpectrumEx s{};
+    s.kind = prgpu::SpecCauchy;
+    s.scale = 1;
+    s.p[0] = A;
+    s.p[1] = B;
+    return AddSpectrum(s);
+}
+
 // Measured optical constants sampled across the visible range (the reference's named metals,
 // src/core/spectrum.cpp:164-189).
 int SceneBuilder::Metal(const std::string &name, bool k) {
@@ -615,7 +639,7 @@ int SceneBuilder::Interface() {
     return int(materials.size() - 1);
 }
 
-int SceneBuilder::AddMesh(const Mesh &mesh, int material, int mIn, int mOut, bool normals) {
+int SceneBuilder::AddMesh(const Mesh &mesh, int material, int mIn, int mOut, bool normals, bool wrapU) {
     int first = int(prims.size());
     for (size_t t = 0; t + 2 < mesh.idx.size(); t += 3) {
         prgpu::Primitive p{};
@@ -643,7 +667,7 @@ int SceneBuilder::AddMesh(const Mesh &mesh, int material, int mIn, int mOut, boo
             // A triangle that straddles the u seam of a wrapped parameterisation keeps a
             // continuous u (its tangent frame stays smooth).
             float lo = std::min(p.u0, std::min(p.u1, p.u2)), hi = std::max(p.u0, std::max(p.u1, p.u2));
-            if (hi - lo > 0.5f) {
+            if (wrapU && hi - lo > 0.5f) {
                 if (p.u0 < 0.5f) p.u0 += 1;
                 if (p.u1 < 0.5f) p.u1 += 1;
                 if (p.u2 < 0.5f) p.u2 += 1;

@@ -81,6 +81,9 @@ class SceneBuilder {
     int Glass(const std::string &name);           // Sellmeier IOR: BK7, SF11, fused_silica, diamond, sapphire
     int Sellmeier(const float *B, const float *C, int n);
     int Piecewise(const std::vector<float> &lambdas, const std::vector<float> &values, float scale = 1.f);
+    int Table95(const std::vector<float> &values);  // 95 values, 360..830 nm every 5 nm
+    int D65Illuminant();                            // CIE D65 normalised to luminance Y = 1 (one per scene)
+    int Cauchy(float A, float Bnm2);                // IOR A + B / lambda^2 (lambda in nm)
 
     // ---- materials ----
     int Diffuse(int reflectance);
@@ -106,7 +109,10 @@ class SceneBuilder {
     void SetSheen(int material, int color, float roughness, float weight);
 
     // ---- geometry ----
-    int AddMesh(const Mesh &m, int material, int mediumInside = -1, int mediumOutside = -1, bool normals = true);
+    // wrapU: a triangle whose u values straddle the seam of a wrapped parameterisation (the
+    // generators' UV spheres) gets a continuous u; meshes from files keep their uvs verbatim.
+    int AddMesh(const Mesh &m, int material, int mediumInside = -1, int mediumOutside = -1, bool normals = true,
+                bool wrapU = true);
     int AddSphere(F3 c, float r, int material, int mediumInside = -1, int mediumOutside = -1);
     // Every triangle of the mesh becomes its own area light (src/scene/scene_loader.cpp:785-801).
     void AddEmitter(const Mesh &m, int spectrum, float scale, bool twoSided = false);
@@ -140,6 +146,7 @@ class SceneBuilder {
 
   private:
     int AddSpectrum(const prgpu::SpectrumEx &s);
+    int d65_ = -1;
     prgpu::SceneDescEx desc_{};
 };
 
@@ -149,6 +156,18 @@ float SpectrumValue(const prgpu::SpectrumEx &s, const float *knots, float lambda
 void FitRGBSigmoid(const float rgb[3], float coeffs[3]);
 // Linear sRGB of a film XYZ value, with the renderer's spectral white adaptation (src/core/film.cpp:170-175).
 void XYZToLinearSRGB(const double xyz[3], double scale, float rgb[3]);
+
+// Settings of a scene file that the flat scene does not hold (src/scene/scene.h:29-42).
+struct SceneFileSettings {
+    float clampLuminance = 0;  // firefly clamp (film.clamp)
+    double timeLimit = 0;      // seconds; 0 = none
+    bool russianRoulette = true;
+    bool lightBVH = true;      // false: light selection by power
+};
+// Loads a .prscene.json file (with its includes, meshes and the given or active light rigs) into
+// sb, at width x height (0: the file's film size). Returns false with the reason in err.
+bool LoadSceneFile(const std::string &path, const std::vector<std::string> *rigs, int width, int height,
+                   SceneBuilder *sb, SceneFileSettings *settings, std::vector<std::string> *warnings, std::string *err);
 
 struct GenOptions {
     int width = 0, height = 0;  // 0: the configuration's resolution

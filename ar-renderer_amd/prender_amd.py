@@ -186,6 +186,7 @@ class Scene:
         self.defaults = defaults or {"spp": 16, "maxDepth": -1, "rrStartDepth": 1, "seed": 1}
         self._handle = handle
         self._host = host
+        self.warnings, self.settings = [], {}
         self.desc = SceneDescEx()
         d = self.desc
         d.apiVersion, d.extVersion = API_VERSION, EXT_VERSION
@@ -243,6 +244,11 @@ class Host:
         L.ph_generate.restype = C.c_void_p
         L.ph_generate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_char_p, C.c_int]
         L.ph_scene_free.argtypes = [C.c_void_p]
+        L.ph_load_scene.restype = C.c_void_p
+        L.ph_load_scene.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_char_p, C.c_int]
+        L.ph_scene_warnings.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+        L.ph_scene_file_settings.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_double),
+                                             C.POINTER(C.c_int32)]
         L.ph_scene_desc.restype = C.POINTER(SceneDescEx)
         L.ph_scene_desc.argtypes = [C.c_void_p]
         L.ph_scene_defaults.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_uint64)]
@@ -278,6 +284,29 @@ class Host:
         h = self.lib.ph_generate(config, width, height, detail, err, 512)
         if not h:
             raise RuntimeError(err.value.decode())
+        return self._take_scene(h)
+
+    def load_scene(self, path, rigs=None, width=0, height=0):
+        """Loads a .prscene.json file (ph_load_scene). rigs: None = the file's active rigs, else a
+        list of rig names. Returns a Scene whose .warnings and .settings hold the loader's warnings and
+        the file's render settings (clamp, time_limit, russian_roulette, light_bvh)."""
+        err = C.create_string_buffer(2048)
+        r = None if rigs is None else ",".join(rigs).encode()
+        h = self.lib.ph_load_scene(str(path).encode(), r, width, height, err, 2048)
+        if not h:
+            raise RuntimeError(err.value.decode())
+        n = self.lib.ph_scene_warnings(h, None, 0)
+        buf = C.create_string_buffer(n + 1)
+        self.lib.ph_scene_warnings(h, buf, n + 1)
+        clamp, limit, flags = C.c_float(), C.c_double(), C.c_int32()
+        self.lib.ph_scene_file_settings(h, C.byref(clamp), C.byref(limit), C.byref(flags))
+        s = self._take_scene(h)
+        s.warnings = [w for w in buf.value.decode().split("\n") if w]
+        s.settings = {"clamp": clamp.value, "time_limit": limit.value, "russian_roulette": bool(flags.value & 1),
+                      "light_bvh": bool(flags.value & 2)}
+        return s
+
+    def _take_scene(self, h):
         d = self.lib.ph_scene_desc(h).contents
 
         def arr(ptr, n, dt):

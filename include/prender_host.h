@@ -12,6 +12,18 @@
 // with geometry density `detail` (1 = full). Returns a handle or null with a message in err.
 PH_API void *ph_generate(int config, int width, int height, float detail, char *err, int errLen);
 PH_API void ph_scene_free(void *scene);
+// Loads a scene file (reference src/scene/scene_loader.cpp:1264-1386): a .prscene.json document with
+// its includes, OBJ / PLY meshes and .vol grids, plus light rigs -- rigs == null: the file's
+// active_rigs; otherwise a comma-separated list replacing them ("" = none). width / height > 0
+// override the film size. Returns a scene handle (ph_scene_desc, ph_scene_defaults and
+// ph_scene_free apply) or null with the reason in err. What the flat scene cannot express is
+// either dropped with a warning, as the reference's GPU exporter drops it, or a load error.
+PH_API void *ph_load_scene(const char *path, const char *rigs, int width, int height, char *err, int errLen);
+// Warnings of a loaded scene, one per line; returns the full length (out may be short or null).
+PH_API int ph_scene_warnings(void *scene, char *out, int outLen);
+// Render settings of a loaded scene: film clamp, time limit (s), flags bit 0 = Russian roulette,
+// bit 1 = light-BVH light selection (else by power).
+PH_API void ph_scene_file_settings(void *scene, float *clampLuminance, double *timeLimit, int32_t *flags);
 PH_API const prgpu::SceneDescEx *ph_scene_desc(void *scene);
 // Render defaults of a generated scene: out = {spp, maxDepth, rrStartDepth}; *seed.
 PH_API void ph_scene_defaults(void *scene, int32_t *out, uint64_t *seed);
