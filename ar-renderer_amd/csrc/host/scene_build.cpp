@@ -721,6 +721,22 @@ int SceneBuilder::AddPointLight(F3 p, int spectrum, float scale) {
     return int(lights.size() - 1);
 }
 
+int SceneBuilder::AddSpotLight(F3 p, F3 dir, int spectrum, float scale, float width, float start) {
+    prgpu::LightEx l{};
+    l.kind = prgpu::LightExSpot;
+    l.primitive = -1;
+    l.spectrum = spectrum;
+    l.scale = scale;
+    l.position = pf(p);
+    float len = std::sqrt(dot3(dir, dir));
+    l.direction = pf(f3(dir.x / len, dir.y / len, dir.z / len));
+    const float rad = kPi / 180.f;
+    l.cosFalloffEnd = std::cos(rad * width);
+    l.cosFalloffStart = std::cos(rad * std::min(start, width));
+    lights.push_back(l);
+    return int(lights.size() - 1);
+}
+
 int SceneBuilder::AddDistantLight(F3 w, int spectrum, float scale) {
     prgpu::LightEx l{};
     l.kind = prgpu::LightExDistant;

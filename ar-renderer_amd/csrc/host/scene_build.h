@@ -120,6 +120,9 @@ class SceneBuilder {
     // ---- lights, media, camera ----
     int AddPointLight(F3 p, int spectrum, float scale);
     int AddDistantLight(F3 towardsLight, int spectrum, float scale);
+    // Spot light at p along dir (normalised here); the cone is given in degrees, as the reference's
+    // SpotLight takes it (src/lights/light.cpp:161-166), and stored as FP32 cosines.
+    int AddSpotLight(F3 p, F3 dir, int spectrum, float scale, float totalWidthDeg, float falloffStartDeg);
     int AddEnvironment(int spectrum, float scale);
     int AddHomogeneous(int sigmaA, int sigmaS, float scale, float g);
     int AddGrid(int sigmaA, int sigmaS, float scale, float g, int nx, int ny, int nz, const std::vector<float> &data,

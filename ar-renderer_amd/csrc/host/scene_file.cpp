@@ -5,8 +5,8 @@
 //
 // What the exporter approximates or drops is approximated or dropped here, with the same warning
 // (checker textures keep both colours and the u scale; roughness textures become the default;
-// normal and bump maps are ignored; spot, IES, sun, sun-sky and image-environment lights are
-// skipped). Distant lights, which the v4 exporter also skips, are kept: the extension has them. Everything the renderer cannot express and the exporter has no fallback for (image and
+// normal and bump maps are ignored; IES, sun, sun-sky and image-environment lights are skipped).
+// Distant and spot lights, which the v4 exporter also skips, are kept: the extension has them. Everything the renderer cannot express and the exporter has no fallback for (image and
 // noise colour textures, orthographic and realistic cameras, glTF / pbrt imports, noise media) is a
 // load error naming the item. Illuminant RGB spectra (sigmoid x D65) are tabulated every 5 nm, as
 // the exporter tabulates every spectrum.
@@ -874,7 +874,7 @@ bool MeshContains(const SceneBuilder &sb, int first, int count, F3 pt) {
 // ---- the loader ----------------------------------------------------------------------------------
 enum class SpecType { Albedo, Unbounded, Illuminant };
 
-const char *const kSkippedLights = "spot, IES, sun, sun-sky and image-environment lights are skipped on the GPU";
+const char *const kSkippedLights = "IES, sun, sun-sky and image-environment lights are skipped on the GPU";
 
 struct Tex {  // a colour input as the exporter flattens it: spectrum, checker second colour, u scale
     int spec = -1, second = -1;
@@ -1510,16 +1510,22 @@ class Loader {
         const std::string type = GetS(l, "type", "");
         const float scaleMul = GetF(l, "scale", 1.f);
         if (type == "point" || type == "spot") {
-            if (type == "spot" || l.has("ies")) {
+            if (type == "point" && l.has("ies")) {
                 Warn(kSkippedLights);
                 return;
             }
             int I = LightSpectrum(l);
             F3 pos = Vec3(l.at("position"), "position");
             float scale = GetF(l, "intensity", 1.f) * scaleMul;
-            if (l.has("power")) scale = GetF(l, "power", 1.f) / (4 * 3.14159265358979323846f * sb_.SpectrumY(I));
             if (EnclosingMedium(pos) >= 0) Warn("light '" + id + "': point lights inside media are not attenuated on the GPU");
-            sb_.AddPointLight(pos, I, scale);
+            if (type == "point") {
+                if (l.has("power")) scale = GetF(l, "power", 1.f) / (4 * 3.14159265358979323846f * sb_.SpectrumY(I));
+                sb_.AddPointLight(pos, I, scale);
+            } else {
+                F3 dir = l.has("look_at") ? Vec3(l.at("look_at"), "look_at") - pos : Vec3(l.at("direction"), "direction");
+                float width = GetF(l, "cone_angle", 30.f), start = GetF(l, "falloff_start", width * 0.8f);
+                sb_.AddSpotLight(pos, dir, I, scale, width, start);
+            }
             return;
         }
         if (type == "distant" || type == "directional") {

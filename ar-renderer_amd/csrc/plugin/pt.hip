@@ -309,7 +309,10 @@ void Validate(const prgpu::SceneDescEx &sd) {
         const prgpu::LightEx &l = sd.lights[i];
         if (!specOK(l.spectrum, false)) bad("light spectrum");
         if (l.kind == prgpu::LightExArea && (l.primitive < 0 || l.primitive >= sd.primCount)) bad("area light primitive");
-        if (l.kind > prgpu::LightExEnvironment) bad("light kind");
+        if (l.kind > prgpu::LightExSpot) bad("light kind");
+        if (l.kind == prgpu::LightExSpot &&
+            !(l.cosFalloffEnd >= -1 && l.cosFalloffEnd <= l.cosFalloffStart && l.cosFalloffStart <= 1))
+            bad("spot light cone");
     }
     for (int i = 0; i < sd.mediumCount; ++i) {
         const prgpu::MediumEx &m = sd.media[i];
@@ -418,6 +421,16 @@ void EnsurePTScene(Device &dev, const prgpu::SceneDescEx &sd, const float *sheen
             x.cosTheta_e = 0;
             lb = x;
             pw = 4 * kPi * l.scale * host_spectrum_y(st, l.spectrum, pt->cieYInt);
+        } else if (l.kind == prgpu::LightExSpot) {  // light.cpp:117-125, 173-175
+            HostLB x;
+            x.bounds = Box3(f3(l.position), f3(l.position));
+            x.w = f3(l.direction);
+            x.phi = 4 * kPi * l.scale * spec_max(st, l.spectrum);
+            x.cosTheta_o = l.cosFalloffStart;
+            x.cosTheta_e = std::cos(safe_acos(l.cosFalloffEnd) - safe_acos(l.cosFalloffStart));
+            lb = x;
+            pw = l.scale * host_spectrum_y(st, l.spectrum, pt->cieYInt) * 2 * kPi *
+                 ((1 - l.cosFalloffStart) + (l.cosFalloffStart - l.cosFalloffEnd) / 2);
         } else if (l.kind == prgpu::LightExDistant) {
             pw = l.scale * host_spectrum_y(st, l.spectrum, pt->cieYInt) * kPi * pt->worldR * pt->worldR;
         } else {

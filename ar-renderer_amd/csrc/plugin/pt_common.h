@@ -32,6 +32,12 @@ PG_HD float lerp(float t, float a, float b) { return (1 - t) * a + t * b; }
 PG_HD float safe_sqrt(float x) { return sqrtf(maxf(0.f, x)); }
 PG_HD float safe_asin(float x) { return asinf(clampf(x, -1.f, 1.f)); }
 PG_HD float safe_acos(float x) { return acosf(clampf(x, -1.f, 1.f)); }
+// Hermite smoothstep of x between a and b, a step at a when a == b (src/core/math.h:32-36).
+PG_HD float smooth_step(float x, float a, float b) {
+    if (a == b) return x < a ? 0.f : 1.f;
+    float t = clampf((x - a) / (b - a), 0.f, 1.f);
+    return t * t * (3 - 2 * t);
+}
 PG_HD bool is_finite(float x) { return __builtin_isfinite(x); }
 PG_HD bool is_inf(float x) { return __builtin_isinf(x); }
 PG_HD bool is_nan(float x) { return __builtin_isnan(x); }

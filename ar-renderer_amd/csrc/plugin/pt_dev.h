@@ -1210,6 +1210,16 @@ PG_HD bool sample_li(const DevScene &s, int li, V3 p, V2 u, const Lambda &lam, L
         *ls = {spec_sample(s.spec, L.spectrum, lam) * (L.scale / d2), d / sqrtf(d2), 1, f3(L.position), V3()};
         return true;
     }
+    if (L.kind == prgpu::LightExSpot) {  // light.cpp:168-192
+        V3 d = f3(L.position) - p;
+        float d2 = len2(d);
+        if (d2 == 0) return false;
+        V3 wi = d / sqrtf(d2);
+        float f = smooth_step(dot(-wi, f3(L.direction)), L.cosFalloffEnd, L.cosFalloffStart);
+        if (f == 0) return false;
+        *ls = {spec_sample(s.spec, L.spectrum, lam) * (L.scale * f / d2), wi, 1, f3(L.position), V3()};
+        return true;
+    }
     if (L.kind == prgpu::LightExDistant) {
         V3 w = f3(L.direction);
         *ls = {spec_sample(s.spec, L.spectrum, lam) * L.scale, w, 1, p + w * (2 * s.worldR), V3()};
@@ -1229,7 +1239,7 @@ PG_HD float pdf_li(const DevScene &s, int li, V3 p, V3 wi) {
 
 PG_HD bool light_is_delta(const DevScene &s, int li) {
     uint32_t k = s.lights[li].kind;
-    return k == prgpu::LightExPoint || k == prgpu::LightExDistant;
+    return k == prgpu::LightExPoint || k == prgpu::LightExDistant || k == prgpu::LightExSpot;
 }
 
 PG_HD float p_infinite(const DevScene &s) {

@@ -165,7 +165,7 @@ struct DeviceInfo {  // gpu_api.h:156-162
 // ---------------------------------------------------------------------------------------------
 // Part 2: extension (no counterpart in the reference ABI; see the file comment).
 
-constexpr int kExtVersion = 1;
+constexpr int kExtVersion = 2;
 
 // Analytic spectrum: value(lambda) = scale * base(lambda). Mirrors the reference Spectrum
 // classes (src/core/spectrum.h:151-231, src/core/rgb2spec.h:13-74) so the device evaluates
@@ -192,17 +192,22 @@ enum LightKindEx : uint32_t {
     LightExPoint = 1,        // PointLight at `position`
     LightExDistant = 2,      // DistantLight arriving from `direction` (src/lights/sky.cpp:52-60)
     LightExEnvironment = 3,  // constant EnvironmentLight (src/lights/light.cpp:215-297)
+    LightExSpot = 4,         // SpotLight at `position` pointing along `direction` (src/lights/light.cpp:117-212)
 };
 
-struct LightEx {  // 48 bytes
+struct LightEx {  // 64 bytes
     uint32_t kind;
     int32_t primitive;
     int32_t spectrum;
     float scale;
-    Float3 position;   // point light
-    Float3 direction;  // distant light: unit vector towards the light
+    Float3 position;   // point and spot lights
+    Float3 direction;  // distant light: unit vector towards the light; spot light: unit axis
     uint32_t twoSided;
     float cosPower;
+    // spot light: cosines of the falloff start and of the total width (cos(radians(deg)) in FP32);
+    // intensity is scaled by smoothstep(cos theta, cosFalloffEnd, cosFalloffStart)
+    float cosFalloffStart, cosFalloffEnd;
+    uint32_t reserved[2];
 };
 
 enum MediumKindEx : uint32_t { MediumHomogeneous = 0, MediumGrid = 1 };
@@ -396,7 +401,7 @@ static_assert(sizeof(prgpu::RenderParams) == 24, "RenderParams");
 static_assert(__builtin_offsetof(prgpu::RenderParams, seed) == 8, "RenderParams.seed");
 static_assert(sizeof(prgpu::DeviceInfo) == 304, "DeviceInfo");
 static_assert(sizeof(prgpu::SpectrumEx) == 48, "SpectrumEx");
-static_assert(sizeof(prgpu::LightEx) == 48, "LightEx");
+static_assert(sizeof(prgpu::LightEx) == 64, "LightEx");
 static_assert(sizeof(prgpu::MediumEx) == 120, "MediumEx");
 static_assert(sizeof(prgpu::SampleRecordEx) == 64, "SampleRecordEx");
 static_assert(sizeof(prgpu::ProfileStatsEx) == 256, "ProfileStatsEx");

@@ -207,12 +207,13 @@ class OracleScene:
 
     def light_sample(self, p, n, u, bvh=True):
         pmf = C.c_float()
-        li = lib().or_light_sample(self.h, 1 if bvh else 0, f32(*p).ctypes.data, f32(*n).ctypes.data, u,
-                                   C.byref(pmf))
+        pv, nv = f32(*p), f32(*n)  # named: the arrays must outlive the call that reads their addresses
+        li = lib().or_light_sample(self.h, 1 if bvh else 0, pv.ctypes.data, nv.ctypes.data, u, C.byref(pmf))
         return li, pmf.value
 
     def light_pmf(self, p, n, light, bvh=True):
-        return lib().or_light_pmf(self.h, 1 if bvh else 0, f32(*p).ctypes.data, f32(*n).ctypes.data, light)
+        pv, nv = f32(*p), f32(*n)
+        return lib().or_light_pmf(self.h, 1 if bvh else 0, pv.ctypes.data, nv.ctypes.data, light)
 
     def camera_ray(self, fx, fy, lx=0.5, ly=0.5):
         out = np.zeros(6, np.float32)

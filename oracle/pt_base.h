@@ -36,6 +36,12 @@ inline float lerpf(float t, float a, float b) { return (1 - t) * a + t * b; }  /
 inline float safe_sqrt(float x) { return std::sqrt(std::max(0.f, x)); }        // math.h:27
 inline float safe_asin(float x) { return std::asin(clampv(x, -1.f, 1.f)); }
 inline float safe_acos(float x) { return std::acos(clampv(x, -1.f, 1.f)); }
+// Hermite smoothstep between a and b; a step at a when a == b (src/core/math.h:32-36).
+inline float smooth_step(float x, float a, float b) {
+    if (a == b) return x < a ? 0.f : 1.f;
+    float t = clampv((x - a) / (b - a), 0.f, 1.f);
+    return t * t * (3 - 2 * t);
+}
 inline float deg2rad(float d) { return (kPi / 180.f) * d; }
 inline float rad2deg(float r) { return (180.f / kPi) * r; }
 

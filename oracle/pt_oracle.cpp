@@ -1477,7 +1477,9 @@ struct Scene;
 struct LightObj {
     LightEx d;
     float area = 0;
-    bool delta() const { return d.kind == prgpu::LightExPoint || d.kind == prgpu::LightExDistant; }
+    bool delta() const {
+        return d.kind == prgpu::LightExPoint || d.kind == prgpu::LightExDistant || d.kind == prgpu::LightExSpot;
+    }
     bool infinite_type() const { return d.kind == prgpu::LightExEnvironment; }
 };
 
@@ -1762,6 +1764,15 @@ struct Scene {
             lb.cosTheta_e = 0;
             return lb;
         }
+        case prgpu::LightExSpot: {  // light.cpp:117-125
+            LightB lb;
+            lb.bounds = Box{v3(L.d.position), v3(L.d.position)};
+            lb.w = v3(L.d.direction);
+            lb.phi = 4 * kPi * L.d.scale * spec.max_value(L.d.spectrum);
+            lb.cosTheta_o = L.d.cosFalloffStart;
+            lb.cosTheta_e = std::cos(safe_acos(L.d.cosFalloffEnd) - safe_acos(L.d.cosFalloffStart));
+            return lb;
+        }
         default: return std::nullopt;  // distant and environment lights
         }
     }
@@ -1773,6 +1784,9 @@ struct Scene {
             return L.d.scale * spec.to_y(L.d.spectrum) * L.area * lobe * (L.d.twoSided ? 2 : 1);
         }
         case prgpu::LightExPoint: return 4 * kPi * L.d.scale * spec.to_y(L.d.spectrum);
+        case prgpu::LightExSpot:  // light.cpp:173-175
+            return L.d.scale * spec.to_y(L.d.spectrum) * 2 * kPi *
+                   ((1 - L.d.cosFalloffStart) + (L.d.cosFalloffStart - L.d.cosFalloffEnd) / 2);
         case prgpu::LightExDistant: return L.d.scale * spec.to_y(L.d.spectrum) * kPi * worldRadius * worldRadius;
         default: return 4 * kPi * kPi * worldRadius * worldRadius * L.d.scale * spec.to_y(L.d.spectrum);
         }
@@ -1808,6 +1822,20 @@ struct Scene {
             LightSample ls;
             ls.wi = d / std::sqrt(d2);
             ls.L = spec.sample(L.d.spectrum, lam) * (L.d.scale / d2);
+            ls.pdf = 1;
+            ls.pLight.p = v3(L.d.position);
+            return ls;
+        }
+        case prgpu::LightExSpot: {  // light.cpp:168-192
+            V3 d = v3(L.d.position) - p;
+            float d2 = len2(d);
+            if (d2 == 0) return std::nullopt;
+            V3 wi = d / std::sqrt(d2);
+            float f = smooth_step(dot(-wi, v3(L.d.direction)), L.d.cosFalloffEnd, L.d.cosFalloffStart);
+            if (f == 0) return std::nullopt;
+            LightSample ls;
+            ls.wi = wi;
+            ls.L = spec.sample(L.d.spectrum, lam) * (L.d.scale * f / d2);
             ls.pdf = 1;
             ls.pLight.p = v3(L.d.position);
             return ls;

@@ -56,6 +56,12 @@ SpectrumPtr MakeSpectrum(const prgpu::SpectrumEx &s, const float *knots) {
         break;
     }
     case prgpu::SpecCauchy: base = std::make_shared<CauchyIOR>(s.p[0], s.p[1]); break;
+    case prgpu::SpecTable95: {  // 95 nodes every 5 nm from 360 nm, linear between them
+        std::vector<float> l(95), v(knots + s.offset, knots + s.offset + 95);
+        for (int i = 0; i < 95; ++i) l[size_t(i)] = 360.f + 5.f * float(i);
+        base = std::make_shared<PiecewiseLinearSpectrum>(l, v);
+        break;
+    }
     default: return nullptr;
     }
     if (s.scale != 1.f) return std::make_shared<ScaledSpectrum>(s.scale, base);
@@ -302,6 +308,11 @@ void *rf_scene_create(const prgpu::SceneDescEx *d, float fovY) {
             light = al;
         } else if (l.kind == prgpu::LightExPoint) {
             light = std::make_shared<PointLight>(V(l.position), sp, l.scale);
+        } else if (l.kind == prgpu::LightExSpot) {
+            // the flat scene keeps the cone as cosines; the reference takes degrees
+            float width = std::acos(l.cosFalloffEnd) * 180.f / 3.14159265358979f;
+            float start = std::acos(l.cosFalloffStart) * 180.f / 3.14159265358979f;
+            light = std::make_shared<SpotLight>(V(l.position), V(l.direction), sp, l.scale, width, start);
         } else if (l.kind == prgpu::LightExDistant) {
             light = std::make_shared<DistantLight>(V(l.direction), sp, l.scale);
         } else {

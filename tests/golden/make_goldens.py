@@ -307,10 +307,45 @@ def main():
     L.rf_scene_destroy(h)
 
     closures(L, out)
+    spot_lights(L, out, host)
 
     path = os.path.join(HERE, "ref_components.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")
+
+
+def spot_lights(L, out, host):
+    """Spot lights (src/lights/light.cpp:117-212) beside point and area lights: light-BVH choice and
+    PMFs, power PMFs and SampleLi / PDF_Li, from the scene file tests/scenes/spots."""
+    rng = np.random.default_rng(4117)
+    scene = host.load_scene(os.path.join(HERE, "..", "scenes", "spots", "scene.prscene.json"))
+    h = L.rf_scene_create(C.byref(scene.desc), 40.0)
+    n_l = len(scene.lights)
+    P = f32(rng.uniform([-3, 0, -3], [3, 2.5, 3], (400, 3)))
+    N = rng.normal(size=(400, 3)).astype(np.float32)
+    N /= np.linalg.norm(N, axis=1, keepdims=True)
+    N[::4] = 0
+    U = f32(rng.random(400))
+    li = np.zeros(400, np.int32)
+    lpmf = np.zeros(400, np.float32)
+    each = np.zeros((400, n_l), np.float32)
+    for i in range(400):
+        pm = np.zeros(1, np.float32)
+        li[i] = L.rf_light_sample(h, ptr(P[i]), ptr(N[i]), float(U[i]), ptr(pm))
+        lpmf[i] = pm[0]
+        for k in range(n_l):
+            each[i, k] = L.rf_light_pmf(h, ptr(P[i]), ptr(N[i]), k)
+    LU = f32(rng.random((400, 2)))
+    LL = f32(rng.random(400))
+    lsi = np.zeros((400, 9), np.float32)
+    for i in range(400):
+        o9 = np.zeros(9, np.float32)
+        L.rf_light_sample_li(h, i % n_l, ptr(P[i]), ptr(N[i]), ptr(LU[i]), float(LL[i]), ptr(o9))
+        lsi[i] = o9
+    out.update(spot_p=P, spot_n=N, spot_u=U, spot_light=li, spot_light_pmf=lpmf, spot_pmf_each=each,
+               spot_power_pmf=f32([L.rf_light_power_pmf(h, k) for k in range(n_l)]), spot_lsu=LU, spot_lsl=LL,
+               spot_lsi=lsi)
+    L.rf_scene_destroy(h)
 
 
 def closures(L, out):

@@ -215,3 +215,39 @@ def test_c1_area_light_sample_li(c1_small):
             np.testing.assert_allclose(out[:8], ref[:8], rtol=2e-5, atol=1e-7, err_msg=f"sample {i}")
             got = L.or_light_pdf_li(o.h, k, p(f32(P[i])), p(f32(ref[4:7])))
             np.testing.assert_allclose(got, G["c1_lpdf"][i], rtol=2e-5)
+
+
+@pytest.fixture(scope="module")
+def spots(host):
+    s = host.load_scene(os.path.join(os.path.dirname(__file__), "scenes", "spots", "scene.prscene.json"))
+    return s, orc.OracleScene(s)
+
+
+def test_spot_lights_match_reference(spots):
+    """Spot lights (reference src/lights/light.cpp:117-212: smoothstep falloff between the cone's
+    cosines, cone bounds in the light BVH, PowerY) next to point and area lights: light-BVH choice
+    and every light's PMF, power PMFs, and SampleLi against the reference's own code."""
+    s, o = spots
+    kinds = s.lights["kind"]
+    assert (kinds == pa.LIGHT_SPOT).sum() == 3
+    P, N, U = G["spot_p"], G["spot_n"], G["spot_u"]
+    for i in range(len(P)):
+        li, pmf = o.light_sample(P[i], N[i], float(U[i]))
+        assert li == G["spot_light"][i], i
+        np.testing.assert_allclose(pmf, G["spot_light_pmf"][i], rtol=1e-5, atol=1e-9)
+        each = [o.light_pmf(P[i], N[i], k) for k in range(len(s.lights))]
+        np.testing.assert_allclose(each, G["spot_pmf_each"][i], rtol=1e-5, atol=1e-9)
+    pw = np.array([L.or_light_power(o.h, k) for k in range(len(s.lights))], np.float64)
+    np.testing.assert_allclose(pw / pw.sum(), G["spot_power_pmf"], rtol=1e-5)
+    lit = 0
+    for i in range(len(P)):
+        k = i % len(s.lights)
+        lam, pdf = sample_visible(G["spot_lsl"][i])
+        out = np.zeros(15, np.float32)
+        L.or_light_sample_li(o.h, k, p(f32(P[i])), p(f32(N[i])), p(f32(G["spot_lsu"][i])), p(lam), p(pdf), p(out))
+        ref = G["spot_lsi"][i]
+        assert out[14] == ref[8], i
+        if ref[8]:
+            lit += kinds[k] == pa.LIGHT_SPOT
+            np.testing.assert_allclose(out[:8], ref[:8], rtol=2e-5, atol=1e-7, err_msg=f"sample {i}")
+    assert lit > 50  # spot lights sampled inside and across their falloff

@@ -68,9 +68,10 @@ def test_studio_counts_rigs_and_settings(host, studio):
     # the lamps rig adds the key rect (2) and the bulb sphere
     assert len(s.prims) == 2 + 5 + 12 + 12 + 9 + 6 + 1 + 2 + 1
     kinds = sorted(s.lights["kind"].tolist())
-    # sign triangle, key rect (2), bulb, sun (distant), sky; the spot is skipped, "off" is disabled
-    assert kinds == sorted([pa.LIGHT_AREA] * 4 + [pa.LIGHT_DISTANT, pa.LIGHT_ENV])
-    assert any("spot" in w for w in s.warnings) and any("roughness textures" in w for w in s.warnings)
+    # sign triangle, key rect (2), bulb, sun (distant), sky, spot; "off" is disabled
+    assert kinds == sorted([pa.LIGHT_AREA] * 4 + [pa.LIGHT_DISTANT, pa.LIGHT_ENV, pa.LIGHT_SPOT])
+    assert any("roughness textures" in w for w in s.warnings)
+    assert not any("spot" in w for w in s.warnings)
     assert (s.width, s.height) == (96, 64)
     assert s.camera.filterType == pa.FILTER_GAUSSIAN and s.camera.filterRadius == pytest.approx(1.25)
     assert s.defaults == {"spp": 32, "maxDepth": 12, "rrStartDepth": 1, "seed": 11}
@@ -280,6 +281,27 @@ def test_two_constant_environments_add(host, tmp_path):
                     for sc in singles)
         got = env[0]["scale"] * host.spectrum_value(both.spectra[env[0]["spectrum"]], both.knots, lam)
         assert got == pytest.approx(total, rel=1e-5)
+
+
+def test_spot_light_cones(host, studio):
+    """cone_angle is the total width, falloff_start defaults to 0.8 of it and is capped at it, both in
+    degrees, stored as FP32 cosines; the axis comes from direction or look_at (scene_loader.cpp:958-962,
+    light.cpp:161-166)."""
+    spot = studio.lights[studio.lights["kind"] == pa.LIGHT_SPOT][0]
+    assert spot["cosFalloffEnd"] == np.float32(math.cos(math.radians(30)))
+    assert spot["cosFalloffStart"] == pytest.approx(math.cos(math.radians(24)), rel=1e-6)
+    assert np.allclose(spot["direction"], [0, -1, 0]) and np.allclose(spot["position"], [0, 3, 0])
+    s = host.load_scene(os.path.join(HERE, "scenes", "spots", "scene.prscene.json"))
+    spots = s.lights[s.lights["kind"] == pa.LIGHT_SPOT]
+    hard = [l for l in spots if np.isclose(l["scale"], 35)][0]
+    assert hard["cosFalloffStart"] == hard["cosFalloffEnd"] == np.float32(math.cos(math.radians(12)))
+    d = np.array([-0.6, -1, -0.2])
+    assert np.allclose(hard["direction"], d / np.linalg.norm(d), atol=1e-7)
+    soft = [l for l in spots if np.isclose(l["scale"], 20)][0]
+    t = np.array([0, 0, 0]) - np.array([-2, 3, 1])
+    assert np.allclose(soft["direction"], t / np.linalg.norm(t), atol=1e-7)
+    assert soft["cosFalloffStart"] == pytest.approx(math.cos(math.radians(30)), rel=1e-6)
+    assert soft["cosFalloffEnd"] == pytest.approx(math.cos(math.radians(45)), rel=1e-6)
 
 
 # ---- camera ----------------------------------------------------------------------------------------

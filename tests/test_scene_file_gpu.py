@@ -38,3 +38,13 @@ def test_sample_radiance_studio_file_in_haze(plugin, host):
     assert s.desc.cameraMedium >= 0
     check_samples("studio+haze", s, pa.render_params(spp=1, first_sample=2, seed=3, max_depth=24, rr_start=1),
                   plugin, SAMPLE_BAR)
+
+
+def test_sample_radiance_spot_lights(plugin, host):
+    """Spot lights (smoothstep falloff, cone bounds in the light BVH) beside point and area lights."""
+    s = host.load_scene(os.path.join(os.path.dirname(STUDIO), "..", "spots", "scene.prscene.json"))
+    assert (s.lights["kind"] == pa.LIGHT_SPOT).sum() == 3
+    for sampler in (pa.LIGHT_SAMPLER_BVH, pa.LIGHT_SAMPLER_POWER):
+        check_samples(f"spots sampler={sampler}", s, pa.render_params(spp=1, first_sample=1, seed=5, max_depth=6,
+                                                                     rr_start=1, light_sampler=sampler),
+                      plugin, 0.99)
