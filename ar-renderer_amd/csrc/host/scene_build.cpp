@@ -728,8 +728,7 @@ int SceneBuilder::AddSpotLight(F3 p, F3 dir, int spectrum, float scale, float wi
     l.spectrum = spectrum;
     l.scale = scale;
     l.position = pf(p);
-    float len = std::sqrt(dot3(dir, dir));
-    l.direction = pf(f3(dir.x / len, dir.y / len, dir.z / len));
+    l.direction = pf(norm3(dir));  // Normalize: v * (1 / |v|) (src/core/math.h:79-82)
     const float rad = kPi / 180.f;
     l.cosFalloffEnd = std::cos(rad * width);
     l.cosFalloffStart = std::cos(rad * std::min(start, width));

@@ -68,6 +68,8 @@ float FBm(F3 p, int octaves, float lacunarity = 2.f, float gain = 0.5f);
 // Camera-to-world for a camera at pos looking at target, +z forward (src/core/math.cpp:87-105).
 void LookAt(F3 pos, F3 target, F3 up, float out12[12]);
 
+struct IesProfile;
+
 class SceneBuilder {
   public:
     // ---- spectra ----
@@ -123,6 +125,9 @@ class SceneBuilder {
     // Spot light at p along dir (normalised here); the cone is given in degrees, as the reference's
     // SpotLight takes it (src/lights/light.cpp:161-166), and stored as FP32 cosines.
     int AddSpotLight(F3 p, F3 dir, int spectrum, float scale, float totalWidthDeg, float falloffStartDeg);
+    // Goniometric light at pos with the luminaire frame {x, y, z} (z = the IES nadir); the profile is
+    // appended to knots as the record prender_gpu.h describes (kIesHeader).
+    int AddGoniometricLight(F3 pos, const F3 frame[3], const IesProfile &p, int spectrum, float scale);
     int AddEnvironment(int spectrum, float scale);
     int AddHomogeneous(int sigmaA, int sigmaS, float scale, float g);
     int AddGrid(int sigmaA, int sigmaS, float scale, float g, int nx, int ny, int nz, const std::vector<float> &data,
@@ -159,6 +164,16 @@ float SpectrumValue(const prgpu::SpectrumEx &s, const float *knots, float lambda
 void FitRGBSigmoid(const float rgb[3], float coeffs[3]);
 // Linear sRGB of a film XYZ value, with the renderer's spectral white adaptation (src/core/film.cpp:170-175).
 void XYZToLinearSRGB(const double xyz[3], double scale, float rgb[3]);
+
+// IESNA LM-63 type C profile (src/lights/ies.cpp): angles in degrees, candela normalised by the peak.
+struct IesProfile {
+    std::vector<float> v, h, cd;  // cd[h * nV + v]
+    float peak = 0, integral = 0;
+};
+bool ParseIES(const std::string &text, IesProfile *p, std::string *err);
+float IesEvaluateAngles(const IesProfile &p, float thetaDeg, float phiDeg);
+float IesEvaluate(const IesProfile &p, F3 wLocal);  // wLocal in the luminaire frame (+z = nadir)
+void IesIntegral(IesProfile *p);
 
 // Settings of a scene file that the flat scene does not hold (src/scene/scene.h:29-42).
 struct SceneFileSettings {

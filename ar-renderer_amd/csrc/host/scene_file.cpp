@@ -5,8 +5,8 @@
 //
 // What the exporter approximates or drops is approximated or dropped here, with the same warning
 // (checker textures keep both colours and the u scale; roughness textures become the default;
-// normal and bump maps are ignored; IES, sun, sun-sky and image-environment lights are skipped).
-// Distant and spot lights, which the v4 exporter also skips, are kept: the extension has them. Everything the renderer cannot express and the exporter has no fallback for (image and
+// normal and bump maps are ignored; sun, sun-sky and image-environment lights are skipped).
+// Distant, spot and IES lights, which the v4 exporter also skips, are kept: the extension has them. Everything the renderer cannot express and the exporter has no fallback for (image and
 // noise colour textures, orthographic and realistic cameras, glTF / pbrt imports, noise media) is a
 // load error naming the item. Illuminant RGB spectra (sigmoid x D65) are tabulated every 5 nm, as
 // the exporter tabulates every spectrum.
@@ -277,7 +277,7 @@ std::string ReadFile(const fs::path &p) {
 void ResolvePaths(J &j, const fs::path &dir) {
     if (j.isObj()) {
         for (auto &[key, v] : j.o) {
-            if (key == "file" && v.isStr()) {
+            if ((key == "file" || key == "ies") && v.isStr()) {
                 fs::path p(v.s);
                 if (p.is_relative()) v.s = (dir / p).lexically_normal().string();
             } else if (key == "rigs" && v.isObj()) {
@@ -874,7 +874,7 @@ bool MeshContains(const SceneBuilder &sb, int first, int count, F3 pt) {
 // ---- the loader ----------------------------------------------------------------------------------
 enum class SpecType { Albedo, Unbounded, Illuminant };
 
-const char *const kSkippedLights = "IES, sun, sun-sky and image-environment lights are skipped on the GPU";
+const char *const kSkippedLights = "sun, sun-sky and image-environment lights are skipped on the GPU";
 
 struct Tex {  // a colour input as the exporter flattens it: spectrum, checker second colour, u scale
     int spec = -1, second = -1;
@@ -1510,14 +1510,44 @@ class Loader {
         const std::string type = GetS(l, "type", "");
         const float scaleMul = GetF(l, "scale", 1.f);
         if (type == "point" || type == "spot") {
-            if (type == "point" && l.has("ies")) {
-                Warn(kSkippedLights);
-                return;
-            }
             int I = LightSpectrum(l);
             F3 pos = Vec3(l.at("position"), "position");
             float scale = GetF(l, "intensity", 1.f) * scaleMul;
             if (EnclosingMedium(pos) >= 0) Warn("light '" + id + "': point lights inside media are not attenuated on the GPU");
+            if (type == "point" && l.has("ies")) {
+                // Goniometric light: the IES nadir points along 'direction' (default down), turned by
+                // 'rotation' degrees about it (scene_loader.cpp:940-955).
+                IesProfile prof;
+                std::string e;
+                const std::string file = GetS(l, "ies", "");
+                if (!ParseIES(ReadFile(file), &prof, &e)) Fail("light '" + id + "': " + file + ": " + e);
+                F3 dir = l.has("look_at")     ? Vec3(l.at("look_at"), "look_at") - pos
+                         : l.has("direction") ? Vec3(l.at("direction"), "direction")
+                                              : f3(0, -1, 0);
+                F3 z = norm3(dir);
+                float sign = std::copysign(1.f, z.z);  // Frame::FromZ (math.h:119-126, 130-135)
+                float a = -1 / (sign + z.z);
+                float b = z.x * z.y * a;
+                F3 fx = f3(1 + sign * z.x * z.x * a, sign * b, -sign * z.x);
+                F3 fy = f3(b, sign + z.y * z.y * a, -z.y);
+                float rot = (3.14159265358979323846f / 180.f) * GetF(l, "rotation", 0.f);
+                F3 xin = fx * std::cos(rot) + fy * std::sin(rot);
+                // Frame::FromXZ: x made orthogonal to z and normalised (v / s is v * (1 / s)), y = z x x
+                // in double (math.h:79-82, 95-99, 137-148)
+                float dz = dot3(xin, z);
+                F3 x = xin - z * dz;
+                float l2 = dot3(x, x);
+                F3 frame[3] = {fx, fy, z};
+                if (!(l2 < 1e-12f) && std::isfinite(l2)) {
+                    x = x * (1.f / std::sqrt(l2));
+                    frame[0] = x;
+                    frame[1] = crossd(z, x);
+                }
+                if (GetB(l, "photometric", false)) scale *= prof.peak;
+                if (l.has("power")) scale = GetF(l, "power", 1.f) / (prof.integral * sb_.SpectrumY(I));
+                sb_.AddGoniometricLight(pos, frame, prof, I, scale);
+                return;
+            }
             if (type == "point") {
                 if (l.has("power")) scale = GetF(l, "power", 1.f) / (4 * 3.14159265358979323846f * sb_.SpectrumY(I));
                 sb_.AddPointLight(pos, I, scale);

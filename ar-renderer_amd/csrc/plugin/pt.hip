@@ -309,7 +309,18 @@ void Validate(const prgpu::SceneDescEx &sd) {
         const prgpu::LightEx &l = sd.lights[i];
         if (!specOK(l.spectrum, false)) bad("light spectrum");
         if (l.kind == prgpu::LightExArea && (l.primitive < 0 || l.primitive >= sd.primCount)) bad("area light primitive");
-        if (l.kind > prgpu::LightExSpot) bad("light kind");
+        if (l.kind > prgpu::LightExGoniometric) bad("light kind");
+        if (l.kind == prgpu::LightExGoniometric) {
+            bool ok = l.profileOffset >= 0 && l.profileSize > prgpu::kIesHeader &&
+                      int64_t(l.profileOffset) + l.profileSize <= sd.knotCount;
+            if (ok) {
+                const float *r = sd.knots + l.profileOffset;
+                int nV = int(r[9]), nH = int(r[10]);
+                ok = nV >= 1 && nH >= 1 && float(nV) == r[9] && float(nH) == r[10] &&
+                     int64_t(prgpu::kIesHeader) + nV + nH + int64_t(nV) * nH == l.profileSize;
+            }
+            if (!ok) bad("goniometric light profile");
+        }
         if (l.kind == prgpu::LightExSpot &&
             !(l.cosFalloffEnd >= -1 && l.cosFalloffEnd <= l.cosFalloffStart && l.cosFalloffStart <= 1))
             bad("spot light cone");
@@ -431,6 +442,14 @@ void EnsurePTScene(Device &dev, const prgpu::SceneDescEx &sd, const float *sheen
             lb = x;
             pw = l.scale * host_spectrum_y(st, l.spectrum, pt->cieYInt) * 2 * kPi *
                  ((1 - l.cosFalloffStart) + (l.cosFalloffStart - l.cosFalloffEnd) / 2);
+        } else if (l.kind == prgpu::LightExGoniometric) {  // light.h:198-206
+            HostLB x;
+            x.bounds = Box3(f3(l.position), f3(l.position));
+            x.phi = 4 * kPi * l.scale * spec_max(st, l.spectrum);
+            x.cosTheta_o = -1;
+            x.cosTheta_e = 0;
+            lb = x;
+            pw = l.scale * host_spectrum_y(st, l.spectrum, pt->cieYInt) * sd.knots[l.profileOffset + 11];
         } else if (l.kind == prgpu::LightExDistant) {
             pw = l.scale * host_spectrum_y(st, l.spectrum, pt->cieYInt) * kPi * pt->worldR * pt->worldR;
         } else {

@@ -32,6 +32,54 @@ PG_HD float lerp(float t, float a, float b) { return (1 - t) * a + t * b; }
 PG_HD float safe_sqrt(float x) { return sqrtf(maxf(0.f, x)); }
 PG_HD float safe_asin(float x) { return asinf(clampf(x, -1.f, 1.f)); }
 PG_HD float safe_acos(float x) { return acosf(clampf(x, -1.f, 1.f)); }
+// Relative intensity of a goniometric light's IES record (layout: prender_gpu.h, kIesHeader) in the
+// direction w given in world space (src/lights/ies.cpp:80-105, src/lights/sky.cpp:12-27).
+PG_HD int ies_interval(const float *a, int n, float x) {  // FindInterval (src/core/math.h:45-54)
+    int first = 1, len = n - 2;
+    while (len > 0) {
+        int half = len >> 1, middle = first + half;
+        bool r = a[middle] <= x;
+        first = r ? middle + 1 : first;
+        len = r ? len - (half + 1) : half;
+    }
+    return clampi(first - 1, 0, n - 2);
+}
+PG_HD float ies_eval(const float *rec, float wx, float wy, float wz) {
+    // to the luminaire frame
+    float lx = wx * rec[0] + wy * rec[1] + wz * rec[2];
+    float ly = wx * rec[3] + wy * rec[4] + wz * rec[5];
+    float lz = wx * rec[6] + wy * rec[7] + wz * rec[8];
+    const int nV = int(rec[9]), nH = int(rec[10]);
+    const float *V = rec + prgpu::kIesHeader, *H = V + nV, *C = H + nH;
+    float theta = (180.f / kPi) * acosf(clampf(lz, -1.f, 1.f));
+    float phi = (180.f / kPi) * atan2f(ly, lx);
+    if (theta < V[0] || theta > V[nV - 1]) return 0;
+    float hLast = H[nH - 1];
+    phi = fmodf(phi, 360.f);
+    if (phi < 0) phi += 360;
+    if (nH == 1 || hLast == 0) phi = 0;
+    else if (hLast <= 90) {
+        phi = fmodf(phi, 180.f);
+        if (phi > 90) phi = 180 - phi;
+    } else if (hLast <= 180) {
+        if (phi > 180) phi = 360 - phi;
+    }
+    int vi = 0;
+    float tv = 0;
+    if (nV > 1) {
+        vi = ies_interval(V, nV, theta);
+        tv = clampf((theta - V[vi]) / maxf(1e-6f, V[vi + 1] - V[vi]), 0.f, 1.f);
+    }
+    auto vertical = [&](int hi) {
+        const float *row = C + size_t(hi) * size_t(nV);
+        return nV == 1 ? row[0] : lerp(tv, row[vi], row[vi + 1]);
+    };
+    if (nH == 1) return vertical(0);
+    int hi = ies_interval(H, nH, phi);
+    float th = clampf((phi - H[hi]) / maxf(1e-6f, H[hi + 1] - H[hi]), 0.f, 1.f);
+    return lerp(th, vertical(hi), vertical(hi + 1));
+}
+
 // Hermite smoothstep of x between a and b, a step at a when a == b (src/core/math.h:32-36).
 PG_HD float smooth_step(float x, float a, float b) {
     if (a == b) return x < a ? 0.f : 1.f;

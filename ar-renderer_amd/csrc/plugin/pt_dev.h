@@ -1220,6 +1220,16 @@ PG_HD bool sample_li(const DevScene &s, int li, V3 p, V2 u, const Lambda &lam, L
         *ls = {spec_sample(s.spec, L.spectrum, lam) * (L.scale * f / d2), wi, 1, f3(L.position), V3()};
         return true;
     }
+    if (L.kind == prgpu::LightExGoniometric) {  // sky.cpp:12-27
+        V3 d = f3(L.position) - p;
+        float d2 = len2(d);
+        if (d2 == 0) return false;
+        V3 wi = d / sqrtf(d2);
+        float prof = ies_eval(s.spec.knots + L.profileOffset, -wi.x, -wi.y, -wi.z);
+        if (prof <= 0) return false;
+        *ls = {spec_sample(s.spec, L.spectrum, lam) * (L.scale * prof / d2), wi, 1, f3(L.position), V3()};
+        return true;
+    }
     if (L.kind == prgpu::LightExDistant) {
         V3 w = f3(L.direction);
         *ls = {spec_sample(s.spec, L.spectrum, lam) * L.scale, w, 1, p + w * (2 * s.worldR), V3()};
@@ -1239,7 +1249,8 @@ PG_HD float pdf_li(const DevScene &s, int li, V3 p, V3 wi) {
 
 PG_HD bool light_is_delta(const DevScene &s, int li) {
     uint32_t k = s.lights[li].kind;
-    return k == prgpu::LightExPoint || k == prgpu::LightExDistant || k == prgpu::LightExSpot;
+    return k == prgpu::LightExPoint || k == prgpu::LightExDistant || k == prgpu::LightExSpot ||
+           k == prgpu::LightExGoniometric;
 }
 
 PG_HD float p_infinite(const DevScene &s) {

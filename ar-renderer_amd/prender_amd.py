@@ -25,7 +25,7 @@ PRIM_TRIANGLE, PRIM_SPHERE = 0, 1
 MAT_DIFFUSE, MAT_CONDUCTOR, MAT_DIELECTRIC, MAT_BLACK, MAT_INTERFACE = 0, 1, 2, 3, 4
 MAT_COATED_DIFFUSE, MAT_COATED_CONDUCTOR = 5, 6
 SPEC_CONSTANT, SPEC_SIGMOID, SPEC_BLACKBODY, SPEC_PIECEWISE, SPEC_SELLMEIER, SPEC_CAUCHY, SPEC_TABLE95 = range(7)
-LIGHT_AREA, LIGHT_POINT, LIGHT_DISTANT, LIGHT_ENV, LIGHT_SPOT = 0, 1, 2, 3, 4
+LIGHT_AREA, LIGHT_POINT, LIGHT_DISTANT, LIGHT_ENV, LIGHT_SPOT, LIGHT_GONIO = 0, 1, 2, 3, 4, 5
 MEDIUM_HOMOGENEOUS, MEDIUM_GRID = 0, 1
 FILTER_BOX, FILTER_GAUSSIAN, FILTER_BH = 0, 1, 2
 LIGHT_SAMPLER_BVH, LIGHT_SAMPLER_POWER = 0, 1
@@ -49,10 +49,10 @@ MATERIAL_DTYPE = np.dtype({
 SPECTRUM_DTYPE = np.dtype({"names": ["kind", "scale", "offset", "count", "p"],
                            "formats": ["<u4", "<f4", "<i4", "<i4", ("<f4", 8)], "itemsize": 48})
 LIGHT_DTYPE = np.dtype({"names": ["kind", "primitive", "spectrum", "scale", "position", "direction", "twoSided",
-                                  "cosPower", "cosFalloffStart", "cosFalloffEnd", "reserved"],
+                                  "cosPower", "cosFalloffStart", "cosFalloffEnd", "profileOffset", "profileSize"],
                         "formats": ["<u4", "<i4", "<i4", "<f4", ("<f4", 3), ("<f4", 3), "<u4", "<f4", "<f4", "<f4",
-                                    ("<u4", 2)],
-                        "offsets": [0, 4, 8, 12, 16, 28, 40, 44, 48, 52, 56], "itemsize": 64})
+                                    "<i4", "<i4"],
+                        "offsets": [0, 4, 8, 12, 16, 28, 40, 44, 48, 52, 56, 60], "itemsize": 64})
 MEDIUM_DTYPE = np.dtype({"names": ["kind", "sigmaA", "sigmaS", "scale", "g", "nx", "ny", "nz", "gridOffset",
                                    "maxDensity", "boundsMin", "boundsMax", "worldToMedium"],
                          "formats": ["<u4", "<i4", "<i4", "<f4", "<f4", "<i4", "<i4", "<i4", "<i8", "<f4",

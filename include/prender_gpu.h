@@ -193,7 +193,15 @@ enum LightKindEx : uint32_t {
     LightExDistant = 2,      // DistantLight arriving from `direction` (src/lights/sky.cpp:52-60)
     LightExEnvironment = 3,  // constant EnvironmentLight (src/lights/light.cpp:215-297)
     LightExSpot = 4,         // SpotLight at `position` pointing along `direction` (src/lights/light.cpp:117-212)
+    LightExGoniometric = 5,  // GoniometricLight (IES type C profile) at `position` (src/lights/sky.cpp:10-50)
 };
+
+// Record of a goniometric light's profile in SceneDescEx::knots, at LightEx::profileOffset:
+// [0..8] the luminaire frame's x, y and z axes in world space (z = the IES nadir), [9] nV, [10] nH,
+// [11] the integral of the relative intensity over the sphere (steradians), then nV vertical
+// angles, nH horizontal angles (degrees, ascending) and nH x nV relative candela values in [0, 1]
+// (row h holds the vertical profile at horizontal angle h). profileSize = 12 + nV + nH + nV nH.
+constexpr int kIesHeader = 12;
 
 struct LightEx {  // 64 bytes
     uint32_t kind;
@@ -207,7 +215,7 @@ struct LightEx {  // 64 bytes
     // spot light: cosines of the falloff start and of the total width (cos(radians(deg)) in FP32);
     // intensity is scaled by smoothstep(cos theta, cosFalloffEnd, cosFalloffStart)
     float cosFalloffStart, cosFalloffEnd;
-    uint32_t reserved[2];
+    int32_t profileOffset, profileSize;  // goniometric light: its profile record in knots (kIesHeader)
 };
 
 enum MediumKindEx : uint32_t { MediumHomogeneous = 0, MediumGrid = 1 };

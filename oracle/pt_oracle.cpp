@@ -1478,7 +1478,8 @@ struct LightObj {
     LightEx d;
     float area = 0;
     bool delta() const {
-        return d.kind == prgpu::LightExPoint || d.kind == prgpu::LightExDistant || d.kind == prgpu::LightExSpot;
+        return d.kind == prgpu::LightExPoint || d.kind == prgpu::LightExDistant || d.kind == prgpu::LightExSpot ||
+               d.kind == prgpu::LightExGoniometric;
     }
     bool infinite_type() const { return d.kind == prgpu::LightExEnvironment; }
 };
@@ -1773,8 +1774,49 @@ struct Scene {
             lb.cosTheta_e = std::cos(safe_acos(L.d.cosFalloffEnd) - safe_acos(L.d.cosFalloffStart));
             return lb;
         }
+        case prgpu::LightExGoniometric: {  // light.h:199-206
+            LightB lb;
+            lb.bounds = Box{v3(L.d.position), v3(L.d.position)};
+            lb.phi = 4 * kPi * L.d.scale * spec.max_value(L.d.spectrum);
+            lb.cosTheta_o = -1;
+            lb.cosTheta_e = 0;
+            return lb;
+        }
         default: return std::nullopt;  // distant and environment lights
         }
+    }
+    // Relative intensity of a goniometric light towards world direction w: the profile's type C
+    // lookup in the luminaire frame, with phi folded by the profile's symmetry (ies.cpp:80-105).
+    float ies_profile(const LightObj &L, V3 w) const {
+        const float *r = spec.knots.data() + L.d.profileOffset;
+        V3 x(r[0], r[1], r[2]), y(r[3], r[4], r[5]), z(r[6], r[7], r[8]);
+        V3 wl(dot(w, x), dot(w, y), dot(w, z));
+        const int nV = int(r[9]), nH = int(r[10]);
+        const float *V = r + prgpu::kIesHeader, *H = V + nV, *C = H + nH;
+        float theta = (180.f / kPi) * safe_acos(wl.z);
+        float phi = (180.f / kPi) * std::atan2(wl.y, wl.x);
+        if (theta < V[0] || theta > V[nV - 1]) return 0;
+        float hLast = H[nH - 1];
+        phi = std::fmod(phi, 360.f);
+        if (phi < 0) phi += 360;
+        if (nH == 1 || hLast == 0) phi = 0;
+        else if (hLast <= 90) {
+            phi = std::fmod(phi, 180.f);
+            if (phi > 90) phi = 180 - phi;
+        } else if (hLast <= 180) {
+            if (phi > 180) phi = 360 - phi;
+        }
+        auto vertical = [&](int hi) {
+            const float *row = C + size_t(hi) * size_t(nV);
+            if (nV == 1) return row[0];
+            int vi = find_interval(nV, [&](int i) { return V[i] <= theta; });
+            float t = (theta - V[vi]) / std::max(1e-6f, V[vi + 1] - V[vi]);
+            return lerpf(clampv(t, 0.f, 1.f), row[vi], row[vi + 1]);
+        };
+        if (nH == 1) return vertical(0);
+        int hi = find_interval(nH, [&](int i) { return H[i] <= phi; });
+        float t = (phi - H[hi]) / std::max(1e-6f, H[hi + 1] - H[hi]);
+        return lerpf(clampv(t, 0.f, 1.f), vertical(hi), vertical(hi + 1));
     }
     float light_power(int li) const {
         const LightObj &L = lights[size_t(li)];
@@ -1784,6 +1826,8 @@ struct Scene {
             return L.d.scale * spec.to_y(L.d.spectrum) * L.area * lobe * (L.d.twoSided ? 2 : 1);
         }
         case prgpu::LightExPoint: return 4 * kPi * L.d.scale * spec.to_y(L.d.spectrum);
+        case prgpu::LightExGoniometric:  // light.h:198: the profile's integral is in its record
+            return L.d.scale * spec.to_y(L.d.spectrum) * spec.knots[size_t(L.d.profileOffset) + 11];
         case prgpu::LightExSpot:  // light.cpp:173-175
             return L.d.scale * spec.to_y(L.d.spectrum) * 2 * kPi *
                    ((1 - L.d.cosFalloffStart) + (L.d.cosFalloffStart - L.d.cosFalloffEnd) / 2);
@@ -1836,6 +1880,20 @@ struct Scene {
             LightSample ls;
             ls.wi = wi;
             ls.L = spec.sample(L.d.spectrum, lam) * (L.d.scale * f / d2);
+            ls.pdf = 1;
+            ls.pLight.p = v3(L.d.position);
+            return ls;
+        }
+        case prgpu::LightExGoniometric: {  // sky.cpp:12-27
+            V3 d = v3(L.d.position) - p;
+            float d2 = len2(d);
+            if (d2 == 0) return std::nullopt;
+            V3 wi = d / std::sqrt(d2);
+            float prof = ies_profile(L, -wi);
+            if (prof <= 0) return std::nullopt;
+            LightSample ls;
+            ls.wi = wi;
+            ls.L = spec.sample(L.d.spectrum, lam) * (L.d.scale * prof / d2);
             ls.pdf = 1;
             ls.pLight.p = v3(L.d.position);
             return ls;

@@ -14,6 +14,7 @@
 #include "geometry/bvh.h"
 #include "geometry/shape.h"
 #include "integrators/integrator.h"
+#include "lights/ies.h"
 #include "lights/light.h"
 #include "lights/light_bvh.h"
 #include "materials/bxdf.h"
@@ -24,6 +25,7 @@
 #include <cstring>
 #include <memory>
 #include <unordered_map>
+#include <sstream>
 #include <vector>
 
 using namespace pr;
@@ -309,10 +311,38 @@ void *rf_scene_create(const prgpu::SceneDescEx *d, float fovY) {
         } else if (l.kind == prgpu::LightExPoint) {
             light = std::make_shared<PointLight>(V(l.position), sp, l.scale);
         } else if (l.kind == prgpu::LightExSpot) {
-            // the flat scene keeps the cone as cosines; the reference takes degrees
-            float width = std::acos(l.cosFalloffEnd) * 180.f / 3.14159265358979f;
-            float start = std::acos(l.cosFalloffStart) * 180.f / 3.14159265358979f;
+            // the flat scene keeps the cone as cosines; the reference takes degrees: the degree value
+            // nearest acos(c) whose cos(Radians(deg)) reproduces c exactly (within 64 ulps)
+            auto degrees = [](float c) {
+                float d = Degrees(std::acos(c));
+                for (int k = 0; k < 64; ++k) {
+                    float lo = d, hi = d;
+                    for (int j = 0; j < k; ++j) lo = std::nextafter(lo, -1e30f), hi = std::nextafter(hi, 1e30f);
+                    if (std::cos(Radians(lo)) == c) return lo;
+                    if (std::cos(Radians(hi)) == c) return hi;
+                }
+                return d;
+            };
+            float width = degrees(l.cosFalloffEnd);
+            float start = degrees(l.cosFalloffStart);
             light = std::make_shared<SpotLight>(V(l.position), V(l.direction), sp, l.scale, width, start);
+        } else if (l.kind == prgpu::LightExGoniometric) {
+            // the reference's IESProfile is built by its own LM-63 parser from the record's angles
+            // and relative candela, written back out as a type C file
+            const float *r = d->knots + l.profileOffset;
+            int nV = int(r[9]), nH = int(r[10]);
+            std::ostringstream t;
+            t.precision(9);
+            t << "IESNA:LM-63-2002\nTILT=NONE\n1 -1 1 " << nV << " " << nH << " 1 2 0 0 0\n1 1 100\n";
+            for (int i = 0; i < nV + nH + nV * nH; ++i) t << r[prgpu::kIesHeader + i] << (i % 10 == 9 ? "\n" : " ");
+            std::string err;
+            auto prof = IESProfile::Parse(t.str(), &err);
+            Frame f;
+            f.x = Vec3f(r[0], r[1], r[2]);
+            f.y = Vec3f(r[3], r[4], r[5]);
+            f.z = Vec3f(r[6], r[7], r[8]);
+            if (prof) light = std::make_shared<GoniometricLight>(V(l.position), f, prof, sp, l.scale);
+            else light = std::make_shared<PointLight>(V(l.position), sp, 0.f);
         } else if (l.kind == prgpu::LightExDistant) {
             light = std::make_shared<DistantLight>(V(l.direction), sp, l.scale);
         } else {

@@ -308,6 +308,7 @@ def main():
 
     closures(L, out)
     spot_lights(L, out, host)
+    light_vectors(L, out, host, "ies", 5107)
 
     path = os.path.join(HERE, "ref_components.npz")
     np.savez_compressed(path, **out)
@@ -317,8 +318,15 @@ def main():
 def spot_lights(L, out, host):
     """Spot lights (src/lights/light.cpp:117-212) beside point and area lights: light-BVH choice and
     PMFs, power PMFs and SampleLi / PDF_Li, from the scene file tests/scenes/spots."""
-    rng = np.random.default_rng(4117)
-    scene = host.load_scene(os.path.join(HERE, "..", "scenes", "spots", "scene.prscene.json"))
+    light_vectors(L, out, host, "spots", 4117, prefix="spot")
+
+
+def light_vectors(L, out, host, name, seed, prefix=None):
+    """Light-BVH choice and PMFs, power PMFs and SampleLi for the scene file tests/scenes/<name>
+    (goniometric lights: src/lights/sky.cpp:10-50 with the profile of src/lights/ies.cpp)."""
+    prefix = prefix or name
+    rng = np.random.default_rng(seed)
+    scene = host.load_scene(os.path.join(HERE, "..", "scenes", name, "scene.prscene.json"))
     h = L.rf_scene_create(C.byref(scene.desc), 40.0)
     n_l = len(scene.lights)
     P = f32(rng.uniform([-3, 0, -3], [3, 2.5, 3], (400, 3)))
@@ -342,9 +350,9 @@ def spot_lights(L, out, host):
         o9 = np.zeros(9, np.float32)
         L.rf_light_sample_li(h, i % n_l, ptr(P[i]), ptr(N[i]), ptr(LU[i]), float(LL[i]), ptr(o9))
         lsi[i] = o9
-    out.update(spot_p=P, spot_n=N, spot_u=U, spot_light=li, spot_light_pmf=lpmf, spot_pmf_each=each,
-               spot_power_pmf=f32([L.rf_light_power_pmf(h, k) for k in range(n_l)]), spot_lsu=LU, spot_lsl=LL,
-               spot_lsi=lsi)
+    vecs = dict(p=P, n=N, u=U, light=li, light_pmf=lpmf, pmf_each=each,
+                power_pmf=f32([L.rf_light_power_pmf(h, k) for k in range(n_l)]), lsu=LU, lsl=LL, lsi=lsi)
+    out.update({f"{prefix}_{k}": v for k, v in vecs.items()})
     L.rf_scene_destroy(h)
 
 

@@ -217,37 +217,46 @@ def test_c1_area_light_sample_li(c1_small):
             np.testing.assert_allclose(got, G["c1_lpdf"][i], rtol=2e-5)
 
 
-@pytest.fixture(scope="module")
-def spots(host):
-    s = host.load_scene(os.path.join(os.path.dirname(__file__), "scenes", "spots", "scene.prscene.json"))
+
+def scene_file(host, name):
+    s = host.load_scene(os.path.join(os.path.dirname(__file__), "scenes", name, "scene.prscene.json"))
     return s, orc.OracleScene(s)
 
 
-def test_spot_lights_match_reference(spots):
+@pytest.mark.parametrize("name,prefix,kind,count", [("spots", "spot", pa.LIGHT_SPOT, 3),
+                                                     ("ies", "ies", pa.LIGHT_GONIO, 3)])
+def test_spot_and_goniometric_lights_match_reference(host, name, prefix, kind, count):
     """Spot lights (reference src/lights/light.cpp:117-212: smoothstep falloff between the cone's
-    cosines, cone bounds in the light BVH, PowerY) next to point and area lights: light-BVH choice
-    and every light's PMF, power PMFs, and SampleLi against the reference's own code."""
-    s, o = spots
+    cosines, cone bounds in the light BVH, PowerY) and goniometric lights (src/lights/sky.cpp:10-50 with
+    the type C profile of src/lights/ies.cpp: the lookup, its symmetry folding, the tabulated integral)
+    next to point, spot and area lights: light-BVH choice and every light's PMF, power PMFs, and
+    SampleLi against the reference's own code. The reference rebuilds each IES profile with its own
+    LM-63 parser, so the power PMFs also pin the host's integral of the profile."""
+    s, o = scene_file(host, name)
     kinds = s.lights["kind"]
-    assert (kinds == pa.LIGHT_SPOT).sum() == 3
-    P, N, U = G["spot_p"], G["spot_n"], G["spot_u"]
+    assert (kinds == kind).sum() == count
+    P, N, U = G[f"{prefix}_p"], G[f"{prefix}_n"], G[f"{prefix}_u"]
     for i in range(len(P)):
         li, pmf = o.light_sample(P[i], N[i], float(U[i]))
-        assert li == G["spot_light"][i], i
-        np.testing.assert_allclose(pmf, G["spot_light_pmf"][i], rtol=1e-5, atol=1e-9)
+        assert li == G[f"{prefix}_light"][i], i
+        np.testing.assert_allclose(pmf, G[f"{prefix}_light_pmf"][i], rtol=1e-5, atol=1e-9)
         each = [o.light_pmf(P[i], N[i], k) for k in range(len(s.lights))]
-        np.testing.assert_allclose(each, G["spot_pmf_each"][i], rtol=1e-5, atol=1e-9)
+        np.testing.assert_allclose(each, G[f"{prefix}_pmf_each"][i], rtol=1e-5, atol=1e-9)
     pw = np.array([L.or_light_power(o.h, k) for k in range(len(s.lights))], np.float64)
-    np.testing.assert_allclose(pw / pw.sum(), G["spot_power_pmf"], rtol=1e-5)
+    np.testing.assert_allclose(pw / pw.sum(), G[f"{prefix}_power_pmf"], rtol=1e-5)
     lit = 0
     for i in range(len(P)):
         k = i % len(s.lights)
-        lam, pdf = sample_visible(G["spot_lsl"][i])
+        lam, pdf = sample_visible(G[f"{prefix}_lsl"][i])
         out = np.zeros(15, np.float32)
-        L.or_light_sample_li(o.h, k, p(f32(P[i])), p(f32(N[i])), p(f32(G["spot_lsu"][i])), p(lam), p(pdf), p(out))
-        ref = G["spot_lsi"][i]
+        L.or_light_sample_li(o.h, k, p(f32(P[i])), p(f32(N[i])), p(f32(G[f"{prefix}_lsu"][i])), p(lam), p(pdf),
+                             p(out))
+        ref = G[f"{prefix}_lsi"][i]
         assert out[14] == ref[8], i
         if ref[8]:
-            lit += kinds[k] == pa.LIGHT_SPOT
-            np.testing.assert_allclose(out[:8], ref[:8], rtol=2e-5, atol=1e-7, err_msg=f"sample {i}")
-    assert lit > 50  # spot lights sampled inside and across their falloff
+            lit += kinds[k] == kind
+            # 1e-4: inside a spot's falloff band the smoothstep turns the few-ulp differences of the
+            # cone cosine (the reference normalises the stored unit axis once more) into up to ~150x
+            # that relative change of the radiance
+            np.testing.assert_allclose(out[:8], ref[:8], rtol=1e-4, atol=1e-7, err_msg=f"sample {i}")
+    assert lit > 50  # the new light type is sampled inside and across its profile

@@ -304,6 +304,86 @@ def test_spot_light_cones(host, studio):
     assert soft["cosFalloffEnd"] == pytest.approx(math.cos(math.radians(45)), rel=1e-6)
 
 
+IES_SCENE = os.path.join(HERE, "scenes", "ies", "scene.prscene.json")
+
+
+def ies_records(s):
+    out = []
+    for l in s.lights[s.lights["kind"] == pa.LIGHT_GONIO]:
+        r = s.knots[l["profileOffset"]:l["profileOffset"] + l["profileSize"]]
+        nV, nH = int(r[9]), int(r[10])
+        assert l["profileSize"] == 12 + nV + nH + nV * nH
+        out.append((l, r[:9].reshape(3, 3), nV, nH, r[11], r[12:12 + nV], r[12 + nV:12 + nV + nH],
+                    r[12 + nV + nH:].reshape(nH, nV)))
+    return out
+
+
+def test_ies_records_frames_and_scales(host):
+    """IES lights (scene_loader.cpp:940-955): the nadir follows direction / look_at (default down),
+    'rotation' turns the frame about it, 'photometric' multiplies by the peak candela and 'power'
+    divides by the profile's integral times the spectrum's luminance. Candela are stored relative to
+    their peak after the multiplier and ballast factor; an INCLUDE tilt block is skipped."""
+    s = host.load_scene(IES_SCENE)
+    recs = {round(float(l["position"][0]), 3): (l, F, nV, nH, integ, V, H, C) for l, F, nV, nH, integ, V, H, C in ies_records(s)}
+    assert len(recs) == 3 and not s.warnings
+    for l, F, nV, nH, integ, V, H, C in recs.values():
+        assert np.allclose(F @ F.T, np.eye(3), atol=1e-6) and np.linalg.det(F) > 0  # right-handed, orthonormal
+        assert np.allclose(F[2], l["direction"]) and C.max() == 1 and C.min() >= 0
+        assert np.all(np.diff(V) > 0) and np.all(np.diff(H) >= 0)
+    down = recs[-1.5]
+    assert np.allclose(down[1][2], [0, -1, 0]) and (down[2], down[3]) == (7, 3)
+    assert down[0]["scale"] == pytest.approx(0.02)
+    assert np.allclose(down[7][0], np.array([900, 880, 760, 520, 260, 80, 0]) / 900)
+    aimed = recs[2.0]
+    z = np.array([0, 0.5, 0]) - np.array([2.0, 2.5, 1.5])
+    assert np.allclose(aimed[1][2], z / np.linalg.norm(z), atol=1e-6)
+    assert (aimed[2], aimed[3]) == (10, 1)  # the INCLUDE tilt block was skipped
+    assert aimed[0]["scale"] == pytest.approx(1.5e-5 * 1200 * 0.5 * 0.8, rel=1e-6)  # photometric: x peak cd
+    # rotation: the frame's x axis is turned 30 degrees about z from the unrotated frame's x
+    zz = aimed[1][2]
+    sign = math.copysign(1, zz[2])
+    a = -1 / (sign + zz[2])
+    b = zz[0] * zz[1] * a
+    fx = np.array([1 + sign * zz[0] ** 2 * a, sign * b, -sign * zz[0]])
+    assert math.degrees(math.acos(np.clip(np.dot(aimed[1][0], fx), -1, 1))) == pytest.approx(30, abs=1e-3)
+    washer = recs[0.5]
+    y = spectrum_y(host, s, washer[0]["spectrum"])
+    assert washer[0]["scale"] == pytest.approx(60 / (washer[4] * y), rel=3e-3)
+
+
+def test_ies_integral_of_an_axial_profile(host):
+    """For a rotationally symmetric profile I(theta) the integral over the sphere is
+    2 pi int_0^pi I(theta) sin(theta) dtheta; the record's value (the reference's 128 x 64 x 4 cell
+    estimate) must agree with fine quadrature of the piecewise-linear profile."""
+    s = host.load_scene(IES_SCENE)
+    l, F, nV, nH, integ, V, H, C = [r for r in ies_records(s) if r[3] == 1][0]
+    th = np.linspace(0, 180, 200001)
+    I = np.interp(th, V, C[0], left=0, right=0)
+    exact = 2 * math.pi * np.trapezoid(I * np.sin(np.radians(th)), np.radians(th))
+    assert integ == pytest.approx(exact, rel=2e-3)
+
+
+def test_ies_parse_errors(host, tmp_path):
+    def load(text):
+        (tmp_path / "x.ies").write_text(text)
+        doc = {"lights": {"g": {"type": "point", "position": [0, 2, 0], "ies": "x.ies"}},
+               "cameras": {"c": {"position": [0, 1, 5], "look_at": [0, 0, 0]}}}
+        f = tmp_path / "g.prscene.json"
+        f.write_text(json.dumps(doc))
+        return host.load_scene(f)
+    with pytest.raises(RuntimeError, match="missing TILT= line"):
+        load("IESNA:LM-63-2002\n1 1 1 1 1 1 2 0 0 0\n")
+    with pytest.raises(RuntimeError, match="only type C photometry"):
+        load("TILT=NONE\n1 1 1 2 1 2 2 0 0 0\n1 1 1\n0 90\n0\n10 5\n")
+    with pytest.raises(RuntimeError, match="truncated candela data"):
+        load("TILT=NONE\n1 1 1 2 1 1 2 0 0 0\n1 1 1\n0 90\n0\n10\n")
+    with pytest.raises(RuntimeError, match="all candela values are zero"):
+        load("TILT=NONE\n1 1 1 2 1 1 2 0 0 0\n1 1 1\n0 90\n0\n0 0\n")
+    ok = load("TILT=NONE\n1 1 2 2 1 1 2 0 0 0\n1 1 1\n0 90\n0\n10 5\n")
+    rec = ies_records(ok)[0]
+    assert np.allclose(rec[7][0], [1, 0.5])
+
+
 # ---- camera ----------------------------------------------------------------------------------------
 
 def test_thin_lens_camera_from_f_stop_and_focus(studio, host, tmp_path):

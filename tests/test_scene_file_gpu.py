@@ -48,3 +48,13 @@ def test_sample_radiance_spot_lights(plugin, host):
         check_samples(f"spots sampler={sampler}", s, pa.render_params(spp=1, first_sample=1, seed=5, max_depth=6,
                                                                      rr_start=1, light_sampler=sampler),
                       plugin, 0.99)
+
+
+def test_sample_radiance_goniometric_lights(plugin, host):
+    """IES (goniometric) lights of the three symmetries, with a spot and a rect light."""
+    s = host.load_scene(os.path.join(os.path.dirname(STUDIO), "..", "ies", "scene.prscene.json"))
+    assert (s.lights["kind"] == pa.LIGHT_GONIO).sum() == 3
+    for sampler in (pa.LIGHT_SAMPLER_BVH, pa.LIGHT_SAMPLER_POWER):
+        check_samples(f"ies sampler={sampler}", s, pa.render_params(spp=1, first_sample=2, seed=9, max_depth=6,
+                                                                   rr_start=1, light_sampler=sampler),
+                      plugin, 0.99)
