@@ -736,6 +736,42 @@ int SceneBuilder::AddSpotLight(F3 p, F3 dir, int spectrum, float scale, float wi
     return int(lights.size() - 1);
 }
 
+int SceneBuilder::AddSunLight(F3 w, float radiusDeg, int spectrum, float scale) {
+    prgpu::LightEx l{};
+    l.kind = prgpu::LightExSun;
+    l.primitive = -1;
+    l.spectrum = spectrum;
+    l.scale = scale;
+    l.direction = pf(norm3(w));
+    l.cosFalloffEnd = std::cos((kPi / 180.f) * std::clamp(radiusDeg, 0.01f, 45.f));
+    lights.push_back(l);
+    return int(lights.size() - 1);
+}
+
+int SceneBuilder::SunRadiance(F3 sunDir, float T, float radiusDeg) {
+    F3 d = norm3(sunDir);
+    float elevDeg = (180.f / kPi) * std::asin(std::clamp(d.y, -1.f, 1.f));
+    float zenithDeg = 90 - elevDeg;
+    float m = zenithDeg < 95 ? 1 / (std::cos((kPi / 180.f) * std::min(zenithDeg, 90.f)) +
+                                    0.50572f * std::pow(96.07995f - zenithDeg, -1.6364f))
+                             : 1e3f;
+    float beta = std::max(0.f, 0.04608f * T - 0.04586f);
+    const prgpu::SpectrumEx bb = spectra[size_t(BlackbodyUnitY(5778.f))];  // bb / Y(bb)
+    spectra.pop_back();
+    float r = 0.2665f / std::max(radiusDeg, 0.01f);
+    float sizeScale = r * r;
+    std::vector<float> l, v;
+    for (int lambda = 360; lambda <= 830; lambda += 5) {
+        float um = float(lambda) * 1e-3f;
+        float tauR = 0.008735f * std::pow(um, -4.08f);
+        float tauA = beta * std::pow(um, -1.3f);
+        float trans = elevDeg < -1 ? 0.f : std::exp(-m * (tauR + tauA));
+        l.push_back(float(lambda));
+        v.push_back(2.0e6f * sizeScale * SpectrumValue(bb, knots.data(), float(lambda)) * trans);
+    }
+    return Piecewise(l, v);
+}
+
 int SceneBuilder::AddDistantLight(F3 w, int spectrum, float scale) {
     prgpu::LightEx l{};
     l.kind = prgpu::LightExDistant;

@@ -128,6 +128,13 @@ class SceneBuilder {
     // Goniometric light at pos with the luminaire frame {x, y, z} (z = the IES nadir); the profile is
     // appended to knots as the record prender_gpu.h describes (kIesHeader).
     int AddGoniometricLight(F3 pos, const F3 frame[3], const IesProfile &p, int spectrum, float scale);
+    // Sun disk towards wSun of the given angular radius (degrees, clamped to [0.01, 45]), radiance
+    // spectrum * scale (src/lights/sky.cpp:84-86).
+    int AddSunLight(F3 wSun, float angularRadiusDeg, int spectrum, float scale);
+    // Solar radiance spectrum for a sun at sunDir through a clear atmosphere of turbidity T: a 5778 K
+    // blackbody at 2e6 cd/m^2 for the nominal disk, attenuated by Rayleigh and aerosol optical depth
+    // over the Kasten-Young air mass, every 5 nm (src/lights/sky.cpp:207-233).
+    int SunRadiance(F3 sunDir, float T, float angularRadiusDeg);
     int AddEnvironment(int spectrum, float scale);
     int AddHomogeneous(int sigmaA, int sigmaS, float scale, float g);
     int AddGrid(int sigmaA, int sigmaS, float scale, float g, int nx, int ny, int nz, const std::vector<float> &data,

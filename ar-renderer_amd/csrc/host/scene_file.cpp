@@ -5,8 +5,9 @@
 //
 // What the exporter approximates or drops is approximated or dropped here, with the same warning
 // (checker textures keep both colours and the u scale; roughness textures become the default;
-// normal and bump maps are ignored; sun, sun-sky and image-environment lights are skipped).
-// Distant, spot and IES lights, which the v4 exporter also skips, are kept: the extension has them. Everything the renderer cannot express and the exporter has no fallback for (image and
+// normal and bump maps are ignored; image-environment lights and the Preetham sky of sun-sky lights
+// are skipped). Distant, spot, IES and sun lights, which the v4 exporter also skips, are kept: the
+// extension has them. Everything the renderer cannot express and the exporter has no fallback for (image and
 // noise colour textures, orthographic and realistic cameras, glTF / pbrt imports, noise media) is a
 // load error naming the item. Illuminant RGB spectra (sigmoid x D65) are tabulated every 5 nm, as
 // the exporter tabulates every spectrum.
@@ -874,7 +875,7 @@ bool MeshContains(const SceneBuilder &sb, int first, int count, F3 pt) {
 // ---- the loader ----------------------------------------------------------------------------------
 enum class SpecType { Albedo, Unbounded, Illuminant };
 
-const char *const kSkippedLights = "sun, sun-sky and image-environment lights are skipped on the GPU";
+const char *const kSkippedLights = "image-environment lights are skipped on the GPU";
 
 struct Tex {  // a colour input as the exporter flattens it: spectrum, checker second colour, u scale
     int spec = -1, second = -1;
@@ -1566,8 +1567,20 @@ class Loader {
             sb_.AddDistantLight(f3(-dir.x, -dir.y, -dir.z), LightSpectrum(l), scale);
             return;
         }
-        if (type == "sun_sky" || type == "sun") {
-            Warn(kSkippedLights);
+        if (type == "sun_sky" || type == "sun") {  // scene_loader.cpp:977-997
+            F3 sunDir;
+            if (l.has("sun_direction")) sunDir = norm3(Vec3(l.at("sun_direction"), "sun_direction"));
+            else {
+                const float rad = 3.14159265358979323846f / 180.f;
+                float el = rad * GetF(l, "elevation", 45.f), az = rad * GetF(l, "azimuth", 0.f);
+                sunDir = f3(std::cos(el) * std::sin(az), std::sin(el), std::cos(el) * std::cos(az));
+            }
+            float turbidity = GetF(l, "turbidity", 3.f);
+            float scale = GetF(l, "intensity", 0.03f) * scaleMul;
+            float radius = 0.2665f * GetF(l, "sun_size", 1.f);
+            if (sunDir.y > -0.02f && GetF(l, "sun_scale", 1.f) > 0)
+                sb_.AddSunLight(sunDir, radius, sb_.SunRadiance(sunDir, turbidity, radius), scale * GetF(l, "sun_scale", 1.f));
+            if (type == "sun_sky") Warn("the Preetham sky of sun_sky lights (an image environment) is skipped on the GPU");
             return;
         }
         if (type == "environment" || type == "env" || type == "sky") {

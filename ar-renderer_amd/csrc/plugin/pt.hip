@@ -309,7 +309,8 @@ void Validate(const prgpu::SceneDescEx &sd) {
         const prgpu::LightEx &l = sd.lights[i];
         if (!specOK(l.spectrum, false)) bad("light spectrum");
         if (l.kind == prgpu::LightExArea && (l.primitive < 0 || l.primitive >= sd.primCount)) bad("area light primitive");
-        if (l.kind > prgpu::LightExGoniometric) bad("light kind");
+        if (l.kind > prgpu::LightExSun) bad("light kind");
+        if (l.kind == prgpu::LightExSun && !(l.cosFalloffEnd > -1 && l.cosFalloffEnd < 1)) bad("sun disk");
         if (l.kind == prgpu::LightExGoniometric) {
             bool ok = l.profileOffset >= 0 && l.profileSize > prgpu::kIesHeader &&
                       int64_t(l.profileOffset) + l.profileSize <= sd.knotCount;
@@ -450,6 +451,9 @@ void EnsurePTScene(Device &dev, const prgpu::SceneDescEx &sd, const float *sheen
             x.cosTheta_e = 0;
             lb = x;
             pw = l.scale * host_spectrum_y(st, l.spectrum, pt->cieYInt) * sd.knots[l.profileOffset + 11];
+        } else if (l.kind == prgpu::LightExSun) {  // sky.cpp:88-90
+            pw = l.scale * host_spectrum_y(st, l.spectrum, pt->cieYInt) * 2 * kPi * (1 - l.cosFalloffEnd) * kPi *
+                 pt->worldR * pt->worldR;
         } else if (l.kind == prgpu::LightExDistant) {
             pw = l.scale * host_spectrum_y(st, l.spectrum, pt->cieYInt) * kPi * pt->worldR * pt->worldR;
         } else {
@@ -569,8 +573,9 @@ DevScene MakeDevScene(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::Re
     s.envCount = 0;
     s.emitters = 0;
     for (int i = 0; i < sd.lightCount; ++i) {
-        s.envCount += sd.lights[i].kind == prgpu::LightExEnvironment ? 1 : 0;
-        s.emitters |= (sd.lights[i].kind == prgpu::LightExArea || sd.lights[i].kind == prgpu::LightExEnvironment) ? 1 : 0;
+        const uint32_t k = sd.lights[i].kind;
+        s.envCount += (k == prgpu::LightExEnvironment || k == prgpu::LightExSun) ? 1 : 0;
+        s.emitters |= (k == prgpu::LightExArea || k == prgpu::LightExEnvironment || k == prgpu::LightExSun) ? 1 : 0;
     }
     s.powerCdf = pt.powerCdf.as<float>();
     s.powerFunc = pt.powerFunc.as<float>();
@@ -1071,7 +1076,9 @@ __global__ __launch_bounds__(kBlock) void k_emit(const DevScene *__restrict__ sp
         if (prim < 0) {
             for (int li = 0; li < s.infCount; ++li) {
                 int L = s.infinite[li];
-                if (s.lights[L].kind == prgpu::LightExEnvironment)
+                if (s.lights[L].kind == prgpu::LightExEnvironment ||
+                    (s.lights[L].kind == prgpu::LightExSun &&  // SunLight::Le (sky.cpp:92-95)
+                     dot(normalize(r.rd), f3(s.lights[L].direction)) >= s.lights[L].cosFalloffEnd))
                     add_emitted(s, r, L, spec_sample(s.spec, s.lights[L].spectrum, r.lam) * s.lights[L].scale, r.rd);
             }
         } else {

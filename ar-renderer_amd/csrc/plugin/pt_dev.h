@@ -1230,6 +1230,15 @@ PG_HD bool sample_li(const DevScene &s, int li, V3 p, V2 u, const Lambda &lam, L
         *ls = {spec_sample(s.spec, L.spectrum, lam) * (L.scale * prof / d2), wi, 1, f3(L.position), V3()};
         return true;
     }
+    if (L.kind == prgpu::LightExSun) {  // sky.cpp:97-105
+        float cosMax = L.cosFalloffEnd;
+        float cosT = (1 - u.x) + u.x * cosMax;  // SampleUniformCone (sampling.h:47-53)
+        V3 wl = spherical_dir(safe_sqrt(1 - cosT * cosT), cosT, u.y * 2 * kPi);
+        V3 wi = Frame::from_z(f3(L.direction)).from_local(wl);
+        *ls = {spec_sample(s.spec, L.spectrum, lam) * L.scale, wi, 1 / (2 * kPi * (1 - cosMax)),
+               p + wi * (2 * s.worldR), V3()};
+        return true;
+    }
     if (L.kind == prgpu::LightExDistant) {
         V3 w = f3(L.direction);
         *ls = {spec_sample(s.spec, L.spectrum, lam) * L.scale, w, 1, p + w * (2 * s.worldR), V3()};
@@ -1244,6 +1253,8 @@ PG_HD float pdf_li(const DevScene &s, int li, V3 p, V3 wi) {
     const prgpu::LightEx &L = s.lights[li];
     if (L.kind == prgpu::LightExArea) return shape_pdf_from(s.prims[L.primitive], s.lightArea[li], p, wi);
     if (L.kind == prgpu::LightExEnvironment) return kInv4Pi;
+    if (L.kind == prgpu::LightExSun)  // sky.cpp:107-109
+        return dot(normalize(wi), f3(L.direction)) >= L.cosFalloffEnd ? 1 / (2 * kPi * (1 - L.cosFalloffEnd)) : 0.f;
     return 0;
 }
 

@@ -25,7 +25,7 @@ PRIM_TRIANGLE, PRIM_SPHERE = 0, 1
 MAT_DIFFUSE, MAT_CONDUCTOR, MAT_DIELECTRIC, MAT_BLACK, MAT_INTERFACE = 0, 1, 2, 3, 4
 MAT_COATED_DIFFUSE, MAT_COATED_CONDUCTOR = 5, 6
 SPEC_CONSTANT, SPEC_SIGMOID, SPEC_BLACKBODY, SPEC_PIECEWISE, SPEC_SELLMEIER, SPEC_CAUCHY, SPEC_TABLE95 = range(7)
-LIGHT_AREA, LIGHT_POINT, LIGHT_DISTANT, LIGHT_ENV, LIGHT_SPOT, LIGHT_GONIO = 0, 1, 2, 3, 4, 5
+LIGHT_AREA, LIGHT_POINT, LIGHT_DISTANT, LIGHT_ENV, LIGHT_SPOT, LIGHT_GONIO, LIGHT_SUN = 0, 1, 2, 3, 4, 5, 6
 MEDIUM_HOMOGENEOUS, MEDIUM_GRID = 0, 1
 FILTER_BOX, FILTER_GAUSSIAN, FILTER_BH = 0, 1, 2
 LIGHT_SAMPLER_BVH, LIGHT_SAMPLER_POWER = 0, 1

@@ -194,6 +194,8 @@ enum LightKindEx : uint32_t {
     LightExEnvironment = 3,  // constant EnvironmentLight (src/lights/light.cpp:215-297)
     LightExSpot = 4,         // SpotLight at `position` pointing along `direction` (src/lights/light.cpp:117-212)
     LightExGoniometric = 5,  // GoniometricLight (IES type C profile) at `position` (src/lights/sky.cpp:10-50)
+    LightExSun = 6,          // SunLight: a disk around `direction` (unit, towards the sun) of angular radius
+                             // acos(cosFalloffEnd), uniform radiance (src/lights/sky.cpp:82-127)
 };
 
 // Record of a goniometric light's profile in SceneDescEx::knots, at LightEx::profileOffset:

@@ -1481,7 +1481,7 @@ struct LightObj {
         return d.kind == prgpu::LightExPoint || d.kind == prgpu::LightExDistant || d.kind == prgpu::LightExSpot ||
                d.kind == prgpu::LightExGoniometric;
     }
-    bool infinite_type() const { return d.kind == prgpu::LightExEnvironment; }
+    bool infinite_type() const { return d.kind == prgpu::LightExEnvironment || d.kind == prgpu::LightExSun; }
 };
 
 class LightBvh {
@@ -1832,6 +1832,9 @@ struct Scene {
             return L.d.scale * spec.to_y(L.d.spectrum) * 2 * kPi *
                    ((1 - L.d.cosFalloffStart) + (L.d.cosFalloffStart - L.d.cosFalloffEnd) / 2);
         case prgpu::LightExDistant: return L.d.scale * spec.to_y(L.d.spectrum) * kPi * worldRadius * worldRadius;
+        case prgpu::LightExSun:  // sky.cpp:88-90
+            return L.d.scale * spec.to_y(L.d.spectrum) * 2 * kPi * (1 - L.d.cosFalloffEnd) * kPi * worldRadius *
+                   worldRadius;
         default: return 4 * kPi * kPi * worldRadius * worldRadius * L.d.scale * spec.to_y(L.d.spectrum);
         }
     }
@@ -1898,6 +1901,17 @@ struct Scene {
             ls.pLight.p = v3(L.d.position);
             return ls;
         }
+        case prgpu::LightExSun: {  // sky.cpp:97-105
+            float cosMax = L.d.cosFalloffEnd;
+            float cosT = (1 - u.x) + u.x * cosMax;
+            V3 wl = spherical_dir(safe_sqrt(1 - cosT * cosT), cosT, u.y * 2 * kPi);
+            LightSample ls;
+            ls.wi = Frame::from_z(v3(L.d.direction)).from_local(wl);
+            ls.pdf = 1 / (2 * kPi * (1 - cosMax));
+            ls.L = spec.sample(L.d.spectrum, lam) * L.d.scale;
+            ls.pLight.p = p + ls.wi * (2 * worldRadius);
+            return ls;
+        }
         case prgpu::LightExDistant: {  // sky.cpp:52-60
             LightSample ls;
             V3 w = v3(L.d.direction);
@@ -1922,6 +1936,9 @@ struct Scene {
         const LightObj &L = lights[size_t(li)];
         if (L.d.kind == prgpu::LightExArea) return shapes[size_t(L.d.primitive)].pdf_from(p, wi);
         if (L.d.kind == prgpu::LightExEnvironment) return kInv4Pi;
+        if (L.d.kind == prgpu::LightExSun)  // sky.cpp:107-109
+            return dot(normalize(wi), v3(L.d.direction)) >= L.d.cosFalloffEnd ? 1 / (2 * kPi * (1 - L.d.cosFalloffEnd))
+                                                                               : 0.f;
         return 0;
     }
     int light_sample(bool useBvh, V3 p, V3 n, float u, float *pmf) const {
@@ -2343,9 +2360,13 @@ static SS li(const Scene &sc, const Settings &st, Ray ray, Lambda &lam, Sampler 
         }
 
         if (!hit) {
-            for (size_t i = 0; i < sc.lights.size(); ++i)
-                if (sc.lights[i].infinite_type())
-                    add_emitted(int(i), sc.spec.sample(sc.lights[i].d.spectrum, lam) * sc.lights[i].d.scale, ray.d);
+            for (size_t i = 0; i < sc.lights.size(); ++i) {
+                const LightObj &Li = sc.lights[i];
+                if (Li.d.kind == prgpu::LightExEnvironment ||
+                    (Li.d.kind == prgpu::LightExSun &&  // SunLight::Le (sky.cpp:92-95)
+                     dot(normalize(ray.d), v3(Li.d.direction)) >= Li.d.cosFalloffEnd))
+                    add_emitted(int(i), sc.spec.sample(Li.d.spectrum, lam) * Li.d.scale, ray.d);
+            }
             break;
         }
 

@@ -72,6 +72,19 @@ SpectrumPtr MakeSpectrum(const prgpu::SpectrumEx &s, const float *knots) {
 
 Vec3f V(const prgpu::Float3 &f) { return {f.x, f.y, f.z}; }
 
+// The degree value nearest acos(c) whose cos(Radians(deg)) reproduces c exactly (within 64 ulps):
+// the flat scene keeps cone angles as cosines, the reference's lights take degrees.
+float ExactDegrees(float c) {
+    float d = Degrees(std::acos(c));
+    for (int k = 0; k < 64; ++k) {
+        float lo = d, hi = d;
+        for (int j = 0; j < k; ++j) lo = std::nextafter(lo, -1e30f), hi = std::nextafter(hi, 1e30f);
+        if (std::cos(Radians(lo)) == c) return lo;
+        if (std::cos(Radians(hi)) == c) return hi;
+    }
+    return d;
+}
+
 struct RefScene {
     Scene scene;
     std::vector<const Light *> lightOrder;
@@ -133,6 +146,11 @@ float rf_named_metal(const char *name, int k, float l) {
 float rf_named_ior(const char *name, float l) {
     SpectrumPtr s = GetNamedIOR(name);
     return s ? (*s)(l) : -1;
+}
+// The reference's solar radiance spectrum (sky.cpp:207-233) at 95 wavelengths from 360 nm, 5 nm apart.
+void rf_sun_radiance(const float *dir, float T, float radiusDeg, float *out95) {
+    SpectrumPtr s = MakeSunRadiance(Vec3f(dir[0], dir[1], dir[2]), T, radiusDeg);
+    for (int i = 0; i < 95; ++i) out95[i] = (*s)(360.f + 5.f * float(i));
 }
 float rf_blackbody_unit_y(float T, float l) {
     auto bb = std::make_shared<BlackbodySpectrum>(T);
@@ -313,18 +331,8 @@ void *rf_scene_create(const prgpu::SceneDescEx *d, float fovY) {
         } else if (l.kind == prgpu::LightExSpot) {
             // the flat scene keeps the cone as cosines; the reference takes degrees: the degree value
             // nearest acos(c) whose cos(Radians(deg)) reproduces c exactly (within 64 ulps)
-            auto degrees = [](float c) {
-                float d = Degrees(std::acos(c));
-                for (int k = 0; k < 64; ++k) {
-                    float lo = d, hi = d;
-                    for (int j = 0; j < k; ++j) lo = std::nextafter(lo, -1e30f), hi = std::nextafter(hi, 1e30f);
-                    if (std::cos(Radians(lo)) == c) return lo;
-                    if (std::cos(Radians(hi)) == c) return hi;
-                }
-                return d;
-            };
-            float width = degrees(l.cosFalloffEnd);
-            float start = degrees(l.cosFalloffStart);
+            float width = ExactDegrees(l.cosFalloffEnd);
+            float start = ExactDegrees(l.cosFalloffStart);
             light = std::make_shared<SpotLight>(V(l.position), V(l.direction), sp, l.scale, width, start);
         } else if (l.kind == prgpu::LightExGoniometric) {
             // the reference's IESProfile is built by its own LM-63 parser from the record's angles
@@ -343,6 +351,8 @@ void *rf_scene_create(const prgpu::SceneDescEx *d, float fovY) {
             f.z = Vec3f(r[6], r[7], r[8]);
             if (prof) light = std::make_shared<GoniometricLight>(V(l.position), f, prof, sp, l.scale);
             else light = std::make_shared<PointLight>(V(l.position), sp, 0.f);
+        } else if (l.kind == prgpu::LightExSun) {
+            light = std::make_shared<SunLight>(V(l.direction), ExactDegrees(l.cosFalloffEnd), sp, l.scale);
         } else if (l.kind == prgpu::LightExDistant) {
             light = std::make_shared<DistantLight>(V(l.direction), sp, l.scale);
         } else {

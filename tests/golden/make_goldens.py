@@ -39,6 +39,7 @@ def load_ref():
         ("rf_named_metal", C.c_float, [C.c_char_p, C.c_int, C.c_float]),
         ("rf_named_ior", C.c_float, [C.c_char_p, C.c_float]),
         ("rf_blackbody_unit_y", C.c_float, [C.c_float, C.c_float]),
+        ("rf_sun_radiance", None, [vp, C.c_float, C.c_float, vp]),
         ("rf_bxdf", None, [vp, C.c_int, vp, vp, C.c_float, vp, vp]),
         ("rf_fr_dielectric", C.c_float, [C.c_float, C.c_float]),
         ("rf_fr_complex", C.c_float, [C.c_float, C.c_float, C.c_float]),
@@ -179,6 +180,16 @@ def main():
         for j, l in enumerate(sl):
             bb[t, j] = L.rf_blackbody_unit_y(float(T), float(l))
     out["bb_temps"], out["bb_unit_y"] = temps, bb
+    # Solar radiance: (elevation, azimuth, turbidity, angular radius) cases, including a sun just
+    # below the horizon (no transmission) and an enlarged disk.
+    sun_cases = f32([[35, 40, 4, 0.2665 * 6], [70, -20, 2, 0.2665], [5, 180, 8, 0.2665 * 2], [-0.5, 10, 3, 0.2665],
+                     [-1.5, 10, 3, 0.2665]])
+    sun = np.zeros((len(sun_cases), 95), np.float32)
+    for i, (el, az, T, r) in enumerate(sun_cases):
+        e, a = np.radians(el, dtype=np.float32), np.radians(az, dtype=np.float32)
+        d = f32([np.cos(e) * np.sin(a), np.sin(e), np.cos(e) * np.cos(a)])
+        L.rf_sun_radiance(ptr(d), float(T), float(r), ptr(sun[i]))
+    out["sun_cases"], out["sun_radiance"] = sun_cases, sun
 
     # BxDFs: [type, alphaX, alphaY, etaD, R0..3, eta0..3, k0..3]
     descs = []
@@ -309,6 +320,7 @@ def main():
     closures(L, out)
     spot_lights(L, out, host)
     light_vectors(L, out, host, "ies", 5107)
+    light_vectors(L, out, host, "sun", 6133)
 
     path = os.path.join(HERE, "ref_components.npz")
     np.savez_compressed(path, **out)
@@ -346,12 +358,16 @@ def light_vectors(L, out, host, name, seed, prefix=None):
     LU = f32(rng.random((400, 2)))
     LL = f32(rng.random(400))
     lsi = np.zeros((400, 9), np.float32)
+    lpdf = np.zeros(400, np.float32)
     for i in range(400):
         o9 = np.zeros(9, np.float32)
         L.rf_light_sample_li(h, i % n_l, ptr(P[i]), ptr(N[i]), ptr(LU[i]), float(LL[i]), ptr(o9))
         lsi[i] = o9
+        if o9[8]:
+            lpdf[i] = L.rf_light_pdf_li(h, i % n_l, ptr(P[i]), ptr(N[i]), ptr(o9[4:7]))
     vecs = dict(p=P, n=N, u=U, light=li, light_pmf=lpmf, pmf_each=each,
-                power_pmf=f32([L.rf_light_power_pmf(h, k) for k in range(n_l)]), lsu=LU, lsl=LL, lsi=lsi)
+                power_pmf=f32([L.rf_light_power_pmf(h, k) for k in range(n_l)]), lsu=LU, lsl=LL, lsi=lsi,
+                lpdf=lpdf)
     out.update({f"{prefix}_{k}": v for k, v in vecs.items()})
     L.rf_scene_destroy(h)
 

@@ -224,13 +224,16 @@ def scene_file(host, name):
 
 
 @pytest.mark.parametrize("name,prefix,kind,count", [("spots", "spot", pa.LIGHT_SPOT, 3),
-                                                     ("ies", "ies", pa.LIGHT_GONIO, 3)])
+                                                     ("ies", "ies", pa.LIGHT_GONIO, 3),
+                                                     ("sun", "sun", pa.LIGHT_SUN, 1)])
 def test_spot_and_goniometric_lights_match_reference(host, name, prefix, kind, count):
     """Spot lights (reference src/lights/light.cpp:117-212: smoothstep falloff between the cone's
     cosines, cone bounds in the light BVH, PowerY) and goniometric lights (src/lights/sky.cpp:10-50 with
     the type C profile of src/lights/ies.cpp: the lookup, its symmetry folding, the tabulated integral)
-    next to point, spot and area lights: light-BVH choice and every light's PMF, power PMFs, and
-    SampleLi against the reference's own code. The reference rebuilds each IES profile with its own
+    and the sun (src/lights/sky.cpp:82-127: a uniform cone over the disk, an infinite light in the
+    light BVH, the attenuated solar spectrum) next to point, spot, environment and area lights:
+    light-BVH choice and every light's PMF, power PMFs, SampleLi and PDF_Li against the
+    reference's own code. The reference rebuilds each IES profile with its own
     LM-63 parser, so the power PMFs also pin the host's integral of the profile."""
     s, o = scene_file(host, name)
     kinds = s.lights["kind"]
@@ -259,4 +262,6 @@ def test_spot_and_goniometric_lights_match_reference(host, name, prefix, kind, c
             # cone cosine (the reference normalises the stored unit axis once more) into up to ~150x
             # that relative change of the radiance
             np.testing.assert_allclose(out[:8], ref[:8], rtol=1e-4, atol=1e-7, err_msg=f"sample {i}")
+            got = L.or_light_pdf_li(o.h, k, p(f32(P[i])), p(f32(ref[4:7])))
+            np.testing.assert_allclose(got, G[f"{prefix}_lpdf"][i], rtol=2e-5, err_msg=f"pdf {i}")
     assert lit > 50  # the new light type is sampled inside and across its profile

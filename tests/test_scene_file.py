@@ -384,6 +384,38 @@ def test_ies_parse_errors(host, tmp_path):
     assert np.allclose(rec[7][0], [1, 0.5])
 
 
+def test_sun_light_and_its_spectrum_match_reference(host, tmp_path):
+    """A sun light (scene_loader.cpp:977-997): direction from elevation / azimuth, disk radius
+    0.2665 deg x sun_size, scale intensity x sun_scale, no light once the sun is 0.02 below the
+    horizon; its radiance spectrum against the reference's MakeSunRadiance (sky.cpp:207-233: air
+    mass, Rayleigh and aerosol depth, blackbody at 2e6 cd/m^2 scaled with the disk's size), golden
+    vectors from the reference's own code. A sun-sky light also warns that its sky is skipped."""
+    G = np.load(os.path.join(HERE, "golden", "ref_components.npz"))
+    for (el, az, T, r), ref in zip(G["sun_cases"], G["sun_radiance"]):
+        doc = {"lights": {"s": {"type": "sun", "elevation": float(el), "azimuth": float(az), "turbidity": float(T),
+                                "sun_size": float(r) / 0.2665, "intensity": 0.5, "sun_scale": 2}},
+               "cameras": {"c": {"position": [0, 1, 5], "look_at": [0, 0, 0]}}}
+        f = tmp_path / "sun.prscene.json"
+        f.write_text(json.dumps(doc))
+        s = host.load_scene(f)
+        e, a = math.radians(el), math.radians(az)
+        if math.sin(e) <= -0.02:
+            assert len(s.lights) == 0 and not ref.any()
+            continue
+        sun = s.lights[0]
+        assert sun["kind"] == pa.LIGHT_SUN and sun["scale"] == pytest.approx(1.0)
+        assert np.allclose(sun["direction"], [math.cos(e) * math.sin(a), math.sin(e), math.cos(e) * math.cos(a)],
+                           atol=1e-6)
+        assert sun["cosFalloffEnd"] == pytest.approx(math.cos(math.radians(r)), abs=1e-7)
+        got = np.array([host.spectrum_value(s.spectra[sun["spectrum"]], s.knots, float(l))
+                        for l in np.arange(360.0, 831.0, 5.0)])
+        np.testing.assert_allclose(got, ref, rtol=2e-4, atol=1e-30, err_msg=f"sun {el} {az} {T} {r}")
+    doc["lights"]["s"].update(type="sun_sky", elevation=30.0)
+    f.write_text(json.dumps(doc))
+    s = host.load_scene(f)
+    assert len(s.lights) == 1 and any("Preetham sky" in w for w in s.warnings)
+
+
 # ---- camera ----------------------------------------------------------------------------------------
 
 def test_thin_lens_camera_from_f_stop_and_focus(studio, host, tmp_path):
@@ -595,7 +627,9 @@ def test_reference_sphere_pyramid(host):
     fog = host.load_scene(path, rigs=["studio", "fog"])
     assert fog.desc.cameraMedium >= 0 and len(fog.prims) == len(s.prims) + 12
     day = host.load_scene(path, rigs=["daylight"])
-    assert len(day.lights) == 1 and any("sun" in w for w in day.warnings)
+    # the sun_sky rig: its sun is a light (its sky is skipped with a warning) beside the glowing ball
+    assert sorted(day.lights["kind"].tolist()) == [pa.LIGHT_AREA, pa.LIGHT_SUN]
+    assert any("Preetham sky" in w for w in day.warnings)
 
 
 @pytest.mark.skipif(not os.path.isdir(REF_SCENES), reason="reference scene files not present")

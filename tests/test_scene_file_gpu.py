@@ -58,3 +58,14 @@ def test_sample_radiance_goniometric_lights(plugin, host):
         check_samples(f"ies sampler={sampler}", s, pa.render_params(spp=1, first_sample=2, seed=9, max_depth=6,
                                                                    rr_start=1, light_sampler=sampler),
                       plugin, 0.99)
+
+
+def test_sample_radiance_sun(plugin, host):
+    """The sun: cone sampling over its disk, emission met by escaping paths inside the disk with MIS,
+    beside a constant sky and a rect light, through a glass sphere."""
+    s = host.load_scene(os.path.join(os.path.dirname(STUDIO), "..", "sun", "scene.prscene.json"))
+    assert (s.lights["kind"] == pa.LIGHT_SUN).sum() == 1
+    for sampler in (pa.LIGHT_SAMPLER_BVH, pa.LIGHT_SAMPLER_POWER):
+        check_samples(f"sun sampler={sampler}", s, pa.render_params(spp=1, first_sample=3, seed=2, max_depth=8,
+                                                                   rr_start=1, light_sampler=sampler),
+                      plugin, 0.99)
