@@ -64,6 +64,15 @@ class BinaryBuilder {
   private:
     int Alloc() { return used.fetch_add(1); }
 
+#ifndef PG_BVH_LEAF_CANDIDATE
+#define PG_BVH_LEAF_CANDIDATE 4
+#endif
+#ifndef PG_BVH_TRAVERSAL_COST
+#define PG_BVH_TRAVERSAL_COST 1.0f
+#endif
+    static constexpr int kLeafCandidate = PG_BVH_LEAF_CANDIDATE;
+    static constexpr float kTraversalCost = PG_BVH_TRAVERSAL_COST;
+
     int Recurse(int b, int e, int depth) {
         int self = Alloc();
         BNode &node = nodes[size_t(self)];
@@ -118,9 +127,11 @@ class BinaryBuilder {
                 }
             }
         }
+        // A range becomes a leaf when it is small and a split would not pay: the SAH cost of the best
+        // split, relative to one primitive test, plus the cost of one more traversal step.
         float area = bounds.area();
-        float splitCost = 1.f + (area > 0 ? bestCost / area : 0.f);
-        if (n <= 4 && (bestAxis < 0 || float(n) <= splitCost)) return leaf();
+        float splitCost = kTraversalCost + (area > 0 ? bestCost / area : 0.f);
+        if (n <= kLeafCandidate && (bestAxis < 0 || float(n) <= splitCost)) return leaf();
         int mid;
         if (bestAxis < 0) {
             if (n <= kMaxLeafPrims) return leaf();
