@@ -62,24 +62,35 @@ Device &Get(int index) {
 constexpr int kTraceBlocksPerCU = PG_TRACE_BLOCKS_PER_CU;
 static_assert(kTraceBlocksPerCU * kStackLds * kTraceBlock * 4 <= 160 * 1024, "traversal blocks exceed the LDS of a CU");
 
+// With kStats (profile mode 2) every lane adds its rays, nodes fetched and leaf records tested
+// to counters[0..2] (closest hit) or counters[3..5] (any hit).
+template <bool kStats>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(TraceCtx c, const prgpu::RayEx *rays, prgpu::HitEx *hits,
-                                                            int64_t n, int anyHit) {
+                                                            int64_t n, int anyHit, unsigned long long *counters) {
     __shared__ uint32_t lds[kStackLds * kTraceBlock];
     const int tid = int(blockIdx.x * blockDim.x + threadIdx.x);
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    TraceStats st;
+    unsigned long long traced = 0;
     for (int64_t i = tid; i < n; i += stride) {
         const prgpu::RayEx rr = rays[i];
         DRay r{rr.o.x, rr.o.y, rr.o.z, rr.d.x, rr.d.y, rr.d.z};
         DHit h;
-        TraceStats st;
-        bool hit = anyHit ? dev_trace<true, false>(c, r, rr.tMax, &h, lds, tid, &st)
-                          : dev_trace<false, false>(c, r, rr.tMax, &h, lds, tid, &st);
+        bool hit = anyHit ? dev_trace<true, kStats>(c, r, rr.tMax, &h, lds, tid, &st)
+                          : dev_trace<false, kStats>(c, r, rr.tMax, &h, lds, tid, &st);
         prgpu::HitEx o;
         o.t = hit ? h.t : 0.f;
         o.prim = hit ? h.prim : -1;
         o.b1 = hit ? h.b1 : 0.f;
         o.b2 = hit ? h.b2 : 0.f;
         hits[i] = o;
+        ++traced;
+    }
+    if (kStats && traced) {
+        unsigned long long *ctr = counters + (anyHit ? 3 : 0);
+        atomicAdd(&ctr[0], traced);
+        atomicAdd(&ctr[1], (unsigned long long)st.nodes);
+        atomicAdd(&ctr[2], (unsigned long long)st.recs);
     }
 }
 
@@ -198,8 +209,14 @@ PRGPU_API int prgpu_trace_rays(int deviceIndex, const prgpu::SceneDescEx *sd, co
         if (n == 0) return 0;
         d.rays.upload(rays, size_t(n), d.stream);
         d.hits.ensure(size_t(n) * sizeof(prgpu::HitEx));
-        hipLaunchKernelGGL(k_trace_rays, dim3(TraceGrid(d, n)), dim3(kTraceBlock), 0, d.stream, MakeTraceCtx(d),
-                           d.rays.as<prgpu::RayEx>(), d.hits.as<prgpu::HitEx>(), n, anyHit);
+        if (d.profileMode >= 2 && d.travCounters.p)
+            hipLaunchKernelGGL(k_trace_rays<true>, dim3(TraceGrid(d, n)), dim3(kTraceBlock), 0, d.stream,
+                               MakeTraceCtx(d), d.rays.as<prgpu::RayEx>(), d.hits.as<prgpu::HitEx>(), n, anyHit,
+                               d.travCounters.as<unsigned long long>());
+        else
+            hipLaunchKernelGGL(k_trace_rays<false>, dim3(TraceGrid(d, n)), dim3(kTraceBlock), 0, d.stream,
+                               MakeTraceCtx(d), d.rays.as<prgpu::RayEx>(), d.hits.as<prgpu::HitEx>(), n, anyHit,
+                               static_cast<unsigned long long *>(nullptr));
         PG_CHECK(hipGetLastError());
         PG_CHECK(hipMemcpyAsync(hits, d.hits.p, size_t(n) * sizeof(prgpu::HitEx), hipMemcpyDeviceToHost, d.stream));
         CheckTraceFlags(d);

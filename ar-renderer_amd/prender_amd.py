@@ -161,6 +161,36 @@ def render_params(spp=1, first_sample=0, seed=1, max_depth=-1, rr_start=1, rr=Tr
     return RenderParamsEx(spp, first_sample, seed, max_depth, clamp, rr_start, 1 if rr else 0, light_sampler, flags)
 
 
+def measurement_rays(scene, n, seed):
+    """Two deterministic ray batches over a scene, for traversal statistics: n primary rays through
+    pixel-uniform film positions of the scene's camera (pinhole; the lens is ignored) and n rays
+    from uniform points of the primitives' bounding box in uniform directions. float32 throughout."""
+    rng = np.random.default_rng(seed)
+    cam = scene.camera
+    m = np.array(cam.cameraToWorld, np.float32).reshape(3, 4)
+    w, h = float(cam.width), float(cam.height)
+    fx = rng.uniform(0, w, n).astype(np.float32)
+    fy = rng.uniform(0, h, n).astype(np.float32)
+    t = np.float32(cam.tanHalfFov)
+    aspect = np.float32(w / h)
+    local = np.stack([(2 * fx / np.float32(w) - 1) * t * aspect, (1 - 2 * fy / np.float32(h)) * t,
+                      np.ones(n, np.float32)], axis=1).astype(np.float32)
+    d = local @ m[:, :3].T
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    cam_rays = np.zeros(n, RAY_DTYPE)
+    cam_rays["o"] = m[:, 3]
+    cam_rays["d"] = d.astype(np.float32)
+    cam_rays["tMax"] = np.inf
+    p = np.concatenate([scene.prims["p0"], scene.prims["p1"], scene.prims["p2"]])
+    lo, hi = p.min(axis=0), p.max(axis=0)
+    o = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+    v = rng.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    rnd = np.zeros(n, RAY_DTYPE)
+    rnd["o"], rnd["d"], rnd["tMax"] = o, v.astype(np.float32), np.inf
+    return cam_rays, rnd
+
+
 def _ptr(a):
     return a.ctypes.data if a is not None and len(a) else None
 

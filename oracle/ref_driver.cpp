@@ -406,6 +406,73 @@ void rf_trace(void *h, const prgpu::RayEx *rays, int64_t n, int anyHit, prgpu::H
     }
 }
 
+// Closest-hit queries on the reference's binary BVH with its traversal statistics: a replay of
+// BVH::Intersect (geometry/bvh.cpp:137-191: slab test with the far distances widened by
+// 1 + 6 * 1.2e-7, near child first by the split axis' direction sign, 64-entry stack) that also
+// counts, per ray, the nodes whose box is tested and the primitives tested. counts: 2 per ray.
+// The hits are those of BVH::Intersect; the caller checks them against rf_trace.
+void rf_trace_counts(void *h, const prgpu::RayEx *rays, int64_t n, prgpu::HitEx *hits, int32_t *counts) {
+    const Scene &sc = static_cast<RefScene *>(h)->scene;
+    const BVH &bvh = sc.GetBVH();
+    const auto &nodes = bvh.Nodes();
+    const auto &order = bvh.Order();
+    const float widen = 1 + 2 * 1.2e-7f * 3;
+    for (int64_t i = 0; i < n; ++i) {
+        Ray r(V(rays[i].o), V(rays[i].d));
+        float tMax = rays[i].tMax;
+        Vec3f inv(1 / r.d.x, 1 / r.d.y, 1 / r.d.z);
+        const int neg[3] = {inv.x < 0, inv.y < 0, inv.z < 0};
+        int stack[64], sp = 0, cur = 0, nodeCount = 0, primCount = 0;
+        prgpu::HitEx &o = hits[i];
+        o = prgpu::HitEx{0.f, -1, 0.f, 0.f};
+        while (!nodes.empty()) {
+            const BVH::Node &nd = nodes[size_t(cur)];
+            ++nodeCount;
+            const Bounds3f &b = nd.bounds;
+            float t0 = (b[neg[0]].x - r.o.x) * inv.x, t1 = (b[1 - neg[0]].x - r.o.x) * inv.x * widen;
+            float y0 = (b[neg[1]].y - r.o.y) * inv.y, y1 = (b[1 - neg[1]].y - r.o.y) * inv.y * widen;
+            bool inside = !(t0 > y1 || y0 > t1);
+            if (inside) {
+                t0 = y0 > t0 ? y0 : t0;
+                t1 = y1 < t1 ? y1 : t1;
+                float z0 = (b[neg[2]].z - r.o.z) * inv.z, z1 = (b[1 - neg[2]].z - r.o.z) * inv.z * widen;
+                inside = !(t0 > z1 || z0 > t1);
+                if (inside) {
+                    t0 = z0 > t0 ? z0 : t0;
+                    t1 = z1 < t1 ? z1 : t1;
+                    inside = t0 < tMax && t1 > 0;
+                }
+            }
+            if (inside && nd.nPrims == 0) {
+                if (neg[nd.axis]) {
+                    stack[sp++] = cur + 1;
+                    cur = nd.offset;
+                } else {
+                    stack[sp++] = nd.offset;
+                    cur = cur + 1;
+                }
+                continue;
+            }
+            if (inside) {
+                for (int k = 0; k < nd.nPrims; ++k) {
+                    int pi = nd.offset + k;
+                    ++primCount;
+                    float t;
+                    HitInfo info;
+                    if (sc.primitives[size_t(order[size_t(pi)])].shape->Intersect(r, tMax, &t, &info)) {
+                        tMax = t;
+                        o = prgpu::HitEx{t, order[size_t(pi)], info.b1, info.b2};
+                    }
+                }
+            }
+            if (sp == 0) break;
+            cur = stack[--sp];
+        }
+        counts[2 * i] = nodeCount;
+        counts[2 * i + 1] = primCount;
+    }
+}
+
 int rf_light_sample(void *h, const float *p, const float *n, float u, float *pmf) {
     RefScene &rs = *static_cast<RefScene *>(h);
     *pmf = 0;

@@ -40,6 +40,7 @@ def load_ref():
         ("rf_named_ior", C.c_float, [C.c_char_p, C.c_float]),
         ("rf_blackbody_unit_y", C.c_float, [C.c_float, C.c_float]),
         ("rf_sun_radiance", None, [vp, C.c_float, C.c_float, vp]),
+        ("rf_trace_counts", None, [vp, vp, C.c_int64, vp, vp]),
         ("rf_bxdf", None, [vp, C.c_int, vp, vp, C.c_float, vp, vp]),
         ("rf_fr_dielectric", C.c_float, [C.c_float, C.c_float]),
         ("rf_fr_complex", C.c_float, [C.c_float, C.c_float, C.c_float]),
@@ -321,10 +322,33 @@ def main():
     spot_lights(L, out, host)
     light_vectors(L, out, host, "ies", 5107)
     light_vectors(L, out, host, "sun", 6133)
+    reference_bvh_counts(L, out, host)
 
     path = os.path.join(HERE, "ref_components.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")
+
+
+def reference_bvh_counts(L, out, host):
+    """Nodes and primitives the reference's binary BVH tests per closest-hit ray (geometry/bvh.cpp)
+    on configuration 2 at 1920x1080, for the two batches of prender_amd.measurement_rays(scene, 8192,
+    7): primary rays and uniform random rays. The replay's hits must equal BVH::Intersect's."""
+    import hashlib
+    scene = host.generate(2)
+    h = L.rf_scene_create(C.byref(scene.desc), 40.0)
+    rows = []
+    for rays in pa.measurement_rays(scene, 8192, 7):
+        hits = np.zeros(len(rays), pa.HIT_DTYPE)
+        ref = np.zeros(len(rays), pa.HIT_DTYPE)
+        counts = np.zeros((len(rays), 2), np.int32)
+        L.rf_trace_counts(h, ptr(rays), len(rays), ptr(hits), ptr(counts))
+        L.rf_trace(h, ptr(rays), len(rays), 0, ptr(ref))
+        assert np.array_equal(hits.view(np.uint8), ref.view(np.uint8)), "replay differs from BVH::Intersect"
+        rows.append([counts[:, 0].mean(), counts[:, 1].mean(), (hits["prim"] >= 0).mean()])
+        out.setdefault("refbvh_ray_sha1", []).append(hashlib.sha1(rays.tobytes()).hexdigest())
+    L.rf_scene_destroy(h)
+    out["refbvh_counts"] = np.array(rows, np.float64)  # [batch][nodes per ray, primitives per ray, hit rate]
+    out["refbvh_ray_sha1"] = np.array(out["refbvh_ray_sha1"])
 
 
 def spot_lights(L, out, host):
