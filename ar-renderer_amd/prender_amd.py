@@ -161,6 +161,16 @@ def render_params(spp=1, first_sample=0, seed=1, max_depth=-1, rr_start=1, rr=Tr
     return RenderParamsEx(spp, first_sample, seed, max_depth, clamp, rr_start, 1 if rr else 0, light_sampler, flags)
 
 
+def flat_scene(npz, name):
+    """A Scene from arrays saved under `name_*` keys (prims, materials, spectra, knots, lights, media,
+    grid, camera bytes, camera_medium, max_depth, seed), as tests/golden/make_sample_scene.py writes them."""
+    g = lambda f: npz[f"{name}_{f}"]  # noqa: E731
+    cam = CameraEx.from_buffer_copy(g("camera").tobytes())
+    defaults = {"spp": 16, "maxDepth": int(g("max_depth")), "rrStartDepth": 1, "seed": int(g("seed"))}
+    return Scene(g("prims"), g("materials"), g("spectra"), g("knots"), g("lights"), g("media"), g("grid"), cam,
+                 int(g("camera_medium")), defaults)
+
+
 def measurement_rays(scene, n, seed):
     """Two deterministic ray batches over a scene, for traversal statistics: n primary rays through
     pixel-uniform film positions of the scene's camera (pinhole; the lens is ignored) and n rays

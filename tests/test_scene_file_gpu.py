@@ -69,3 +69,19 @@ def test_sample_radiance_sun(plugin, host):
         check_samples(f"sun sampler={sampler}", s, pa.render_params(spp=1, first_sample=3, seed=2, max_depth=8,
                                                                    rr_start=1, light_sampler=sampler),
                       plugin, 0.99)
+
+
+def test_sample_radiance_reference_sample_scene(plugin):
+    """The reference's sample scene (scenes/sphere-pyramid as the loader flattens it: glass with
+    dispersion, metals, subsurface, coats, sheen, an emissive sphere; studio rig, and studio + fog),
+    at a quarter of its film. The per-sample bar is that of the coated materials."""
+    import numpy as np
+    g = np.load(os.path.join(os.path.dirname(STUDIO), "..", "..", "golden", "sample_scene.npz"))
+    for name in ("studio", "studio_fog"):
+        full = pa.flat_scene(g, name)
+        cam = pa.CameraEx.from_buffer_copy(bytes(full.camera))
+        cam.width, cam.height = 240, 135
+        s = pa.Scene(full.prims, full.materials, full.spectra, full.knots, full.lights, full.media, full.grid, cam,
+                     full.desc.cameraMedium, full.defaults)
+        check_samples(f"sample scene {name}", s, pa.render_params(spp=1, first_sample=4, seed=1, max_depth=64,
+                                                                 rr_start=1), plugin, SAMPLE_BAR)
