@@ -127,7 +127,7 @@ void EnsureGeometry(Device &d, const prgpu::Primitive *prims, int count) {
         bvh.depth = 1;
     }
     d.nodes.upload(bvh.nodes.data(), bvh.nodes.size(), d.stream);
-    d.recs.upload(bvh.recs.data(), std::max<size_t>(1, bvh.recs.size()), d.stream);
+    d.recs.upload(bvh.recs.data(), bvh.recs.size(), d.stream);  // an empty scene gets a small unread buffer
     d.depth = bvh.depth;
     d.nodeCount = int(bvh.nodes.size());
     d.recCount = int(bvh.recs.size());
