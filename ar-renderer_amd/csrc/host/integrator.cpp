@@ -255,6 +255,12 @@ PH_API void ph_resolve_srgb(const double *xyz, int64_t pixels, double scale, flo
 
 PH_API void ph_fit_sigmoid(const float *rgb, float *coeffs) { phost::FitRGBSigmoid(rgb, coeffs); }
 
+PH_API int ph_preetham_sky(const float *sunDir, float turbidity, float groundAlbedo, int width, int height, float *rgb) {
+    if (!sunDir || !rgb || width <= 0 || height <= 0) return 1;
+    phost::PreethamSky(phost::f3(sunDir[0], sunDir[1], sunDir[2]), turbidity, groundAlbedo, width, height, rgb);
+    return 0;
+}
+
 PH_API float ph_spectrum_value(const prgpu::SpectrumEx *s, const float *knots, float lambda) {
     return phost::SpectrumValue(*s, knots, lambda);
 }

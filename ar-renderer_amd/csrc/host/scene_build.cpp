@@ -244,6 +244,8 @@ static const M3 &WhiteAdaptation() {
     return w;
 }
 
+void XYZToSRGB(float X, float Y, float Z, float rgb[3]) { kXYZToSRGB.apply(X, Y, Z, rgb); }
+
 void XYZToLinearSRGB(const double xyz[3], double scale, float rgb[3]) {
     float X = float(xyz[0] * scale), Y = float(xyz[1] * scale), Z = float(xyz[2] * scale);
     float a[3];

@@ -171,6 +171,12 @@ float SpectrumValue(const prgpu::SpectrumEx &s, const float *knots, float lambda
 void FitRGBSigmoid(const float rgb[3], float coeffs[3]);
 // Linear sRGB of a film XYZ value, with the renderer's spectral white adaptation (src/core/film.cpp:170-175).
 void XYZToLinearSRGB(const double xyz[3], double scale, float rgb[3]);
+// Linear sRGB of an XYZ value by the sRGB matrix alone (IEC 61966-2-1, D65 white).
+void XYZToSRGB(float X, float Y, float Z, float rgb[3]);
+// Preetham et al. (1999) daylight as an equirectangular linear-sRGB map of width x height texels
+// (kcd/m^2), below-horizon texels holding the Lambertian ground lit by sun and sky
+// (src/lights/sky.cpp:146-208). Writes 3 * width * height floats, rows top to bottom.
+void PreethamSky(F3 sunDir, float turbidity, float groundAlbedo, int width, int height, float *rgb);
 
 // IESNA LM-63 type C profile (src/lights/ies.cpp): angles in degrees, candela normalised by the peak.
 struct IesProfile {

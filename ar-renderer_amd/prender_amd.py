@@ -303,6 +303,7 @@ class Host:
                                 C.c_char_p, C.c_int]
         L.ph_resolve_srgb.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.c_void_p]
         L.ph_fit_sigmoid.argtypes = [C.c_void_p, C.c_void_p]
+        L.ph_preetham_sky.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_int, C.c_int, C.c_void_p]
         L.ph_spectrum_value.restype = C.c_float
         L.ph_spectrum_value.argtypes = [C.c_void_p, C.c_void_p, C.c_float]
         L.ph_scene_fingerprint.restype = C.c_uint64
@@ -378,6 +379,14 @@ class Host:
         rgb = np.ascontiguousarray(rgb, dtype=np.float32)
         out = np.zeros(3, np.float32)
         self.lib.ph_fit_sigmoid(rgb.ctypes.data, out.ctypes.data)
+        return out
+
+    def preetham_sky(self, sun_dir, turbidity, ground_albedo, width, height):
+        """Preetham daylight map (height, width, 3) in linear sRGB, kcd/m^2, zenith row first."""
+        d = np.ascontiguousarray(sun_dir, dtype=np.float32)
+        out = np.zeros((height, width, 3), np.float32)
+        if self.lib.ph_preetham_sky(d.ctypes.data, turbidity, ground_albedo, width, height, out.ctypes.data):
+            raise ValueError("invalid sky arguments")
         return out
 
     def spectrum_value(self, spec_row, knots, lam):

@@ -105,4 +105,8 @@ PH_API int ph_checkpoint_read(const char *path, const prgpu::SceneDescEx *scene,
 PH_API void ph_resolve_srgb(const double *xyz, int64_t pixels, double scale, float *rgb);
 // Sigmoid-polynomial coefficients of an sRGB reflectance (RGB uplift used by the generators).
 PH_API void ph_fit_sigmoid(const float *rgb, float *coeffs);
+// Preetham daylight map of a sun_sky light: width x height linear-sRGB texels (kcd/m^2),
+// equirectangular with v = 0 at the zenith, ground below the horizon (replaces MakePreethamSky,
+// src/lights/light.h:265-268, src/lights/sky.cpp:146-208). 0 on success, 1 on bad arguments.
+PH_API int ph_preetham_sky(const float *sunDir, float turbidity, float groundAlbedo, int width, int height, float *rgb);
 PH_API float ph_spectrum_value(const prgpu::SpectrumEx *s, const float *knots, float lambda);

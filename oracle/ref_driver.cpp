@@ -148,6 +148,17 @@ float rf_named_ior(const char *name, float l) {
     return s ? (*s)(l) : -1;
 }
 // The reference's solar radiance spectrum (sky.cpp:207-233) at 95 wavelengths from 360 nm, 5 nm apart.
+// The reference's Preetham daylight map (MakePreethamSky), width x height x RGB, zenith row first.
+void rf_preetham_sky(const float *dir, float T, float albedo, int w, int h, float *out) {
+    std::shared_ptr<Image> img = MakePreethamSky(Vec3f(dir[0], dir[1], dir[2]), T, albedo, w, h);
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            RGB c = img->Get(x, y);
+            float *o = out + 3 * (size_t(y) * w + x);
+            o[0] = c.r, o[1] = c.g, o[2] = c.b;
+        }
+}
+
 void rf_sun_radiance(const float *dir, float T, float radiusDeg, float *out95) {
     SpectrumPtr s = MakeSunRadiance(Vec3f(dir[0], dir[1], dir[2]), T, radiusDeg);
     for (int i = 0; i < 95; ++i) out95[i] = (*s)(360.f + 5.f * float(i));

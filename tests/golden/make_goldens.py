@@ -40,6 +40,7 @@ def load_ref():
         ("rf_named_ior", C.c_float, [C.c_char_p, C.c_float]),
         ("rf_blackbody_unit_y", C.c_float, [C.c_float, C.c_float]),
         ("rf_sun_radiance", None, [vp, C.c_float, C.c_float, vp]),
+        ("rf_preetham_sky", None, [vp, C.c_float, C.c_float, C.c_int, C.c_int, vp]),
         ("rf_trace_counts", None, [vp, vp, C.c_int64, vp, vp]),
         ("rf_bxdf", None, [vp, C.c_int, vp, vp, C.c_float, vp, vp]),
         ("rf_fr_dielectric", C.c_float, [C.c_float, C.c_float]),
@@ -323,10 +324,23 @@ def main():
     light_vectors(L, out, host, "ies", 5107)
     light_vectors(L, out, host, "sun", 6133)
     reference_bvh_counts(L, out, host)
+    preetham_sky(L, out)
 
     path = os.path.join(HERE, "ref_components.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")
+
+
+def preetham_sky(L, out):
+    """Preetham daylight maps (MakePreethamSky) at 48 x 24 texels: (sun x, y, z, turbidity, ground
+    albedo) cases with the sun high, low, just below the horizon (the fading daylight), at the
+    zenith, and with the turbidity clamped from below and above."""
+    cases = f32([[0.3, 0.8, 0.2, 2.5, 0.2], [-0.5, 0.12, 0.7, 6.0, 0.35], [0.0, -0.05, 1.0, 3.0, 0.1],
+                 [0.1, 1.0, 0.0, 1.2, 0.5], [0.9, 0.3, -0.4, 12.0, 0.0], [0.2, -0.4, 0.5, 4.0, 0.3]])
+    maps = np.zeros((len(cases), 24, 48, 3), np.float32)
+    for i, (x, y, z, T, albedo) in enumerate(cases):
+        L.rf_preetham_sky(ptr(f32([x, y, z])), float(T), float(albedo), 48, 24, ptr(maps[i]))
+    out["sky_cases"], out["sky_maps"] = cases, maps
 
 
 def reference_bvh_counts(L, out, host):

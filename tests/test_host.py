@@ -179,3 +179,21 @@ def test_checkpoint_round_trip_and_errors(host, tmp_path):
     with pytest.raises(RuntimeError, match="truncated or corrupt"):
         host.checkpoint_read(cut, s, 8, 1)
     assert not (tmp_path / "render.ckpt.tmp").exists()
+
+
+def test_preetham_sky_matches_reference_maps(host):
+    """The Preetham daylight map of sun_sky lights against the reference's MakePreethamSky
+    (src/lights/sky.cpp:146-208) at 48 x 24 texels: sun high, low, just below the horizon, at the
+    zenith, turbidity clamped from both sides. Same float operations in the same order, so the
+    maps agree to a few ulps (the ground term sums the sky's irradiance in double)."""
+    for case, ref in zip(G["sky_cases"], G["sky_maps"]):
+        sky = host.preetham_sky(case[:3], float(case[3]), float(case[4]), 48, 24)
+        np.testing.assert_allclose(sky, ref, rtol=1e-5, atol=1e-6, err_msg=str(case))
+        # Independent of the reference: the lower half is one grey ground value, the sky is
+        # non-negative, and a sun below -0.1 in y leaves no daylight above the horizon.
+        ground = sky[12:]
+        assert np.all(ground == ground[0, 0, 0]) and np.all(sky >= 0)
+        if case[1] / np.linalg.norm(case[:3]) <= -0.1:
+            assert not sky[:12].any()
+    with pytest.raises(ValueError):
+        host.preetham_sky((0, 1, 0), 3.0, 0.2, 0, 8)
