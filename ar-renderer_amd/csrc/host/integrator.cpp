@@ -255,6 +255,19 @@ PH_API void ph_resolve_srgb(const double *xyz, int64_t pixels, double scale, flo
 
 PH_API void ph_fit_sigmoid(const float *rgb, float *coeffs) { phost::FitRGBSigmoid(rgb, coeffs); }
 
+PH_API int ph_envmap_sample(const float *rgb, int width, int height, const float *u, int n, float *out, float *avgY) {
+    if (!rgb || width <= 0 || height <= 0 || n < 0 || (n > 0 && (!u || !out))) return 1;
+    phost::EnvMapDistribution d;
+    d.Build(rgb, width, height);
+    for (int i = 0; i < n; ++i) {
+        float *o = out + 4 * size_t(i);
+        d.Sample(u[2 * size_t(i)], u[2 * size_t(i) + 1], &o[0], &o[1], &o[2]);
+        o[3] = d.Pdf(o[0], o[1]);
+    }
+    if (avgY) *avgY = d.avgY;
+    return 0;
+}
+
 PH_API int ph_preetham_sky(const float *sunDir, float turbidity, float groundAlbedo, int width, int height, float *rgb) {
     if (!sunDir || !rgb || width <= 0 || height <= 0) return 1;
     phost::PreethamSky(phost::f3(sunDir[0], sunDir[1], sunDir[2]), turbidity, groundAlbedo, width, height, rgb);

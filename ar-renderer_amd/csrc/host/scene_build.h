@@ -178,6 +178,18 @@ void XYZToSRGB(float X, float Y, float Z, float rgb[3]);
 // (src/lights/sky.cpp:146-208). Writes 3 * width * height floats, rows top to bottom.
 void PreethamSky(F3 sunDir, float turbidity, float groundAlbedo, int width, int height, float *rgb);
 
+// Sampling tables of an equirectangular image environment (src/lights/light.cpp:221-239,
+// src/core/sampling.cpp:5-67): weights luminance x sin(theta) + 1e-6 per texel, per-row
+// conditional CDFs and integrals, the marginal CDF; continuous sampling and density on [0, 1]^2.
+struct EnvMapDistribution {
+    int nu = 0, nv = 0;
+    std::vector<float> func, condCdf, rowInt, margCdf;
+    float margInt = 0, avgY = 0;
+    void Build(const float *rgb, int w, int h);
+    void Sample(float u0, float u1, float *u, float *v, float *pdf) const;
+    float Pdf(float u, float v) const;
+};
+
 // IESNA LM-63 type C profile (src/lights/ies.cpp): angles in degrees, candela normalised by the peak.
 struct IesProfile {
     std::vector<float> v, h, cd;  // cd[h * nV + v]

@@ -304,6 +304,7 @@ class Host:
         L.ph_resolve_srgb.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.c_void_p]
         L.ph_fit_sigmoid.argtypes = [C.c_void_p, C.c_void_p]
         L.ph_preetham_sky.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_int, C.c_int, C.c_void_p]
+        L.ph_envmap_sample.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.ph_spectrum_value.restype = C.c_float
         L.ph_spectrum_value.argtypes = [C.c_void_p, C.c_void_p, C.c_float]
         L.ph_scene_fingerprint.restype = C.c_uint64
@@ -388,6 +389,18 @@ class Host:
         if self.lib.ph_preetham_sky(d.ctypes.data, turbidity, ground_albedo, width, height, out.ctypes.data):
             raise ValueError("invalid sky arguments")
         return out
+
+    def envmap_sample(self, rgb, u):
+        """Samples of an equirectangular RGB map (height, width, 3) at pairs u (n, 2): rows of
+        (u, v, pdf of the sample, density at (u, v)), and the map's mean luminance."""
+        rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+        u = np.ascontiguousarray(u, dtype=np.float32).reshape(-1, 2)
+        out = np.zeros((len(u), 4), np.float32)
+        avg = C.c_float()
+        if self.lib.ph_envmap_sample(rgb.ctypes.data, rgb.shape[1], rgb.shape[0], u.ctypes.data, len(u),
+                                     out.ctypes.data, C.byref(avg)):
+            raise ValueError("invalid environment map")
+        return out, avg.value
 
     def spectrum_value(self, spec_row, knots, lam):
         s = np.ascontiguousarray(np.array([spec_row], dtype=SPECTRUM_DTYPE))

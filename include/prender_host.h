@@ -109,4 +109,9 @@ PH_API void ph_fit_sigmoid(const float *rgb, float *coeffs);
 // equirectangular with v = 0 at the zenith, ground below the horizon (replaces MakePreethamSky,
 // src/lights/light.h:265-268, src/lights/sky.cpp:146-208). 0 on success, 1 on bad arguments.
 PH_API int ph_preetham_sky(const float *sunDir, float turbidity, float groundAlbedo, int width, int height, float *rgb);
+// Sampling of an equirectangular image environment (replaces the Distribution2D of
+// EnvironmentLight, src/lights/light.cpp:221-239, 279-296; src/core/sampling.cpp:5-67): for each of
+// the n pairs in u, writes (u, v, pdf of the sample, density at (u, v)) to out; avgY gets the map's
+// sin-weighted mean luminance. 0 on success, 1 on bad arguments.
+PH_API int ph_envmap_sample(const float *rgb, int width, int height, const float *u, int n, float *out, float *avgY);
 PH_API float ph_spectrum_value(const prgpu::SpectrumEx *s, const float *knots, float lambda);

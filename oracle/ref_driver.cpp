@@ -148,6 +148,33 @@ float rf_named_ior(const char *name, float l) {
     return s ? (*s)(l) : -1;
 }
 // The reference's solar radiance spectrum (sky.cpp:207-233) at 95 wavelengths from 360 nm, 5 nm apart.
+// The reference's image-environment sampling on an RGB map: its Distribution2D over the weights
+// the EnvironmentLight constructor forms (restated here: luminance x sin(theta) + 1e-6), sampled at
+// n pairs u; out per pair: (u, v, pdf, Distribution2D::PDF(u, v), EnvironmentLight::PDF_Li of the
+// sampled direction), the last from the reference light itself, which builds its own table.
+void rf_envmap_sample(const float *rgb, int w, int h, const float *u, int n, float *out) {
+    auto img = std::make_shared<Image>(w, h);
+    std::vector<float> f(size_t(w) * h);
+    for (int y = 0; y < h; ++y) {
+        float sinTheta = std::sin(Pi * (y + 0.5f) / h);
+        for (int x = 0; x < w; ++x) {
+            const float *c = rgb + 3 * (size_t(y) * w + x);
+            img->Set(x, y, RGB(c[0], c[1], c[2]));
+            float lum = 0.2126f * c[0] + 0.7152f * c[1] + 0.0722f * c[2];
+            f[size_t(y) * w + x] = std::max(lum, 0.f) * sinTheta + 1e-6f;
+        }
+    }
+    Distribution2D d(f.data(), w, h);
+    EnvironmentLight light(img, 1.f, 0.f);
+    for (int i = 0; i < n; ++i) {
+        float pdf;
+        Vec2f s = d.SampleContinuous(Vec2f(u[2 * i], u[2 * i + 1]), &pdf);
+        float *o = out + 5 * size_t(i);
+        o[0] = s.x, o[1] = s.y, o[2] = pdf, o[3] = d.PDF(s);
+        o[4] = light.PDF_Li(LightSampleContext(), EquirectToDir(s));
+    }
+}
+
 // The reference's Preetham daylight map (MakePreethamSky), width x height x RGB, zenith row first.
 void rf_preetham_sky(const float *dir, float T, float albedo, int w, int h, float *out) {
     std::shared_ptr<Image> img = MakePreethamSky(Vec3f(dir[0], dir[1], dir[2]), T, albedo, w, h);

@@ -41,6 +41,7 @@ def load_ref():
         ("rf_blackbody_unit_y", C.c_float, [C.c_float, C.c_float]),
         ("rf_sun_radiance", None, [vp, C.c_float, C.c_float, vp]),
         ("rf_preetham_sky", None, [vp, C.c_float, C.c_float, C.c_int, C.c_int, vp]),
+        ("rf_envmap_sample", None, [vp, C.c_int, C.c_int, vp, C.c_int, vp]),
         ("rf_trace_counts", None, [vp, vp, C.c_int64, vp, vp]),
         ("rf_bxdf", None, [vp, C.c_int, vp, vp, C.c_float, vp, vp]),
         ("rf_fr_dielectric", C.c_float, [C.c_float, C.c_float]),
@@ -325,6 +326,7 @@ def main():
     light_vectors(L, out, host, "sun", 6133)
     reference_bvh_counts(L, out, host)
     preetham_sky(L, out)
+    envmap_sampling(L, out)
 
     path = os.path.join(HERE, "ref_components.npz")
     np.savez_compressed(path, **out)
@@ -341,6 +343,17 @@ def preetham_sky(L, out):
     for i, (x, y, z, T, albedo) in enumerate(cases):
         L.rf_preetham_sky(ptr(f32([x, y, z])), float(T), float(albedo), 48, 24, ptr(maps[i]))
     out["sky_cases"], out["sky_maps"] = cases, maps
+
+
+def envmap_sampling(L, out):
+    """Image-environment sampling on the first sky map: 2000 random pairs plus the corners and
+    centre of [0, 1)^2; per pair (u, v, pdf, Distribution2D::PDF, EnvironmentLight::PDF_Li)."""
+    rgb = np.ascontiguousarray(out["sky_maps"][0])
+    u = np.random.default_rng(7).random((2000, 2)).astype(np.float32)
+    u[:4] = [[0, 0], [0.99999994, 0.99999994], [0.5, 0.5], [0.25, 0.75]]
+    res = np.zeros((len(u), 5), np.float32)
+    L.rf_envmap_sample(ptr(rgb), rgb.shape[1], rgb.shape[0], ptr(u), len(u), ptr(res))
+    out["env_u"], out["env_ref"] = u, res
 
 
 def reference_bvh_counts(L, out, host):
