@@ -327,10 +327,62 @@ def main():
     reference_bvh_counts(L, out, host)
     preetham_sky(L, out)
     envmap_sampling(L, out)
+    thin_lens_camera(L, out, host)
+    c2_light_bvh(L, out, host)
 
     path = os.path.join(HERE, "ref_components.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")
+
+
+def thin_lens_camera(L, out, host):
+    """Camera rays of configuration 5's f/2.8 thin-lens camera (src/cameras/camera.cpp:36-48:
+    concentric-disk lens sample, focus plane at the focal distance) for random film positions and lens
+    samples, including the lens centre and its rim."""
+    rng = np.random.default_rng(8311)
+    scene = host.generate(5, 160, 90, 0.25)
+    assert scene.camera.lensRadius > 0
+    h = L.rf_scene_create(C.byref(scene.desc), 30.0)
+    film = f32(rng.uniform(-1, 161, (300, 2)) * [1, 90 / 160])
+    lens = f32(rng.random((300, 2)))
+    lens[:4] = [[0.5, 0.5], [0.0, 0.5], [1.0, 0.0], [0.999, 0.999]]
+    rays = np.zeros((300, 6), np.float32)
+    for i in range(300):
+        o6 = np.zeros(6, np.float32)
+        L.rf_camera_ray(h, float(film[i, 0]), float(film[i, 1]), float(lens[i, 0]), float(lens[i, 1]), ptr(o6))
+        rays[i] = o6
+    out.update(c5_cam_film=film, c5_cam_lens=lens, c5_cam_rays=rays)
+    L.rf_scene_destroy(h)
+
+
+def c2_light_bvh(L, out, host):
+    """Light-BVH choice and PMFs (src/lights/light_bvh.cpp:103-234) on configuration 2's 64 quad lights
+    (128 emitting triangles, blackbody 2700-9000 K), at points over the terrain with surface normals and
+    at medium points (zero normal); the scene geometry is generated at reduced detail, which leaves the
+    lights as they are."""
+    rng = np.random.default_rng(9127)
+    scene = host.generate(2, 160, 90, 0.1)
+    n_l = len(scene.lights)
+    h = L.rf_scene_create(C.byref(scene.desc), 40.0)
+    lo = np.array([-50, -2, -25], np.float32)
+    hi = np.array([50, 12, 25], np.float32)
+    P = f32(rng.uniform(lo, hi, (200, 3)))
+    N = rng.normal(size=(200, 3)).astype(np.float32)
+    N /= np.linalg.norm(N, axis=1, keepdims=True)
+    N[::5] = 0
+    U = f32(rng.random(200))
+    li = np.zeros(200, np.int32)
+    lpmf = np.zeros(200, np.float32)
+    each = np.zeros((200, n_l), np.float32)
+    for i in range(200):
+        pm = np.zeros(1, np.float32)
+        li[i] = L.rf_light_sample(h, ptr(P[i]), ptr(N[i]), float(U[i]), ptr(pm))
+        lpmf[i] = pm[0]
+        for k in range(n_l):
+            each[i, k] = L.rf_light_pmf(h, ptr(P[i]), ptr(N[i]), k)
+    out.update(c2l_p=P, c2l_n=N, c2l_u=U, c2l_light=li, c2l_light_pmf=lpmf, c2l_pmf_each=each,
+               c2l_power_pmf=f32([L.rf_light_power_pmf(h, k) for k in range(n_l)]))
+    L.rf_scene_destroy(h)
 
 
 def preetham_sky(L, out):

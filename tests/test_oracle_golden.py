@@ -218,6 +218,36 @@ def test_c1_area_light_sample_li(c1_small):
 
 
 
+def test_c5_thin_lens_camera_rays(host):
+    """Configuration 5's f/2.8 thin-lens camera (src/cameras/camera.cpp:36-48) against the reference's
+    rays for the same film positions and lens samples."""
+    s = host.generate(5, 160, 90, 0.25)
+    o = orc.OracleScene(s)
+    for (fx, fy), (lx, ly), ref in zip(G["c5_cam_film"], G["c5_cam_lens"], G["c5_cam_rays"]):
+        np.testing.assert_allclose(o.camera_ray(float(fx), float(fy), float(lx), float(ly)), ref, rtol=2e-6, atol=2e-6)
+    # the lens matters: the centre sample and a rim sample of one film position leave different points
+    a = o.camera_ray(80.0, 45.0, 0.5, 0.5)
+    b = o.camera_ray(80.0, 45.0, 1.0, 0.5)
+    assert np.linalg.norm(a[:3] - b[:3]) > 0.1 * s.camera.lensRadius
+
+
+def test_c2_light_bvh_sampling_and_pmf(host):
+    """The light BVH over configuration 2's 128 emitting triangles (src/lights/light_bvh.cpp:103-234):
+    chosen light, its PMF and every light's PMF at 200 points, and the power PMFs."""
+    s = host.generate(2, 160, 90, 0.1)
+    o = orc.OracleScene(s)
+    assert len(s.lights) == 128
+    P, N, U = G["c2l_p"], G["c2l_n"], G["c2l_u"]
+    for i in range(len(P)):
+        li, pmf = o.light_sample(P[i], N[i], float(U[i]))
+        assert li == G["c2l_light"][i], i
+        np.testing.assert_allclose(pmf, G["c2l_light_pmf"][i], rtol=1e-5, atol=1e-9)
+        each = [o.light_pmf(P[i], N[i], k) for k in range(len(s.lights))]
+        np.testing.assert_allclose(each, G["c2l_pmf_each"][i], rtol=1e-5, atol=1e-9)
+    pw = np.array([L.or_light_power(o.h, k) for k in range(len(s.lights))], np.float64)
+    np.testing.assert_allclose(pw / pw.sum(), G["c2l_power_pmf"], rtol=1e-5)
+
+
 def scene_file(host, name):
     s = host.load_scene(os.path.join(os.path.dirname(__file__), "scenes", name, "scene.prscene.json"))
     return s, orc.OracleScene(s)
