@@ -58,3 +58,26 @@ def test_image_3sigma_media(plugin, c3):
 def test_media_power_light_sampler_and_depth_cap(plugin, c3):
     p = pa.render_params(spp=1, first_sample=2, seed=3, max_depth=4, light_sampler=pa.LIGHT_SAMPLER_POWER)
     check_samples("c3 depth 4 power", c3, p, plugin, 0.90)
+
+
+@pytest.fixture(scope="module")
+def c3_full_grid(host):
+    """Configuration 3 at its own grid resolution (detail 1: the 256^3 FP32 density grid, 64 MiB)
+    on a small film."""
+    s = host.generate(3, 64, 36, 1.0)
+    assert s.grid.size == 256 ** 3
+    return s
+
+
+def test_sample_radiance_grid_medium_256(plugin, c3_full_grid):
+    for si in (0, 9):
+        check_samples(f"c3 256^3 grid si={si}", c3_full_grid,
+                      pa.render_params(spp=1, first_sample=si, seed=1, max_depth=256), plugin, 0.90)
+
+
+def test_image_3sigma_media_256(plugin, c3_full_grid):
+    s = c3_full_grid
+    spp = 128
+    g_sum, g_sq = plugin.render_ex(s, pa.render_params(spp=spp, first_sample=0, seed=1, max_depth=256))
+    c_sum, c_sq, _ = orc.OracleScene(s).render(pa.render_params(spp=spp, first_sample=spp, seed=2, max_depth=256))
+    three_sigma("c3 256^3 image", g_sum, g_sq, spp, c_sum, c_sq, spp, splat_correlation(s))
