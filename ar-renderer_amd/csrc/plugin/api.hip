@@ -23,6 +23,16 @@ std::mutex gMutex;
 std::vector<std::unique_ptr<Device>> gDevices;
 bool gEnumerated = false;
 
+// Scratch of the allocation-free ray query (prgpu_trace_rays_noalloc), one copy per GPU, held
+// until the process ends. Words [0, kScratchHeadWords) hold the traversal stack-overflow flag
+// (word 0); the rest holds the per-lane spill slabs of the traversal stacks. Host code never
+// passes `scratch` itself to a kernel: it uses the device address that hipGetSymbolAddress
+// returns for the current device (Device::scratch).
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr size_t kScratchWords = kScratchBytes / sizeof(uint32_t);
+constexpr size_t kScratchHeadWords = 64;  // 256 B, so the spill slabs start 256-B aligned
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+
 void SetErr(char *err, int len, const std::string &m) {
     if (err && len > 0) std::snprintf(err, size_t(len), "%s", m.c_str());
 }
@@ -50,8 +60,17 @@ Device &Get(int index) {
         d.cus = prop.multiProcessorCount;
         d.flags.ensure(64);
         PG_CHECK(hipMemset(d.flags.p, 0, 64));
+        // Device address of this GPU's copy of the allocation-free query's scratch array.
+        void *sp = nullptr;
+        PG_CHECK(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+        d.scratch = static_cast<uint32_t *>(sp);
     }
     return d;
+}
+
+__global__ void k_scratch_clear(uint32_t *words, int n) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) words[i] = 0u;
 }
 
 // Resident traversal blocks per CU (the persistent grid): LDS holds kStackLds * 256 * 4 B per
@@ -220,6 +239,77 @@ PRGPU_API int prgpu_trace_rays(int deviceIndex, const prgpu::SceneDescEx *sd, co
         PG_CHECK(hipGetLastError());
         PG_CHECK(hipMemcpyAsync(hits, d.hits.p, size_t(n) * sizeof(prgpu::HitEx), hipMemcpyDeviceToHost, d.stream));
         CheckTraceFlags(d);
+        return 0;
+    } catch (const std::exception &e) {
+        SetErr(err, errLen, e.what());
+        return 1;
+    }
+}
+
+namespace {
+// True when [p, p + bytes) lies inside one device allocation of device `id`.
+bool DeviceRange(const void *p, size_t bytes, int id) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    if (a.type != hipMemoryTypeDevice || a.device != id) return false;
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, hipDeviceptr_t(const_cast<void *>(p))) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    const uintptr_t b = uintptr_t(base), q = uintptr_t(p);
+    return q >= b && q - b <= size && bytes <= size - (q - b);
+}
+}  // namespace
+
+PRGPU_API int prgpu_trace_rays_noalloc(int deviceIndex, const prgpu::SceneDescEx *sd, const prgpu::RayEx *rays,
+                                       int64_t n, int anyHit, prgpu::HitEx *hits, char *err, int errLen) {
+    std::lock_guard<std::mutex> lock(gMutex);
+    try {
+        if (!sd || sd->apiVersion != prgpu::kApiVersion || sd->extVersion != prgpu::kExtVersion)
+            throw std::runtime_error("API version mismatch");
+        if (deviceIndex < 0 || deviceIndex >= int(gDevices.size()) || !gDevices[size_t(deviceIndex)]->stream ||
+            !gDevices[size_t(deviceIndex)]->scratch)
+            throw std::runtime_error("device not initialised: make the scene resident with prgpu_trace_rays or "
+                                     "prgpu_render_ex on this device first");
+        Device &d = *gDevices[size_t(deviceIndex)];
+        PG_CHECK(hipSetDevice(d.id));
+        if (!d.nodes.p || GeometryKey(sd->prims, sd->primCount) != d.geomKey)
+            throw std::runtime_error("scene geometry is not resident on this device: make it resident with "
+                                     "prgpu_trace_rays or prgpu_render_ex first");
+        if (n < 0) throw std::runtime_error("invalid ray batch");
+        if (n == 0) return 0;
+        if (!DeviceRange(rays, size_t(n) * sizeof(prgpu::RayEx), d.id) ||
+            !DeviceRange(hits, size_t(n) * sizeof(prgpu::HitEx), d.id))
+            throw std::runtime_error("rays and hits must be device allocations of this GPU holding n records");
+        // The spill slabs of all lanes must fit the scratch: cap the grid to what it holds.
+        const int64_t slabWords = int64_t(kScratchWords - kScratchHeadWords);
+        int grid = TraceGrid(d, n);
+        if (d.spillCap > 0) {
+            const int64_t maxBlocks = slabWords / (int64_t(d.spillCap) * kTraceBlock);
+            if (maxBlocks < 1)
+                throw std::runtime_error("BVH too deep for the allocation-free query: its traversal stacks need " +
+                                         std::to_string(int64_t(d.spillCap) * kTraceBlock * 4) + " B of spill per block, "
+                                         "the scratch holds " + std::to_string(slabWords * 4) + " B");
+            grid = int(std::min<int64_t>(grid, maxBlocks));
+        }
+        TraceCtx c = MakeTraceCtx(d);
+        c.spill = d.scratch + kScratchHeadWords;
+        c.spillStride = grid * kTraceBlock;
+        c.errorFlag = reinterpret_cast<int *>(d.scratch);
+        // The scratch keeps the previous call's contents: the overflow flag must start at zero.
+        hipLaunchKernelGGL(k_scratch_clear, dim3(1), dim3(64), 0, d.stream, d.scratch, 1);
+        hipLaunchKernelGGL(k_trace_rays<false>, dim3(grid), dim3(kTraceBlock), 0, d.stream, c, rays, hits, n, anyHit,
+                           static_cast<unsigned long long *>(nullptr));
+        PG_CHECK(hipGetLastError());
+        int f = 0;
+        PG_CHECK(hipMemcpyAsync(&f, d.scratch, sizeof(int), hipMemcpyDeviceToHost, d.stream));
+        PG_CHECK(hipStreamSynchronize(d.stream));
+        if (f) throw std::runtime_error("BVH traversal stack overflow");
         return 0;
     } catch (const std::exception &e) {
         SetErr(err, errLen, e.what());

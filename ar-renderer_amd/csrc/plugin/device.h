@@ -64,6 +64,7 @@ struct Device {
     DevBuf spill;
     int spillCap = 0, spillStride = 0;
     DevBuf flags;  // [0] traversal stack overflow
+    uint32_t *scratch = nullptr;  // this GPU's copy of the allocation-free query's scratch array
     // Path tracing
     uint64_t ptKey = 0;
     std::unique_ptr<PTScene> pt;

@@ -547,6 +547,14 @@ class Plugin:
                                               hits.ctypes.data if len(hits) else None, err, 1024), err)
         return hits
 
+    def trace_noalloc(self, scene, rays_dev, n, hits_dev, any_hit=False, device=0):
+        """prgpu_trace_rays_noalloc: rays_dev / hits_dev are device addresses (ints) of this GPU's
+        memory holding n RayEx / HitEx records; the scene's geometry must already be resident."""
+        f = self.lib.prgpu_trace_rays_noalloc
+        f.argtypes = [C.c_int, C.POINTER(SceneDescEx), C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_char_p, C.c_int]
+        err = C.create_string_buffer(1024)
+        self._check(f(device, scene.desc_ptr(), rays_dev, n, 1 if any_hit else 0, hits_dev, err, 1024), err)
+
     def render_ex(self, scene, params, device=0, variance=True):
         n = scene.width * scene.height * 3
         s = np.zeros(n, np.float64)

@@ -349,6 +349,17 @@ PRGPU_API int prgpu_render_ex(int deviceIndex, const prgpu::SceneDescEx *scene, 
 // host arrays. For any-hit queries only HitEx::prim (>= 0: occluded) is meaningful.
 PRGPU_API int prgpu_trace_rays(int deviceIndex, const prgpu::SceneDescEx *scene, const prgpu::RayEx *rays, int64_t n,
                                int anyHit, prgpu::HitEx *hits, char *err, int errLen);
+// Allocation-free closest-hit / any-hit query, for callers that cannot allocate device memory
+// (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No
+// reference counterpart. rays and hits are DEVICE arrays of this GPU holding n records each;
+// the scene's geometry must already be resident (a previous prgpu_trace_rays or prgpu_render_ex
+// with the same scene on this device), else the call fails. It calls no hipMalloc / hipFree:
+// its traversal-stack spill and overflow flag live in a fixed 64 MiB scratch array per GPU (the
+// grid is capped to the lanes whose spill slabs fit; a tree too deep for one block's slabs is
+// rejected). Calls that overlap on one GPU share that scratch and race; the call is synchronous
+// (the hits are written when it returns). Results equal prgpu_trace_rays' on the same rays.
+PRGPU_API int prgpu_trace_rays_noalloc(int deviceIndex, const prgpu::SceneDescEx *scene, const prgpu::RayEx *rays,
+                                       int64_t n, int anyHit, prgpu::HitEx *hits, char *err, int errLen);
 // Radiance of one camera sample per extended-film pixel for sample index params->firstSample
 // (params->spp is ignored): out has (width + 2m) * (height + 2m) records in raster order.
 PRGPU_API int prgpu_sample_radiance(int deviceIndex, const prgpu::SceneDescEx *scene, const prgpu::RenderParamsEx *params,
