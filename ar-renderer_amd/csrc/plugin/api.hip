@@ -4,10 +4,16 @@
 // src/gpu/prender_gpu.cpp:1576-1730 for the contract).
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
+
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <string>
 
 #include "device.h"
 #include "guide_host.h"
@@ -18,6 +24,14 @@
 namespace pgpu {
 
 namespace {
+
+// Most resident traversal blocks per CU (the persistent grid; the spill slabs are sized for it): LDS holds kStackLds * 256 * 4 B per
+// block, 160 KiB per CU.
+#ifndef PG_TRACE_BLOCKS_PER_CU
+#define PG_TRACE_BLOCKS_PER_CU 8
+#endif
+constexpr int kTraceBlocksPerCU = PG_TRACE_BLOCKS_PER_CU;
+static_assert(kTraceBlocksPerCU * kStackLds * kTraceBlock * 4 <= 160 * 1024, "traversal blocks exceed the LDS of a CU");
 
 std::mutex gMutex;
 std::vector<std::unique_ptr<Device>> gDevices;
@@ -37,14 +51,104 @@ void SetErr(char *err, int len, const std::string &m) {
     if (err && len > 0) std::snprintf(err, size_t(len), "%s", m.c_str());
 }
 
+// ---- run-time tuning ----------------------------------------------------------------------------
+// Keys of the tuning file and of prgpu_tune_set / prgpu_tune_get, with their admissible ranges.
+struct TuneKey {
+    const char *name;
+    double lo, hi;
+    bool integer;
+};
+const TuneKey kTuneKeys[] = {
+    {"trace_blocks_per_cu", 1, kTraceBlocksPerCU, true}, {"shade_blocks_per_cu", 1, 64, true},
+    {"path_budget", 1 << 16, 1 << 26, true},           {"guide_batch", 256, 1 << 20, true},
+    {"guide_steps", 1, 64, true},                      {"guide_min_steps", 0, 64, true},
+    {"guide_lr", 1e-6, 1.0, false},                    {"guide_bounces", 0, 64, true},
+    {"guide_blend", 0.0, 1.0, false},
+};
+
+bool TuneSet(Tuning &t, const std::string &key, double v) {
+    for (const TuneKey &k : kTuneKeys) {
+        if (key != k.name) continue;
+        if (!(v >= k.lo && v <= k.hi) || (k.integer && v != std::floor(v))) return false;
+        if (key == "trace_blocks_per_cu") t.traceBlocksPerCU = int(v);
+        else if (key == "shade_blocks_per_cu") t.shadeBlocksPerCU = int(v);
+        else if (key == "path_budget") t.pathBudget = int64_t(v);
+        else if (key == "guide_batch") t.guideBatch = int(v);
+        else if (key == "guide_steps") t.guideSteps = int(v);
+        else if (key == "guide_min_steps") t.guideMinSteps = int(v);
+        else if (key == "guide_lr") t.guideLr = float(v);
+        else if (key == "guide_bounces") t.guideBounces = int(v);
+        else if (key == "guide_blend") t.guideBlend = float(v);
+        return true;
+    }
+    return false;
+}
+
+bool TuneGet(const Tuning &t, const std::string &key, double *v) {
+    if (key == "trace_blocks_per_cu") *v = t.traceBlocksPerCU;
+    else if (key == "shade_blocks_per_cu") *v = t.shadeBlocksPerCU;
+    else if (key == "path_budget") *v = double(t.pathBudget);
+    else if (key == "guide_batch") *v = t.guideBatch;
+    else if (key == "guide_steps") *v = t.guideSteps;
+    else if (key == "guide_min_steps") *v = t.guideMinSteps;
+    else if (key == "guide_lr") *v = t.guideLr;
+    else if (key == "guide_bounces") *v = t.guideBounces;
+    else if (key == "guide_blend") *v = t.guideBlend;
+    else return false;
+    return true;
+}
+
+Tuning gTuning;           // defaults for every device, after the tuning file
+std::string gTuningFile;  // the file read ("" when none)
+std::string gTuningError; // the first problem found in it ("" when none)
+
+// The tuning file: $PRGPU_TUNING_FILE, else prgpu_tuning.txt beside this library. Lines of
+// "key value"; '#' starts a comment. An unreadable line or value is reported and skipped.
+void LoadTuningFile() {
+    std::string path;
+    if (const char *e = std::getenv("PRGPU_TUNING_FILE")) {
+        path = e;
+    } else {
+        Dl_info info;
+        if (dladdr(reinterpret_cast<const void *>(&LoadTuningFile), &info) && info.dli_fname) {
+            std::string lib = info.dli_fname;
+            const size_t slash = lib.find_last_of('/');
+            path = (slash == std::string::npos ? std::string(".") : lib.substr(0, slash)) + "/prgpu_tuning.txt";
+        }
+    }
+    if (path.empty()) return;
+    std::FILE *f = std::fopen(path.c_str(), "r");
+    if (!f) return;
+    gTuningFile = path;
+    char line[512];
+    int no = 0;
+    while (std::fgets(line, sizeof(line), f)) {
+        ++no;
+        std::string s(line);
+        const size_t hash = s.find('#');
+        if (hash != std::string::npos) s.resize(hash);
+        char key[128];
+        double v = 0;
+        char extra[8];
+        const int got = std::sscanf(s.c_str(), " %127s %lf %7s", key, &v, extra);
+        if (got <= 0) continue;  // blank or comment
+        if (got != 2 || !TuneSet(gTuning, key, v)) {
+            if (gTuningError.empty()) gTuningError = path + ":" + std::to_string(no) + ": invalid tuning line";
+        }
+    }
+    std::fclose(f);
+}
+
 void Enumerate() {
     if (gEnumerated) return;
     gEnumerated = true;
+    LoadTuningFile();
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
     for (int i = 0; i < n; ++i) {
         auto d = std::make_unique<Device>();
         d->id = i;
+        d->tune = gTuning;
         gDevices.push_back(std::move(d));
     }
 }
@@ -73,13 +177,6 @@ __global__ void k_scratch_clear(uint32_t *words, int n) {
     if (i < n) words[i] = 0u;
 }
 
-// Resident traversal blocks per CU (the persistent grid): LDS holds kStackLds * 256 * 4 B per
-// block, 160 KiB per CU.
-#ifndef PG_TRACE_BLOCKS_PER_CU
-#define PG_TRACE_BLOCKS_PER_CU 8
-#endif
-constexpr int kTraceBlocksPerCU = PG_TRACE_BLOCKS_PER_CU;
-static_assert(kTraceBlocksPerCU * kStackLds * kTraceBlock * 4 <= 160 * 1024, "traversal blocks exceed the LDS of a CU");
 
 // With kStats (profile mode 2) every lane adds its rays, nodes fetched and leaf records tested
 // to counters[0..2] (closest hit) or counters[3..5] (any hit).
@@ -176,7 +273,8 @@ TraceCtx MakeTraceCtx(Device &d) {
 
 int TraceGrid(const Device &d, int64_t n) {
     int64_t blocks = (n + kTraceBlock - 1) / kTraceBlock;
-    return int(std::max<int64_t>(1, std::min<int64_t>(blocks, int64_t(d.cus) * kTraceBlocksPerCU)));
+    const int perCU = std::max(1, std::min(d.tune.traceBlocksPerCU, kTraceBlocksPerCU));
+    return int(std::max<int64_t>(1, std::min<int64_t>(blocks, int64_t(d.cus) * perCU)));
 }
 
 void CheckTraceFlags(Device &d) {
@@ -192,6 +290,28 @@ void CheckTraceFlags(Device &d) {
 }  // namespace pgpu
 
 using namespace pgpu;
+
+PRGPU_API int prgpu_tune_set(int deviceIndex, const char *key, double value) {
+    std::lock_guard<std::mutex> lock(gMutex);
+    Enumerate();
+    if (!key || deviceIndex < 0 || deviceIndex >= int(gDevices.size())) return 1;
+    return TuneSet(gDevices[size_t(deviceIndex)]->tune, key, value) ? 0 : 1;
+}
+
+PRGPU_API int prgpu_tune_get(int deviceIndex, const char *key, double *value) {
+    std::lock_guard<std::mutex> lock(gMutex);
+    Enumerate();
+    if (!key || !value || deviceIndex < 0 || deviceIndex >= int(gDevices.size())) return 1;
+    return TuneGet(gDevices[size_t(deviceIndex)]->tune, key, value) ? 0 : 1;
+}
+
+PRGPU_API int prgpu_tune_source(char *path, int pathLen, char *error, int errorLen) {
+    std::lock_guard<std::mutex> lock(gMutex);
+    Enumerate();
+    SetErr(path, pathLen, gTuningFile);
+    SetErr(error, errorLen, gTuningError);
+    return gTuningFile.empty() ? 1 : 0;
+}
 
 PRGPU_API int prgpu_api_version() { return prgpu::kApiVersion; }
 PRGPU_API int prgpu_api_ext_version() { return prgpu::kExtVersion; }
@@ -541,6 +661,7 @@ PRGPU_API int prgpu_guide_train(int deviceIndex, const float *records, int n, in
         if (n <= 0 || !records || steps < 0 || (steps == 0 && !grad)) throw std::runtime_error("invalid arguments");
         Device &d = Get(deviceIndex);
         pgpu::GuideState &g = EnsureGuide(d);
+        g.blend = d.tune.guideBlend;
         DevBuf in;
         in.upload(records, size_t(n) * 16, d.stream);
         if (steps == 0) {
@@ -554,6 +675,27 @@ PRGPU_API int prgpu_guide_train(int deviceIndex, const float *records, int n, in
         return 0;
     } catch (const std::exception &) {
         return 1;
+    }
+}
+
+PRGPU_API int prgpu_guide_records(int deviceIndex, float *records, int max, int *produced) {
+    std::lock_guard<std::mutex> lock(gMutex);
+    try {
+        Enumerate();
+        Device &d = Get(deviceIndex);
+        if (!d.guide || max < 0 || (max > 0 && !records)) return -1;
+        int counts[2] = {0, 0};
+        PG_CHECK(hipMemcpyAsync(counts, d.guide->trainCount.p, 2 * sizeof(int), hipMemcpyDeviceToHost, d.stream));
+        PG_CHECK(hipStreamSynchronize(d.stream));
+        const int n = std::max(0, std::min({counts[0], int(d.guide->trainCap), max}));
+        if (n > 0)
+            PG_CHECK(hipMemcpyAsync(records, d.guide->train.p, size_t(n) * 16 * sizeof(float), hipMemcpyDeviceToHost,
+                                    d.stream));
+        PG_CHECK(hipStreamSynchronize(d.stream));
+        if (produced) *produced = counts[1];
+        return n;
+    } catch (const std::exception &) {
+        return -1;
     }
 }
 

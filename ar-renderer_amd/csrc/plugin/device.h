@@ -47,12 +47,28 @@ struct DevBuf {
     }
 };
 
+// Launch and batching parameters chosen at run time: these defaults, then the tuning file that
+// the autotuner writes (read once, when the plugin first enumerates devices), then
+// prgpu_tune_set per device.
+struct Tuning {
+    int traceBlocksPerCU = 8;                // persistent traversal grid per CU (1..8: LDS-bound)
+    int shadeBlocksPerCU = 16;               // grid of the surface, emission and material kernels
+    int64_t pathBudget = int64_t(8) << 20;   // paths in flight per batch: sizes every queue
+    int guideBatch = 1 << 18;                // training records per Adam step
+    int guideSteps = 4;                      // Adam steps per batch of paths, at most
+    int guideMinSteps = 1;                   // ... and at least, cycling over the records
+    float guideLr = 1e-3f;                   // Adam learning rate
+    int guideBounces = 2;                    // bounces whose diffuse vertices use the guide (0: all)
+    float guideBlend = 0.5f;                 // weight of the MIS-proposal term in the training loss
+};
+
 struct PTScene;     // path-tracing data (pt.hip)
 struct GuideState;  // learned path-guiding model (guide_host.h)
 
 struct Device {
     int id = 0;
     int cus = 0;
+    Tuning tune;
     hipStream_t stream = nullptr;
     // Geometry
     uint64_t geomKey = 0;

@@ -5,8 +5,9 @@
 // that mixture instead of the BSDF. A guided vertex samples the one-sample MIS proposal
 //     q(w) = s * sum_k pi_k vMF(w; mu_k, kappa_k) + (1 - s) * p_bsdf(w)
 // and uses q as its scattering pdf (in the throughput and in both MIS weights), which keeps the
-// estimator unbiased whatever the network predicts. Training minimises the KL divergence of q
-// from the incident radiance (Mueller et al. 2019, "Neural Importance Sampling").
+// estimator unbiased whatever the network predicts. Training minimises a blend of the KL
+// divergences of q and of the mixture from the incident radiance (Mueller et al. 2019, "Neural
+// Importance Sampling", where the selection probability is learned through the MIS proposal).
 //
 // Network: 64 features -> 4 x 64 ReLU -> 41 outputs (row-padded to 48), BF16 weights stored
 // transposed (Wt[out][in]) with FP32 biases and FP32 accumulation. Parameters live in one FP32
@@ -24,6 +25,17 @@ constexpr int kGIn = 64, kGHidden = 64, kGOutRows = 48, kGOut = 41, kGLobes = 8,
 #define PG_GUIDE_SEL_MIN 0.1f
 #endif
 constexpr float kGSelMin = PG_GUIDE_SEL_MIN, kGSelMax = 0.9f, kGKappaMin = 0.01f, kGKappaMax = 1000.f;
+// Training loss per record: -w [kGBlend log q + (1 - kGBlend) log p_guide]. The first term fits the
+// one-sample MIS proposal q (and is the only one that moves s); the second fits the mixture itself,
+// so its lobes keep learning the incident radiance while s is small (with q alone, the lobes'
+// gradient scales with s, and a poor early mixture drives s to its floor and stalls there).
+constexpr float kGBlend = 0.5f;
+// Training records a path keeps during a training render: one per guided vertex, for its first
+// kGuideRecords guided vertices.
+#ifndef PG_GUIDE_RECORDS
+#define PG_GUIDE_RECORDS 2
+#endif
+constexpr int kGuideRecords = PG_GUIDE_RECORDS;
 
 PG_HD constexpr int guide_rows(int l) { return l == kGLayers - 1 ? kGOutRows : kGHidden; }
 PG_HD constexpr int guide_woff(int l) {

@@ -232,9 +232,9 @@ __global__ __launch_bounds__(kGBlock) void k_guide_eval(const short *wt, const f
 }
 
 // ---- training ----------------------------------------------------------------------------------
-// d(-wt log q) / d(raw outputs) for one record, as the FP64 model defines it (the weight is not
-// yet normalised by the batch's summed weight).
-__device__ void output_grad(const float *o, const float *rec, float *dout, float *lossOut) {
+// d(-wt (b log q + (1 - b) log p_guide)) / d(raw outputs) for one record, b = blend, as the FP64
+// model defines it (the weight is not yet normalised by the batch's summed weight).
+__device__ void output_grad(const float *o, const float *rec, float blend, float *dout, float *lossOut) {
     for (int i = 0; i < kGOutRows; ++i) dout[i] = 0.f;
     const float wt = rec[13], pb = rec[12];
     *lossOut = 0.f;
@@ -251,12 +251,12 @@ __device__ void output_grad(const float *o, const float *rec, float *dout, float
         pm += pi[k] * pk[k];
     }
     const float q = m.sel * pm + (1.f - m.sel) * pb;
-    if (!(q > 0.f)) return;
-    *lossOut = -wt * logf(q);
-    const float dq = -wt / q;
+    if (!(q > 0.f) || !(pm > 0.f)) return;
+    *lossOut = -wt * (blend * logf(q) + (1.f - blend) * logf(pm));
+    const float dq = -wt * blend / q;
     const float sig = (m.sel - kGSelMin) / (kGSelMax - kGSelMin);
     dout[40] = dq * (pm - pb) * (kGSelMax - kGSelMin) * sig * (1.f - sig);
-    const float dpm = dq * m.sel;
+    const float dpm = dq * m.sel - wt * (1.f - blend) / pm;
     for (int k = 0; k < kGLobes; ++k) {
         const float *ok = o + 5 * k;
         dout[5 * k + 4] = dpm * pi[k] * (pk[k] - pm);
@@ -346,7 +346,8 @@ __device__ __forceinline__ void input_grad(const GuideLds &L, int l, const f32x4
 // One pass over n records (16 floats each): per block, the summed gradient of sum_r -wt_r log q_r
 // (unnormalised), the summed weight and loss, written to part[block * (kGParams + 2) ..].
 __global__ __launch_bounds__(kGBlock) void k_guide_train(const short *wt, const float *bias, const float *recs,
-                                                         const int *count, int limit, int first, int n, float *part) {
+                                                         const int *count, int limit, int first, int n, float blend,
+                                                         float *part) {
     extern __shared__ __align__(16) unsigned char smem[];
     TrainLds &T = *reinterpret_cast<TrainLds *>(smem);
     load_model(T.m, wt, bias);
@@ -378,7 +379,7 @@ __global__ __launch_bounds__(kGBlock) void k_guide_train(const short *wt, const 
         wave_sync();
         if (lane < 16) {
             float dout[kGOutRows], loss;
-            output_grad(tl[s], rec, dout, &loss);
+            output_grad(tl[s], rec, blend, dout, &loss);
             for (int i = 0; i < kGOutRows; ++i) tl[s][i] = dout[i];
             if (valid && rec[13] > 0.f) {
                 atomicAdd(&T.wsum, rec[13]);
@@ -509,8 +510,8 @@ void GuideState::Init(hipStream_t s) {
     grad.ensure(size_t(kGParams + 2) * sizeof(float));
     stats.ensure(4 * sizeof(float));
     part.ensure(size_t(kTrainBlocks) * (kGParams + 2) * sizeof(float));
-    trainCount.ensure(sizeof(int));
-    PG_CHECK(hipMemsetAsync(trainCount.p, 0, sizeof(int), s));
+    trainCount.ensure(2 * sizeof(int));
+    PG_CHECK(hipMemsetAsync(trainCount.p, 0, 2 * sizeof(int), s));
     static bool attr = false;
     if (!attr) {
         PG_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(k_guide_train),
@@ -522,7 +523,7 @@ void GuideState::Init(hipStream_t s) {
 void GuideState::EnsureBatch(size_t n) {
     if (cap >= n) return;
     out.ensure(n * 48 * sizeof(float));
-    rec.ensure(n * 6 * sizeof(float4));
+    rec.ensure(n * 6 * size_t(kGuideRecords) * sizeof(float4));
     cap = n;
 }
 
@@ -544,7 +545,7 @@ void GuideState::Infer(const int *queue, const int *count, int maxCount, const f
 
 void GuideState::Gradient(const float *recs, const int *count, int first, int n, hipStream_t s) {
     hipLaunchKernelGGL(k_guide_train, dim3(kTrainBlocks), dim3(kGBlock), sizeof(TrainLds), s, wt.as<short>(),
-                       bias.as<float>(), recs, count, count ? int(trainCap) : n, first, n, part.as<float>());
+                       bias.as<float>(), recs, count, count ? int(trainCap) : n, first, n, blend, part.as<float>());
     PG_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_guide_reduce, dim3((kGParams + 2 + 255) / 256), dim3(256), 0, s, part.as<float>(),
                        kTrainBlocks, grad.as<float>(), stats.as<float>());

@@ -19,9 +19,11 @@ struct GuideState {
     DevBuf wt, bias;       // BF16 weights (Wt[out][in] per layer) and FP32 biases
     DevBuf grad, stats, part;
     DevBuf out, rec;       // per batch: raw outputs per guided vertex, training record per path
-    DevBuf train, trainCount;
+    DevBuf train, trainCount;  // compacted training records; [0] written, [1] produced
     size_t cap = 0, trainCap = 0;
     int steps = 0;
+    double recordsPerPath = 0;  // training records the last training batch produced per path
+    float blend = kGBlend;      // weight of the MIS-proposal term in the training loss
 
     void Init(hipStream_t s);
     void SetParams(const float *P, hipStream_t s);  // also resets the Adam state

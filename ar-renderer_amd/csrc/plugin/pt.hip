@@ -633,6 +633,7 @@ struct Params {
     uint64_t seed;
     int firstSample;
     int guide, train;  // learned path guiding: sample with the model / collect training records
+    float trainKeep;   // probability that a training record is kept (a uniform subsample over the image)
     GuideBox box;      // the box vertex positions are normalised to for the model
 };
 
@@ -653,9 +654,10 @@ struct Bufs {
     float4 *xyz;
     // shadow records
     float4 *so, *sd, *sp, *sn, *sf, *sL, *sb;
-    // guiding: decoded mixture per diffuse-queue entry (12 float4), training record per path
-    // (6 float4 planes of qcap: {position, state}, {normal, p_bsdf}, wo, w, throughput, radiance
-    // snapshot), compacted training records (16 floats) and their counter
+    // guiding: decoded mixture per diffuse-queue entry (12 float4), kGuideRecords training
+    // records per path (record j: 6 float4 planes of qcap at 6 j qcap: {position, state},
+    // {normal, p_bsdf}, wo, w, throughput, radiance snapshot), compacted training records (16
+    // floats) and their counter
     int4 *qe;       // emission queue: slot, depth and flags on arrival, primitive (-1: escaped)
     float4 *qeLpdf; // and the wavelength pdfs on arrival
     const float4 *gout;
@@ -747,7 +749,8 @@ __global__ __launch_bounds__(kBlock) void k_camera(const DevScene *__restrict__ 
     b.st[slot] = make_int4(0, 0, s.cameraMedium, 0);
     b.pfilm[slot] = make_float2(fx, fy);
     b.q0[slot] = slot;
-    if (p.train) b.trec[slot] = make_float4(0, 0, 0, 0);  // no guided vertex yet
+    if (p.train)  // no guided vertex yet
+        for (int j = 0; j < kGuideRecords; ++j) b.trec[6 * j * size_t(b.qcap) + slot] = make_float4(0, 0, 0, 0);
 }
 
 // Closest hits of the queued paths. Persistent: each lane takes a new ray from its XCD's part of
@@ -917,14 +920,16 @@ __global__ __launch_bounds__(kBlock) PG_WAVES(kSurfaceWaves) void k_surface(cons
         const bool lam0 = r.lam.terminated();
         // Training: the radiance gathered before this vertex closes the record of the guided
         // vertex the path just left (its own next-event estimate included).
-        if (p.train) {
-            float4 t0 = b.trec[slot];
-            if (t0.w == 1.f) {
-                b.trec[5 * size_t(b.qcap) + slot] = b.Lh[slot];
-                t0.w = 2.f;
-                b.trec[slot] = t0;
+        if (p.train)
+            for (int j = 0; j < kGuideRecords; ++j) {
+                const size_t base = 6 * j * size_t(b.qcap);
+                float4 t0 = b.trec[base + slot];
+                if (t0.w == 1.f) {
+                    b.trec[base + 5 * size_t(b.qcap) + slot] = b.Lh[slot];
+                    t0.w = 2.f;
+                    b.trec[base + slot] = t0;
+                }
             }
-        }
         const float4 hv = b.hit[slot];
         const int prim = __float_as_int(hv.y);
         const bool hit = prim >= 0;
@@ -1170,14 +1175,19 @@ __global__ __launch_bounds__(kBlock) PG_WAVES(kType == QLayered ? 0 : kMaterialW
                 r.ro = offset_origin(si.p, si.n, wi);
                 r.rd = wi;
                 if (kMedia) r.medium = dot(wi, si.n) > 0 ? mOut : mIn;
-                if (p.train && b.trec[slot].w == 0.f) {  // first guided vertex of the path
+                if (p.train) {  // the path's first free record slot takes this guided vertex
                     const size_t c = size_t(b.qcap);
-                    b.trec[slot] = make_float4((si.p.x - p.box.lo[0]) * p.box.inv[0], (si.p.y - p.box.lo[1]) * p.box.inv[1],
-                                               (si.p.z - p.box.lo[2]) * p.box.inv[2], 1.f);
-                    b.trec[c + slot] = make_float4(si.n.x, si.n.y, si.n.z, pb);
-                    b.trec[2 * c + slot] = make_float4(si.wo.x, si.wo.y, si.wo.z, 0.f);
-                    b.trec[3 * c + slot] = make_float4(wi.x, wi.y, wi.z, 0.f);
-                    st4(b.trec, 4 * c + slot, beta0);
+                    int j = 0;
+                    while (j < kGuideRecords && b.trec[6 * j * c + slot].w != 0.f) ++j;
+                    if (j < kGuideRecords) {
+                        float4 *t = b.trec + 6 * j * c;
+                        t[slot] = make_float4((si.p.x - p.box.lo[0]) * p.box.inv[0], (si.p.y - p.box.lo[1]) * p.box.inv[1],
+                                              (si.p.z - p.box.lo[2]) * p.box.inv[2], 1.f);
+                        t[c + slot] = make_float4(si.n.x, si.n.y, si.n.z, pb);
+                        t[2 * c + slot] = make_float4(si.wo.x, si.wo.y, si.wo.z, 0.f);
+                        t[3 * c + slot] = make_float4(wi.x, wi.y, wi.z, 0.f);
+                        st4(t, 4 * c + slot, beta0);
+                    }
                 }
                 if (!r.beta.nonzero() || r.beta.has_nan()) alive = false;
                 else if (roulette(p.rr, p.rrStart, r)) alive = false;
@@ -1334,32 +1344,42 @@ __global__ __launch_bounds__(kBlock) void k_finish(const DevScene *__restrict__ 
     Lambda lam = {{l.x, l.y, l.z, l.w}, {lp.x, lp.y, lp.z, lp.w}};
     SS L = lam.terminated() ? ld4(b.Lh, slot) : ld4(b.L, slot);
     L = L * 1.f;  // camera ray weight
-    if (p.train && b.trec[slot].w == 2.f) {
-        // Training weight of the path's guided vertex: the radiance gathered beyond it divided by
-        // the throughput that reached it, f L_i / q in the KL gradient (averaged over the hero
-        // wavelengths).
-        const size_t c = size_t(b.qcap);
-        const SS Lh = ld4(b.Lh, slot), snap = ld4(b.trec, 5 * c + slot), beta0 = ld4(b.trec, 4 * c + slot);
-        float w = 0.f;
-        int k = 0;
-        for (int j = 0; j < 4; ++j)
-            if (beta0.v[j] > 0.f && lam.pdf[j] > 0.f) {
-                w += (Lh.v[j] - snap.v[j]) / beta0.v[j];
-                ++k;
-            }
-        w = k ? w / float(k) : 0.f;
-        if (w > 0.f && is_finite(w)) {
-            const int at = wave_append(b.trainCount);
-            if (at < b.trainCap) {
-                float4 a = b.trec[slot], n1 = b.trec[c + slot], wo = b.trec[2 * c + slot], wi = b.trec[3 * c + slot];
-                float4 *dst = reinterpret_cast<float4 *>(b.train + size_t(at) * 16);
-                dst[0] = make_float4(a.x, a.y, a.z, n1.x);
-                dst[1] = make_float4(n1.y, n1.z, wo.x, wo.y);
-                dst[2] = make_float4(wo.z, wi.x, wi.y, wi.z);
-                dst[3] = make_float4(n1.w, w, 0.f, 0.f);
+    if (p.train)
+        for (int r = 0; r < kGuideRecords; ++r) {
+            // Training weight of a guided vertex of the path: the radiance gathered beyond it divided
+            // by the throughput that reached it, f L_i / q in the KL gradient (averaged over the hero
+            // wavelengths).
+            const size_t c = size_t(b.qcap);
+            const float4 *t = b.trec + 6 * r * c;
+            if (t[slot].w != 2.f) continue;
+            const SS Lh = ld4(b.Lh, slot), snap = ld4(t, 5 * c + slot), beta0 = ld4(t, 4 * c + slot);
+            float w = 0.f;
+            int k = 0;
+            for (int j = 0; j < 4; ++j)
+                if (beta0.v[j] > 0.f && lam.pdf[j] > 0.f) {
+                    w += (Lh.v[j] - snap.v[j]) / beta0.v[j];
+                    ++k;
+                }
+            w = k ? w / float(k) : 0.f;
+            if (w > 0.f && is_finite(w)) {
+                // Every record is counted (trainCount[1]); a uniform subsample of probability
+                // trainKeep, decided by a hash of the sample, is written, so that the records a
+                // batch trains on cover the whole image rather than its first rows.
+                wave_append(b.trainCount + 1);
+                const uint64_t h = mix_bits((uint64_t(uint32_t(p.firstSample)) << 36) ^ (uint64_t(slot) << 2) ^ uint64_t(r));
+                const float u = float(uint32_t(h >> 40)) * 0x1p-24f;
+                if (!(u < p.trainKeep)) continue;
+                const int at = wave_append(b.trainCount);
+                if (at < b.trainCap) {
+                    float4 a = t[slot], n1 = t[c + slot], wo = t[2 * c + slot], wi = t[3 * c + slot];
+                    float4 *dst = reinterpret_cast<float4 *>(b.train + size_t(at) * 16);
+                    dst[0] = make_float4(a.x, a.y, a.z, n1.x);
+                    dst[1] = make_float4(n1.y, n1.z, wo.x, wo.y);
+                    dst[2] = make_float4(wo.z, wi.x, wi.y, wi.z);
+                    dst[3] = make_float4(n1.w, w, 0.f, 0.f);
+                }
             }
         }
-    }
     float2 pf = b.pfilm[slot];
     if (rec) {
         prgpu::SampleRecordEx &o = rec[slot];
@@ -1521,6 +1541,7 @@ Params MakeParams(const prgpu::RenderParamsEx &rp) {
     p.firstSample = rp.firstSample;
     p.guide = (rp.flags & prgpu::RenderGuide) ? 1 : 0;
     p.train = p.guide && (rp.flags & prgpu::RenderGuideTrain) ? 1 : 0;
+    p.trainKeep = 1.f;
     p.box = GuideBox{};
     return p;
 }
@@ -1575,10 +1596,7 @@ struct Timer {
 
 // Traces one batch of `passes` passes (samples firstSample .. firstSample + passes - 1) to
 // completion; the per-sample results are left in the xyz / record buffers.
-#ifndef PG_GUIDE_BOUNCES
-#define PG_GUIDE_BOUNCES 2
-#endif
-constexpr int kGuideBounces = PG_GUIDE_BOUNCES;
+
 
 // Bounce rounds per captured graph (even: a replay ends on the queue it started from); 0 = no
 // graphs. Off by default: on configuration 2 at 1080p, replaying 4 captured rounds took 27.9 ms per
@@ -1629,7 +1647,7 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
     int active = n;
     pt.countsHost[0] = n;
     PG_CHECK(hipMemcpyAsync(b.counts, pt.countsHost, sizeof(int), hipMemcpyHostToDevice, st));
-    const int shadeGrid = std::max(1, std::min((active + kBlock - 1) / kBlock, dev.cus * 16));
+    const int shadeGrid = std::max(1, std::min((active + kBlock - 1) / kBlock, dev.cus * dev.tune.shadeBlocksPerCU));
     // One bounce: closest hits of the current queue, the surface and emission stages, the
     // material families (which fill the next queue), the shadow rays. Every kernel reads its
     // queue length from device memory, so rounds can be queued without reading them back.
@@ -1645,8 +1663,8 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
                 hipLaunchKernelGGL(k_intersect<false>, dim3(tgrid), dim3(kTraceBlock), 0, st, ds, b, cur, b.counts + curIdx,
                                    b.counts + kFetchIntersect, ctr);
         });
-        // Guiding applies to the vertices of the first kGuideBounces bounces (0: all).
-        const bool guided = p.guide && (kGuideBounces == 0 || bounce < kGuideBounces);
+        // Guiding applies to the vertices of the first guideBounces bounces (0: all).
+        const bool guided = p.guide && (dev.tune.guideBounces == 0 || bounce < dev.tune.guideBounces);
         timer.Run(prgpu::KernSurface, active, nullptr, [&] {
             auto *k = media ? (interfaces ? k_surface<true, true> : k_surface<true, false>)
                             : (interfaces ? k_surface<false, true> : k_surface<false, false>);
@@ -1757,28 +1775,12 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
     CheckTraceFlags(dev);
 }
 
-// Guided rendering trains the model after every batch: up to kGuideSteps Adam steps of
-// kGuideBatch records each (the records of the batch's paths, in the order they were written).
-#ifndef PG_GUIDE_BATCH
-#define PG_GUIDE_BATCH (1 << 16)
-#endif
-#ifndef PG_GUIDE_STEPS
-#define PG_GUIDE_STEPS 8
-#endif
-#ifndef PG_GUIDE_LR
-#define PG_GUIDE_LR 1e-2f
-#endif
-#ifndef PG_GUIDE_MIN_STEPS
-#define PG_GUIDE_MIN_STEPS 1
-#endif
-constexpr int kGuideBatch = PG_GUIDE_BATCH, kGuideSteps = PG_GUIDE_STEPS, kGuideTrainCap = kGuideBatch * kGuideSteps;
-constexpr int kGuideMinSteps = PG_GUIDE_MIN_STEPS;
-constexpr float kGuideLearningRate = PG_GUIDE_LR;
-
-// Passes per batch: bounded by a path budget so that the state fits comfortably in HBM.
-int PassesPerBatch(int npix, int spp) {
-    const int64_t budget = int64_t(8) << 20;  // paths in flight
-    return int(std::max<int64_t>(1, std::min<int64_t>(spp, budget / std::max(1, npix))));
+// Guided rendering trains the model after every batch: up to guideSteps Adam steps of guideBatch
+// records each (the records of the batch's paths, in the order they were written).
+// Passes per batch: bounded by the path budget (the queue capacity), so that the state fits
+// comfortably in HBM.
+int PassesPerBatch(const Device &dev, int npix, int spp) {
+    return int(std::max<int64_t>(1, std::min<int64_t>(spp, dev.tune.pathBudget / std::max(1, npix))));
 }
 
 }  // namespace
@@ -1790,7 +1792,7 @@ void RenderExImpl(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::Render
     DevScene s = MakeDevScene(dev, sd, rp);
     const DevScene *ds = UploadDevScene(dev, s);
     const int sw = s.width + 2 * s.margin, sh = s.height + 2 * s.margin, npix = sw * sh;
-    const int per = PassesPerBatch(npix, rp.spp);
+    const int per = PassesPerBatch(dev, npix, rp.spp);
     PTScene &pt = *dev.pt;
     EnsureBatch(pt, size_t(per) * size_t(npix));
     const size_t filmN = size_t(s.width) * s.height * 3;
@@ -1816,7 +1818,8 @@ void RenderExImpl(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::Render
             dev.guide->Init(dev.stream);
         }
         dev.guide->EnsureBatch(pt.cap);  // per-path planes are indexed with the batch capacity
-        if (p.train) dev.guide->EnsureTrain(size_t(kGuideTrainCap));
+        dev.guide->blend = dev.tune.guideBlend;
+        if (p.train) dev.guide->EnsureTrain(size_t(dev.tune.guideBatch) * size_t(dev.tune.guideSteps));
         for (int a = 0; a < 3; ++a) {
             p.box.lo[a] = dev.sceneLo[a];
             const float ext = dev.sceneHi[a] - dev.sceneLo[a];
@@ -1826,7 +1829,16 @@ void RenderExImpl(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::Render
     for (int done = 0; done < rp.spp; done += per) {
         int passes = std::min(per, rp.spp - done);
         p.firstSample = rp.firstSample + done;
-        if (p.train) PG_CHECK(hipMemsetAsync(dev.guide->trainCount.p, 0, sizeof(int), dev.stream));
+        if (p.train) {
+            PG_CHECK(hipMemsetAsync(dev.guide->trainCount.p, 0, 2 * sizeof(int), dev.stream));
+            // Keep about as many records as the training steps use: from the records the last
+            // batch produced per path, or, for the first, an estimate of one per path.
+            const double cap = double(std::min<size_t>(dev.guide->trainCap,
+                                                       size_t(dev.tune.guideBatch) * size_t(dev.tune.guideSteps)));
+            const double expected = dev.guide->recordsPerPath > 0 ? dev.guide->recordsPerPath * double(passes) * npix
+                                                                  : double(passes) * npix;
+            p.trainKeep = float(std::min(1.0, cap / std::max(1.0, expected)));
+        }
         TraceBatch(dev, s, ds, p, passes, npix, sw);
         Bufs b = MakeBufs(pt);
         if (p.guide) AttachGuide(dev, b);
@@ -1843,18 +1855,22 @@ void RenderExImpl(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::Render
         });
         PG_CHECK(hipGetLastError());
         if (p.train) {
-            // One Adam step per slice of kGuideBatch records of the batch (at most kGuideSteps), and at
-            // least kGuideMinSteps steps, cycling over the slices, when the batch has records.
-            int records = 0;
-            PG_CHECK(hipMemcpyAsync(&records, dev.guide->trainCount.p, sizeof(int), hipMemcpyDeviceToHost, dev.stream));
+            // One Adam step per slice of guideBatch records of the batch (at most guideSteps), and at
+            // least guideMinSteps steps, cycling over the slices, when the batch has records.
+            int counts[2] = {0, 0};  // records written, records produced
+            PG_CHECK(hipMemcpyAsync(counts, dev.guide->trainCount.p, 2 * sizeof(int), hipMemcpyDeviceToHost, dev.stream));
             PG_CHECK(hipStreamSynchronize(dev.stream));
-            const int slices = std::min(kGuideSteps, (std::min(records, kGuideTrainCap) + kGuideBatch - 1) / kGuideBatch);
-            const int steps = slices > 0 ? std::max(slices, kGuideMinSteps) : 0;
+            const int records = counts[0];
+            dev.guide->recordsPerPath = double(counts[1]) / double(std::max(1, passes * npix));
+            const Tuning &t = dev.tune;
+            const int cap = int(std::min<size_t>(dev.guide->trainCap, size_t(t.guideBatch) * size_t(t.guideSteps)));
+            const int slices = std::min(t.guideSteps, (std::min(records, cap) + t.guideBatch - 1) / t.guideBatch);
+            const int steps = slices > 0 ? std::max(slices, t.guideMinSteps) : 0;
             if (steps > 0)
                 timer.Run(prgpu::KernGuide, steps, nullptr, [&] {
                     for (int k = 0; k < steps; ++k)
                         dev.guide->Step(dev.guide->train.as<float>(), dev.guide->trainCount.as<int>(),
-                                        (k % slices) * kGuideBatch, kGuideBatch, kGuideLearningRate, dev.stream);
+                                        (k % slices) * t.guideBatch, t.guideBatch, t.guideLr, dev.stream);
                 });
         }
         if (dev.profileMode) {

@@ -547,6 +547,29 @@ class Plugin:
                                               hits.ctypes.data if len(hits) else None, err, 1024), err)
         return hits
 
+    TUNE_KEYS = ("trace_blocks_per_cu", "shade_blocks_per_cu", "path_budget", "guide_batch", "guide_steps",
+                 "guide_min_steps", "guide_lr", "guide_bounces", "guide_blend")
+
+    def tune_set(self, key, value, device=0):
+        self.lib.prgpu_tune_set.argtypes = [C.c_int, C.c_char_p, C.c_double]
+        if self.lib.prgpu_tune_set(device, key.encode(), float(value)) != 0:
+            raise ValueError(f"tuning key {key!r}: unknown or value {value!r} out of range")
+
+    def tune_get(self, key, device=0):
+        self.lib.prgpu_tune_get.argtypes = [C.c_int, C.c_char_p, C.POINTER(C.c_double)]
+        v = C.c_double()
+        if self.lib.prgpu_tune_get(device, key.encode(), C.byref(v)) != 0:
+            raise ValueError(f"unknown tuning key {key!r}")
+        return v.value
+
+    def tuning(self, device=0):
+        """The device's run-time tuning, and the tuning file it started from (None if none)."""
+        self.lib.prgpu_tune_source.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_int]
+        path, err = C.create_string_buffer(1024), C.create_string_buffer(1024)
+        found = self.lib.prgpu_tune_source(path, 1024, err, 1024) == 0
+        return {"values": {k: self.tune_get(k, device) for k in self.TUNE_KEYS},
+                "file": path.value.decode() if found else None, "file_error": err.value.decode() or None}
+
     def trace_noalloc(self, scene, rays_dev, n, hits_dev, any_hit=False, device=0):
         """prgpu_trace_rays_noalloc: rays_dev / hits_dev are device addresses (ints) of this GPU's
         memory holding n RayEx / HitEx records; the scene's geometry must already be resident."""
@@ -614,6 +637,17 @@ class Plugin:
         if self.lib.prgpu_guide_train(device, R.ctypes.data, len(R), 0, 0.0, g.ctypes.data, st.ctypes.data) != 0:
             raise RuntimeError("prgpu_guide_train failed")
         return g, st
+
+    def guide_records(self, max_records, device=0):
+        """The training records the last training batch kept (n x 16 float32), and how many it produced."""
+        f = self.lib.prgpu_guide_records
+        f.argtypes = [C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+        out = np.zeros((max_records, 16), np.float32)
+        produced = C.c_int()
+        n = f(device, out.ctypes.data, max_records, C.byref(produced))
+        if n < 0:
+            raise RuntimeError("prgpu_guide_records failed")
+        return out[:n], produced.value
 
     def guide_stats(self, device=0):
         """(summed weight, summed weighted loss) of the model's last step, and its step count."""

@@ -374,6 +374,19 @@ PRGPU_API int prgpu_film_read(int deviceIndex, double *sumXYZ, double *sumSqXYZ)
 // totals since the last reset.
 PRGPU_API int prgpu_profile(int deviceIndex, int mode);
 PRGPU_API int prgpu_profile_read(int deviceIndex, prgpu::ProfileStatsEx *out, int reset);
+// Run-time tuning (no reference counterpart): launch and batching parameters, set per device.
+// Keys: trace_blocks_per_cu (persistent traversal blocks per CU, 1..8), shade_blocks_per_cu
+// (blocks per CU of the surface / material kernels), path_budget (paths in flight per batch: the
+// queue capacity), guide_batch (records per Adam step), guide_steps and guide_min_steps (Adam
+// steps per batch of paths), guide_lr, guide_bounces (bounces that sample the guide, 0 = all),
+// guide_blend (weight of the MIS-proposal term in the training loss, the rest fits the mixture).
+// Defaults come from the tuning file the autotuner writes ($PRGPU_TUNING_FILE, else
+// prgpu_tuning.txt beside the library: "key value" lines), read when devices are first
+// enumerated. set / get return 0, or 1 for an unknown key, a value out of range or no device.
+PRGPU_API int prgpu_tune_set(int deviceIndex, const char *key, double value);
+PRGPU_API int prgpu_tune_get(int deviceIndex, const char *key, double *value);
+// The tuning file read (0) or none (1), and the first invalid line found in it (empty if none).
+PRGPU_API int prgpu_tune_source(char *path, int pathLen, char *error, int errorLen);
 // Frees every device buffer the plugin holds for the device.
 PRGPU_API int prgpu_release(int deviceIndex);
 
@@ -392,6 +405,11 @@ PRGPU_API int prgpu_guide_eval(int deviceIndex, const float *records, int n, flo
 // lr) on the records and returns the stats of the last step.
 PRGPU_API int prgpu_guide_train(int deviceIndex, const float *records, int n, int steps, float lr, float *grad,
                                 float *stats);
+// The training records (16 floats each, as prgpu_guide_train takes them) that the last batch of
+// a training render kept: copies at most max into records, returns the count copied (-1 when the
+// device has no model) and sets *produced (may be null) to the records the batch produced before
+// the uniform subsample kept about guide_batch x guide_steps of them.
+PRGPU_API int prgpu_guide_records(int deviceIndex, float *records, int max, int *produced);
 // State of the device's model: stats = {summed weight, summed weighted loss} of its last training
 // step, *steps = Adam steps taken since it was created or reset. 1 when no model exists.
 PRGPU_API int prgpu_guide_stats(int deviceIndex, float *stats, int *steps);

@@ -11,9 +11,10 @@
 //   outputs:       8 von Mises-Fisher lobes (direction 3, concentration 1, weight logit 1)
 //                  and the probability s of sampling the guide instead of the BSDF
 //   proposal:      q(w) = s * sum_k pi_k vMF(w; mu_k, kappa_k) + (1 - s) * p_bsdf(w)
-//   loss:          -weight * log q(w) per training record, normalised by the summed weights
-//                  (the KL divergence of the proposal from the incident radiance; Mueller et al.
-//                  2019, "Neural Importance Sampling", one-sample MIS with learned selection)
+//   loss:          -weight * (b log q(w) + (1 - b) log p_guide(w)) per training record, b = 0.5,
+//                  normalised by the summed weights: a blend of the KL divergences of the
+//                  proposal and of the mixture from the incident radiance (Mueller et al. 2019,
+//                  "Neural Importance Sampling", one-sample MIS with learned selection)
 //
 // Parameters are one float vector: for each layer l = 0..4 the weight matrix stored as
 // Wt[out][in] (rows: 64, 64, 64, 64, 48 (41 used); 64 columns), then its bias (64 or 48).
@@ -27,6 +28,7 @@ namespace {
 constexpr int kIn = 64, kHidden = 64, kOutRows = 48, kOut = 41, kLobes = 8, kLayers = 5;
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kSelMin = 0.1, kSelMax = 0.9, kKappaMin = 0.01, kKappaMax = 1000.0;
+double gBlend = 0.5;  // weight of the MIS-proposal term in the loss (or_guide_set_blend)
 
 int Rows(int l) { return l == kLayers - 1 ? kOutRows : kHidden; }
 size_t WOff(int l) {
@@ -160,9 +162,12 @@ void or_guide_mix_sample(const float *o41, float ul, float u1, float u2, float *
     for (int i = 0; i < 3; ++i) w[i] = float(sinT * std::cos(phi) * t[i] + sinT * std::sin(phi) * s[i] + cosT * mu[i]);
 }
 
-// Loss sum_r -wt_r log q_r / sum_r wt_r and its gradient with respect to the parameters, for
+// Loss sum_r -wt_r (b log q_r + (1 - b) log p_guide,r) / sum_r wt_r and its gradient with respect
+// to the parameters, for
 // records of 16 floats: position (3, normalised), normal (3), wo (3), sampled direction (3),
 // p_bsdf of that direction (1), weight (1), 2 unused. Returns the loss.
+void or_guide_set_blend(double b) { gBlend = b; }
+
 double or_guide_loss_grad(const float *P, const float *recs, int n, float *grad) {
     const size_t np = ParamCount();
     std::vector<double> g(np, 0.0);
@@ -189,13 +194,14 @@ double or_guide_loss_grad(const float *P, const float *recs, int n, float *grad)
             pm += m.pi[k] * pk[k];
         }
         const double q = m.sel * pm + (1 - m.sel) * pb;
-        loss += -wt * std::log(q);
+        if (!(q > 0) || !(pm > 0)) continue;
+        loss += -wt * (gBlend * std::log(q) + (1 - gBlend) * std::log(pm));
         // d loss / d outputs
         double dout[kOutRows] = {0};
-        const double dq = -wt / q;
+        const double dq = -wt * gBlend / q;
         const double sig = (m.sel - kSelMin) / (kSelMax - kSelMin);
         dout[40] = dq * (pm - pb) * (kSelMax - kSelMin) * sig * (1 - sig);
-        const double dpm = dq * m.sel;
+        const double dpm = dq * m.sel - wt * (1 - gBlend) / pm;
         for (int k = 0; k < kLobes; ++k) {
             const double *o = fw.out + 5 * k;
             dout[5 * k + 4] = dpm * m.pi[k] * (pk[k] - pm);  // softmax

@@ -146,6 +146,12 @@ def guide_mix_sample(out41, ul, u1, u2):
     return w
 
 
+def guide_set_blend(b):
+    """Weight of the MIS-proposal term in the training loss (the rest is the mixture's own term)."""
+    lib().or_guide_set_blend.argtypes = [C.c_double]
+    lib().or_guide_set_blend(float(b))
+
+
 def guide_loss_grad(params, recs16):
     P = np.ascontiguousarray(params, dtype=np.float32)
     R = np.ascontiguousarray(recs16, dtype=np.float32).reshape(-1, 16)
