@@ -63,7 +63,7 @@ const TuneKey kTuneKeys[] = {
     {"path_budget", 1 << 16, 1 << 26, true},           {"guide_batch", 256, 1 << 20, true},
     {"guide_steps", 1, 64, true},                      {"guide_min_steps", 0, 64, true},
     {"guide_lr", 1e-6, 1.0, false},                    {"guide_bounces", 0, 64, true},
-    {"guide_blend", 0.0, 1.0, false},
+    {"guide_blend", 0.0, 1.0, false},                  {"graph_rounds", 0, 16, true},
 };
 
 bool TuneSet(Tuning &t, const std::string &key, double v) {
@@ -79,6 +79,10 @@ bool TuneSet(Tuning &t, const std::string &key, double v) {
         else if (key == "guide_lr") t.guideLr = float(v);
         else if (key == "guide_bounces") t.guideBounces = int(v);
         else if (key == "guide_blend") t.guideBlend = float(v);
+        else if (key == "graph_rounds") {
+            if (int(v) % 2) return false;  // a replay must end on the queue it started from
+            t.graphRounds = int(v);
+        }
         return true;
     }
     return false;
@@ -94,6 +98,7 @@ bool TuneGet(const Tuning &t, const std::string &key, double *v) {
     else if (key == "guide_lr") *v = t.guideLr;
     else if (key == "guide_bounces") *v = t.guideBounces;
     else if (key == "guide_blend") *v = t.guideBlend;
+    else if (key == "graph_rounds") *v = t.graphRounds;
     else return false;
     return true;
 }

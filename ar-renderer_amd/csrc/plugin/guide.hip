@@ -2,16 +2,15 @@
 // bounce, and a fused BF16 forward / backward training kernel with FP32 accumulation, followed
 // by an Adam step. The model and the mixture are described in guide.h.
 //
-// Matrix layout. Every layer is computed transposed, Y^T = W^T X^T, with 16x16x16 BF16 MFMAs:
-// the A operand is a 16x16 block of Wt[out][in] (lane l holds row l % 16, columns
-// 4 (l / 16) .. +3: one 8-byte LDS read), the B operand a 16-feature x 16-sample block of the
-// layer input (lane l holds sample l % 16, features 4 (l / 16) .. +3), and the accumulator holds
-// 16 outputs x 16 samples with lane l owning sample l % 16, outputs 4 (l / 16) .. +3: exactly
-// the B-operand layout of the next layer, so activations stay in registers from layer to layer.
-// A wave evaluates 16 vertices at a time; the 19 456 BF16 weights and 304 biases sit in LDS.
-// LDS rows of weights (and of the training kernel's staging tiles) are padded by 8 bytes: the
-// 16 lanes of an A-operand read take 16 consecutive rows, which at a 128-byte stride would all
-// fall in one LDS bank (a 16-way conflict); at 136 bytes they cover the 32 banks once.
+// Matrix layout. Every layer is computed transposed, Y^T = W^T X^T, with gfx950's 16x16x32 BF16
+// MFMAs: the A operand is a 16-row block of Wt[out][in] (lane l holds row l % 16 and 8 of the 32
+// K elements), the B operand a 32-feature x 16-vertex block of the layer input (lane l holds vertex
+// l % 16 and 8 features), and the accumulator holds 16 outputs x 16 vertices with lane l owning
+// vertex l % 16, outputs 4 (l / 16) .. +3. With the K order chosen below, the accumulators of two
+// output blocks are, lane for lane, the B operand of the next layer: activations stay in registers
+// from layer to layer; weights sit in LDS. A wave evaluates 16 vertices at a time.
+// LDS rows of weights and staging tiles are padded by 16 bytes, so a row starts 16-byte aligned and
+// the 16 rows an A-operand read takes do not all fall in the same LDS banks.
 #include "guide_host.h"
 
 #include <cmath>
@@ -21,11 +20,10 @@ namespace pgpu {
 
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kGBlock = 256;
-constexpr int kInferBlock = 256;       // inference: 4 waves share one LDS copy of the model (3 blocks per CU)
+constexpr int kInferBlock = 512;       // inference: 8 waves share one LDS copy of the model (2 blocks per CU)
 constexpr int kGTileRow = kGOutRows + 1;  // floats per row of the raw-output tiles (padded)
 constexpr int kGBiases = 4 * kGHidden + kGOutRows;  // 304
 constexpr int kGRec = 16;                            // floats per training record
@@ -47,80 +45,133 @@ __device__ __forceinline__ short f2bf(float f) {  // round to nearest even
 }
 __device__ __forceinline__ float bf2f(short h) { return __uint_as_float(uint32_t(uint16_t(h)) << 16); }
 
-__device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
-}
-
-constexpr int kGLdsStride = kGIn + 4;                 // bf16 per LDS weight row
-constexpr int kGRows = kGWeights / kGIn;               // 304 weight rows, all layers
-__host__ __device__ constexpr int lds_row(int l, int r) { return (l * kGHidden + r) * kGLdsStride; }
-
-struct GuideLds {
-    short w[kGRows * kGLdsStride];
-    float b[kGBiases];
-};
-
-__device__ __forceinline__ void load_model(GuideLds &L, const short *wt, const float *bias) {
-    for (int i = threadIdx.x; i < kGWeights / 4; i += blockDim.x)
-        *reinterpret_cast<uint2 *>(&L.w[(i >> 4) * kGLdsStride + 4 * (i & 15)]) = reinterpret_cast<const uint2 *>(wt)[i];
-    for (int i = threadIdx.x; i < kGBiases; i += blockDim.x) L.b[i] = bias[i];
-}
-
-__device__ __forceinline__ bf16x4 wblock(const GuideLds &L, int l, int j, int kb, int lane) {
-    return *reinterpret_cast<const bf16x4 *>(&L.w[lds_row(l, j * 16 + (lane & 15)) + kb * 16 + 4 * (lane >> 4)]);
-}
-
-// Features of this lane's vertex (sample lane % 16) in B layout: lane group q = lane / 16 holds
-// features 16 kb + 4 q .. +3, i.e. frequencies 2q and 2q + 1 (sin, cos) of axis kb for kb < 3,
-// and for kb = 3 the normal and outgoing direction (q = 0, 1) or zeros.
-__device__ __forceinline__ void features(const float *pos, const float *nrm, const float *wo, int lane, bf16x4 x[4]) {
-    const int q = lane >> 4;
-    const float f0 = float(1 << (2 * q)), f1 = 2.f * f0;
-#pragma unroll
-    for (int kb = 0; kb < 3; ++kb) {
-        float s0, c0, s1, c1;
-        sincospif(pos[kb] * f0, &s0, &c0);
-        sincospif(pos[kb] * f1, &s1, &c1);
-        x[kb][0] = f2bf(s0);
-        x[kb][1] = f2bf(c0);
-        x[kb][2] = f2bf(s1);
-        x[kb][3] = f2bf(c1);
-    }
-    const bool g0 = q == 0, g1 = q == 1;
-    x[3][0] = f2bf(g0 ? nrm[0] : (g1 ? wo[1] : 0.f));
-    x[3][1] = f2bf(g0 ? nrm[1] : (g1 ? wo[2] : 0.f));
-    x[3][2] = f2bf(g0 ? nrm[2] : 0.f);
-    x[3][3] = f2bf(g0 ? wo[0] : 0.f);
-}
-
-// One hidden layer: x_out = relu(W x_in + b), B layout in and out.
-__device__ __forceinline__ void hidden(const GuideLds &L, int l, const bf16x4 xin[4], bf16x4 xout[4], int lane) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kb = 0; kb < 4; ++kb) acc = mfma(wblock(L, l, j, kb, lane), xin[kb], acc);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) xout[j][i] = f2bf(fmaxf(acc[i] + L.b[bl_off(l) + j * 16 + 4 * (lane >> 4) + i], 0.f));
-    }
-}
-
-// The output layer: 48 raw outputs (rows 41..47 zero), accumulator layout.
-__device__ __forceinline__ void output(const GuideLds &L, const bf16x4 xin[4], f32x4 out[3], int lane) {
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kb = 0; kb < 4; ++kb) acc = mfma(wblock(L, 4, j, kb, lane), xin[kb], acc);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) acc[i] += L.b[bl_off(4) + j * 16 + 4 * (lane >> 4) + i];
-        out[j] = acc;
-    }
-}
+constexpr int kGRows = kGWeights / kGIn;  // 304 weight rows, all layers
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
+}
+
+// ---- forward pass on the gfx950 16x16x32 BF16 matrix cores ---------------------------------------
+// The K dimension of one 16x16x32 MFMA is 32 wide; lane l supplies 8 of its elements (k = 8 (l / 16)
+// .. +7). The accumulator of a layer holds, in lane l, rows 4 (l / 16) .. +3 of each 16-row output
+// block j. Taking the K order of block m as {32 m + 4 q .. +3, 32 m + 16 + 4 q .. +3} for lane group
+// q = l / 16 (any K order works when A and B agree), the accumulators of output blocks 2 m and
+// 2 m + 1 are, lane for lane, the B operand of K-block m of the next layer. The weights are stored
+// in LDS with every row's 64 columns in that order, so one 16-byte LDS read fetches a lane's A
+// operand.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kG32Stride = kGIn + 8;  // bf16 per LDS weight row (16-byte padding: rows start 16-byte aligned)
+
+struct GuideLds32 {
+    alignas(16) short w[kGRows * kG32Stride];
+    float b[kGBiases];
+};
+
+__host__ __device__ constexpr int row32(int l, int r) { return (l * kGHidden + r) * kG32Stride; }
+// Stored column of input column c in the K order above.
+__host__ __device__ constexpr int perm32(int c) {
+    return (c & ~31) + 8 * ((c & 15) >> 2) + (c & 3) + ((c & 16) ? 4 : 0);
+}
+
+__device__ __forceinline__ f32x4 mfma32(s16x8 a, s16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0,
+                                                   0);
+}
+
+// Loads the BF16 weights (Wt[out][in], 4 consecutive columns per 8-byte word, which stay consecutive
+// in the stored order) and the FP32 biases.
+__device__ __forceinline__ void load_model32(GuideLds32 &L, const short *wt, const float *bias) {
+    for (int i = threadIdx.x; i < kGWeights / 4; i += blockDim.x) {
+        const int row = i >> 4, c = 4 * (i & 15);
+        *reinterpret_cast<uint2 *>(&L.w[row * kG32Stride + perm32(c)]) = reinterpret_cast<const uint2 *>(wt)[i];
+    }
+    for (int i = threadIdx.x; i < kGBiases; i += blockDim.x) L.b[i] = bias[i];
+}
+
+__device__ __forceinline__ s16x8 wblock32(const GuideLds32 &L, int l, int j, int m, int lane) {
+    return *reinterpret_cast<const s16x8 *>(&L.w[row32(l, j * 16 + (lane & 15)) + 32 * m + 8 * (lane >> 4)]);
+}
+
+__device__ __forceinline__ s16x8 pack8(f32x4 a, f32x4 b) {
+    s16x8 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        r[i] = f2bf(a[i]);
+        r[4 + i] = f2bf(b[i]);
+    }
+    return r;
+}
+
+// Features of this lane's vertex in the K order: B[0] = axis 0 then axis 1 frequencies 2q, 2q + 1
+// (sin, cos each), B[1] = axis 2 frequencies, then the normal and outgoing direction (q = 0, 1) or
+// zeros. Feature f of the model (guide.h) is element (f & 3) + 4 ((f >> 4) & 1) of block f >> 5.
+__device__ __forceinline__ void features32(const float *pos, const float *nrm, const float *wo, int lane, s16x8 x[2]) {
+    const int q = lane >> 4;
+    const float f0 = float(1 << (2 * q)), f1 = 2.f * f0;
+    float v[3][4];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        sincospif(pos[a] * f0, &v[a][0], &v[a][1]);
+        sincospif(pos[a] * f1, &v[a][2], &v[a][3]);
+    }
+    const bool g0 = q == 0, g1 = q == 1;
+    const float e[4] = {g0 ? nrm[0] : (g1 ? wo[1] : 0.f), g0 ? nrm[1] : (g1 ? wo[2] : 0.f), g0 ? nrm[2] : 0.f,
+                        g0 ? wo[0] : 0.f};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        x[0][i] = f2bf(v[0][i]);
+        x[0][4 + i] = f2bf(v[1][i]);
+        x[1][i] = f2bf(v[2][i]);
+        x[1][4 + i] = f2bf(e[i]);
+    }
+}
+
+// One hidden layer: relu(W x + b), B layout (K order above) in and out. The accumulators start at
+// the bias.
+__device__ __forceinline__ void hidden32(const GuideLds32 &L, int l, const s16x8 xin[2], s16x8 xout[2], int lane) {
+    f32x4 acc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float4 bj = *reinterpret_cast<const float4 *>(&L.b[bl_off(l) + j * 16 + 4 * (lane >> 4)]);
+        acc[j] = f32x4{bj.x, bj.y, bj.z, bj.w};
+    }
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[j] = mfma32(wblock32(L, l, j, m, lane), xin[m], acc[j]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[j][i] = fmaxf(acc[j][i], 0.f);
+    xout[0] = pack8(acc[0], acc[1]);
+    xout[1] = pack8(acc[2], acc[3]);
+}
+
+// The output layer: 48 raw outputs (rows 41..47 zero), accumulator layout.
+__device__ __forceinline__ void output32(const GuideLds32 &L, const s16x8 xin[2], f32x4 out[3], int lane) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const float4 bj = *reinterpret_cast<const float4 *>(&L.b[bl_off(4) + j * 16 + 4 * (lane >> 4)]);
+        out[j] = f32x4{bj.x, bj.y, bj.z, bj.w};
+    }
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) out[j] = mfma32(wblock32(L, 4, j, m, lane), xin[m], out[j]);
+}
+
+__device__ __forceinline__ void forward32(const GuideLds32 &L, const float *pos, const float *nrm, const float *wo,
+                                          int lane, f32x4 o[3]) {
+    s16x8 xa[2], xb[2];
+    features32(pos, nrm, wo, lane, xa);
+    hidden32(L, 0, xa, xb, lane);
+    hidden32(L, 1, xb, xa, lane);
+    hidden32(L, 2, xa, xb, lane);
+    hidden32(L, 3, xb, xa, lane);
+    output32(L, xa, o, lane);
 }
 
 // ---- inference -------------------------------------------------------------------------------
@@ -130,9 +181,9 @@ __device__ __forceinline__ void wave_sync() {
 __global__ __launch_bounds__(kInferBlock) void k_guide_infer(const short *wt, const float *bias, const int *queue,
                                                              const int *count, const float4 *h0, const float4 *h1,
                                                              const float4 *dir, GuideBox box, float4 *out) {
-    __shared__ GuideLds L;
+    __shared__ GuideLds32 L;
     __shared__ float tile[kInferBlock / 64][16][kGTileRow];
-    load_model(L, wt, bias);
+    load_model32(L, wt, bias);
     __syncthreads();
     const int n = *count, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int waves = int(gridDim.x) * (kInferBlock / 64);
@@ -150,14 +201,8 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(const short *wt, co
             float il = -1.f / sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
             wo[0] = d.x * il, wo[1] = d.y * il, wo[2] = d.z * il;
         }
-        bf16x4 xa[4], xb[4];
-        features(pos, nrm, wo, lane, xa);
-        hidden(L, 0, xa, xb, lane);
-        hidden(L, 1, xb, xa, lane);
-        hidden(L, 2, xa, xb, lane);
-        hidden(L, 3, xb, xa, lane);
         f32x4 o[3];
-        output(L, xa, o, lane);
+        forward32(L, pos, nrm, wo, lane, o);
         // Decode the mixture with all 64 lanes: the four lanes of a vertex (lane % 16) take two lobes
         // each and share the softmax normaliser and the lobe CDF through shuffles.
 #pragma unroll
@@ -201,8 +246,8 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(const short *wt, co
 // Raw outputs for explicit records (9 floats: normalised position, normal, outgoing direction).
 __global__ __launch_bounds__(kGBlock) void k_guide_eval(const short *wt, const float *bias, const float *recs, int n,
                                                         float *out41) {
-    __shared__ GuideLds L;
-    load_model(L, wt, bias);
+    __shared__ GuideLds32 L;
+    load_model32(L, wt, bias);
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int waves = int(gridDim.x) * (kGBlock / 64);
@@ -212,14 +257,8 @@ __global__ __launch_bounds__(kGBlock) void k_guide_eval(const short *wt, const f
         float r[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         if (valid)
             for (int k = 0; k < 9; ++k) r[k] = recs[size_t(idx) * 9 + k];
-        bf16x4 xa[4], xb[4];
-        features(r, r + 3, r + 6, lane, xa);
-        hidden(L, 0, xa, xb, lane);
-        hidden(L, 1, xb, xa, lane);
-        hidden(L, 2, xa, xb, lane);
-        hidden(L, 3, xb, xa, lane);
         f32x4 o[3];
-        output(L, xa, o, lane);
+        forward32(L, r, r + 3, r + 6, lane, o);
         if (valid)
 #pragma unroll
             for (int j = 0; j < 3; ++j)
@@ -276,101 +315,125 @@ __device__ void output_grad(const float *o, const float *rec, float blend, float
     }
 }
 
-constexpr int kGStage = 16 + 4;  // bf16 per staging row (16 samples, padded)
+// Training, one block per CU, 4 waves, 64 records per pass (16 per wave):
+//  - forward with the 16x16x32 layers above, every layer's input kept in registers;
+//  - the loss gradient of the 48 raw outputs per record (output_grad);
+//  - per layer, from the output: the layer input X and output gradient dY of the block's 64
+//    records are staged in LDS as [feature][record]; the weight gradient dW = dY X^T (K = the 64
+//    records) is an MFMA product whose 16x16 tiles are split among the 4 waves and accumulated in
+//    their registers over all passes (no atomics); the input gradient dX = W^T dY (a second,
+//    transposed copy of the weights in LDS, columns in the K order) masked by the ReLU, in registers.
+// At the end every wave writes its tiles, and 64 threads their bias sums, to the block's partial.
+constexpr int kTWaves = 4;
+constexpr int kTRecs = 16 * kTWaves;   // records per pass of a block
+constexpr int kTStage = kTRecs + 8;    // bf16 per staging row (64 records, 16-byte padding)
+constexpr int kTTiles[kGLayers] = {16, 16, 16, 16, 12};  // 16x16 weight-gradient tiles per layer
 
 struct TrainLds {
-    GuideLds m;
-    float acc[kGParams];            // this block's gradient, in the parameter layout
-    short stage[4][2][kGIn * kGStage];  // per wave: layer input X[in][s] and output gradient dY[out][s]
-    float tile[4][16][kGTileRow];  // per wave: raw outputs, then their gradients
+    GuideLds32 m;                                   // Wt[out][in], columns in the K order
+    alignas(16) short wT[4][kGIn * kG32Stride];     // layers 1..4: W^T[in][out], out in the K order
+    alignas(16) short sx[kGIn * kTStage];           // X_l[in][record]
+    alignas(16) short sd[kGIn * kTStage];           // dY_l[out][record]
+    float tile[kTWaves][16][kGTileRow];             // raw outputs, then their gradients
     float wsum, loss;
 };
 
 static_assert(sizeof(TrainLds) <= 160 * 1024, "training LDS exceeds a CU");
 
-// dW_l += dY_l X_l^T over this wave's 16 samples and db_l += sum_s dY_l (LDS atomics), with the
-// layer's input X (B layout, bf16) and output gradient dY (accumulator layout, fp32, nb blocks).
-__device__ __forceinline__ void weight_grad(TrainLds &T, int wave, int lane, int l, const bf16x4 x[4], const f32x4 *dy,
-                                            int nb) {
-    short *sx = T.stage[wave][0], *sd = T.stage[wave][1];
-    const int s = lane & 15, q = lane >> 4;
+__device__ __forceinline__ s16x8 lds8(const short *p) { return *reinterpret_cast<const s16x8 *>(p); }
+
+// Stages this wave's X (B layout: K order of the 64 inputs) and dY (accumulator layout, nb blocks)
+// as [feature][record] rows; record column = 16 wave + s.
+__device__ __forceinline__ void stage(TrainLds &T, int wave, int lane, const s16x8 x[2], const f32x4 *dy, int nb) {
+    const int s = 16 * wave + (lane & 15), q = lane >> 4;
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb)
+    for (int m = 0; m < 2; ++m)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) sx[(kb * 16 + 4 * q + i) * kGStage + s] = x[kb][i];
+        for (int i = 0; i < 8; ++i) T.sx[(32 * m + 4 * q + (i & 3) + (i & 4 ? 16 : 0)) * kTStage + s] = x[m][i];
     for (int j = 0; j < nb; ++j)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) sd[(j * 16 + 4 * q + i) * kGStage + s] = f2bf(dy[j][i]);
-    wave_sync();
-    float *gw = T.acc + guide_woff(l), *gb = T.acc + guide_boff(l);
-    for (int j = 0; j < nb; ++j) {
-        const bf16x4 a = *reinterpret_cast<const bf16x4 *>(&sd[(j * 16 + s) * kGStage + 4 * q]);
+        for (int i = 0; i < 4; ++i) T.sd[(16 * j + 4 * q + i) * kTStage + s] = f2bf(dy[j][i]);
+    for (int j = nb; j < 4; ++j)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const bf16x4 b = *reinterpret_cast<const bf16x4 *>(&sx[(c * 16 + s) * kGStage + 4 * q]);
-            f32x4 d = mfma(a, b, f32x4{0.f, 0.f, 0.f, 0.f});
-#pragma unroll
-            for (int i = 0; i < 4; ++i) atomicAdd(&gw[(j * 16 + 4 * q + i) * kGIn + c * 16 + s], d[i]);
-        }
-    }
-    if (lane < nb * 16) {
-        float sum = 0.f;
-        for (int k = 0; k < 16; ++k) sum += bf2f(sd[lane * kGStage + k]);
-        atomicAdd(&gb[lane], sum);
-    }
-    wave_sync();
+        for (int i = 0; i < 4; ++i) T.sd[(16 * j + 4 * q + i) * kTStage + s] = 0;
 }
 
-// dX = W^T dY (accumulator layout in and out), masked by the ReLU of the layer input x.
-__device__ __forceinline__ void input_grad(const GuideLds &L, int l, const f32x4 *dy, int nb, const bf16x4 x[4],
-                                           f32x4 dx[4], int lane) {
-    bf16x4 b[4];
-    for (int kb = 0; kb < nb; ++kb)
+// This wave's share of dW_l += dY X^T over the staged records: tiles t = wave, wave + 4, .. of the
+// layer's (rows / 16) x 4 tiles, tile t = (row block t / 4, column block t % 4).
+__device__ __forceinline__ void weight_grad32(const TrainLds &T, int l, int wave, int lane, f32x4 *acc) {
+    const int r = lane & 15, q = lane >> 4;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) b[kb][i] = f2bf(dy[kb][i]);
+    for (int k = 0; k < 4; ++k) {
+        const int t = wave + 4 * k;
+        if (t >= kTTiles[l]) break;
+        const int jb = t >> 2, cb = t & 3;
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb) {
+            const s16x8 a = lds8(&T.sd[(16 * jb + r) * kTStage + 32 * kb + 8 * q]);
+            const s16x8 b = lds8(&T.sx[(16 * cb + r) * kTStage + 32 * kb + 8 * q]);
+            acc[k] = mfma32(a, b, acc[k]);
+        }
+    }
+}
+
+// dX = W_l^T dY, accumulator layout in (nb blocks of dY rows) and out (4 blocks of 16 inputs),
+// masked by the ReLU of the layer input x (B layout).
+__device__ __forceinline__ void input_grad32(const TrainLds &T, int l, const f32x4 *dy, int nb, const s16x8 x[2],
+                                             f32x4 dx[4], int lane) {
+    const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+    const s16x8 b0 = pack8(dy[0], dy[1]), b1 = pack8(dy[2], nb > 3 ? dy[3] : zero);
+    const short *w = T.wT[l - 1];
 #pragma unroll
     for (int jb = 0; jb < 4; ++jb) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        for (int kb = 0; kb < nb; ++kb) {
-            bf16x4 a;
+        const short *row = &w[(16 * jb + (lane & 15)) * kG32Stride + 8 * (lane >> 4)];
+        f32x4 acc = mfma32(lds8(row), b0, zero);
+        acc = mfma32(lds8(row + 32), b1, acc);
+        const int m = jb >> 1, h = 4 * (jb & 1);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = L.w[lds_row(l, kb * 16 + 4 * (lane >> 4) + i) + jb * 16 + (lane & 15)];
-            acc = mfma(a, b[kb], acc);
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) acc[i] = bf2f(x[jb][i]) > 0.f ? acc[i] : 0.f;
+        for (int i = 0; i < 4; ++i) acc[i] = bf2f(x[m][h + i]) > 0.f ? acc[i] : 0.f;
         dx[jb] = acc;
     }
 }
 
-// One pass over n records (16 floats each): per block, the summed gradient of sum_r -wt_r log q_r
-// (unnormalised), the summed weight and loss, written to part[block * (kGParams + 2) ..].
+// One pass over records [first, first + n) (16 floats each, within the *count written when count
+// is given, at most `limit`): per block, the summed gradient of the unnormalised loss (see
+// output_grad), the summed weight and loss, written to part[block * (kGParams + 2) ..].
 __global__ __launch_bounds__(kGBlock) void k_guide_train(const short *wt, const float *bias, const float *recs,
                                                          const int *count, int limit, int first, int n, float blend,
                                                          float *part) {
     extern __shared__ __align__(16) unsigned char smem[];
     TrainLds &T = *reinterpret_cast<TrainLds *>(smem);
-    load_model(T.m, wt, bias);
-    for (int i = threadIdx.x; i < kGParams; i += blockDim.x) T.acc[i] = 0.f;
+    load_model32(T.m, wt, bias);
+    // The transposed weights of layers 1..4 (rows = inputs, columns = outputs in the K order; the
+    // output layer's outputs 48..63 are zero).
+    for (int i = threadIdx.x; i < 4 * kGIn * kGIn; i += blockDim.x) {
+        const int l = 1 + i / (kGIn * kGIn), e = i % (kGIn * kGIn), out = e / kGIn, in = e % kGIn;
+        const short v = out < guide_rows(l) ? wt[wl_off(l) + out * kGIn + in] : short(0);
+        T.wT[l - 1][in * kG32Stride + perm32(out)] = v;
+    }
     if (threadIdx.x == 0) T.wsum = T.loss = 0.f;
     __syncthreads();
-    // Records [first, first + n), within the *count written (at most `limit`, the buffer's size).
     const int avail = count ? min(n, max(0, min(*count, limit) - first)) : n;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, s = lane & 15;
-    const int waves = int(gridDim.x) * (kGBlock / 64);
-    for (int tile = int(blockIdx.x) * (kGBlock / 64) + wave; tile * 16 < avail; tile += waves) {
-        const int idx = tile * 16 + s;
+    f32x4 gacc[kGLayers][4];
+#pragma unroll
+    for (int l = 0; l < kGLayers; ++l)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) gacc[l][k] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float bacc[kGLayers] = {0.f, 0.f, 0.f, 0.f, 0.f};  // thread t < 64: bias gradient of output t of each layer
+    for (int pass = int(blockIdx.x); pass * kTRecs < avail; pass += int(gridDim.x)) {
+        const int idx = pass * kTRecs + 16 * wave + s;
         const bool valid = idx < avail;
         float rec[kGRec];
         for (int k = 0; k < kGRec; ++k) rec[k] = valid ? recs[size_t(first + idx) * kGRec + k] : 0.f;
-        bf16x4 x[5][4];
-        features(rec, rec + 3, rec + 6, lane, x[0]);
-        hidden(T.m, 0, x[0], x[1], lane);
-        hidden(T.m, 1, x[1], x[2], lane);
-        hidden(T.m, 2, x[2], x[3], lane);
-        hidden(T.m, 3, x[3], x[4], lane);
+        s16x8 x[kGLayers][2];
+        features32(rec, rec + 3, rec + 6, lane, x[0]);
+        hidden32(T.m, 0, x[0], x[1], lane);
+        hidden32(T.m, 1, x[1], x[2], lane);
+        hidden32(T.m, 2, x[2], x[3], lane);
+        hidden32(T.m, 3, x[3], x[4], lane);
         f32x4 o[3];
-        output(T.m, x[4], o, lane);
+        output32(T.m, x[4], o, lane);
         float(*tl)[kGTileRow] = T.tile[wave];
 #pragma unroll
         for (int j = 0; j < 3; ++j)
@@ -380,7 +443,7 @@ __global__ __launch_bounds__(kGBlock) void k_guide_train(const short *wt, const 
         if (lane < 16) {
             float dout[kGOutRows], loss;
             output_grad(tl[s], rec, blend, dout, &loss);
-            for (int i = 0; i < kGOutRows; ++i) tl[s][i] = dout[i];
+            for (int i = 0; i < kGOutRows; ++i) tl[s][i] = valid ? dout[i] : 0.f;
             if (valid && rec[13] > 0.f) {
                 atomicAdd(&T.wsum, rec[13]);
                 atomicAdd(&T.loss, loss);
@@ -392,37 +455,80 @@ __global__ __launch_bounds__(kGBlock) void k_guide_train(const short *wt, const 
         for (int j = 0; j < 3; ++j)
 #pragma unroll
             for (int i = 0; i < 4; ++i) dy[j][i] = tl[s][j * 16 + 4 * (lane >> 4) + i];
-        wave_sync();
-        // Layer 4 (output, 48 rows), then the hidden layers 3..0.
-        weight_grad(T, wave, lane, 4, x[4], dy, 3);
-        f32x4 dx[4];
-        input_grad(T.m, 4, dy, 3, x[4], dx, lane);
-        for (int l = 3; l >= 0; --l) {
-            weight_grad(T, wave, lane, l, x[l], dx, 4);
-            if (l == 0) break;
-            f32x4 nd[4];
-            input_grad(T.m, l, dx, 4, x[l], nd, lane);
+        dy[3] = f32x4{0.f, 0.f, 0.f, 0.f};
+        // Layers 4 .. 0: stage, weight and bias gradients, input gradient.
 #pragma unroll
-            for (int j = 0; j < 4; ++j) dx[j] = nd[j];
+        for (int l = kGLayers - 1; l >= 0; --l) {
+            const int nb = l == kGLayers - 1 ? 3 : 4;
+            __syncthreads();  // the previous layer's staging has been read
+            stage(T, wave, lane, x[l], dy, nb);
+            __syncthreads();
+            weight_grad32(T, l, wave, lane, gacc[l]);
+            if (threadIdx.x < guide_rows(l)) {
+                float sum = 0.f;
+                const short *row = &T.sd[threadIdx.x * kTStage];
+                for (int k = 0; k < kTRecs; k += 8) {
+                    const s16x8 v = lds8(row + k);
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) sum += bf2f(v[i]);
+                }
+                bacc[l] += sum;
+            }
+            if (l > 0) {
+                f32x4 dx[4];
+                input_grad32(T, l, dy, nb, x[l], dx, lane);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) dy[j] = dx[j];
+            }
         }
     }
-    __syncthreads();
+    // Tiles: lane l holds column 16 cb + l % 16 (input) and rows 16 jb + 4 (l / 16) .. +3 (outputs).
     float *out = part + size_t(blockIdx.x) * (kGParams + 2);
-    for (int i = threadIdx.x; i < kGParams; i += blockDim.x) out[i] = T.acc[i];
+    const int q = lane >> 4;
+#pragma unroll
+    for (int l = 0; l < kGLayers; ++l)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int t = wave + 4 * k;
+            if (t >= kTTiles[l]) break;
+            const int jb = t >> 2, cb = t & 3;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) out[guide_woff(l) + (16 * jb + 4 * q + i) * kGIn + 16 * cb + s] = gacc[l][k][i];
+        }
+    if (threadIdx.x < kGIn)
+#pragma unroll
+        for (int l = 0; l < kGLayers; ++l)
+            if (int(threadIdx.x) < guide_rows(l)) out[guide_boff(l) + threadIdx.x] = bacc[l];
+    __syncthreads();
     if (threadIdx.x == 0) {
         out[kGParams] = T.wsum;
         out[kGParams + 1] = T.loss;
     }
 }
 
-// Sums the block partials: gradient / summed weight (grad), and summed weight and loss.
-__global__ void k_guide_reduce(const float *part, int blocks, float *grad, float *stats) {
+// Sums the block partials: gradient / summed weight (grad), and summed weight and loss. Each block
+// first reduces the summed weight over the partials, then one thread per parameter sums its column
+// of the partials with eight independent accumulators.
+__global__ __launch_bounds__(256) void k_guide_reduce(const float *part, int blocks, float *grad, float *stats) {
+    __shared__ float ws[256];
+    float w = 0.f;
+    for (int b = int(threadIdx.x); b < blocks; b += 256) w += part[size_t(b) * (kGParams + 2) + kGParams];
+    ws[threadIdx.x] = w;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (int(threadIdx.x) < h) ws[threadIdx.x] += ws[threadIdx.x + h];
+        __syncthreads();
+    }
+    w = ws[0];
     const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
     if (i > kGParams + 1) return;
-    float w = 0.f;
-    for (int b = 0; b < blocks; ++b) w += part[size_t(b) * (kGParams + 2) + kGParams];
-    float s = 0.f;
-    for (int b = 0; b < blocks; ++b) s += part[size_t(b) * (kGParams + 2) + i];
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int b = 0;
+    for (; b + 8 <= blocks; b += 8)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc[k] += part[size_t(b + k) * (kGParams + 2) + i];
+    for (; b < blocks; ++b) acc[0] += part[size_t(b) * (kGParams + 2) + i];
+    const float s = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     if (i < kGParams) grad[i] = w > 0.f ? s / w : 0.f;
     else stats[i - kGParams] = s;
 }

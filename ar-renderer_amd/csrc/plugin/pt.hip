@@ -844,20 +844,24 @@ __device__ __forceinline__ void sample_ld(const DevScene &s, const Bufs &b, int 
     if (!sample_li(s, li, p, uL, r.lam, &ls) || !ls.L.nonzero() || ls.pdf == 0) return;
     V3 wi = ls.wi;
     SS f;
-    float scatterPdf;
+    // A delta light takes no MIS weight: the scattering pdf is needed only for the others.
+    const bool delta = light_is_delta(s, li);
+    float scatterPdf = 0.f;
     if (bsdf) {
         f = bsdf->f(wo, wi) * absdot(wi, ns);
         if (!f.nonzero()) return;
-        scatterPdf = bsdf->pdf(wo, wi);
-        // A guided vertex scatters with the one-sample MIS proposal: that is the pdf to weigh against.
-        if (gm) scatterPdf = gm->sel * guide_pdf(*gm, wi) + (1.f - gm->sel) * scatterPdf;
+        if (!delta) {
+            scatterPdf = bsdf->pdf(wo, wi);
+            // A guided vertex scatters with the one-sample MIS proposal: that is the pdf to weigh against.
+            if (gm) scatterPdf = gm->sel * guide_pdf(*gm, wi) + (1.f - gm->sel) * scatterPdf;
+        }
     } else {
         float ph = hg_phase(dot(wo, wi), g);
         f = SS(ph);
         scatterPdf = ph;
     }
     float lightPdf = pmf * ls.pdf;
-    float k = light_is_delta(s, li) ? 1.f / lightPdf : power_heuristic(lightPdf, scatterPdf) / lightPdf;
+    float k = delta ? 1.f / lightPdf : power_heuristic(lightPdf, scatterPdf) / lightPdf;
     // Shadow ray p0.SpawnRayTo(pLight) (src/scene/interaction.h:43-47) and its starting medium.
     V3 o = surface ? offset_origin(p, n, ls.pl - p) : p;
     V3 target = offset_origin(ls.pl, ls.nl, o - ls.pl);
@@ -1598,16 +1602,11 @@ struct Timer {
 // completion; the per-sample results are left in the xyz / record buffers.
 
 
-// Bounce rounds per captured graph (even: a replay ends on the queue it started from); 0 = no
-// graphs. Off by default: on configuration 2 at 1080p, replaying 4 captured rounds took 27.9 ms per
-// step against 25.4 ms for the pipelined eager loop (late, nearly empty bounces still launch the
-// whole-batch grids a graph bakes in, and a replay costs more than it saves in synchronisation).
-#ifndef PG_GRAPH_ROUNDS
-#define PG_GRAPH_ROUNDS 0
-#endif
-constexpr int kGraphRounds = PG_GRAPH_ROUNDS;
-static_assert(kGraphRounds % 2 == 0, "graph rounds must alternate back to the first queue");
-
+// Bounce rounds per captured graph (tuning key graph_rounds; even: a replay ends on the queue it
+// started from); 0 = no graphs, the default. On configuration 2 at 1080p, replaying 4 captured
+// rounds took 27.9 ms per step against 25.4 ms for the pipelined eager loop (late, nearly empty
+// bounces still launch the whole-batch grids a graph bakes in, and a replay costs more than it
+// saves in synchronisation).
 // What a captured bounce graph bakes in: the kernel variants, grids, buffer addresses and the
 // parameters the bounce kernels read (not the sample range, seed or clamp, which only the camera
 // and finish kernels use).
@@ -1699,19 +1698,21 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
         });
     };
 
-    if (!dev.profileMode && !p.guide && kGraphRounds > 0) {
-        // Unprofiled, unguided renders replay kGraphRounds captured bounces per launch (an even
+    const int graphRounds = dev.tune.graphRounds;
+    if (!dev.profileMode && !p.guide && graphRounds > 0) {
+        // Unprofiled, unguided renders replay graphRounds captured bounces per launch (an even
         // count, so every replay starts from queue A) with grids sized for the whole batch, and
         // read the path count back once per replay instead of once per bounce.
         const int tgrid = TraceGrid(dev, n);
-        const uint64_t key = BounceKey(p, b, ds, tgrid, shadeGrid, media, interfaces, s.emitters, pt.hasLayered);
+        const uint64_t key = BounceKey(p, b, ds, tgrid, shadeGrid, media, interfaces, s.emitters, pt.hasLayered) ^
+                             (uint64_t(graphRounds) << 56);
         if (!pt.bounceExec || pt.bounceKey != key) {
             if (pt.bounceExec) PG_CHECK(hipGraphExecDestroy(pt.bounceExec));
             pt.bounceExec = nullptr;
             hipGraph_t g = nullptr;
             PG_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             try {
-                for (int k = 0; k < kGraphRounds; ++k) round(k & 1, k, tgrid, shadeGrid);
+                for (int k = 0; k < graphRounds; ++k) round(k & 1, k, tgrid, shadeGrid);
             } catch (...) {
                 (void)hipStreamEndCapture(st, &g);
                 if (g) (void)hipGraphDestroy(g);

@@ -548,7 +548,7 @@ class Plugin:
         return hits
 
     TUNE_KEYS = ("trace_blocks_per_cu", "shade_blocks_per_cu", "path_budget", "guide_batch", "guide_steps",
-                 "guide_min_steps", "guide_lr", "guide_bounces", "guide_blend")
+                 "guide_min_steps", "guide_lr", "guide_bounces", "guide_blend", "graph_rounds")
 
     def tune_set(self, key, value, device=0):
         self.lib.prgpu_tune_set.argtypes = [C.c_int, C.c_char_p, C.c_double]
@@ -567,8 +567,12 @@ class Plugin:
         self.lib.prgpu_tune_source.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_int]
         path, err = C.create_string_buffer(1024), C.create_string_buffer(1024)
         found = self.lib.prgpu_tune_source(path, 1024, err, 1024) == 0
-        return {"values": {k: self.tune_get(k, device) for k in self.TUNE_KEYS},
-                "file": path.value.decode() if found else None, "file_error": err.value.decode() or None}
+        try:
+            values = {k: self.tune_get(k, device) for k in self.TUNE_KEYS}
+        except ValueError:  # no such device
+            values = None
+        return {"values": values, "file": path.value.decode() if found else None,
+                "file_error": err.value.decode() or None}
 
     def trace_noalloc(self, scene, rays_dev, n, hits_dev, any_hit=False, device=0):
         """prgpu_trace_rays_noalloc: rays_dev / hits_dev are device addresses (ints) of this GPU's

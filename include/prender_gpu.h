@@ -379,7 +379,8 @@ PRGPU_API int prgpu_profile_read(int deviceIndex, prgpu::ProfileStatsEx *out, in
 // (blocks per CU of the surface / material kernels), path_budget (paths in flight per batch: the
 // queue capacity), guide_batch (records per Adam step), guide_steps and guide_min_steps (Adam
 // steps per batch of paths), guide_lr, guide_bounces (bounces that sample the guide, 0 = all),
-// guide_blend (weight of the MIS-proposal term in the training loss, the rest fits the mixture).
+// guide_blend (weight of the MIS-proposal term in the training loss, the rest fits the mixture),
+// graph_rounds (bounce rounds replayed per captured HIP graph, even, 0 = eager launches).
 // Defaults come from the tuning file the autotuner writes ($PRGPU_TUNING_FILE, else
 // prgpu_tuning.txt beside the library: "key value" lines), read when devices are first
 // enumerated. set / get return 0, or 1 for an unknown key, a value out of range or no device.

@@ -27,7 +27,8 @@ namespace {
 
 constexpr int kIn = 64, kHidden = 64, kOutRows = 48, kOut = 41, kLobes = 8, kLayers = 5;
 constexpr double kPi = 3.14159265358979323846;
-constexpr double kSelMin = 0.1, kSelMax = 0.9, kKappaMin = 0.01, kKappaMax = 1000.0;
+double kSelMin = 0.1;  // floor of the selection probability (or_guide_set_sel_min)
+constexpr double kSelMax = 0.9, kKappaMin = 0.01, kKappaMax = 1000.0;
 double gBlend = 0.5;  // weight of the MIS-proposal term in the loss (or_guide_set_blend)
 
 int Rows(int l) { return l == kLayers - 1 ? kOutRows : kHidden; }
@@ -42,9 +43,32 @@ size_t ParamCount() { return WOff(kLayers); }
 double Softplus(double x) { return x > 30 ? x : std::log1p(std::exp(x)); }
 double Sigmoid(double x) { return 1 / (1 + std::exp(-x)); }
 
+int gEncoding = 0;  // experiments: 0 frequency encoding, 1 one-blob
+
 void Features(const float *rec, double *f) {
     // rec: position in [0,1]^3 (already normalised), normal (3), wo (3)
     int k = 0;
+    if (gEncoding == 1) {
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 16; ++b) {
+                const double c = (b + 0.5) / 16, d = (double(rec[a]) - c) * 16;
+                f[k++] = std::exp(-0.5 * d * d);
+            }
+        const double th = std::acos(std::fmax(-1.0, std::fmin(1.0, double(rec[5])))) / kPi;
+        double ph = std::atan2(double(rec[4]), double(rec[3])) / (2 * kPi);
+        if (ph < 0) ph += 1;
+        for (int b = 0; b < 8; ++b) {
+            const double c = (b + 0.5) / 8, d = (th - c) * 8;
+            f[k++] = std::exp(-0.5 * d * d);
+        }
+        for (int b = 0; b < 8; ++b) {
+            double d = ph - (b + 0.5) / 8;
+            d -= std::round(d);  // azimuth wraps
+            d *= 8;
+            f[k++] = std::exp(-0.5 * d * d);
+        }
+        return;
+    }
     for (int a = 0; a < 3; ++a)
         for (int fr = 0; fr < 8; ++fr) {
             double arg = std::ldexp(kPi * double(rec[a]), fr);
@@ -167,6 +191,8 @@ void or_guide_mix_sample(const float *o41, float ul, float u1, float u2, float *
 // records of 16 floats: position (3, normalised), normal (3), wo (3), sampled direction (3),
 // p_bsdf of that direction (1), weight (1), 2 unused. Returns the loss.
 void or_guide_set_blend(double b) { gBlend = b; }
+void or_guide_set_sel_min(double s) { kSelMin = s; }
+void or_guide_set_encoding(int e) { gEncoding = e; }
 
 double or_guide_loss_grad(const float *P, const float *recs, int n, float *grad) {
     const size_t np = ParamCount();

@@ -152,6 +152,17 @@ def guide_set_blend(b):
     lib().or_guide_set_blend(float(b))
 
 
+def guide_set_sel_min(s):
+    """Floor of the selection probability s (its ceiling is 0.9)."""
+    lib().or_guide_set_sel_min.argtypes = [C.c_double]
+    lib().or_guide_set_sel_min(float(s))
+
+
+def guide_set_encoding(e):
+    lib().or_guide_set_encoding.argtypes = [C.c_int]
+    lib().or_guide_set_encoding(int(e))
+
+
 def guide_loss_grad(params, recs16):
     P = np.ascontiguousarray(params, dtype=np.float32)
     R = np.ascontiguousarray(recs16, dtype=np.float32).reshape(-1, 16)

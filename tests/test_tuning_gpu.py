@@ -1,0 +1,57 @@
+"""Run-time tuning on the MI355X: every launch shape and queue capacity renders the same film.
+
+The path of every camera sample is a pure function of (pixel, sample index, seed), so the grids of
+the traversal and shading kernels, the queue capacity (paths in flight per batch, which splits a
+call into batches) and replaying captured bounce graphs instead of launching eagerly must all give
+the film of the default settings, bit for bit. Captured graphs are replayed many times per call
+(one replay per graph_rounds bounces), with the eager renders of the same sample ranges between
+them, and every call is checked.
+"""
+import numpy as np
+import pytest
+
+import prender_amd as pa
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture()
+def defaults(plugin):
+    saved = {k: plugin.tune_get(k) for k in plugin.TUNE_KEYS}
+    yield saved
+    for k, v in saved.items():
+        plugin.tune_set(k, v)
+
+
+def _film(plugin, scene, first, spp=4):
+    s, _ = plugin.render_ex(scene, pa.render_params(spp=spp, first_sample=first, seed=1, max_depth=8, rr_start=3),
+                            variance=False)
+    return s
+
+
+def test_graph_replays_equal_eager_launches(plugin, host, defaults):
+    s = host.generate(1, 96, 96)
+    for rounds in (2, 4):
+        for call in range(3):
+            plugin.tune_set("graph_rounds", 0)
+            eager = _film(plugin, s, 8 * call)
+            plugin.tune_set("graph_rounds", rounds)
+            graphed = _film(plugin, s, 8 * call)
+            assert np.array_equal(graphed, eager), f"graph_rounds={rounds}, call {call}: films differ"
+        print(f"graph_rounds={rounds}: 3 calls equal the eager films", flush=True)
+
+
+@pytest.mark.parametrize("key,value", [("trace_blocks_per_cu", 3), ("shade_blocks_per_cu", 5),
+                                       ("path_budget", 65536)])
+def test_launch_shape_and_queue_capacity_keep_the_film(plugin, host, defaults, key, value):
+    s = host.generate(2, 160, 90)
+    ref = _film(plugin, s, 0, spp=16)
+    plugin.tune_set(key, value)
+    assert plugin.tune_get(key) == value
+    assert np.array_equal(_film(plugin, s, 0, spp=16), ref)
+
+
+def test_out_of_range_values_are_refused(plugin, defaults):
+    for key, value in (("trace_blocks_per_cu", 0), ("graph_rounds", 3), ("guide_blend", 1.5), ("no_such_key", 1)):
+        with pytest.raises(ValueError):
+            plugin.tune_set(key, value)
