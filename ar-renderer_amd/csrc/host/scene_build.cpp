@@ -3,7 +3,9 @@
 #include <algorithm>
 #include <array>
 #include <cstring>
+#include <atomic>
 #include <map>
+#include <thread>
 
 namespace phost {
 
@@ -386,6 +388,57 @@ void FitRGBSigmoid(const float rgbIn[3], float out[3]) {
     out[0] = float(A * c1 * c1);
     out[1] = float(B * c1 - 2 * A * c0 * c1 * c1);
     out[2] = float(C - B * c0 * c1 + A * c0 * c0 * c1 * c1);
+}
+
+// The table's nodes, as the reference's generator places and fits them (rgb2spec.cpp:243-276):
+// z nodes smoothstep(smoothstep(k / (res - 1))); for maximum component l and the other two at
+// x, y in [0, 1] of it, each (l, y, x) column of z is fitted from z index res / 5 upwards and then
+// downwards, every fit starting from the previous one's coefficients.
+const std::vector<float> &RgbToSpectrumTableRecord() {
+    static const std::vector<float> rec = [] {
+        const int R = prgpu::kRgbTableRes;
+        std::vector<float> out(1 + size_t(R) + size_t(3) * R * R * R * 3, 0.f);
+        out[0] = float(R);
+        float *z = out.data() + 1, *co = z + R;
+        auto smoothstep = [](double x) { return x * x * (3.0 - 2.0 * x); };
+        for (int k = 0; k < R; ++k) z[k] = float(smoothstep(smoothstep(double(k) / (R - 1))));
+        const FitData &t = Fit();
+        const double c0 = 360.0, c1 = 1.0 / (830.0 - 360.0);
+        auto column = [&](int l, int j, int i) {
+            const double y = double(j) / (R - 1), x = double(i) / (R - 1);
+            auto fitRange = [&](int kStart, int kEnd, int step) {
+                double c[3] = {0, 0, 0};
+                for (int k = kStart; k != kEnd; k += step) {
+                    const double b = z[k];
+                    double rgb[3];
+                    rgb[l] = b;
+                    rgb[(l + 1) % 3] = x * b;
+                    rgb[(l + 2) % 3] = y * b;
+                    GaussNewton(t, rgb, c, 15);
+                    const size_t idx = ((size_t(l) * R + size_t(k)) * R + size_t(j)) * R + size_t(i);
+                    const double A = c[0], B = c[1], C = c[2];
+                    co[3 * idx + 0] = float(A * c1 * c1);
+                    co[3 * idx + 1] = float(B * c1 - 2 * A * c0 * c1 * c1);
+                    co[3 * idx + 2] = float(C - B * c0 * c1 + A * c0 * c0 * c1 * c1);
+                }
+            };
+            fitRange(R / 5, R, 1);
+            fitRange(R / 5, -1, -1);
+        };
+        const int jobs = 3 * R;
+        std::atomic<int> next{0};
+        auto worker = [&] {
+            for (int lj = next++; lj < jobs; lj = next++)
+                for (int i = 0; i < R; ++i) column(lj / R, lj % R, i);
+        };
+        const unsigned n = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        for (unsigned k = 1; k < n; ++k) pool.emplace_back(worker);
+        worker();
+        for (std::thread &th : pool) th.join();
+        return out;
+    }();
+    return rec;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -791,6 +844,38 @@ int SceneBuilder::AddEnvironment(int spectrum, float scale) {
     l.primitive = -1;
     l.spectrum = spectrum;
     l.scale = scale;
+    lights.push_back(l);
+    return int(lights.size() - 1);
+}
+
+int SceneBuilder::AddEnvironmentMap(const float *rgb, int w, int h, float scale, float rotationDeg) {
+    // EnvironmentLight(map, scale, rotation) (src/lights/light.cpp:221-239): world directions go
+    // into the map's frame by the inverse of Rotate(rotation, +y).
+    EnvMapDistribution dist;
+    dist.Build(rgb, w, h);
+    if (rgbTable_ < 0) {
+        const std::vector<float> &t = RgbToSpectrumTableRecord();
+        rgbTable_ = int(knots.size());
+        knots.insert(knots.end(), t.begin(), t.end());
+    }
+    const int d65 = D65Illuminant();
+    const float rad = (3.14159265358979323846f / 180.f) * rotationDeg;
+    const int off = int(knots.size());
+    const float header[prgpu::kEnvHeader] = {float(w), float(h), std::cos(rad), std::sin(rad), dist.margInt,
+                                             dist.avgY, float(d65), float(rgbTable_)};
+    knots.insert(knots.end(), header, header + prgpu::kEnvHeader);
+    knots.insert(knots.end(), rgb, rgb + size_t(w) * size_t(h) * 3);
+    knots.insert(knots.end(), dist.func.begin(), dist.func.end());
+    knots.insert(knots.end(), dist.condCdf.begin(), dist.condCdf.end());
+    knots.insert(knots.end(), dist.rowInt.begin(), dist.rowInt.end());
+    knots.insert(knots.end(), dist.margCdf.begin(), dist.margCdf.end());
+    prgpu::LightEx l{};
+    l.kind = prgpu::LightExEnvironment;
+    l.primitive = -1;
+    l.spectrum = d65;
+    l.scale = scale;
+    l.profileOffset = off;
+    l.profileSize = int(knots.size()) - off;
     lights.push_back(l);
     return int(lights.size() - 1);
 }

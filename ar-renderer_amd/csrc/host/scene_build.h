@@ -136,6 +136,9 @@ class SceneBuilder {
     // over the Kasten-Young air mass, every 5 nm (src/lights/sky.cpp:207-233).
     int SunRadiance(F3 sunDir, float T, float angularRadiusDeg);
     int AddEnvironment(int spectrum, float scale);
+    // Image environment (an equirectangular linear-RGB map, rows from the zenith down, w x h):
+    // its record (prgpu::kEnvHeader) and, once per scene, the RGB-to-spectrum table go to knots.
+    int AddEnvironmentMap(const float *rgb, int w, int h, float scale, float rotationDeg);
     int AddHomogeneous(int sigmaA, int sigmaS, float scale, float g);
     int AddGrid(int sigmaA, int sigmaS, float scale, float g, int nx, int ny, int nz, const std::vector<float> &data,
                 F3 boundsMin, F3 boundsMax);
@@ -162,6 +165,7 @@ class SceneBuilder {
   private:
     int AddSpectrum(const prgpu::SpectrumEx &s);
     int d65_ = -1;
+    int rgbTable_ = -1;  // knots offset of the RGB-to-spectrum table record, once added
     prgpu::SceneDescEx desc_{};
 };
 
@@ -181,6 +185,11 @@ void PreethamSky(F3 sunDir, float turbidity, float groundAlbedo, int width, int 
 // Sampling tables of an equirectangular image environment (src/lights/light.cpp:221-239,
 // src/core/sampling.cpp:5-67): weights luminance x sin(theta) + 1e-6 per texel, per-row
 // conditional CDFs and integrals, the marginal CDF; continuous sampling and density on [0, 1]^2.
+// The RGB-to-spectrum table (prgpu::kRgbTableRes^3 nodes per maximum component, the layout of
+// the reference's RGBToSpectrumTable, src/core/rgb2spec.cpp:243-276), as its knots record:
+// resolution, z nodes, coefficients. Built once per process.
+const std::vector<float> &RgbToSpectrumTableRecord();
+
 struct EnvMapDistribution {
     int nu = 0, nv = 0;
     std::vector<float> func, condCdf, rowInt, margCdf;

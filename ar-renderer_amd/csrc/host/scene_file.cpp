@@ -5,8 +5,8 @@
 //
 // What the exporter approximates or drops is approximated or dropped here, with the same warning
 // (checker textures keep both colours and the u scale; roughness textures become the default;
-// normal and bump maps are ignored; image-environment lights and the Preetham sky of sun-sky lights
-// are skipped). Distant, spot, IES and sun lights, which the v4 exporter also skips, are kept: the
+// normal and bump maps are ignored). Distant, spot, IES and sun lights, image environments (PFM and
+// PPM maps) and the Preetham sky of sun-sky lights, which the v4 exporter skips, are kept: the
 // extension has them. Everything the renderer cannot express and the exporter has no fallback for (image and
 // noise colour textures, orthographic and realistic cameras, glTF / pbrt imports, noise media) is a
 // load error naming the item. Illuminant RGB spectra (sigmoid x D65) are tabulated every 5 nm, as
@@ -275,6 +275,94 @@ std::string ReadFile(const fs::path &p) {
 }
 
 // Relative "file" values (and bare rig paths) are resolved against the file that names them.
+// Images for environment lights (src/core/image.cpp:31-101): PFM (linear float, "PF" RGB or "Pf"
+// grey, bottom row first, negative scale = little-endian) and binary PPM ("P6", 8-bit, decoded from
+// sRGB). Rows come out top row first, linear RGB. Other formats are a named error.
+bool ReadImageRGB(const std::string &path, int *w, int *h, std::vector<float> *rgb, std::string *err) {
+    std::string ext = fs::path(path).extension().string();
+    for (char &c : ext) c = char(std::tolower(static_cast<unsigned char>(c)));
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) {
+        *err = "cannot open " + path;
+        return false;
+    }
+    bool ok = false;
+    if (ext == ".pfm") {
+        char hdr[3] = {0};
+        float scale = 0;
+        ok = std::fscanf(f, "%2s %d %d %f", hdr, w, h, &scale) == 4;
+        std::fgetc(f);
+        const int nc = (ok && std::strcmp(hdr, "PF") == 0) ? 3 : (ok && std::strcmp(hdr, "Pf") == 0 ? 1 : 0);
+        if (!ok || nc == 0 || *w <= 0 || *h <= 0 || int64_t(*w) * *h > (int64_t(1) << 26)) {
+            *err = "bad PFM header";
+            ok = false;
+        } else {
+            std::vector<float> data(size_t(*w) * size_t(*h) * size_t(nc));
+            ok = std::fread(data.data(), sizeof(float), data.size(), f) == data.size();
+            if (!ok) *err = "truncated PFM";
+            if (ok && scale > 0)  // big-endian
+                for (float &v : data) {
+                    uint32_t u;
+                    std::memcpy(&u, &v, 4);
+                    u = (u >> 24) | ((u >> 8) & 0xff00) | ((u << 8) & 0xff0000) | (u << 24);
+                    std::memcpy(&v, &u, 4);
+                }
+            if (ok) {
+                rgb->assign(size_t(*w) * size_t(*h) * 3, 0.f);
+                for (int y = 0; y < *h; ++y)
+                    for (int x = 0; x < *w; ++x) {
+                        const float *q = &data[(size_t(*h - 1 - y) * size_t(*w) + size_t(x)) * size_t(nc)];
+                        float *o = &(*rgb)[3 * (size_t(y) * size_t(*w) + size_t(x))];
+                        o[0] = q[0];
+                        o[1] = nc == 3 ? q[1] : q[0];
+                        o[2] = nc == 3 ? q[2] : q[0];
+                    }
+            }
+        }
+    } else if (ext == ".ppm") {
+        char magic[3] = {0};
+        int maxv = 0;
+        ok = std::fscanf(f, "%2s", magic) == 1 && std::strcmp(magic, "P6") == 0;
+        auto num = [&](int *v) {  // skips whitespace and # comments
+            int c = std::fgetc(f);
+            while (c != EOF && (std::isspace(c) || c == '#')) {
+                if (c == '#')
+                    while (c != EOF && c != '\n') c = std::fgetc(f);
+                c = std::fgetc(f);
+            }
+            if (c == EOF || !std::isdigit(c)) return false;
+            *v = 0;
+            while (c != EOF && std::isdigit(c)) {
+                *v = *v * 10 + (c - '0');
+                c = std::fgetc(f);
+            }
+            return true;
+        };
+        ok = ok && num(w) && num(h) && num(&maxv) && maxv == 255 && *w > 0 && *h > 0 &&
+             int64_t(*w) * *h <= (int64_t(1) << 26);
+        if (!ok) {
+            *err = "bad PPM header (binary P6, 8 bits)";
+        } else {
+            std::vector<unsigned char> data(size_t(*w) * size_t(*h) * 3);
+            ok = std::fread(data.data(), 1, data.size(), f) == data.size();
+            if (!ok) *err = "truncated PPM";
+            float lut[256];
+            for (int i = 0; i < 256; ++i) {  // SRGBDecode (src/core/color.h)
+                float v = float(i) / 255.f;
+                lut[i] = v <= 0.04045f ? v / 12.92f : std::pow((v + 0.055f) / 1.055f, 2.4f);
+            }
+            if (ok) {
+                rgb->resize(data.size());
+                for (size_t i = 0; i < data.size(); ++i) (*rgb)[i] = lut[data[i]];
+            }
+        }
+    } else {
+        *err = "image format '" + ext + "' is not read by this loader (PFM and PPM are)";
+    }
+    std::fclose(f);
+    return ok;
+}
+
 void ResolvePaths(J &j, const fs::path &dir) {
     if (j.isObj()) {
         for (auto &[key, v] : j.o) {
@@ -875,7 +963,6 @@ bool MeshContains(const SceneBuilder &sb, int first, int count, F3 pt) {
 // ---- the loader ----------------------------------------------------------------------------------
 enum class SpecType { Albedo, Unbounded, Illuminant };
 
-const char *const kSkippedLights = "image-environment lights are skipped on the GPU";
 
 struct Tex {  // a colour input as the exporter flattens it: spectrum, checker second colour, u scale
     int spec = -1, second = -1;
@@ -1580,12 +1667,22 @@ class Loader {
             float radius = 0.2665f * GetF(l, "sun_size", 1.f);
             if (sunDir.y > -0.02f && GetF(l, "sun_scale", 1.f) > 0)
                 sb_.AddSunLight(sunDir, radius, sb_.SunRadiance(sunDir, turbidity, radius), scale * GetF(l, "sun_scale", 1.f));
-            if (type == "sun_sky") Warn("the Preetham sky of sun_sky lights (an image environment) is skipped on the GPU");
+            if (type == "sun_sky") {  // MakePreethamSky at resolution x resolution / 2
+                const int res = int(GetF(l, "resolution", 512));
+                if (res < 2) Fail("light '" + id + "': sky resolution must be at least 2");
+                std::vector<float> sky(size_t(res) * size_t(res / 2) * 3);
+                PreethamSky(sunDir, turbidity, GetF(l, "ground_albedo", 0.3f), res, res / 2, sky.data());
+                sb_.AddEnvironmentMap(sky.data(), res, res / 2, scale * GetF(l, "sky_scale", 1.f), 0.f);
+            }
             return;
         }
         if (type == "environment" || type == "env" || type == "sky") {
             if (l.has("file")) {
-                Warn(kSkippedLights);
+                int w = 0, h = 0;
+                std::vector<float> img;
+                std::string err;
+                if (!ReadImageRGB(GetS(l, "file", ""), &w, &h, &img, &err)) Fail("light '" + id + "': " + err);
+                sb_.AddEnvironmentMap(img.data(), w, h, GetF(l, "intensity", 1.f) * scaleMul, GetF(l, "rotation", 0.f));
                 return;
             }
             const int spec = LightSpectrum(l);

@@ -64,6 +64,7 @@ const TuneKey kTuneKeys[] = {
     {"guide_steps", 1, 64, true},                      {"guide_min_steps", 0, 64, true},
     {"guide_lr", 1e-6, 1.0, false},                    {"guide_bounces", 0, 64, true},
     {"guide_blend", 0.0, 1.0, false},                  {"graph_rounds", 0, 16, true},
+    {"guide_replay", 0, 1 << 24, true},
 };
 
 bool TuneSet(Tuning &t, const std::string &key, double v) {
@@ -79,6 +80,7 @@ bool TuneSet(Tuning &t, const std::string &key, double v) {
         else if (key == "guide_lr") t.guideLr = float(v);
         else if (key == "guide_bounces") t.guideBounces = int(v);
         else if (key == "guide_blend") t.guideBlend = float(v);
+        else if (key == "guide_replay") t.guideReplay = int(v);
         else if (key == "graph_rounds") {
             if (int(v) % 2) return false;  // a replay must end on the queue it started from
             t.graphRounds = int(v);
@@ -99,6 +101,7 @@ bool TuneGet(const Tuning &t, const std::string &key, double *v) {
     else if (key == "guide_bounces") *v = t.guideBounces;
     else if (key == "guide_blend") *v = t.guideBlend;
     else if (key == "graph_rounds") *v = t.graphRounds;
+    else if (key == "guide_replay") *v = t.guideReplay;
     else return false;
     return true;
 }
@@ -692,7 +695,7 @@ PRGPU_API int prgpu_guide_records(int deviceIndex, float *records, int max, int 
         int counts[2] = {0, 0};
         PG_CHECK(hipMemcpyAsync(counts, d.guide->trainCount.p, 2 * sizeof(int), hipMemcpyDeviceToHost, d.stream));
         PG_CHECK(hipStreamSynchronize(d.stream));
-        const int n = std::max(0, std::min({counts[0], int(d.guide->trainCap), max}));
+        const int n = std::max(0, std::min(int(d.guide->ringValid), max));
         if (n > 0)
             PG_CHECK(hipMemcpyAsync(records, d.guide->train.p, size_t(n) * 16 * sizeof(float), hipMemcpyDeviceToHost,
                                     d.stream));

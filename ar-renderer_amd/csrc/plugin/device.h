@@ -60,6 +60,7 @@ struct Tuning {
     float guideLr = 1e-3f;                   // Adam learning rate
     int guideBounces = 2;                    // bounces whose diffuse vertices use the guide (0: all)
     float guideBlend = 0.5f;                 // weight of the MIS-proposal term in the training loss
+    int guideReplay = 1 << 20;               // training records kept across batches (0: the batch's only)
     int graphRounds = 0;                     // bounce rounds per captured HIP graph (even; 0: none)
 };
 

@@ -637,6 +637,7 @@ void GuideState::EnsureTrain(size_t n) {
     if (trainCap >= n) return;
     train.ensure(n * kGRec * sizeof(float));
     trainCap = n;
+    ringHead = ringValid = 0;  // the ring starts over in the new buffer
 }
 
 void GuideState::Infer(const int *queue, const int *count, int maxCount, const float4 *h0, const float4 *h1,

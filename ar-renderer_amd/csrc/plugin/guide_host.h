@@ -23,6 +23,9 @@ struct GuideState {
     size_t cap = 0, trainCap = 0;
     int steps = 0;
     double recordsPerPath = 0;  // training records the last training batch produced per path
+    size_t ringHead = 0, ringValid = 0;  // training-record ring: next write position, records held
+    size_t batchAdd = 0;                 // records the current batch may add to the ring
+    size_t rotor = 0;                    // next slice of the ring to train on
     float blend = kGBlend;      // weight of the MIS-proposal term in the training loss
 
     void Init(hipStream_t s);

@@ -325,6 +325,24 @@ void Validate(const prgpu::SceneDescEx &sd) {
         if (l.kind == prgpu::LightExSpot &&
             !(l.cosFalloffEnd >= -1 && l.cosFalloffEnd <= l.cosFalloffStart && l.cosFalloffStart <= 1))
             bad("spot light cone");
+        if (l.kind == prgpu::LightExEnvironment && l.profileSize != 0) {  // image environment record
+            bool ok = l.profileOffset >= 0 && l.profileSize > prgpu::kEnvHeader &&
+                      int64_t(l.profileOffset) + l.profileSize <= sd.knotCount;
+            if (ok) {
+                const float *r = sd.knots + l.profileOffset;
+                const int64_t w = int64_t(r[0]), h = int64_t(r[1]), d65 = int64_t(r[6]), tab = int64_t(r[7]);
+                ok = w >= 1 && h >= 1 && float(w) == r[0] && float(h) == r[1] && w * h <= (int64_t(1) << 26) &&
+                     int64_t(prgpu::kEnvHeader) + 3 * w * h + w * h + h * (w + 1) + h + (h + 1) == l.profileSize &&
+                     float(d65) == r[6] && specOK(int(d65), false) && float(tab) == r[7] && tab >= 0 &&
+                     tab < sd.knotCount;
+                if (ok) {
+                    const int64_t n = int64_t(sd.knots[tab]);
+                    ok = n >= 2 && n <= 256 && float(n) == sd.knots[tab] &&
+                         tab + 1 + n + 3 * n * n * n * 3 <= sd.knotCount;
+                }
+            }
+            if (!ok) bad("environment map record");
+        }
     }
     for (int i = 0; i < sd.mediumCount; ++i) {
         const prgpu::MediumEx &m = sd.media[i];
@@ -456,6 +474,8 @@ void EnsurePTScene(Device &dev, const prgpu::SceneDescEx &sd, const float *sheen
                  pt->worldR * pt->worldR;
         } else if (l.kind == prgpu::LightExDistant) {
             pw = l.scale * host_spectrum_y(st, l.spectrum, pt->cieYInt) * kPi * pt->worldR * pt->worldR;
+        } else if (l.profileSize > 0) {  // image environment: its mean luminance (light.cpp:241)
+            pw = 4 * kPi * kPi * pt->worldR * pt->worldR * l.scale * sd.knots[l.profileOffset + 5];
         } else {
             pw = 4 * kPi * kPi * pt->worldR * pt->worldR * l.scale * host_spectrum_y(st, l.spectrum, pt->cieYInt);
         }
@@ -664,7 +684,9 @@ struct Bufs {
     float4 *trec;
     float *train;
     int *trainCount;
-    int trainCap;
+    int trainCap;    // records the batch may add
+    int trainBase;   // ring position of the batch's first record
+    int trainRing;   // ring capacity (records)
 };
 
 constexpr int kBlock = 256;
@@ -1085,9 +1107,10 @@ __global__ __launch_bounds__(kBlock) void k_emit(const DevScene *__restrict__ sp
         if (prim < 0) {
             for (int li = 0; li < s.infCount; ++li) {
                 int L = s.infinite[li];
-                if (s.lights[L].kind == prgpu::LightExEnvironment ||
-                    (s.lights[L].kind == prgpu::LightExSun &&  // SunLight::Le (sky.cpp:92-95)
-                     dot(normalize(r.rd), f3(s.lights[L].direction)) >= s.lights[L].cosFalloffEnd))
+                if (s.lights[L].kind == prgpu::LightExEnvironment)
+                    add_emitted(s, r, L, env_L(s, s.lights[L], r.rd, r.lam), r.rd);
+                else if (s.lights[L].kind == prgpu::LightExSun &&  // SunLight::Le (sky.cpp:92-95)
+                         dot(normalize(r.rd), f3(s.lights[L].direction)) >= s.lights[L].cosFalloffEnd)
                     add_emitted(s, r, L, spec_sample(s.spec, s.lights[L].spectrum, r.lam) * s.lights[L].scale, r.rd);
             }
         } else {
@@ -1376,7 +1399,8 @@ __global__ __launch_bounds__(kBlock) void k_finish(const DevScene *__restrict__ 
                 const int at = wave_append(b.trainCount);
                 if (at < b.trainCap) {
                     float4 a = t[slot], n1 = t[c + slot], wo = t[2 * c + slot], wi = t[3 * c + slot];
-                    float4 *dst = reinterpret_cast<float4 *>(b.train + size_t(at) * 16);
+                    const size_t ring = size_t((int64_t(b.trainBase) + at) % b.trainRing);
+                    float4 *dst = reinterpret_cast<float4 *>(b.train + ring * 16);
                     dst[0] = make_float4(a.x, a.y, a.z, n1.x);
                     dst[1] = make_float4(n1.y, n1.z, wo.x, wo.y);
                     dst[2] = make_float4(wo.z, wi.x, wi.y, wi.z);
@@ -1521,6 +1545,8 @@ Bufs MakeBufs(PTScene &pt) {
     b.train = nullptr;
     b.trainCount = nullptr;
     b.trainCap = 0;
+    b.trainBase = 0;
+    b.trainRing = 1;
     return b;
 }
 
@@ -1532,7 +1558,9 @@ void AttachGuide(Device &dev, Bufs &b) {
     b.trec = g.rec.as<float4>();
     b.train = g.train.as<float>();
     b.trainCount = g.trainCount.as<int>();
-    b.trainCap = int(g.trainCap);
+    b.trainCap = int(std::min<size_t>(g.trainCap, size_t(g.batchAdd)));
+    b.trainBase = int(g.ringHead);
+    b.trainRing = int(std::max<size_t>(1, g.trainCap));
 }
 
 Params MakeParams(const prgpu::RenderParamsEx &rp) {
@@ -1820,7 +1848,9 @@ void RenderExImpl(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::Render
         }
         dev.guide->EnsureBatch(pt.cap);  // per-path planes are indexed with the batch capacity
         dev.guide->blend = dev.tune.guideBlend;
-        if (p.train) dev.guide->EnsureTrain(size_t(dev.tune.guideBatch) * size_t(dev.tune.guideSteps));
+        if (p.train)
+            dev.guide->EnsureTrain(std::max(size_t(dev.tune.guideBatch) * size_t(dev.tune.guideSteps),
+                                            size_t(dev.tune.guideReplay)));
         for (int a = 0; a < 3; ++a) {
             p.box.lo[a] = dev.sceneLo[a];
             const float ext = dev.sceneHi[a] - dev.sceneLo[a];
@@ -1831,14 +1861,18 @@ void RenderExImpl(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::Render
         int passes = std::min(per, rp.spp - done);
         p.firstSample = rp.firstSample + done;
         if (p.train) {
-            PG_CHECK(hipMemsetAsync(dev.guide->trainCount.p, 0, 2 * sizeof(int), dev.stream));
-            // Keep about as many records as the training steps use: from the records the last
-            // batch produced per path, or, for the first, an estimate of one per path.
-            const double cap = double(std::min<size_t>(dev.guide->trainCap,
-                                                       size_t(dev.tune.guideBatch) * size_t(dev.tune.guideSteps)));
-            const double expected = dev.guide->recordsPerPath > 0 ? dev.guide->recordsPerPath * double(passes) * npix
-                                                                  : double(passes) * npix;
-            p.trainKeep = float(std::min(1.0, cap / std::max(1.0, expected)));
+            GuideState &g = *dev.guide;
+            PG_CHECK(hipMemsetAsync(g.trainCount.p, 0, 2 * sizeof(int), dev.stream));
+            // The records live in a ring of trainCap records that keeps the last batches' records
+            // (guide_replay): a batch adds up to a quarter of the ring, or the whole of what the
+            // training steps use when there is no replay. A uniform subsample keeps about that
+            // many, from the records the last batch produced per path (for the first, an estimate
+            // of one per path).
+            const size_t use = size_t(dev.tune.guideBatch) * size_t(dev.tune.guideSteps);
+            g.batchAdd = dev.tune.guideReplay > 0 ? std::max<size_t>(1, g.trainCap / 4) : std::min(use, g.trainCap);
+            if (dev.tune.guideReplay == 0) g.ringHead = 0, g.ringValid = 0;
+            const double expected = g.recordsPerPath > 0 ? g.recordsPerPath * double(passes) * npix : double(passes) * npix;
+            p.trainKeep = float(std::min(1.0, double(g.batchAdd) / std::max(1.0, expected)));
         }
         TraceBatch(dev, s, ds, p, passes, npix, sw);
         Bufs b = MakeBufs(pt);
@@ -1858,20 +1892,29 @@ void RenderExImpl(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::Render
         if (p.train) {
             // One Adam step per slice of guideBatch records of the batch (at most guideSteps), and at
             // least guideMinSteps steps, cycling over the slices, when the batch has records.
+            GuideState &g = *dev.guide;
             int counts[2] = {0, 0};  // records written, records produced
-            PG_CHECK(hipMemcpyAsync(counts, dev.guide->trainCount.p, 2 * sizeof(int), hipMemcpyDeviceToHost, dev.stream));
+            PG_CHECK(hipMemcpyAsync(counts, g.trainCount.p, 2 * sizeof(int), hipMemcpyDeviceToHost, dev.stream));
             PG_CHECK(hipStreamSynchronize(dev.stream));
-            const int records = counts[0];
-            dev.guide->recordsPerPath = double(counts[1]) / double(std::max(1, passes * npix));
+            const size_t added = std::min<size_t>(size_t(std::max(0, counts[0])), g.batchAdd);
+            g.recordsPerPath = double(counts[1]) / double(std::max(1, passes * npix));
+            g.ringHead = (g.ringHead + added) % std::max<size_t>(1, g.trainCap);
+            g.ringValid = std::min(g.trainCap, g.ringValid + added);
+            // Adam steps on slices of guideBatch records of the ring: one per slice of the batch's
+            // records (at most guideSteps) and at least guideMinSteps, the slices rotating over the
+            // whole ring from call to call.
             const Tuning &t = dev.tune;
-            const int cap = int(std::min<size_t>(dev.guide->trainCap, size_t(t.guideBatch) * size_t(t.guideSteps)));
-            const int slices = std::min(t.guideSteps, (std::min(records, cap) + t.guideBatch - 1) / t.guideBatch);
+            const int valid = int(g.ringValid);
+            const int slices = std::min(t.guideSteps, (int(std::min<size_t>(added, g.trainCap)) + t.guideBatch - 1) / t.guideBatch);
             const int steps = slices > 0 ? std::max(slices, t.guideMinSteps) : 0;
+            const int ringSlices = std::max(1, valid / std::max(1, t.guideBatch));
             if (steps > 0)
                 timer.Run(prgpu::KernGuide, steps, nullptr, [&] {
-                    for (int k = 0; k < steps; ++k)
-                        dev.guide->Step(dev.guide->train.as<float>(), dev.guide->trainCount.as<int>(),
-                                        (k % slices) * t.guideBatch, t.guideBatch, t.guideLr, dev.stream);
+                    for (int k = 0; k < steps; ++k) {
+                        const int first = int((g.rotor++ % size_t(ringSlices)) * size_t(t.guideBatch));
+                        g.Step(g.train.as<float>(), nullptr, first, std::min(t.guideBatch, valid - first), t.guideLr,
+                               dev.stream);
+                    }
                 });
         }
         if (dev.profileMode) {

@@ -278,6 +278,123 @@ PG_HD float planck(float lambdaNm, float T) {
     return float((2 * h * c * c) / (pow(l, 5.0) * (exp((h * c) / (l * kb * T)) - 1)));
 }
 
+// ---- image environment light (record layout: prgpu::kEnvHeader) ----------------------------------
+// The map's lookup (src/lights/light.cpp:249-261, light.h:287-297), the RGB-to-spectrum table
+// lookup (src/core/rgb2spec.cpp:278-304) and the piecewise-constant 2D distribution over the map
+// (src/core/sampling.cpp:24-67), as float operations in the reference's order.
+struct EnvMap {
+    const float *hdr;
+    int w, h;
+    const float *rgb, *func, *cond, *rowInt, *marg;
+};
+PG_HD EnvMap env_map(const float *rec) {
+    EnvMap e;
+    e.hdr = rec;
+    e.w = int(rec[0]);
+    e.h = int(rec[1]);
+    e.rgb = rec + prgpu::kEnvHeader;
+    e.func = e.rgb + 3 * size_t(e.w) * size_t(e.h);
+    e.cond = e.func + size_t(e.w) * size_t(e.h);
+    e.rowInt = e.cond + size_t(e.h) * size_t(e.w + 1);
+    e.marg = e.rowInt + e.h;
+    return e;
+}
+// World direction to the map's frame (Rotate(r, +y) inverted) and back.
+PG_HD V3 env_to_light(const EnvMap &e, V3 w) { return V3(e.hdr[2] * w.x - e.hdr[3] * w.z, w.y, e.hdr[3] * w.x + e.hdr[2] * w.z); }
+PG_HD V3 env_to_world(const EnvMap &e, V3 w) { return V3(e.hdr[2] * w.x + e.hdr[3] * w.z, w.y, -e.hdr[3] * w.x + e.hdr[2] * w.z); }
+// DirToEquirect of a unit direction: (phi / 2 pi, theta / pi), theta from +y.
+PG_HD void dir_to_equirect(V3 d, float *u, float *v) {
+    float theta = safe_acos(d.y);
+    float phi = atan2f(d.z, d.x);
+    if (phi < 0) phi += 2 * kPi;
+    *u = phi * kInv2Pi;
+    *v = theta * kInvPi;
+}
+PG_HD V3 equirect_to_dir(float u, float v) {
+    float theta = v * kPi, phi = u * 2 * kPi;
+    float s = sinf(theta);
+    return V3(s * cosf(phi), cosf(theta), s * sinf(phi));
+}
+// Bilinear RGB with horizontal wrap and vertical clamp at map coordinates (u, v).
+PG_HD void env_rgb(const EnvMap &e, float u, float v, float out[3]) {
+    float x = u * float(e.w) - 0.5f, y = v * float(e.h) - 0.5f;
+    int x0 = int(floorf(x)), y0 = int(floorf(y));
+    float dx = x - float(x0), dy = y - float(y0);
+    auto tex = [&](int xi, int yi) {
+        xi = ((xi % e.w) + e.w) % e.w;
+        yi = clampi(yi, 0, e.h - 1);
+        return e.rgb + 3 * (size_t(yi) * size_t(e.w) + size_t(xi));
+    };
+    const float *a = tex(x0, y0), *b = tex(x0 + 1, y0), *c = tex(x0, y0 + 1), *d = tex(x0 + 1, y0 + 1);
+    for (int i = 0; i < 3; ++i)
+        out[i] = (1 - dx) * (1 - dy) * a[i] + dx * (1 - dy) * b[i] + (1 - dx) * dy * c[i] + dx * dy * d[i];
+}
+// Sigmoid-polynomial coefficients of an RGB in [0, 1]^3 from the table record (resolution n, z
+// nodes, coefficients [max component][z][y][x][3]).
+PG_HD void rgb_sigmoid_coeffs(const float *table, const float rgbIn[3], float c[3]) {
+    float rgb[3] = {clampf(rgbIn[0], 0.f, 1.f), clampf(rgbIn[1], 0.f, 1.f), clampf(rgbIn[2], 0.f, 1.f)};
+    if (rgb[0] == rgb[1] && rgb[1] == rgb[2]) {
+        float v = rgb[0];
+        c[0] = c[1] = 0;
+        c[2] = v <= 0 ? -__builtin_huge_valf() : (v >= 1 ? __builtin_huge_valf() : (v - .5f) / sqrtf(v * (1 - v)));
+        return;
+    }
+    const int n = int(table[0]);
+    const float *zn = table + 1, *co = zn + n;
+    int maxc = (rgb[0] > rgb[1]) ? ((rgb[0] > rgb[2]) ? 0 : 2) : ((rgb[1] > rgb[2]) ? 1 : 2);
+    float z = rgb[maxc];
+    float x = rgb[(maxc + 1) % 3] * float(n - 1) / z;
+    float y = rgb[(maxc + 2) % 3] * float(n - 1) / z;
+    int xi = int(x) < n - 2 ? int(x) : n - 2, yi = int(y) < n - 2 ? int(y) : n - 2;
+    int first = 1, len = n - 2;  // FindInterval(n, zNodes[i] < z)
+    while (len > 0) {
+        int half = len >> 1, middle = first + half;
+        bool r = zn[middle] < z;
+        first = r ? middle + 1 : first;
+        len = r ? len - (half + 1) : half;
+    }
+    int zi = clampi(first - 1, 0, n - 2);
+    float dx = x - float(xi), dy = y - float(yi), dz = (z - zn[zi]) / (zn[zi + 1] - zn[zi]);
+    auto at = [&](int zz, int yy, int xx, int i) {
+        return co[3 * (((size_t(maxc) * size_t(n) + size_t(zz)) * size_t(n) + size_t(yy)) * size_t(n) + size_t(xx)) + i];
+    };
+    for (int i = 0; i < 3; ++i)
+        c[i] = lerp(dz, lerp(dy, lerp(dx, at(zi, yi, xi, i), at(zi, yi, xi + 1, i)),
+                             lerp(dx, at(zi, yi + 1, xi, i), at(zi, yi + 1, xi + 1, i))),
+                    lerp(dy, lerp(dx, at(zi + 1, yi, xi, i), at(zi + 1, yi, xi + 1, i)),
+                         lerp(dx, at(zi + 1, yi + 1, xi, i), at(zi + 1, yi + 1, xi + 1, i))));
+}
+// The sigmoid polynomial at wavelength l (RGBSigmoidPolynomial, src/core/rgb2spec.h:13-27).
+PG_HD float sigmoid_poly(const float c[3], float l) {
+    float x = fmaf(l, fmaf(l, c[0], c[1]), c[2]);
+    if (is_inf(x)) return x > 0 ? 1.f : 0.f;
+    return 0.5f + x / (2 * sqrtf(1 + x * x));
+}
+// Distribution1D::SampleContinuous over n cells with its CDF (n + 1 entries) and integral.
+PG_HD float env_sample_cells(const float *f, const float *cdf, int n, float integral, float u, float *pdf, int *cell) {
+    int o = ies_interval(cdf, n + 1, u);
+    float du = u - cdf[o];
+    if (cdf[o + 1] - cdf[o] > 0) du /= (cdf[o + 1] - cdf[o]);
+    *pdf = integral > 0 ? f[o] / integral : 1.f;
+    *cell = o;
+    return minf((float(o) + du) / float(n), kOneMinusEps);
+}
+// Distribution2D::SampleContinuous: the row by the marginal (u1), then the column (u0).
+PG_HD void env_sample_uv(const EnvMap &e, float u0, float u1, float *u, float *v, float *pdf) {
+    float pv, pu;
+    int row, col;
+    *v = env_sample_cells(e.rowInt, e.marg, e.h, e.hdr[4], u1, &pv, &row);
+    *u = env_sample_cells(e.func + size_t(row) * size_t(e.w), e.cond + size_t(row) * size_t(e.w + 1), e.w, e.rowInt[row],
+                          u0, &pu, &col);
+    *pdf = pu * pv;
+}
+// Distribution2D::PDF at map coordinates (u, v).
+PG_HD float env_pdf_uv(const EnvMap &e, float u, float v) {
+    int iu = clampi(int(u * float(e.w)), 0, e.w - 1), iv = clampi(int(v * float(e.h)), 0, e.h - 1);
+    if (e.hdr[4] == 0) return 0;
+    return e.func[size_t(iv) * size_t(e.w) + size_t(iu)] / e.hdr[4];
+}
+
 struct SpecTable {
     const prgpu::SpectrumEx *s;
     const float *knots;

@@ -1191,6 +1191,32 @@ struct LightSample {
     V3 pl, nl;  // point on the light (and its normal, zero for non-surfaces)
 };
 
+// Radiance of an image environment in world direction w (EnvironmentLight::Lookup,
+// src/lights/light.cpp:249-261): the bilinear map value times the light's scale, as an RGB
+// illuminant (RGBIlluminantSample, src/core/rgb2spec.cpp:322-343): 2 m sigmoid(rgb / (2 m)) D65.
+PG_HD SS env_image_L(const DevScene &s, const prgpu::LightEx &L, V3 w, const Lambda &lam) {
+    const EnvMap e = env_map(s.spec.knots + L.profileOffset);
+    float u, v, rgb[3];
+    dir_to_equirect(normalize(env_to_light(e, w)), &u, &v);
+    env_rgb(e, u, v, rgb);
+    for (int i = 0; i < 3; ++i) rgb[i] = rgb[i] * L.scale;
+    const float m = maxf(0.f, maxf(rgb[0], maxf(rgb[1], rgb[2])));
+    if (m == 0) return SS(0.f);
+    const float scale = 2 * m, inv = 1 / scale;
+    const float q[3] = {rgb[0] * inv, rgb[1] * inv, rgb[2] * inv};
+    float c[3];
+    rgb_sigmoid_coeffs(s.spec.knots + int(e.hdr[7]), q, c);
+    const SS d65 = spec_sample(s.spec, int(e.hdr[6]), lam);
+    SS out;
+    for (int i = 0; i < 4; ++i) out.v[i] = scale * sigmoid_poly(c, lam.l[i]) * d65.v[i];
+    return out;
+}
+PG_HD bool env_is_image(const prgpu::LightEx &L) { return L.kind == prgpu::LightExEnvironment && L.profileSize > 0; }
+// Radiance an escaping ray in direction w receives from environment light L.
+PG_HD SS env_L(const DevScene &s, const prgpu::LightEx &L, V3 w, const Lambda &lam) {
+    return env_is_image(L) ? env_image_L(s, L, w, lam) : spec_sample(s.spec, L.spectrum, lam) * L.scale;
+}
+
 PG_HD bool sample_li(const DevScene &s, int li, V3 p, V2 u, const Lambda &lam, LightSample *ls) {
     const prgpu::LightEx &L = s.lights[li];
     if (L.kind == prgpu::LightExArea) {
@@ -1244,6 +1270,17 @@ PG_HD bool sample_li(const DevScene &s, int li, V3 p, V2 u, const Lambda &lam, L
         *ls = {spec_sample(s.spec, L.spectrum, lam) * L.scale, w, 1, p + w * (2 * s.worldR), V3()};
         return true;
     }
+    if (env_is_image(L)) {  // EnvironmentLight::SampleLi (src/lights/light.cpp:267-289)
+        const EnvMap e = env_map(s.spec.knots + L.profileOffset);
+        float mu, mv, mapPdf;
+        env_sample_uv(e, u.x, u.y, &mu, &mv, &mapPdf);
+        if (mapPdf == 0) return false;
+        float sinTheta = sinf(mv * kPi);
+        if (sinTheta == 0) return false;
+        V3 wi = env_to_world(e, equirect_to_dir(mu, mv));
+        *ls = {env_image_L(s, L, wi, lam), wi, mapPdf / (2 * kPi * kPi * sinTheta), p + wi * (2 * s.worldR), V3()};
+        return true;
+    }
     V3 wi = uniform_sphere(u);
     *ls = {spec_sample(s.spec, L.spectrum, lam) * L.scale, wi, kInv4Pi, p + wi * (2 * s.worldR), V3()};
     return true;
@@ -1252,6 +1289,14 @@ PG_HD bool sample_li(const DevScene &s, int li, V3 p, V2 u, const Lambda &lam, L
 PG_HD float pdf_li(const DevScene &s, int li, V3 p, V3 wi) {
     const prgpu::LightEx &L = s.lights[li];
     if (L.kind == prgpu::LightExArea) return shape_pdf_from(s.prims[L.primitive], s.lightArea[li], p, wi);
+    if (env_is_image(L)) {  // EnvironmentLight::PDF_Li (src/lights/light.cpp:291-297)
+        const EnvMap e = env_map(s.spec.knots + L.profileOffset);
+        float mu, mv;
+        dir_to_equirect(normalize(env_to_light(e, wi)), &mu, &mv);
+        float sinTheta = sinf(mv * kPi);
+        if (sinTheta == 0) return 0;
+        return env_pdf_uv(e, mu, mv) / (2 * kPi * kPi * sinTheta);
+    }
     if (L.kind == prgpu::LightExEnvironment) return kInv4Pi;
     if (L.kind == prgpu::LightExSun)  // sky.cpp:107-109
         return dot(normalize(wi), f3(L.direction)) >= L.cosFalloffEnd ? 1 / (2 * kPi * (1 - L.cosFalloffEnd)) : 0.f;

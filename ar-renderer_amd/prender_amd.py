@@ -18,7 +18,7 @@ PLUGIN_PATH = os.path.join(HERE, "libprender_gpu.so")
 HOST_PATH = os.path.join(HERE, "libprender_host.so")
 
 API_VERSION = 4
-EXT_VERSION = 2
+EXT_VERSION = 3
 
 # ---- enums (include/prender_gpu.h) ---------------------------------------------------------------
 PRIM_TRIANGLE, PRIM_SPHERE = 0, 1
@@ -548,7 +548,8 @@ class Plugin:
         return hits
 
     TUNE_KEYS = ("trace_blocks_per_cu", "shade_blocks_per_cu", "path_budget", "guide_batch", "guide_steps",
-                 "guide_min_steps", "guide_lr", "guide_bounces", "guide_blend", "graph_rounds")
+                 "guide_min_steps", "guide_lr", "guide_bounces", "guide_blend", "graph_rounds",
+                 "guide_replay")
 
     def tune_set(self, key, value, device=0):
         self.lib.prgpu_tune_set.argtypes = [C.c_int, C.c_char_p, C.c_double]

@@ -165,7 +165,7 @@ struct DeviceInfo {  // gpu_api.h:156-162
 // ---------------------------------------------------------------------------------------------
 // Part 2: extension (no counterpart in the reference ABI; see the file comment).
 
-constexpr int kExtVersion = 2;
+constexpr int kExtVersion = 3;
 
 // Analytic spectrum: value(lambda) = scale * base(lambda). Mirrors the reference Spectrum
 // classes (src/core/spectrum.h:151-231, src/core/rgb2spec.h:13-74) so the device evaluates
@@ -204,6 +204,20 @@ enum LightKindEx : uint32_t {
 // angles, nH horizontal angles (degrees, ascending) and nH x nV relative candela values in [0, 1]
 // (row h holds the vertical profile at horizontal angle h). profileSize = 12 + nV + nH + nV nH.
 constexpr int kIesHeader = 12;
+
+// Record of an image environment light in SceneDescEx::knots, at LightEx::profileOffset
+// (profileSize floats; a LightExEnvironment whose profileSize is 0 is a constant environment with
+// its spectrum): [0] width w, [1] height h, [2] cos r and [3] sin r of its rotation r about +y
+// (a world direction d is looked up at (cos r dx - sin r dz, dy, sin r dx + cos r dz)), [4] the
+// integral of the sampling function, [5] the map's sin-weighted mean luminance, [6] the index of
+// the D65 spectrum (unit luminance) in spectra, [7] the knots offset of the RGB-to-spectrum table
+// record; then w x h linear sRGB texels (rows from the zenith down), the sampling function
+// luminance x sin(theta) + 1e-6 (w x h), per row its conditional CDF (w + 1), the row integrals
+// (h) and the marginal CDF (h + 1).
+constexpr int kEnvHeader = 8;
+// RGB-to-spectrum table record: [0] the resolution n = kRgbTableRes, n z nodes, then the sigmoid
+// polynomial coefficients (3 per node) of 3 x n x n x n nodes, [max component][z][y][x].
+constexpr int kRgbTableRes = 64;
 
 struct LightEx {  // 64 bytes
     uint32_t kind;
@@ -380,7 +394,8 @@ PRGPU_API int prgpu_profile_read(int deviceIndex, prgpu::ProfileStatsEx *out, in
 // queue capacity), guide_batch (records per Adam step), guide_steps and guide_min_steps (Adam
 // steps per batch of paths), guide_lr, guide_bounces (bounces that sample the guide, 0 = all),
 // guide_blend (weight of the MIS-proposal term in the training loss, the rest fits the mixture),
-// graph_rounds (bounce rounds replayed per captured HIP graph, even, 0 = eager launches).
+// graph_rounds (bounce rounds replayed per captured HIP graph, even, 0 = eager launches),
+// guide_replay (training records kept across batches in a ring, 0 = train on each batch's own).
 // Defaults come from the tuning file the autotuner writes ($PRGPU_TUNING_FILE, else
 // prgpu_tuning.txt beside the library: "key value" lines), read when devices are first
 // enumerated. set / get return 0, or 1 for an unknown key, a value out of range or no device.
@@ -406,10 +421,11 @@ PRGPU_API int prgpu_guide_eval(int deviceIndex, const float *records, int n, flo
 // lr) on the records and returns the stats of the last step.
 PRGPU_API int prgpu_guide_train(int deviceIndex, const float *records, int n, int steps, float lr, float *grad,
                                 float *stats);
-// The training records (16 floats each, as prgpu_guide_train takes them) that the last batch of
-// a training render kept: copies at most max into records, returns the count copied (-1 when the
-// device has no model) and sets *produced (may be null) to the records the batch produced before
-// the uniform subsample kept about guide_batch x guide_steps of them.
+// The training records (16 floats each, as prgpu_guide_train takes them) that the device's
+// record ring holds (the records its training renders kept, at most guide_replay of them or, with
+// no replay, the last batch's): copies at most max into records, returns the count copied (-1
+// when the device has no model) and sets *produced (may be null) to the records the last batch
+// produced before its uniform subsample.
 PRGPU_API int prgpu_guide_records(int deviceIndex, float *records, int max, int *produced);
 // State of the device's model: stats = {summed weight, summed weighted loss} of its last training
 // step, *steps = Adam steps taken since it was created or reset. 1 when no model exists.

@@ -1474,9 +1474,12 @@ struct LightSample {
 
 struct Scene;
 
+struct EnvImage;
+
 struct LightObj {
     LightEx d;
     float area = 0;
+    std::shared_ptr<EnvImage> env;  // image environment (d.profileSize > 0)
     bool delta() const {
         return d.kind == prgpu::LightExPoint || d.kind == prgpu::LightExDistant || d.kind == prgpu::LightExSpot ||
                d.kind == prgpu::LightExGoniometric;
@@ -1658,6 +1661,147 @@ struct Distrib1D {
         *p = pmf(off);
         return off;
     }
+    float sample_continuous(float u, float *pdf, int *offOut) const {  // sampling.cpp:24-32
+        int n = int(func.size());
+        int off = find_interval(int(cdf.size()), [&](int i) { return cdf[size_t(i)] <= u; });
+        *offOut = off;
+        float du = u - cdf[size_t(off)];
+        if (cdf[size_t(off) + 1] - cdf[size_t(off)] > 0) du /= (cdf[size_t(off) + 1] - cdf[size_t(off)]);
+        *pdf = funcInt > 0 ? func[size_t(off)] / funcInt : 1.f;
+        return std::min((off + du) / n, kOneMinusEps);
+    }
+};
+
+// Image environment light (src/lights/light.cpp:221-297): the map and its rotation from the
+// record, the sampling distribution rebuilt from the texels (not read from the record), the
+// RGB-to-spectrum table from its record (src/core/rgb2spec.cpp:278-304, 316-343).
+struct EnvImage {
+    int w = 0, h = 0, d65 = -1;
+    float c = 1, s = 0, avgY = 0;
+    std::vector<float> rgb, table;
+    std::vector<Distrib1D> cond;
+    Distrib1D marg;
+    void init(const float *rec, const float *knots) {
+        w = int(rec[0]);
+        h = int(rec[1]);
+        c = rec[2];
+        s = rec[3];
+        d65 = int(rec[6]);
+        rgb.assign(rec + prgpu::kEnvHeader, rec + prgpu::kEnvHeader + size_t(3) * w * h);
+        const float *t = knots + int(rec[7]);
+        const int n = int(t[0]);
+        table.assign(t, t + 1 + n + size_t(3) * n * n * n * 3);
+        std::vector<float> f(size_t(w) * h);
+        double sumY = 0, sumW = 0;
+        for (int y = 0; y < h; ++y) {  // light.cpp:226-237
+            float sinTheta = std::sin(kPi * (y + 0.5f) / h);
+            for (int x = 0; x < w; ++x) {
+                const float *px = &rgb[3 * (size_t(y) * w + x)];
+                float lum = 0.2126f * px[0] + 0.7152f * px[1] + 0.0722f * px[2];
+                f[size_t(y) * w + x] = std::max(lum, 0.f) * sinTheta + 1e-6f;
+                sumY += lum * sinTheta;
+                sumW += sinTheta;
+            }
+        }
+        avgY = float(sumY / std::max(1e-9, sumW));
+        cond.resize(size_t(h));
+        std::vector<float> m(static_cast<size_t>(h), 0.f);
+        for (int y = 0; y < h; ++y) {  // Distribution2D (sampling.cpp:44-50)
+            cond[size_t(y)].init(std::vector<float>(f.begin() + size_t(y) * w, f.begin() + size_t(y + 1) * w));
+            m[size_t(y)] = cond[size_t(y)].funcInt;
+        }
+        marg.init(m);
+    }
+    V3 to_light(V3 v) const { return V3(c * v.x + 0.f * v.y + (-s) * v.z, v.y, s * v.x + 0.f * v.y + c * v.z); }
+    V3 to_world(V3 v) const { return V3(c * v.x + 0.f * v.y + s * v.z, v.y, (-s) * v.x + 0.f * v.y + c * v.z); }
+    static void dir_to_equirect(V3 d, float *u, float *v) {  // light.h:292-297
+        float theta = safe_acos(d.y);
+        float phi = std::atan2(d.z, d.x);
+        if (phi < 0) phi += 2 * kPi;
+        *u = phi * kInv2Pi;
+        *v = theta * kInvPi;
+    }
+    static V3 equirect_to_dir(float u, float v) {  // light.h:287-291
+        float theta = v * kPi, phi = u * 2 * kPi;
+        float st = std::sin(theta);
+        return V3(st * std::cos(phi), std::cos(theta), st * std::sin(phi));
+    }
+    void coeffs(float r, float g, float b, float co[3]) const {  // RGBToSpectrumTable::operator()
+        float rgbc[3] = {clampv(r, 0.f, 1.f), clampv(g, 0.f, 1.f), clampv(b, 0.f, 1.f)};
+        if (rgbc[0] == rgbc[1] && rgbc[1] == rgbc[2]) {
+            float v = rgbc[0];
+            co[0] = co[1] = 0;
+            co[2] = v <= 0 ? -std::numeric_limits<float>::infinity()
+                           : (v >= 1 ? std::numeric_limits<float>::infinity() : (v - .5f) / std::sqrt(v * (1 - v)));
+            return;
+        }
+        const int n = int(table[0]);
+        const float *zNodes = table.data() + 1, *cf = zNodes + n;
+        int maxc = (rgbc[0] > rgbc[1]) ? ((rgbc[0] > rgbc[2]) ? 0 : 2) : ((rgbc[1] > rgbc[2]) ? 1 : 2);
+        float z = rgbc[maxc];
+        float x = rgbc[(maxc + 1) % 3] * (n - 1) / z;
+        float y = rgbc[(maxc + 2) % 3] * (n - 1) / z;
+        int xi = std::min(int(x), n - 2), yi = std::min(int(y), n - 2);
+        int zi = find_interval(n, [&](int i) { return zNodes[i] < z; });
+        float dx = x - xi, dy = y - yi, dz = (z - zNodes[zi]) / (zNodes[zi + 1] - zNodes[zi]);
+        auto at = [&](int zz, int yy, int xx, int i) {
+            return cf[3 * (((size_t(maxc) * n + zz) * n + yy) * n + xx) + i];
+        };
+        for (int i = 0; i < 3; ++i)
+            co[i] = lerpf(dz,
+                          lerpf(dy, lerpf(dx, at(zi, yi, xi, i), at(zi, yi, xi + 1, i)),
+                                lerpf(dx, at(zi, yi + 1, xi, i), at(zi, yi + 1, xi + 1, i))),
+                          lerpf(dy, lerpf(dx, at(zi + 1, yi, xi, i), at(zi + 1, yi, xi + 1, i)),
+                                lerpf(dx, at(zi + 1, yi + 1, xi, i), at(zi + 1, yi + 1, xi + 1, i))));
+    }
+    SS lookup(const SpectrumTable &spec, float scale, V3 wWorld, const Lambda &lam) const {  // light.cpp:249-261
+        float u, v;
+        dir_to_equirect(normalize(to_light(wWorld)), &u, &v);
+        float x = u * w - 0.5f, y = v * h - 0.5f;
+        int x0 = int(std::floor(x)), y0 = int(std::floor(y));
+        float dx = x - x0, dy = y - y0;
+        auto tex = [&](int xi, int yi) {
+            xi = ((xi % w) + w) % w;
+            yi = clampv(yi, 0, h - 1);
+            return &rgb[3 * (size_t(yi) * w + xi)];
+        };
+        const float *a = tex(x0, y0), *b = tex(x0 + 1, y0), *cc = tex(x0, y0 + 1), *d = tex(x0 + 1, y0 + 1);
+        float px[3];
+        for (int i = 0; i < 3; ++i)
+            px[i] = (1 - dx) * (1 - dy) * a[i] + dx * (1 - dy) * b[i] + (1 - dx) * dy * cc[i] + dx * dy * d[i];
+        for (int i = 0; i < 3; ++i) px[i] = px[i] * scale;
+        // RGBIlluminantSample = RGBUnboundedSample x D65 (rgb2spec.cpp:326-343)
+        float m = std::max(0.f, std::max(px[0], std::max(px[1], px[2])));
+        if (m == 0) return SS(0.f);
+        float sc = 2 * m, inv = 1 / sc, co[3];
+        coeffs(px[0] * inv, px[1] * inv, px[2] * inv, co);
+        SS out;
+        for (int i = 0; i < 4; ++i)
+            out.v[i] = sc * sigmoid_s(std::fma(lam.l[i], std::fma(lam.l[i], co[0], co[1]), co[2])) * spec.eval(d65, lam.l[i]);
+        return out;
+    }
+    bool sample(V2 u, V3 *wi, float *pdf) const {  // light.cpp:272-280, Distribution2D::SampleContinuous
+        float pdfs[2];
+        int row, col;
+        float d1 = marg.sample_continuous(u.y, &pdfs[1], &row);
+        float d0 = cond[size_t(row)].sample_continuous(u.x, &pdfs[0], &col);
+        float mapPdf = pdfs[0] * pdfs[1];
+        if (mapPdf == 0) return false;
+        float sinTheta = std::sin(d1 * kPi);
+        if (sinTheta == 0) return false;
+        *wi = to_world(equirect_to_dir(d0, d1));
+        *pdf = mapPdf / (2 * kPi * kPi * sinTheta);
+        return true;
+    }
+    float pdf(V3 wi) const {  // light.cpp:291-297, Distribution2D::PDF
+        float u, v;
+        dir_to_equirect(normalize(to_light(wi)), &u, &v);
+        float sinTheta = std::sin(v * kPi);
+        if (sinTheta == 0) return 0;
+        int iu = clampv(int(u * w), 0, w - 1), iv = clampv(int(v * h), 0, h - 1);
+        if (marg.funcInt == 0) return 0;
+        return cond[size_t(iv)].func[size_t(iu)] / marg.funcInt / (2 * kPi * kPi * sinTheta);
+    }
 };
 
 // =============================================================================================
@@ -1835,7 +1979,9 @@ struct Scene {
         case prgpu::LightExSun:  // sky.cpp:88-90
             return L.d.scale * spec.to_y(L.d.spectrum) * 2 * kPi * (1 - L.d.cosFalloffEnd) * kPi * worldRadius *
                    worldRadius;
-        default: return 4 * kPi * kPi * worldRadius * worldRadius * L.d.scale * spec.to_y(L.d.spectrum);
+        default:  // environment (light.cpp:241): the map's mean luminance, or the constant's
+            return 4 * kPi * kPi * worldRadius * worldRadius * L.d.scale *
+                   (L.env ? L.env->avgY : spec.to_y(L.d.spectrum));
         }
     }
     SS area_L(const LightObj &L, V3 n, V3 w, const Lambda &lam) const {  // light.cpp:20-25
@@ -1921,7 +2067,14 @@ struct Scene {
             ls.pLight.p = p + w * (2 * worldRadius);
             return ls;
         }
-        default: {  // constant environment (light.cpp:258-277)
+        default: {  // environment (light.cpp:258-289)
+            if (L.env) {
+                LightSample ls;
+                if (!L.env->sample(u, &ls.wi, &ls.pdf)) return std::nullopt;
+                ls.L = L.env->lookup(spec, L.d.scale, ls.wi, lam);
+                ls.pLight.p = p + ls.wi * (2 * worldRadius);
+                return ls;
+            }
             V3 wi = sample_uniform_sphere(u);
             LightSample ls;
             ls.wi = wi;
@@ -1935,7 +2088,7 @@ struct Scene {
     float pdf_li(int li, V3 p, V3 wi) const {
         const LightObj &L = lights[size_t(li)];
         if (L.d.kind == prgpu::LightExArea) return shapes[size_t(L.d.primitive)].pdf_from(p, wi);
-        if (L.d.kind == prgpu::LightExEnvironment) return kInv4Pi;
+        if (L.d.kind == prgpu::LightExEnvironment) return L.env ? L.env->pdf(wi) : kInv4Pi;
         if (L.d.kind == prgpu::LightExSun)  // sky.cpp:107-109
             return dot(normalize(wi), v3(L.d.direction)) >= L.d.cosFalloffEnd ? 1 / (2 * kPi * (1 - L.d.cosFalloffEnd))
                                                                                : 0.f;
@@ -2362,9 +2515,12 @@ static SS li(const Scene &sc, const Settings &st, Ray ray, Lambda &lam, Sampler 
         if (!hit) {
             for (size_t i = 0; i < sc.lights.size(); ++i) {
                 const LightObj &Li = sc.lights[i];
-                if (Li.d.kind == prgpu::LightExEnvironment ||
-                    (Li.d.kind == prgpu::LightExSun &&  // SunLight::Le (sky.cpp:92-95)
-                     dot(normalize(ray.d), v3(Li.d.direction)) >= Li.d.cosFalloffEnd))
+                if (Li.d.kind == prgpu::LightExEnvironment)  // EnvironmentLight::Le (light.cpp:263-265)
+                    add_emitted(int(i), Li.env ? Li.env->lookup(sc.spec, Li.d.scale, ray.d, lam)
+                                               : sc.spec.sample(Li.d.spectrum, lam) * Li.d.scale,
+                                ray.d);
+                else if (Li.d.kind == prgpu::LightExSun &&  // SunLight::Le (sky.cpp:92-95)
+                         dot(normalize(ray.d), v3(Li.d.direction)) >= Li.d.cosFalloffEnd)
                     add_emitted(int(i), sc.spec.sample(Li.d.spectrum, lam) * Li.d.scale, ray.d);
             }
             break;
@@ -2484,6 +2640,10 @@ static Scene *build_scene(const prgpu::SceneDescEx *d) {
         LightObj L;
         L.d = d->lights[i];
         if (L.d.kind == prgpu::LightExArea) L.area = sc->shapes[size_t(L.d.primitive)].area();
+        if (L.d.kind == prgpu::LightExEnvironment && L.d.profileSize > 0) {
+            L.env = std::make_shared<EnvImage>();
+            L.env->init(d->knots + L.d.profileOffset, d->knots);
+        }
         sc->lights.push_back(L);
     }
     // Preprocess of distant/environment lights uses the same bounds: radius = max(r, 1e-3).

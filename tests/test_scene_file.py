@@ -389,7 +389,8 @@ def test_sun_light_and_its_spectrum_match_reference(host, tmp_path):
     0.2665 deg x sun_size, scale intensity x sun_scale, no light once the sun is 0.02 below the
     horizon; its radiance spectrum against the reference's MakeSunRadiance (sky.cpp:207-233: air
     mass, Rayleigh and aerosol depth, blackbody at 2e6 cd/m^2 scaled with the disk's size), golden
-    vectors from the reference's own code. A sun-sky light also warns that its sky is skipped."""
+    vectors from the reference's own code. A sun-sky light adds its Preetham sky as an image
+    environment of resolution x resolution / 2 (scene_loader.cpp:991-996)."""
     G = np.load(os.path.join(HERE, "golden", "ref_components.npz"))
     for (el, az, T, r), ref in zip(G["sun_cases"], G["sun_radiance"]):
         doc = {"lights": {"s": {"type": "sun", "elevation": float(el), "azimuth": float(az), "turbidity": float(T),
@@ -410,10 +411,13 @@ def test_sun_light_and_its_spectrum_match_reference(host, tmp_path):
         got = np.array([host.spectrum_value(s.spectra[sun["spectrum"]], s.knots, float(l))
                         for l in np.arange(360.0, 831.0, 5.0)])
         np.testing.assert_allclose(got, ref, rtol=2e-4, atol=1e-30, err_msg=f"sun {el} {az} {T} {r}")
-    doc["lights"]["s"].update(type="sun_sky", elevation=30.0)
+    doc["lights"]["s"].update(type="sun_sky", elevation=30.0, resolution=32)
     f.write_text(json.dumps(doc))
     s = host.load_scene(f)
-    assert len(s.lights) == 1 and any("Preetham sky" in w for w in s.warnings)
+    assert list(s.lights["kind"]) == [pa.LIGHT_SUN, pa.LIGHT_ENV]
+    sky = s.lights[1]
+    assert sky["profileSize"] > 0 and tuple(s.knots[sky["profileOffset"]:sky["profileOffset"] + 2]) == (32, 16)
+    assert not any("Preetham sky" in w for w in s.warnings)
 
 
 # ---- camera ----------------------------------------------------------------------------------------
@@ -627,9 +631,9 @@ def test_reference_sphere_pyramid(host):
     fog = host.load_scene(path, rigs=["studio", "fog"])
     assert fog.desc.cameraMedium >= 0 and len(fog.prims) == len(s.prims) + 12
     day = host.load_scene(path, rigs=["daylight"])
-    # the sun_sky rig: its sun is a light (its sky is skipped with a warning) beside the glowing ball
-    assert sorted(day.lights["kind"].tolist()) == [pa.LIGHT_AREA, pa.LIGHT_SUN]
-    assert any("Preetham sky" in w for w in day.warnings)
+    # the sun_sky rig: its sun and its Preetham sky map beside the glowing ball
+    assert sorted(day.lights["kind"].tolist()) == sorted([pa.LIGHT_AREA, pa.LIGHT_SUN, pa.LIGHT_ENV])
+    assert not any("Preetham sky" in w for w in day.warnings)
 
 
 @pytest.mark.skipif(not os.path.isdir(REF_SCENES), reason="reference scene files not present")

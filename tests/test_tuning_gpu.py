@@ -1,9 +1,10 @@
 """Run-time tuning on the MI355X: every launch shape and queue capacity renders the same film.
 
 The path of every camera sample is a pure function of (pixel, sample index, seed), so the grids of
-the traversal and shading kernels, the queue capacity (paths in flight per batch, which splits a
-call into batches) and replaying captured bounce graphs instead of launching eagerly must all give
-the film of the default settings, bit for bit. Captured graphs are replayed many times per call
+the traversal and shading kernels and replaying captured bounce graphs instead of launching eagerly
+must give the film of the default settings bit for bit, and a smaller queue capacity (paths in
+flight per batch, which splits a call into batches) the same film up to the grouping of its FP64
+sums. Captured graphs are replayed many times per call
 (one replay per graph_rounds bounces), with the eager renders of the same sample ranges between
 them, and every call is checked.
 """
@@ -48,7 +49,12 @@ def test_launch_shape_and_queue_capacity_keep_the_film(plugin, host, defaults, k
     ref = _film(plugin, s, 0, spp=16)
     plugin.tune_set(key, value)
     assert plugin.tune_get(key) == value
-    assert np.array_equal(_film(plugin, s, 0, spp=16), ref)
+    film = _film(plugin, s, 0, spp=16)
+    if key == "path_budget":
+        # 16 passes in batches of 4: the same samples, their FP64 film sums grouped per batch
+        np.testing.assert_allclose(film, ref, rtol=1e-12, atol=1e-300)
+    else:
+        assert np.array_equal(film, ref)
 
 
 def test_out_of_range_values_are_refused(plugin, defaults):
