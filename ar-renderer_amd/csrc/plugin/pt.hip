@@ -192,18 +192,22 @@ Box3 PrimBounds(const prgpu::Primitive &p) {
 // Device-resident path-tracing scene and batch buffers
 
 // Minimum waves per SIMD a kernel is compiled for (its register budget: 8 waves = 64 VGPRs,
-// 6 = 80, 5 = 96, 4 = 128); 0 leaves the choice to the compiler. The traversal is latency-bound
-// and its LDS stacks allow 8 waves per SIMD, which 64 VGPRs reach without spilling.
+// 6 = 80, 5 = 96, 4 = 128, 3 = 168); 0 leaves the choice to the compiler. The traversal is
+// latency-bound and its LDS stacks allow 8 waves per SIMD, which 64 VGPRs reach without spilling.
+// The plain material families take 175-185 VGPRs unconstrained (2 waves); measured on
+// configuration 2 at 1920x1080, 4 spp: 5.13 ms per step at 2 waves, 4.69 at 3, 4.80 at 4 (the
+// spills of a 128-VGPR budget cost more than the occupancy gains). The surface stage is fastest
+// unconstrained (92 VGPRs: 1.61 ms; 1.71 at 6 waves, 2.23 at 8).
 #define PG_WAVES(n) __attribute__((amdgpu_waves_per_eu((n) > 0 ? (n) : 1, 8)))
 #ifndef PG_SURFACE_WAVES
 #define PG_SURFACE_WAVES 0
 #endif
 #ifndef PG_MATERIAL_WAVES
-#define PG_MATERIAL_WAVES 0
+#define PG_MATERIAL_WAVES 3
 #endif
 constexpr int kSurfaceWaves = PG_SURFACE_WAVES, kMaterialWaves = PG_MATERIAL_WAVES;
 
-// Bufs::counts: queue lengths and the per-XCD ray fetch counters of the persistent traversals.
+// Bufs::counts: queue lengths and the ray fetch counters of the persistent traversals.
 constexpr int kFetchIntersect = 16, kFetchShadow = 32, kCountsInts = 48, kCountsBytes = kCountsInts * 4;
 static_assert(kFetchParts <= 16, "fetch counters overlap");
 
@@ -219,9 +223,9 @@ struct PTScene {
     // batch buffers
     size_t cap = 0;
     DevBuf o, d, rng, lam, lpdf, beta, rpath, L, Lh, prevP, prevN, st, hit, q0, q1, qm, pfilm, xyz;
-    DevBuf h0, h1, h2, h3, qe, qeLpdf;
+    DevBuf h0, h1, h2, h3, qe, qeLpdf, qes;
     DevBuf sceneBuf;  // the DevScene the kernels read through a pointer
-    DevBuf so, sd, sp, sn, sf, sL, sb, sq;
+    DevBuf so, sd, sp, sn, sf, sL, sb, sq;  // shadow records and the shadow queue
     int *countsHost = nullptr;
     DevBuf counts;
     // The captured bounce rounds (see TraceBatch) and the key of what they were captured with.
@@ -672,14 +676,16 @@ struct Bufs {
                   // [32..32+P) shadow fetch counters (P = kFetchParts)
     float2 *pfilm;
     float4 *xyz;
-    // shadow records
+    // shadow records, one per path slot (a path makes at most one shadow ray per bounce)
     float4 *so, *sd, *sp, *sn, *sf, *sL, *sb;
     // guiding: decoded mixture per diffuse-queue entry (12 float4), kGuideRecords training
     // records per path (record j: 6 float4 planes of qcap at 6 j qcap: {position, state},
     // {normal, p_bsdf}, wo, w, throughput, radiance snapshot), compacted training records (16
     // floats) and their counter
-    int4 *qe;       // emission queue: slot, depth and flags on arrival, primitive (-1: escaped)
+    int *qes;       // emission queue (path slots)
+    int4 *qe;       // per slot: slot, depth and flags on arrival at the emitter, primitive (-1: escaped)
     float4 *qeLpdf; // and the wavelength pdfs on arrival
+    int *sq;        // shadow queue (path slots; the shadow records are indexed by slot)
     const float4 *gout;
     float4 *trec;
     float *train;
@@ -702,6 +708,98 @@ __device__ __forceinline__ int wave_append(int *counter) {
     if (lane == leader) base = atomicAdd(counter, __popcll(mask));
     base = __shfl(base, leader);
     return base + prefix;
+}
+
+// ---- queue appends aggregated per tile --------------------------------------------------------
+// A device-scope atomic on one cache line completes about every 12 ns on MI355X, one after the
+// other, so one atomic per wave and queue (a launch of 880k paths: ~14k waves, up to 6 queues)
+// serializes into hundreds of microseconds (measured on configuration 2: the surface stage took
+// 5.8 ms per step with per-wave appends, 1.6 ms with per-tile ones). The shading kernels therefore
+// take their items in tiles of kAppendK items per thread: each item's queue entries (at most two)
+// get a position within the tile from LDS counters, one atomic instruction (a lane per queue; the
+// counters share a cache line) reserves every queue's range for the whole tile, and the entries
+// are then scattered. Queue order does not affect any path's result. One item per thread is the
+// fastest: larger tiles shrink the grid (material kernels 5.2, 6.6 and 9.3 ms per step at 1, 2
+// and 4 items per thread).
+#ifndef PG_APPEND_K
+#define PG_APPEND_K 1
+#endif
+constexpr int kAppendK = PG_APPEND_K;
+constexpr int kTile = kAppendK * kBlock;
+static_assert(kTile <= 2048, "tile positions are 11 bits");
+// Queue ids: the material families 0..3, then the next bounce, emission and shadow queues.
+enum QueueId : int { QNext = 4, QEmit = 5, QShadow = 6, QIds = 7 };
+constexpr uint32_t kNoQueue = 7;
+
+struct TileStage {
+    int cnt[8];             // entries of the current tile per queue
+    int base[8];            // each queue's reserved range for the tile
+    int slot[kTile];        // the item's path slot
+    uint32_t code[kTile];   // two entries of 14 bits: queue id << 11 | position in the tile's range
+};
+
+__device__ __forceinline__ void tile_begin(TileStage &t) {
+    if (threadIdx.x < 8) t.cnt[threadIdx.x] = 0;
+    __syncthreads();
+}
+
+// Position of this lane's entry among the tile's entries of queue q (q < 0: no entry). Every lane
+// of the wave calls it; the loop runs once per distinct queue in the wave.
+__device__ __forceinline__ int lds_append(int *cnt, int q) {
+    int pos = 0;
+    unsigned long long pending = __ballot(q >= 0);
+    const int lane = __lane_id();
+    while (pending) {
+        const int leader = __ffsll(pending) - 1;
+        const int qq = __shfl(q, leader);
+        const unsigned long long m = __ballot(q == qq);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&cnt[qq], __popcll(m));
+        base = __shfl(base, leader);
+        if (q == qq) pos = base + __popcll(m & ((1ull << lane) - 1ull));
+        pending &= ~m;
+    }
+    return pos;
+}
+
+__device__ __forceinline__ void tile_stage(TileStage &t, int j, int slot, int q1, int q2) {
+    const int p1 = lds_append(t.cnt, q1), p2 = lds_append(t.cnt, q2);
+    const uint32_t c1 = q1 < 0 ? (kNoQueue << 11) : (uint32_t(q1) << 11) | uint32_t(p1);
+    const uint32_t c2 = q2 < 0 ? (kNoQueue << 11) : (uint32_t(q2) << 11) | uint32_t(p2);
+    t.slot[j] = slot;
+    t.code[j] = c1 | (c2 << 16);
+}
+
+__device__ __forceinline__ int *queue_ptr(const Bufs &b, int *next, int q) {
+    return q < QNext ? b.qm + q * b.qcap : (q == QNext ? next : (q == QEmit ? b.qes : b.sq));
+}
+// Index in Bufs::counts of material family q's queue length (the layered family's is 10).
+__device__ __host__ constexpr int mat_count_index(int q) { return q == 3 ? 10 : 3 + q; }
+__device__ __forceinline__ int *count_ptr(const Bufs &b, int *nextCount, int q) {
+    return q < QNext ? b.counts + mat_count_index(q) : (q == QNext ? nextCount : (q == QEmit ? b.counts + 9 : b.counts + 2));
+}
+
+// Reserves the tile's ranges (one atomic instruction) and scatters the staged entries.
+__device__ __forceinline__ void tile_flush(TileStage &t, const Bufs &b, int *next, int *nextCount) {
+    __syncthreads();
+    if (threadIdx.x < QIds) {
+        const int q = int(threadIdx.x), c = t.cnt[q];
+        t.base[q] = c > 0 ? atomicAdd(count_ptr(b, nextCount, q), c) : 0;
+        t.cnt[q] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kAppendK; ++k) {
+        const int j = k * kBlock + int(threadIdx.x);
+        const uint32_t code = t.code[j];
+        const int slot = t.slot[j];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const uint32_t ce = (code >> (16 * e)) & 0x3fffu;
+            const int q = int(ce >> 11);
+            if (q != int(kNoQueue)) queue_ptr(b, next, q)[t.base[q] + int(ce & 0x7ffu)] = slot;
+        }
+    }
 }
 
 __device__ __forceinline__ Pcg load_rng(const Bufs &b, int slot) {
@@ -775,9 +873,9 @@ __global__ __launch_bounds__(kBlock) void k_camera(const DevScene *__restrict__ 
         for (int j = 0; j < kGuideRecords; ++j) b.trec[6 * j * size_t(b.qcap) + slot] = make_float4(0, 0, 0, 0);
 }
 
-// Closest hits of the queued paths. Persistent: each lane takes a new ray from its XCD's part of
-// the queue as soon as its previous one completes (fetch_rays_parts), so a wave's lanes stay busy
-// while ray lengths differ and neighbouring rays share one L2.
+// Closest hits of the queued paths. Persistent: each lane takes a new ray from the queue front as
+// soon as its previous one completes (fetch_rays_parts), so a wave's lanes stay busy while ray
+// lengths differ and the rays in flight stay a narrow, coherent part of the queue.
 template <bool kStats>
 __global__ __launch_bounds__(kTraceBlock) PG_WAVES(8) void k_intersect(const DevScene *__restrict__ sp, Bufs b, const int *queue,
                                                            const int *count, int *fetch, unsigned long long *counters) {
@@ -853,17 +951,18 @@ __device__ __forceinline__ bool roulette(int rr, int rrStart, PathRegs &r) {
 }
 
 // Next-event estimation up to the transmittance (path.cpp:82-111): samples a light and, when
-// the unoccluded contribution is nonzero, appends a shadow record that k_shadow completes.
+// the unoccluded contribution is nonzero, writes the path's shadow record, which k_shadow
+// completes; returns whether it did (the caller then appends the slot to the shadow queue).
 template <bool kMedia, class B = BSDF>
-__device__ __forceinline__ void sample_ld(const DevScene &s, const Bufs &b, int slot, PathRegs &r, V3 p, V3 n, V3 wo, const B *bsdf,
+__device__ __forceinline__ bool sample_ld(const DevScene &s, const Bufs &b, int slot, PathRegs &r, V3 p, V3 n, V3 wo, const B *bsdf,
                           float g, V3 ns, bool surface, int mIn, int mOut, const GuideMix *gm = nullptr) {
     float uLight = r.rng.uniform();
     V2 uL = get2(r.rng);
     float pmf;
     int li = light_sample(s, p, n, uLight, &pmf);
-    if (li < 0 || pmf == 0) return;
+    if (li < 0 || pmf == 0) return false;
     LightSample ls;
-    if (!sample_li(s, li, p, uL, r.lam, &ls) || !ls.L.nonzero() || ls.pdf == 0) return;
+    if (!sample_li(s, li, p, uL, r.lam, &ls) || !ls.L.nonzero() || ls.pdf == 0) return false;
     V3 wi = ls.wi;
     SS f;
     // A delta light takes no MIS weight: the scattering pdf is needed only for the others.
@@ -871,7 +970,7 @@ __device__ __forceinline__ void sample_ld(const DevScene &s, const Bufs &b, int 
     float scatterPdf = 0.f;
     if (bsdf) {
         f = bsdf->f(wo, wi) * absdot(wi, ns);
-        if (!f.nonzero()) return;
+        if (!f.nonzero()) return false;
         if (!delta) {
             scatterPdf = bsdf->pdf(wo, wi);
             // A guided vertex scatters with the one-sample MIS proposal: that is the pdf to weigh against.
@@ -889,7 +988,7 @@ __device__ __forceinline__ void sample_ld(const DevScene &s, const Bufs &b, int 
     V3 target = offset_origin(ls.pl, ls.nl, o - ls.pl);
     V3 d = target - o;
     int medium = kMedia ? (surface ? (dot(d, n) > 0 ? mOut : mIn) : r.medium) : -1;
-    int rec = wave_append(&b.counts[2]);
+    const int rec = slot;
     b.so[rec] = make_float4(o.x, o.y, o.z, __int_as_float(medium));
     b.sd[rec] = make_float4(d.x, d.y, d.z, __int_as_float(slot));
     b.sp[rec] = make_float4(ls.pl.x, ls.pl.y, ls.pl.z, k);
@@ -897,6 +996,7 @@ __device__ __forceinline__ void sample_ld(const DevScene &s, const Bufs &b, int 
     st4(b.sf, rec, f);
     st4(b.sL, rec, ls.L);
     st4(b.sb, rec, r.beta);
+    return true;
 }
 
 __device__ __forceinline__ void add_emitted(const DevScene &s, PathRegs &r, int li, const SS &Le, V3 dir) {  // path.cpp:143-153
@@ -918,165 +1018,177 @@ __device__ __forceinline__ void add_emitted(const DevScene &s, PathRegs &r, int 
 // primitive, or escape with an environment light) are appended to the emission queue, whose
 // MIS-weighted contributions k_emit adds before the BSDF stage changes the throughput.
 enum MatQueue : int { QDiffuse = 0, QConductor = 1, QDielectric = 2, QLayered = 3, QCount = 4 };
-// Index in Bufs::counts of each material queue's length.
-__device__ __host__ constexpr int mat_count_index(int q) { return q == QLayered ? 10 : 3 + q; }
+static_assert(int(QCount) == int(QNext), "the next-bounce queue id follows the material families");
+
+// One path's surface stage; *q1 is the queue the path continues in (a material family, QNext:
+// straight to the next bounce, -1: ended) and *q2 the emission or shadow queue (or -1).
+template <bool kMedia, bool kInterfaces>
+__device__ __forceinline__ void surface_vertex(const DevScene &s, const Params &p, const Bufs &b, int slot, int *q1, int *q2) {
+    PathRegs r;
+    float4 o = b.o[slot], d = b.d[slot];
+    r.ro = V3(o.x, o.y, o.z);
+    r.prevPdf = o.w;
+    r.rd = V3(d.x, d.y, d.z);
+    r.etaScale = d.w;
+    float4 lp = b.lpdf[slot];
+    const float4 lpIn = lp;
+    r.lam.pdf[0] = lp.x, r.lam.pdf[1] = lp.y, r.lam.pdf[2] = lp.z, r.lam.pdf[3] = lp.w;
+    int4 st = b.st[slot];
+    r.depth = st.x;
+    r.flags = st.y;
+    r.medium = st.z;
+    r.crossings = st.w;
+    const bool lam0 = r.lam.terminated();
+    // Training: the radiance gathered before this vertex closes the record of the guided
+    // vertex the path just left (its own next-event estimate included).
+    if (p.train)
+        for (int j = 0; j < kGuideRecords; ++j) {
+            const size_t base = 6 * j * size_t(b.qcap);
+            float4 t0 = b.trec[base + slot];
+            if (t0.w == 1.f) {
+                b.trec[base + 5 * size_t(b.qcap) + slot] = b.Lh[slot];
+                t0.w = 2.f;
+                b.trec[base + slot] = t0;
+            }
+        }
+    const float4 hv = b.hit[slot];
+    const int prim = __float_as_int(hv.y);
+    const bool hit = prim >= 0;
+    const float tHit = hv.x;
+    bool alive = true, toMaterial = false, done = false, emits = false;
+    int mq = 0;
+
+    if (kMedia && r.medium >= 0) {
+        // The medium vertex needs the whole path state (throughput, MIS ratios, radiance).
+        float4 l = b.lam[slot];
+        r.lam.l[0] = l.x, r.lam.l[1] = l.y, r.lam.l[2] = l.z, r.lam.l[3] = l.w;
+        r.beta = ld4(b.beta, slot);
+        r.rpath = ld4(b.rpath, slot);
+        r.L = ld4(b.L, slot);
+        r.Lh = ld4(b.Lh, slot);
+        float4 pp = b.prevP[slot], pn = b.prevN[slot];
+        r.prevP = V3(pp.x, pp.y, pp.z);
+        r.prevN = V3(pn.x, pn.y, pn.z);
+        r.rng = load_rng(b, slot);
+        DistSample ds = medium_sample(s, r.medium, r.ro, r.rd, hit ? tHit : kInfF, r.rng.uniform(), r.lam);
+        r.beta = r.beta * ds.weight;
+        r.rpath = r.rpath * ds.pdfRatio;
+        if (!r.beta.nonzero()) {
+            alive = false;
+            done = true;
+        } else if (ds.scattered) {
+            done = true;
+            if (r.depth++ >= p.maxDepth) {
+                alive = false;
+            } else {
+                V3 mp = r.ro + r.rd * ds.t;
+                V3 mwo = -r.rd;
+                float g = clampf(s.media[r.medium].g, -0.99f, 0.99f);
+                if (sample_ld<kMedia>(s, b, slot, r, mp, V3(), mwo, static_cast<const BSDF *>(nullptr), g, V3(), false, -1, -1))
+                *q2 = QShadow;
+                float phasePdf;
+                V3 wi = sample_hg(mwo, g, get2(r.rng), &phasePdf);
+                if (phasePdf == 0) {
+                    alive = false;
+                } else {
+                    r.prevPdf = phasePdf;
+                    r.prevP = mp;
+                    r.prevN = V3();
+                    r.flags &= ~kSpecularFlag;
+                    r.ro = mp;
+                    r.rd = wi;
+                    if (roulette(p.rr, p.rrStart, r)) alive = false;
+                }
+            }
+        }
+        b.rng[slot] = make_ulonglong2(r.rng.state, r.rng.inc);
+        st4(b.beta, slot, r.beta);
+        st4(b.rpath, slot, r.rpath);
+        st4(b.L, slot, r.L);
+        st4(b.Lh, slot, r.Lh);
+        b.prevP[slot] = make_float4(r.prevP.x, r.prevP.y, r.prevP.z, 0.f);
+        b.prevN[slot] = make_float4(r.prevN.x, r.prevN.y, r.prevN.z, 0.f);
+    }
+
+    const int depthIn = r.depth, flagsIn = r.flags;
+    if (!done) {
+        if (!hit) {
+            emits = s.envCount > 0;
+            alive = false;
+        } else {
+            const prgpu::Primitive &pr = s.prims[prim];
+            emits = pr.light >= 0;
+            if (kInterfaces && is_interface(s, pr)) {
+                if (++r.crossings > 1000) {
+                    alive = false;
+                } else {
+                    Surf si;
+                    interaction(pr, r.ro, r.rd, tHit, hv.z, hv.w, &si);
+                    r.medium = dot(r.rd, si.n) > 0 ? pr.mediumOutside : pr.mediumInside;
+                    r.ro = offset_origin(si.p, si.n, r.rd);
+                }
+            } else if (!has_bsdf(s, pr)) {
+                alive = false;
+            } else {
+                const prgpu::Material &m = s.mats[pr.material];
+                // A dispersive dielectric or coat keeps only the hero wavelength
+                // (material.cpp:106-110, 120-124), also when the depth limit ends the path here.
+                if (m.type == prgpu::MatDielectric && !spec_is_constant(s.spec, m.eta)) r.lam.terminate_secondary();
+                if (is_coated(m) && !spec_is_constant(s.spec, m.coatEta)) r.lam.terminate_secondary();
+                if (r.depth++ >= p.maxDepth) {
+                    alive = false;
+                } else {
+                    Surf si;
+                    interaction(pr, r.ro, r.rd, tHit, hv.z, hv.w, &si);
+                    toMaterial = true;
+                    mq = (is_coated(m) || has_sheen(m)) ? QLayered
+                         : m.type == prgpu::MatDiffuse ? QDiffuse
+                         : (m.type == prgpu::MatConductor ? QConductor : QDielectric);
+                    uint32_t media = uint32_t(pr.mediumInside & 0xffff) | (uint32_t(pr.mediumOutside & 0xffff) << 16);
+                    b.h0[slot] = make_float4(si.p.x, si.p.y, si.p.z, __int_as_float(pr.material));
+                    b.h1[slot] = make_float4(si.n.x, si.n.y, si.n.z, si.u);
+                    b.h2[slot] = make_float4(si.ns.x, si.ns.y, si.ns.z, si.v);
+                    b.h3[slot] = make_float4(si.dpdus.x, si.dpdus.y, si.dpdus.z, __uint_as_float(media));
+                }
+            }
+        }
+    }
+    if (emits) {
+        // The emission is evaluated with the wavelength pdfs and depth the path arrived with.
+        b.qe[slot] = make_int4(slot, depthIn, flagsIn, prim);
+        b.qeLpdf[slot] = lpIn;
+        *q2 = QEmit;
+    }
+
+    if (r.lam.terminated() != lam0)
+        b.lpdf[slot] = make_float4(r.lam.pdf[0], r.lam.pdf[1], r.lam.pdf[2], r.lam.pdf[3]);
+    b.st[slot] = make_int4(r.depth, r.flags, r.medium, r.crossings);
+    if (kMedia || kInterfaces) {
+        b.o[slot] = make_float4(r.ro.x, r.ro.y, r.ro.z, r.prevPdf);
+        b.d[slot] = make_float4(r.rd.x, r.rd.y, r.rd.z, r.etaScale);
+    }
+    *q1 = toMaterial ? mq : (alive ? int(QNext) : -1);
+}
 
 template <bool kMedia, bool kInterfaces>
 __global__ __launch_bounds__(kBlock) PG_WAVES(kSurfaceWaves) void k_surface(const DevScene *__restrict__ sp, Params p, Bufs b, const int *queue,
                                                     const int *count, int *nextQueue, int *nextCount) {
+    __shared__ TileStage ts;
     const DevScene &s = *sp;
     const int n = *count;
-    const int stride = int(gridDim.x * blockDim.x);
-    for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += stride) {
-        const int slot = queue[i];
-        PathRegs r;
-        float4 o = b.o[slot], d = b.d[slot];
-        r.ro = V3(o.x, o.y, o.z);
-        r.prevPdf = o.w;
-        r.rd = V3(d.x, d.y, d.z);
-        r.etaScale = d.w;
-        float4 lp = b.lpdf[slot];
-        const float4 lpIn = lp;
-        r.lam.pdf[0] = lp.x, r.lam.pdf[1] = lp.y, r.lam.pdf[2] = lp.z, r.lam.pdf[3] = lp.w;
-        int4 st = b.st[slot];
-        r.depth = st.x;
-        r.flags = st.y;
-        r.medium = st.z;
-        r.crossings = st.w;
-        const bool lam0 = r.lam.terminated();
-        // Training: the radiance gathered before this vertex closes the record of the guided
-        // vertex the path just left (its own next-event estimate included).
-        if (p.train)
-            for (int j = 0; j < kGuideRecords; ++j) {
-                const size_t base = 6 * j * size_t(b.qcap);
-                float4 t0 = b.trec[base + slot];
-                if (t0.w == 1.f) {
-                    b.trec[base + 5 * size_t(b.qcap) + slot] = b.Lh[slot];
-                    t0.w = 2.f;
-                    b.trec[base + slot] = t0;
-                }
+    tile_begin(ts);
+    for (int t0 = int(blockIdx.x) * kTile; t0 < n; t0 += int(gridDim.x) * kTile) {
+#pragma unroll 1
+        for (int k = 0; k < kAppendK; ++k) {
+            const int j = k * kBlock + int(threadIdx.x), i = t0 + j;
+            int slot = 0, q1 = -1, q2 = -1;
+            if (i < n) {
+                slot = queue[i];
+                surface_vertex<kMedia, kInterfaces>(s, p, b, slot, &q1, &q2);
             }
-        const float4 hv = b.hit[slot];
-        const int prim = __float_as_int(hv.y);
-        const bool hit = prim >= 0;
-        const float tHit = hv.x;
-        bool alive = true, toMaterial = false, done = false, emits = false;
-        int mq = 0;
-
-        if (kMedia && r.medium >= 0) {
-            // The medium vertex needs the whole path state (throughput, MIS ratios, radiance).
-            float4 l = b.lam[slot];
-            r.lam.l[0] = l.x, r.lam.l[1] = l.y, r.lam.l[2] = l.z, r.lam.l[3] = l.w;
-            r.beta = ld4(b.beta, slot);
-            r.rpath = ld4(b.rpath, slot);
-            r.L = ld4(b.L, slot);
-            r.Lh = ld4(b.Lh, slot);
-            float4 pp = b.prevP[slot], pn = b.prevN[slot];
-            r.prevP = V3(pp.x, pp.y, pp.z);
-            r.prevN = V3(pn.x, pn.y, pn.z);
-            r.rng = load_rng(b, slot);
-            DistSample ds = medium_sample(s, r.medium, r.ro, r.rd, hit ? tHit : kInfF, r.rng.uniform(), r.lam);
-            r.beta = r.beta * ds.weight;
-            r.rpath = r.rpath * ds.pdfRatio;
-            if (!r.beta.nonzero()) {
-                alive = false;
-                done = true;
-            } else if (ds.scattered) {
-                done = true;
-                if (r.depth++ >= p.maxDepth) {
-                    alive = false;
-                } else {
-                    V3 mp = r.ro + r.rd * ds.t;
-                    V3 mwo = -r.rd;
-                    float g = clampf(s.media[r.medium].g, -0.99f, 0.99f);
-                    sample_ld<kMedia>(s, b, slot, r, mp, V3(), mwo, static_cast<const BSDF *>(nullptr), g, V3(), false, -1, -1);
-                    float phasePdf;
-                    V3 wi = sample_hg(mwo, g, get2(r.rng), &phasePdf);
-                    if (phasePdf == 0) {
-                        alive = false;
-                    } else {
-                        r.prevPdf = phasePdf;
-                        r.prevP = mp;
-                        r.prevN = V3();
-                        r.flags &= ~kSpecularFlag;
-                        r.ro = mp;
-                        r.rd = wi;
-                        if (roulette(p.rr, p.rrStart, r)) alive = false;
-                    }
-                }
-            }
-            b.rng[slot] = make_ulonglong2(r.rng.state, r.rng.inc);
-            st4(b.beta, slot, r.beta);
-            st4(b.rpath, slot, r.rpath);
-            st4(b.L, slot, r.L);
-            st4(b.Lh, slot, r.Lh);
-            b.prevP[slot] = make_float4(r.prevP.x, r.prevP.y, r.prevP.z, 0.f);
-            b.prevN[slot] = make_float4(r.prevN.x, r.prevN.y, r.prevN.z, 0.f);
+            tile_stage(ts, j, slot, q1, q2);
         }
-
-        const int depthIn = r.depth, flagsIn = r.flags;
-        if (!done) {
-            if (!hit) {
-                emits = s.envCount > 0;
-                alive = false;
-            } else {
-                const prgpu::Primitive &pr = s.prims[prim];
-                emits = pr.light >= 0;
-                if (kInterfaces && is_interface(s, pr)) {
-                    if (++r.crossings > 1000) {
-                        alive = false;
-                    } else {
-                        Surf si;
-                        interaction(pr, r.ro, r.rd, tHit, hv.z, hv.w, &si);
-                        r.medium = dot(r.rd, si.n) > 0 ? pr.mediumOutside : pr.mediumInside;
-                        r.ro = offset_origin(si.p, si.n, r.rd);
-                    }
-                } else if (!has_bsdf(s, pr)) {
-                    alive = false;
-                } else {
-                    const prgpu::Material &m = s.mats[pr.material];
-                    // A dispersive dielectric or coat keeps only the hero wavelength
-                    // (material.cpp:106-110, 120-124), also when the depth limit ends the path here.
-                    if (m.type == prgpu::MatDielectric && !spec_is_constant(s.spec, m.eta)) r.lam.terminate_secondary();
-                    if (is_coated(m) && !spec_is_constant(s.spec, m.coatEta)) r.lam.terminate_secondary();
-                    if (r.depth++ >= p.maxDepth) {
-                        alive = false;
-                    } else {
-                        Surf si;
-                        interaction(pr, r.ro, r.rd, tHit, hv.z, hv.w, &si);
-                        toMaterial = true;
-                        mq = (is_coated(m) || has_sheen(m)) ? QLayered
-                             : m.type == prgpu::MatDiffuse ? QDiffuse
-                             : (m.type == prgpu::MatConductor ? QConductor : QDielectric);
-                        uint32_t media = uint32_t(pr.mediumInside & 0xffff) | (uint32_t(pr.mediumOutside & 0xffff) << 16);
-                        b.h0[slot] = make_float4(si.p.x, si.p.y, si.p.z, __int_as_float(pr.material));
-                        b.h1[slot] = make_float4(si.n.x, si.n.y, si.n.z, si.u);
-                        b.h2[slot] = make_float4(si.ns.x, si.ns.y, si.ns.z, si.v);
-                        b.h3[slot] = make_float4(si.dpdus.x, si.dpdus.y, si.dpdus.z, __uint_as_float(media));
-                    }
-                }
-            }
-        }
-        if (emits) {
-            // The emission is evaluated with the wavelength pdfs and depth the path arrived with.
-            const int e = wave_append(&b.counts[9]);
-            b.qe[e] = make_int4(slot, depthIn, flagsIn, prim);
-            b.qeLpdf[e] = lpIn;
-        }
-
-        if (r.lam.terminated() != lam0)
-            b.lpdf[slot] = make_float4(r.lam.pdf[0], r.lam.pdf[1], r.lam.pdf[2], r.lam.pdf[3]);
-        b.st[slot] = make_int4(r.depth, r.flags, r.medium, r.crossings);
-        if (kMedia || kInterfaces) {
-            b.o[slot] = make_float4(r.ro.x, r.ro.y, r.ro.z, r.prevPdf);
-            b.d[slot] = make_float4(r.rd.x, r.rd.y, r.rd.z, r.etaScale);
-        }
-        const int target = toMaterial ? mq : (alive ? QCount : -1);  // QCount: straight to the next bounce
-        if (target == QDiffuse) b.qm[QDiffuse * b.qcap + wave_append(&b.counts[mat_count_index(QDiffuse)])] = slot;
-        if (target == QConductor) b.qm[QConductor * b.qcap + wave_append(&b.counts[mat_count_index(QConductor)])] = slot;
-        if (target == QDielectric) b.qm[QDielectric * b.qcap + wave_append(&b.counts[mat_count_index(QDielectric)])] = slot;
-        if (target == QLayered) b.qm[QLayered * b.qcap + wave_append(&b.counts[mat_count_index(QLayered)])] = slot;
-        if (target == QCount) nextQueue[wave_append(nextCount)] = slot;
+        tile_flush(ts, b, nextQueue, nextCount);
     }
 }
 
@@ -1087,10 +1199,11 @@ __global__ __launch_bounds__(kBlock) void k_emit(const DevScene *__restrict__ sp
     const int n = b.counts[9];
     const int stride = int(gridDim.x * blockDim.x);
     for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += stride) {
-        const int4 e = b.qe[i];
-        const int slot = e.x, prim = e.w;
+        const int slot = b.qes[i];
+        const int4 e = b.qe[slot];
+        const int prim = e.w;
         PathRegs r;
-        float4 o = b.o[slot], d = b.d[slot], l = b.lam[slot], lp = b.qeLpdf[i];
+        float4 o = b.o[slot], d = b.d[slot], l = b.lam[slot], lp = b.qeLpdf[slot];
         r.ro = V3(o.x, o.y, o.z);
         r.prevPdf = o.w;
         r.rd = V3(d.x, d.y, d.z);
@@ -1128,137 +1241,155 @@ __global__ __launch_bounds__(kBlock) void k_emit(const DevScene *__restrict__ sp
 // ---- stage 2: one BSDF family (path.cpp:208-235) ----------------------------------------------------
 // Builds the BSDF, next-event estimation when it has a non-specular lobe, BSDF sampling,
 // throughput update, spawn, Russian roulette.
+// One path's BSDF stage (i: its position in the family's queue); *q1 is QNext when the path
+// continues, *q2 QShadow when it made a shadow ray.
+template <int kType, bool kMedia, bool kGuided>
+__device__ __forceinline__ void material_vertex(const DevScene &s, const Params &p, const Bufs &b, int i, int slot, int *q1,
+                                                int *q2) {
+    PathRegs r;
+    float4 d = b.d[slot];
+    r.rd = V3(d.x, d.y, d.z);
+    r.etaScale = d.w;
+    r.rng = load_rng(b, slot);
+    float4 l = b.lam[slot], lp = b.lpdf[slot];
+    r.lam = {{l.x, l.y, l.z, l.w}, {lp.x, lp.y, lp.z, lp.w}};
+    r.beta = ld4(b.beta, slot);
+    r.rpath = ld4(b.rpath, slot);
+    int4 st = b.st[slot];
+    r.depth = st.x;
+    r.flags = st.y;
+    r.medium = st.z;
+    float4 h0 = b.h0[slot], h1 = b.h1[slot], h2 = b.h2[slot], h3 = b.h3[slot];
+    Surf si;
+    si.p = V3(h0.x, h0.y, h0.z);
+    si.n = V3(h1.x, h1.y, h1.z);
+    si.ns = V3(h2.x, h2.y, h2.z);
+    si.dpdus = V3(h3.x, h3.y, h3.z);
+    si.u = h1.w;
+    si.v = h2.w;
+    si.wo = -normalize(r.rd);
+    const prgpu::Material &m = s.mats[__float_as_int(h0.w)];
+    uint32_t media = __float_as_uint(h3.w);
+    int mIn = int(int16_t(media & 0xffff)), mOut = int(int16_t(media >> 16));
+    auto bsdf = [&] {
+        if constexpr (kType == QLayered) return eval_closure(s, m, si, r.lam);
+        else return eval_surface<kType>(s, m, si, r.lam);
+    }();
+    bool alive = true;
+    if constexpr (kGuided) {
+        // One-sample MIS between the learned mixture and the BSDF (q = s p_guide + (1 - s) p_bsdf).
+        const GuideMix gm = guide_mix(b.gout + size_t(i) * 12);
+        const SS beta0 = r.beta;
+        if (sample_ld<kMedia>(s, b, slot, r, si.p, si.n, si.wo, &bsdf, 0, si.ns, true, mIn, mOut, &gm)) *q2 = QShadow;
+        const float usel = r.rng.uniform(), uc = r.rng.uniform();
+        const V2 u2 = get2(r.rng);
+        V3 wi;
+        SS f;
+        float pb;
+        if (usel < gm.sel) {
+            wi = guide_sample(gm, uc, u2.x, u2.y);
+            f = bsdf.f(si.wo, wi);
+            pb = bsdf.pdf(si.wo, wi);
+        } else {
+            BSample bs;
+            if (bsdf.sample(si.wo, uc, u2, &bs)) {
+                wi = bs.wi;
+                f = bs.f;
+                pb = bs.pdf;
+            } else {
+                f = SS(0.f);
+                pb = 0.f;
+            }
+        }
+        const float q = gm.sel * guide_pdf(gm, wi) + (1.f - gm.sel) * pb;
+        if (!(q > 0.f) || !f.nonzero()) {
+            alive = false;
+        } else {
+            r.beta = r.beta * (f * (absdot(wi, si.ns) / q));
+            r.prevPdf = q;
+            r.flags &= ~kSpecularFlag;
+            r.ro = offset_origin(si.p, si.n, wi);
+            r.rd = wi;
+            if (kMedia) r.medium = dot(wi, si.n) > 0 ? mOut : mIn;
+            if (p.train) {  // the path's first free record slot takes this guided vertex
+                const size_t c = size_t(b.qcap);
+                int j = 0;
+                while (j < kGuideRecords && b.trec[6 * j * c + slot].w != 0.f) ++j;
+                if (j < kGuideRecords) {
+                    float4 *t = b.trec + 6 * j * c;
+                    t[slot] = make_float4((si.p.x - p.box.lo[0]) * p.box.inv[0], (si.p.y - p.box.lo[1]) * p.box.inv[1],
+                                          (si.p.z - p.box.lo[2]) * p.box.inv[2], 1.f);
+                    t[c + slot] = make_float4(si.n.x, si.n.y, si.n.z, pb);
+                    t[2 * c + slot] = make_float4(si.wo.x, si.wo.y, si.wo.z, 0.f);
+                    t[3 * c + slot] = make_float4(wi.x, wi.y, wi.z, 0.f);
+                    st4(t, 4 * c + slot, beta0);
+                }
+            }
+            if (!r.beta.nonzero() || r.beta.has_nan()) alive = false;
+            else if (roulette(p.rr, p.rrStart, r)) alive = false;
+        }
+    } else {
+        if (bsdf.non_specular() && sample_ld<kMedia>(s, b, slot, r, si.p, si.n, si.wo, &bsdf, 0, si.ns, true, mIn, mOut))
+            *q2 = QShadow;
+        float uc = r.rng.uniform();
+        V2 u2 = get2(r.rng);
+        BSample bs;
+        if (!bsdf.sample(si.wo, uc, u2, &bs)) {
+            alive = false;
+        } else {
+            r.beta = r.beta * (bs.f * (absdot(bs.wi, si.ns) / bs.pdf));
+            r.prevPdf = bs.pdf;
+            // A stochastic closure's sampling pdf is only proportional: MIS uses its pdf (path.cpp:220).
+            if constexpr (kType == QLayered)
+                if (bs.proportional) r.prevPdf = bsdf.pdf(si.wo, bs.wi);
+            r.flags = (bs.flags & kSpecular) ? (r.flags | kSpecularFlag) : (r.flags & ~kSpecularFlag);
+            if (bs.flags & kTrans) r.etaScale *= sq(bs.eta);
+            r.ro = offset_origin(si.p, si.n, bs.wi);
+            r.rd = bs.wi;
+            if (kMedia) r.medium = dot(bs.wi, si.n) > 0 ? mOut : mIn;
+            if (!r.beta.nonzero() || r.beta.has_nan()) alive = false;
+            else if (roulette(p.rr, p.rrStart, r)) alive = false;
+        }
+    }
+    b.rng[slot] = make_ulonglong2(r.rng.state, r.rng.inc);
+    if (alive) {
+        b.o[slot] = make_float4(r.ro.x, r.ro.y, r.ro.z, r.prevPdf);
+        b.d[slot] = make_float4(r.rd.x, r.rd.y, r.rd.z, r.etaScale);
+        st4(b.beta, slot, r.beta);
+        b.prevP[slot] = make_float4(si.p.x, si.p.y, si.p.z, 0.f);
+        b.prevN[slot] = make_float4(si.n.x, si.n.y, si.n.z, 0.f);
+        b.st[slot] = make_int4(r.depth, r.flags, r.medium, st.w);
+        *q1 = QNext;
+    }
+}
+
 template <int kType, bool kMedia, bool kGuided = false>
 __global__ __launch_bounds__(kBlock) PG_WAVES(kType == QLayered ? 0 : kMaterialWaves) void k_material(const DevScene *__restrict__ sp, Params p, Bufs b, int *nextQueue,
                                                      int *nextCount) {
+    __shared__ TileStage ts;
     const DevScene &s = *sp;
     const int n = b.counts[mat_count_index(kType)];
     const int *queue = b.qm + kType * b.qcap;
-    const int stride = int(gridDim.x * blockDim.x);
-    for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += stride) {
-        const int slot = queue[i];
-        PathRegs r;
-        float4 d = b.d[slot];
-        r.rd = V3(d.x, d.y, d.z);
-        r.etaScale = d.w;
-        r.rng = load_rng(b, slot);
-        float4 l = b.lam[slot], lp = b.lpdf[slot];
-        r.lam = {{l.x, l.y, l.z, l.w}, {lp.x, lp.y, lp.z, lp.w}};
-        r.beta = ld4(b.beta, slot);
-        r.rpath = ld4(b.rpath, slot);
-        int4 st = b.st[slot];
-        r.depth = st.x;
-        r.flags = st.y;
-        r.medium = st.z;
-        float4 h0 = b.h0[slot], h1 = b.h1[slot], h2 = b.h2[slot], h3 = b.h3[slot];
-        Surf si;
-        si.p = V3(h0.x, h0.y, h0.z);
-        si.n = V3(h1.x, h1.y, h1.z);
-        si.ns = V3(h2.x, h2.y, h2.z);
-        si.dpdus = V3(h3.x, h3.y, h3.z);
-        si.u = h1.w;
-        si.v = h2.w;
-        si.wo = -normalize(r.rd);
-        const prgpu::Material &m = s.mats[__float_as_int(h0.w)];
-        uint32_t media = __float_as_uint(h3.w);
-        int mIn = int(int16_t(media & 0xffff)), mOut = int(int16_t(media >> 16));
-        auto bsdf = [&] {
-            if constexpr (kType == QLayered) return eval_closure(s, m, si, r.lam);
-            else return eval_surface<kType>(s, m, si, r.lam);
-        }();
-        bool alive = true;
-        if constexpr (kGuided) {
-            // One-sample MIS between the learned mixture and the BSDF (q = s p_guide + (1 - s) p_bsdf).
-            const GuideMix gm = guide_mix(b.gout + size_t(i) * 12);
-            const SS beta0 = r.beta;
-            sample_ld<kMedia>(s, b, slot, r, si.p, si.n, si.wo, &bsdf, 0, si.ns, true, mIn, mOut, &gm);
-            const float usel = r.rng.uniform(), uc = r.rng.uniform();
-            const V2 u2 = get2(r.rng);
-            V3 wi;
-            SS f;
-            float pb;
-            if (usel < gm.sel) {
-                wi = guide_sample(gm, uc, u2.x, u2.y);
-                f = bsdf.f(si.wo, wi);
-                pb = bsdf.pdf(si.wo, wi);
-            } else {
-                BSample bs;
-                if (bsdf.sample(si.wo, uc, u2, &bs)) {
-                    wi = bs.wi;
-                    f = bs.f;
-                    pb = bs.pdf;
-                } else {
-                    f = SS(0.f);
-                    pb = 0.f;
-                }
+    tile_begin(ts);
+    for (int t0 = int(blockIdx.x) * kTile; t0 < n; t0 += int(gridDim.x) * kTile) {
+#pragma unroll 1
+        for (int k = 0; k < kAppendK; ++k) {
+            const int j = k * kBlock + int(threadIdx.x), i = t0 + j;
+            int slot = 0, q1 = -1, q2 = -1;
+            if (i < n) {
+                slot = queue[i];
+                material_vertex<kType, kMedia, kGuided>(s, p, b, i, slot, &q1, &q2);
             }
-            const float q = gm.sel * guide_pdf(gm, wi) + (1.f - gm.sel) * pb;
-            if (!(q > 0.f) || !f.nonzero()) {
-                alive = false;
-            } else {
-                r.beta = r.beta * (f * (absdot(wi, si.ns) / q));
-                r.prevPdf = q;
-                r.flags &= ~kSpecularFlag;
-                r.ro = offset_origin(si.p, si.n, wi);
-                r.rd = wi;
-                if (kMedia) r.medium = dot(wi, si.n) > 0 ? mOut : mIn;
-                if (p.train) {  // the path's first free record slot takes this guided vertex
-                    const size_t c = size_t(b.qcap);
-                    int j = 0;
-                    while (j < kGuideRecords && b.trec[6 * j * c + slot].w != 0.f) ++j;
-                    if (j < kGuideRecords) {
-                        float4 *t = b.trec + 6 * j * c;
-                        t[slot] = make_float4((si.p.x - p.box.lo[0]) * p.box.inv[0], (si.p.y - p.box.lo[1]) * p.box.inv[1],
-                                              (si.p.z - p.box.lo[2]) * p.box.inv[2], 1.f);
-                        t[c + slot] = make_float4(si.n.x, si.n.y, si.n.z, pb);
-                        t[2 * c + slot] = make_float4(si.wo.x, si.wo.y, si.wo.z, 0.f);
-                        t[3 * c + slot] = make_float4(wi.x, wi.y, wi.z, 0.f);
-                        st4(t, 4 * c + slot, beta0);
-                    }
-                }
-                if (!r.beta.nonzero() || r.beta.has_nan()) alive = false;
-                else if (roulette(p.rr, p.rrStart, r)) alive = false;
-            }
-        } else {
-            if (bsdf.non_specular())
-                sample_ld<kMedia>(s, b, slot, r, si.p, si.n, si.wo, &bsdf, 0, si.ns, true, mIn, mOut);
-            float uc = r.rng.uniform();
-            V2 u2 = get2(r.rng);
-            BSample bs;
-            if (!bsdf.sample(si.wo, uc, u2, &bs)) {
-                alive = false;
-            } else {
-                r.beta = r.beta * (bs.f * (absdot(bs.wi, si.ns) / bs.pdf));
-                r.prevPdf = bs.pdf;
-                // A stochastic closure's sampling pdf is only proportional: MIS uses its pdf (path.cpp:220).
-                if constexpr (kType == QLayered)
-                    if (bs.proportional) r.prevPdf = bsdf.pdf(si.wo, bs.wi);
-                r.flags = (bs.flags & kSpecular) ? (r.flags | kSpecularFlag) : (r.flags & ~kSpecularFlag);
-                if (bs.flags & kTrans) r.etaScale *= sq(bs.eta);
-                r.ro = offset_origin(si.p, si.n, bs.wi);
-                r.rd = bs.wi;
-                if (kMedia) r.medium = dot(bs.wi, si.n) > 0 ? mOut : mIn;
-                if (!r.beta.nonzero() || r.beta.has_nan()) alive = false;
-                else if (roulette(p.rr, p.rrStart, r)) alive = false;
-            }
+            tile_stage(ts, j, slot, q1, q2);
         }
-        b.rng[slot] = make_ulonglong2(r.rng.state, r.rng.inc);
-        if (alive) {
-            b.o[slot] = make_float4(r.ro.x, r.ro.y, r.ro.z, r.prevPdf);
-            b.d[slot] = make_float4(r.rd.x, r.rd.y, r.rd.z, r.etaScale);
-            st4(b.beta, slot, r.beta);
-            b.prevP[slot] = make_float4(si.p.x, si.p.y, si.p.z, 0.f);
-            b.prevN[slot] = make_float4(si.n.x, si.n.y, si.n.z, 0.f);
-            b.st[slot] = make_int4(r.depth, r.flags, r.medium, st.w);
-            nextQueue[wave_append(nextCount)] = slot;
-        }
+        tile_flush(ts, b, nextQueue, nextCount);
     }
 }
 
 // The deferred add() of a visible shadow record: beta * (f * Tr * L * k), and the rPath
 // average at sampling time for the spectral-MIS estimate.
-__device__ __forceinline__ void shadow_add(const Bufs &b, int i, int slot, const SS &T, float k, float a) {
-    SS c = ld4(b.sb, i) * (ld4(b.sf, i) * T * ld4(b.sL, i) * k);
+__device__ __forceinline__ void shadow_add(const Bufs &b, int slot, const SS &T, float k, float a) {
+    SS c = ld4(b.sb, slot) * (ld4(b.sf, slot) * T * ld4(b.sL, slot) * k);
     SS Lh = ld4(b.Lh, slot) + c;
     st4(b.Lh, slot, Lh);
     if (a > 0 && is_finite(a)) st4(b.L, slot, ld4(b.L, slot) + c / a);
@@ -1278,14 +1409,15 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(const DevScene *__restri
     uint32_t rays = 0;
     const float tMax = 1 - 1e-4f;
     if constexpr (!kMedia && !kInterfaces) {
-        int idx = -1;
+        int idx = -1, slot = 0;
         bool started = false;
         DRay r{};
         TravState t{};
         FetchState fs = fetch_begin();
         while (fetch_rays_parts(fetch, n, idx, fs)) {
             if (idx >= 0 && !started) {
-                float4 so = b.so[idx], sd = b.sd[idx];
+                slot = b.sq[idx];
+                float4 so = b.so[slot], sd = b.sd[slot];
                 r = DRay{so.x, so.y, so.z, sd.x, sd.y, sd.z};
                 trace_begin(t, r, tMax);
                 started = true;
@@ -1294,9 +1426,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(const DevScene *__restri
             if (trace_step<true, kStats>(s.trace, r, t, lds, tid, &ts)) {
                 ++rays;
                 if (!t.found) {
-                    const int slot = __float_as_int(b.sd[idx].w);
-                    float4 spk = b.sp[idx], snk = b.sn[idx];
-                    shadow_add(b, idx, slot, SS(1.f), spk.w, snk.w);
+                    float4 spk = b.sp[slot], snk = b.sn[slot];
+                    shadow_add(b, slot, SS(1.f), spk.w, snk.w);
                 }
                 idx = -1;
                 started = false;
@@ -1306,8 +1437,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(const DevScene *__restri
         const int stride = int(gridDim.x * blockDim.x);
         for (int i = tid; i < n; i += stride) {
             ++rays;
-            float4 so = b.so[i], sd = b.sd[i], spt = b.sp[i], sn = b.sn[i];
-            const int slot = __float_as_int(sd.w);
+            const int slot = b.sq[i];
+            float4 so = b.so[slot], sd = b.sd[slot], spt = b.sp[slot], sn = b.sn[slot];
             V3 ro(so.x, so.y, so.z), rd(sd.x, sd.y, sd.z), pl(spt.x, spt.y, spt.z), nl(sn.x, sn.y, sn.z);
             int medium = __float_as_int(so.w);
             Lambda lam;
@@ -1351,7 +1482,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(const DevScene *__restri
                 rd = target - ro;
             }
             if (!T.nonzero()) continue;
-            shadow_add(b, i, slot, T, spt.w, sn.w);
+            shadow_add(b, slot, T, spt.w, sn.w);
         }
     }
     if (kStats && rays) {
@@ -1515,6 +1646,8 @@ void EnsureBatch(PTScene &pt, size_t n) {
     f4(pt.o); f4(pt.d); f4(pt.lam); f4(pt.lpdf); f4(pt.beta); f4(pt.rpath); f4(pt.L); f4(pt.Lh);
     f4(pt.prevP); f4(pt.prevN); f4(pt.hit); f4(pt.xyz);
     f4(pt.h0); f4(pt.h1); f4(pt.h2); f4(pt.h3); f4(pt.qe); f4(pt.qeLpdf);
+    pt.qes.ensure(n * sizeof(int));
+    pt.sq.ensure(n * sizeof(int));
     pt.qm.ensure(n * QCount * sizeof(int));
     f4(pt.so); f4(pt.sd); f4(pt.sp); f4(pt.sn); f4(pt.sf); f4(pt.sL); f4(pt.sb);
     pt.rng.ensure(n * sizeof(ulonglong2));
@@ -1534,6 +1667,8 @@ Bufs MakeBufs(PTScene &pt) {
     b.hit = pt.hit.as<float4>(); b.q0 = pt.q0.as<int>(); b.q1 = pt.q1.as<int>(); b.counts = pt.counts.as<int>();
     b.h0 = pt.h0.as<float4>(); b.h1 = pt.h1.as<float4>(); b.h2 = pt.h2.as<float4>(); b.h3 = pt.h3.as<float4>();
     b.qm = pt.qm.as<int>();
+    b.qes = pt.qes.as<int>();
+    b.sq = pt.sq.as<int>();
     b.qe = pt.qe.as<int4>();
     b.qeLpdf = pt.qeLpdf.as<float4>();
     b.qcap = int(pt.cap);
@@ -1674,7 +1809,7 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
     int active = n;
     pt.countsHost[0] = n;
     PG_CHECK(hipMemcpyAsync(b.counts, pt.countsHost, sizeof(int), hipMemcpyHostToDevice, st));
-    const int shadeGrid = std::max(1, std::min((active + kBlock - 1) / kBlock, dev.cus * dev.tune.shadeBlocksPerCU));
+    const int shadeGrid = std::max(1, std::min((active + kTile - 1) / kTile, dev.cus * dev.tune.shadeBlocksPerCU));
     // One bounce: closest hits of the current queue, the surface and emission stages, the
     // material families (which fill the next queue), the shadow rays. Every kernel reads its
     // queue length from device memory, so rounds can be queued without reading them back.
@@ -1770,7 +1905,7 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
         hipEvent_t ready[2] = {dev.pipeEvents[0], dev.pipeEvents[1]};
         int bound = n, curIdx = 0, k = 0;
         auto launch = [&](int r, int curI, int bnd) {
-            round(curI, r, TraceGrid(dev, bnd), std::max(1, std::min((bnd + kBlock - 1) / kBlock, shadeGrid)));
+            round(curI, r, TraceGrid(dev, bnd), std::max(1, std::min((bnd + kTile - 1) / kTile, shadeGrid)));
             PG_CHECK(hipMemcpyAsync(slot + (r & 1), b.counts + (1 - curI), sizeof(int), hipMemcpyDeviceToHost, st));
             PG_CHECK(hipEventRecord(ready[r & 1], st));
         };
@@ -1790,7 +1925,7 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
         int curIdx = 0, bounce = 0;
         while (active > 0) {
             const int tgrid = TraceGrid(dev, active);
-            const int grid = std::max(1, std::min((active + kBlock - 1) / kBlock, shadeGrid));
+            const int grid = std::max(1, std::min((active + kTile - 1) / kTile, shadeGrid));
             round(curIdx, bounce, tgrid, grid);
             PG_CHECK(hipGetLastError());
             PG_CHECK(hipMemcpyAsync(pt.countsHost, b.counts, 3 * sizeof(int), hipMemcpyDeviceToHost, st));

@@ -352,8 +352,11 @@ __device__ __forceinline__ bool trace_turn(bool busy, uint32_t cur) {
 // runs on XCD b % 8): a wave drains its own XCD's part first, then the others in turn. Measured on
 // configuration 2 this is slower (8 parts: closest hit 8.9 ms per step against 6.7 ms with one
 // part): one queue front keeps every XCD on the same rays, whose nodes then stay in the shared
-// MALL. The default is one part. `part` and `left` are wave-uniform; the loop ends when every
-// part is drained and no lane holds a ray.
+// MALL. The default is one part. Per-wave reservoirs of larger chunks (a static share per wave,
+// then guided chunks of 64 to 500 rays) were slower too (8.9 to 13.4 ms against 6.4 ms): they
+// widen the part of the queue in flight, and the fetch atomics (about two per wave and launch) do
+// not limit the traversal. `part` and `left` are wave-uniform; the loop ends when every part is
+// drained and no lane holds a ray.
 constexpr int kFetchParts = PG_FETCH_PARTS;
 struct FetchState {
     int part, left;
