@@ -671,9 +671,11 @@ struct Bufs {
     int *q0, *q1;
     int *qm;     // material queues, QCount x qcap
     int qcap;
-    int *counts;  // [0] queue A, [1] queue B, [2] shadow queue, [3..5] plain material queues, [9] emission
-                  // queue, [10] coated / sheen material queue, [16..16+P) closest-hit fetch counters,
-                  // [32..32+P) shadow fetch counters (P = kFetchParts)
+    int *counts;  // [0] queue A, [1] queue B, [2] and [11] shadow queues (alternating bounces when the
+                  // shadow rays are traced with the next bounce's closest hits), [3..5] plain
+                  // material queues, [9] emission queue, [10] coated / sheen material queue,
+                  // [16..16+P) closest-hit fetch counters, [32..32+P) shadow fetch counters
+                  // (P = kFetchParts)
     float2 *pfilm;
     float4 *xyz;
     // shadow records, one per path slot (a path makes at most one shadow ray per bounce)
@@ -686,6 +688,7 @@ struct Bufs {
     int4 *qe;       // per slot: slot, depth and flags on arrival at the emitter, primitive (-1: escaped)
     float4 *qeLpdf; // and the wavelength pdfs on arrival
     int *sq;        // shadow queue (path slots; the shadow records are indexed by slot)
+    int shadowIdx;  // index in counts of the shadow queue's length this bounce appends to
     const float4 *gout;
     float4 *trec;
     float *train;
@@ -776,7 +779,7 @@ __device__ __forceinline__ int *queue_ptr(const Bufs &b, int *next, int q) {
 // Index in Bufs::counts of material family q's queue length (the layered family's is 10).
 __device__ __host__ constexpr int mat_count_index(int q) { return q == 3 ? 10 : 3 + q; }
 __device__ __forceinline__ int *count_ptr(const Bufs &b, int *nextCount, int q) {
-    return q < QNext ? b.counts + mat_count_index(q) : (q == QNext ? nextCount : (q == QEmit ? b.counts + 9 : b.counts + 2));
+    return q < QNext ? b.counts + mat_count_index(q) : (q == QNext ? nextCount : (q == QEmit ? b.counts + 9 : b.counts + b.shadowIdx));
 }
 
 // Reserves the tile's ranges (one atomic instruction) and scatters the staged entries.
@@ -817,9 +820,9 @@ __device__ __forceinline__ V2 get2(Pcg &r) {
 
 // Start of a bounce: every queue length and fetch counter but the current path queue's is zeroed
 // (a kernel rather than a memset, so that the bounce can be captured in a graph).
-__global__ __launch_bounds__(64) void k_reset_counts(int *counts, int keep) {
+__global__ __launch_bounds__(64) void k_reset_counts(int *counts, int keep, int keep2) {
     for (int i = int(threadIdx.x); i < kCountsInts; i += 64)
-        if (i != keep) counts[i] = 0;
+        if (i != keep && i != keep2) counts[i] = 0;
 }
 
 // Camera samples (src/integrators/path.cpp:61-70, src/cameras/camera.cpp:36-48).
@@ -1395,6 +1398,100 @@ __device__ __forceinline__ void shadow_add(const Bufs &b, int slot, const SS &T,
     if (a > 0 && is_finite(a)) st4(b.L, slot, ld4(b.L, slot) + c / a);
 }
 
+// Closest hits of the current queue and, in the same persistent launch, the shadow rays of the
+// previous bounce (scenes without media or interfaces, where a shadow ray is one any-hit query).
+// The two are independent: a shadow ray only adds to its path's radiance, which this bounce's
+// surface and emission stages read after the launch, so contributions are added in the same order
+// as with a launch of their own and every sample's result is unchanged. A wave takes closest rays
+// until that queue is drained, then shadow rays, so a bounce has one traversal tail instead of
+// two; late bounces, where a few thousand rays take the whole launch latency, cost half as much.
+#ifndef PG_FUSED_SHADOW
+#define PG_FUSED_SHADOW 1
+#endif
+constexpr bool kFusedShadow = PG_FUSED_SHADOW != 0;
+constexpr int kShadowCountB = 11;  // the second shadow queue length (see Bufs::counts)
+__host__ __device__ constexpr int shadow_count_index(int curIdx) { return curIdx ? kShadowCountB : 2; }
+
+template <bool kStats>
+__global__ __launch_bounds__(kTraceBlock) PG_WAVES(8) void k_trace_bounce(const DevScene *__restrict__ sp, Bufs b,
+                                                                        const int *queue, const int *count,
+                                                                        const int *shadowCount, int *fetch,
+                                                                        int *shadowFetch, unsigned long long *counters) {
+    const DevScene &s = *sp;
+    __shared__ uint32_t lds[kStackLds * kTraceBlock];
+    const int n = *count, ns = *shadowCount;
+    const int tid = int(blockIdx.x * blockDim.x + threadIdx.x);
+    TraceStats stc, sts;
+    uint32_t raysC = 0, raysS = 0;
+    int idx = -1, slot = 0;
+    bool shadow = false, started = false;
+    DRay r{};
+    TravState t{};
+    FetchState fc = fetch_begin(), fs = fetch_begin();
+    const float tMaxShadow = 1 - 1e-4f;
+    while (true) {
+        bool more;
+        if (!fc.done) {
+            const bool idle = idx < 0;
+            more = fetch_rays_parts(fetch, n, idx, fc);
+            if (idle && idx >= 0) shadow = false;
+        }
+        if (fc.done) {
+            const bool idle = idx < 0;
+            more = fetch_rays_parts(shadowFetch, ns, idx, fs);
+            if (idle && idx >= 0) shadow = true;
+        }
+        if (!more) break;
+        if (idx >= 0 && !started) {
+            if (shadow) {
+                slot = b.sq[idx];
+                const float4 so = b.so[slot], sd = b.sd[slot];
+                r = DRay{so.x, so.y, so.z, sd.x, sd.y, sd.z};
+                trace_begin(t, r, tMaxShadow);
+                t.any = true;
+            } else {
+                slot = queue[idx];
+                const float4 o = b.o[slot], d = b.d[slot];
+                r = DRay{o.x, o.y, o.z, d.x, d.y, d.z};
+                trace_begin(t, r, kInfF);
+            }
+            started = true;
+        }
+        if (!trace_turn(idx >= 0, t.cur)) continue;
+        TraceStats one;
+        const bool fin = trace_step<false, kStats, true>(s.trace, r, t, lds, tid, &one);
+        if (kStats) {
+            sts.nodes += shadow ? one.nodes : 0u;
+            sts.recs += shadow ? one.recs : 0u;
+            stc.nodes += shadow ? 0u : one.nodes;
+            stc.recs += shadow ? 0u : one.recs;
+        }
+        if (fin) {
+            if (shadow) {
+                ++raysS;
+                if (!t.found) {
+                    const float4 spk = b.sp[slot], snk = b.sn[slot];
+                    shadow_add(b, slot, SS(1.f), spk.w, snk.w);
+                }
+            } else {
+                ++raysC;
+                b.hit[slot] = t.found ? make_float4(t.hit.t, __int_as_float(t.hit.prim), t.hit.b1, t.hit.b2)
+                                      : make_float4(0, __int_as_float(-1), 0, 0);
+            }
+            idx = -1;
+            started = false;
+        }
+    }
+    if (kStats && (raysC || raysS)) {
+        atomicAdd(&counters[0], (unsigned long long)raysC);
+        atomicAdd(&counters[1], (unsigned long long)stc.nodes);
+        atomicAdd(&counters[2], (unsigned long long)stc.recs);
+        atomicAdd(&counters[3], (unsigned long long)raysS);
+        atomicAdd(&counters[4], (unsigned long long)sts.nodes);
+        atomicAdd(&counters[5], (unsigned long long)sts.recs);
+    }
+}
+
 // Transmittance of the shadow records (src/scene/scene.cpp:41-59) and the deferred add().
 // Without media or interfaces a shadow ray is one any-hit query, traversed persistently as in
 // k_intersect; otherwise each ray walks its interface crossings to completion.
@@ -1669,6 +1766,7 @@ Bufs MakeBufs(PTScene &pt) {
     b.qm = pt.qm.as<int>();
     b.qes = pt.qes.as<int>();
     b.sq = pt.sq.as<int>();
+    b.shadowIdx = 2;
     b.qe = pt.qe.as<int4>();
     b.qeLpdf = pt.qeLpdf.as<float4>();
     b.qcap = int(pt.cap);
@@ -1807,32 +1905,47 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
     });
     PG_CHECK(hipGetLastError());
     int active = n;
+    hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, st, b.counts, -1, -1);
     pt.countsHost[0] = n;
     PG_CHECK(hipMemcpyAsync(b.counts, pt.countsHost, sizeof(int), hipMemcpyHostToDevice, st));
     const int shadeGrid = std::max(1, std::min((active + kTile - 1) / kTile, dev.cus * dev.tune.shadeBlocksPerCU));
     // One bounce: closest hits of the current queue, the surface and emission stages, the
     // material families (which fill the next queue), the shadow rays. Every kernel reads its
     // queue length from device memory, so rounds can be queued without reading them back.
+    // Without media or interfaces a bounce's shadow rays are traced in the next bounce's traversal
+    // launch (k_trace_bounce): a bounce appends them to shadow queue shadow_count_index(curIdx) and
+    // the next one, whose curIdx is the other, consumes them. After the last bounce a round over the
+    // empty path queue traces the last shadow rays.
+    const bool fused = kFusedShadow && !media && !interfaces;
+    int pendingShadow = 0;  // shadow rays the next round traces (profiled rounds: read back)
     auto round = [&](int curIdx, int bounce, int tgrid, int grid) {
         const int nxtIdx = 1 - curIdx;
         int *cur = curIdx ? b.q1 : b.q0, *nxt = curIdx ? b.q0 : b.q1;
-        hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, st, b.counts, curIdx);
-        timer.Run(prgpu::KernIntersect, active, nullptr, [&] {
-            if (stats)
-                hipLaunchKernelGGL(k_intersect<true>, dim3(tgrid), dim3(kTraceBlock), 0, st, ds, b, cur, b.counts + curIdx,
+        Bufs rb = b;
+        const int sIn = shadow_count_index(1 - curIdx);
+        rb.shadowIdx = fused ? shadow_count_index(curIdx) : 2;
+        hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, st, b.counts, curIdx, fused ? sIn : -1);
+        timer.Run(prgpu::KernIntersect, int64_t(active) + (fused ? pendingShadow : 0), nullptr, [&] {
+            if (fused) {
+                auto *k = stats ? k_trace_bounce<true> : k_trace_bounce<false>;
+                hipLaunchKernelGGL(k, dim3(tgrid), dim3(kTraceBlock), 0, st, ds, rb, cur, b.counts + curIdx, b.counts + sIn,
+                                   b.counts + kFetchIntersect, b.counts + kFetchShadow, ctr);
+            } else if (stats) {
+                hipLaunchKernelGGL(k_intersect<true>, dim3(tgrid), dim3(kTraceBlock), 0, st, ds, rb, cur, b.counts + curIdx,
                                    b.counts + kFetchIntersect, ctr);
-            else
-                hipLaunchKernelGGL(k_intersect<false>, dim3(tgrid), dim3(kTraceBlock), 0, st, ds, b, cur, b.counts + curIdx,
+            } else {
+                hipLaunchKernelGGL(k_intersect<false>, dim3(tgrid), dim3(kTraceBlock), 0, st, ds, rb, cur, b.counts + curIdx,
                                    b.counts + kFetchIntersect, ctr);
+            }
         });
         // Guiding applies to the vertices of the first guideBounces bounces (0: all).
         const bool guided = p.guide && (dev.tune.guideBounces == 0 || bounce < dev.tune.guideBounces);
         timer.Run(prgpu::KernSurface, active, nullptr, [&] {
             auto *k = media ? (interfaces ? k_surface<true, true> : k_surface<true, false>)
                             : (interfaces ? k_surface<false, true> : k_surface<false, false>);
-            hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, st, ds, p, b, cur, b.counts + curIdx, nxt,
+            hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, st, ds, p, rb, cur, b.counts + curIdx, nxt,
                                b.counts + nxtIdx);
-            if (s.emitters) hipLaunchKernelGGL(k_emit, dim3(grid), dim3(kBlock), 0, st, ds, b);
+            if (s.emitters) hipLaunchKernelGGL(k_emit, dim3(grid), dim3(kBlock), 0, st, ds, rb);
         });
         if (guided)
             timer.Run(prgpu::KernGuide, 0, nullptr, [&] {
@@ -1844,21 +1957,23 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
                               : (media ? k_material<QDiffuse, true> : k_material<QDiffuse, false>);
             auto *kc = media ? k_material<QConductor, true> : k_material<QConductor, false>;
             auto *ke = media ? k_material<QDielectric, true> : k_material<QDielectric, false>;
-            hipLaunchKernelGGL(kd, dim3(grid), dim3(kBlock), 0, st, ds, p, b, nxt, b.counts + nxtIdx);
-            hipLaunchKernelGGL(kc, dim3(grid), dim3(kBlock), 0, st, ds, p, b, nxt, b.counts + nxtIdx);
-            hipLaunchKernelGGL(ke, dim3(grid), dim3(kBlock), 0, st, ds, p, b, nxt, b.counts + nxtIdx);
+            hipLaunchKernelGGL(kd, dim3(grid), dim3(kBlock), 0, st, ds, p, rb, nxt, b.counts + nxtIdx);
+            hipLaunchKernelGGL(kc, dim3(grid), dim3(kBlock), 0, st, ds, p, rb, nxt, b.counts + nxtIdx);
+            hipLaunchKernelGGL(ke, dim3(grid), dim3(kBlock), 0, st, ds, p, rb, nxt, b.counts + nxtIdx);
             if (pt.hasLayered) {
                 auto *kl = media ? k_material<QLayered, true> : k_material<QLayered, false>;
-                hipLaunchKernelGGL(kl, dim3(grid), dim3(kBlock), 0, st, ds, p, b, nxt, b.counts + nxtIdx);
+                hipLaunchKernelGGL(kl, dim3(grid), dim3(kBlock), 0, st, ds, p, rb, nxt, b.counts + nxtIdx);
             }
         });
-        timer.Run(prgpu::KernShadow, 0, &pt.countsHost[2], [&] {
-            auto *k = stats ? (media ? (interfaces ? k_shadow<true, true, true> : k_shadow<true, true, false>)
-                                     : (interfaces ? k_shadow<true, false, true> : k_shadow<true, false, false>))
-                            : (media ? (interfaces ? k_shadow<false, true, true> : k_shadow<false, true, false>)
-                                     : (interfaces ? k_shadow<false, false, true> : k_shadow<false, false, false>));
-            hipLaunchKernelGGL(k, dim3(tgrid), dim3(kTraceBlock), 0, st, ds, b, b.counts + 2, b.counts + kFetchShadow, ctr);
-        });
+        if (!fused)
+            timer.Run(prgpu::KernShadow, 0, &pt.countsHost[2], [&] {
+                auto *k = stats ? (media ? (interfaces ? k_shadow<true, true, true> : k_shadow<true, true, false>)
+                                         : (interfaces ? k_shadow<true, false, true> : k_shadow<true, false, false>))
+                                : (media ? (interfaces ? k_shadow<false, true, true> : k_shadow<false, true, false>)
+                                         : (interfaces ? k_shadow<false, false, true> : k_shadow<false, false, false>));
+                hipLaunchKernelGGL(k, dim3(tgrid), dim3(kTraceBlock), 0, st, ds, rb, b.counts + 2, b.counts + kFetchShadow,
+                                   ctr);
+            });
     };
 
     const int graphRounds = dev.tune.graphRounds;
@@ -1893,6 +2008,7 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
             PG_CHECK(hipStreamSynchronize(st));
             active = pt.countsHost[0];
         }
+        if (fused) round(0, graphRounds, tgrid, 1);  // the last replay's shadow rays
     } else if (!dev.profileMode) {
         // Pipelined: round k + 1 is queued before the host waits for the path count of round k,
         // so the device never idles on a read-back. A round's grids are sized by the count the host
@@ -1904,8 +2020,11 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
             if (!e) PG_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         hipEvent_t ready[2] = {dev.pipeEvents[0], dev.pipeEvents[1]};
         int bound = n, curIdx = 0, k = 0;
+        // With fused shadow rays a round's traversal also takes the previous round's shadow rays,
+        // at most one per path that entered it (<= bnd as well).
         auto launch = [&](int r, int curI, int bnd) {
-            round(curI, r, TraceGrid(dev, bnd), std::max(1, std::min((bnd + kTile - 1) / kTile, shadeGrid)));
+            round(curI, r, TraceGrid(dev, int64_t(bnd) * (fused && r > 0 ? 2 : 1)),
+                  std::max(1, std::min((bnd + kTile - 1) / kTile, shadeGrid)));
             PG_CHECK(hipMemcpyAsync(slot + (r & 1), b.counts + (1 - curI), sizeof(int), hipMemcpyDeviceToHost, st));
             PG_CHECK(hipEventRecord(ready[r & 1], st));
         };
@@ -1923,14 +2042,17 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
         PG_CHECK(hipStreamSynchronize(st));
     } else {
         int curIdx = 0, bounce = 0;
-        while (active > 0) {
-            const int tgrid = TraceGrid(dev, active);
+        // A round runs while paths remain or, with fused shadow rays, while the last round left
+        // shadow rays (the final round's path queue is empty).
+        while (active > 0 || pendingShadow > 0) {
+            const int tgrid = TraceGrid(dev, int64_t(active) + pendingShadow);
             const int grid = std::max(1, std::min((active + kTile - 1) / kTile, shadeGrid));
             round(curIdx, bounce, tgrid, grid);
             PG_CHECK(hipGetLastError());
-            PG_CHECK(hipMemcpyAsync(pt.countsHost, b.counts, 3 * sizeof(int), hipMemcpyDeviceToHost, st));
+            PG_CHECK(hipMemcpyAsync(pt.countsHost, b.counts, (kShadowCountB + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
             PG_CHECK(hipStreamSynchronize(st));
             timer.Collect();
+            pendingShadow = fused ? pt.countsHost[shadow_count_index(curIdx)] : 0;
             curIdx = 1 - curIdx;
             active = pt.countsHost[curIdx];
             ++bounce;

@@ -150,6 +150,7 @@ struct TravState {
     uint32_t cur;
     int sp;
     bool found;
+    bool any;  // any-hit query (read by trace_step<..., kRuntimeAny = true>)
     DHit hit;
 };
 
@@ -175,6 +176,7 @@ __device__ __forceinline__ void trace_begin(TravState &t, const DRay &r, float t
     t.cur = 0;  // root node
     t.sp = 0;
     t.found = false;
+    t.any = false;
     t.hit.prim = -1;
 }
 
@@ -187,8 +189,9 @@ __device__ __forceinline__ uint32_t trace_pop(const TraceCtx &c, TravState &t, c
 // One traversal step: one wide node (slab-test its children, push the hit ones far to near and
 // continue with the nearest) or one leaf (test its records). Returns true when the traversal of
 // this ray is complete. `lds` is the block's stack area (kStackLds * blockDim.x entries), `tid`
-// the global thread index for the spill slab.
-template <bool kAny, bool kStats>
+// the global thread index for the spill slab. An any-hit query (kAny, or t.any with kRuntimeAny)
+// completes at its first hit.
+template <bool kAny, bool kStats, bool kRuntimeAny = false>
 __device__ __forceinline__ bool trace_step(const TraceCtx &c, const DRay &r, TravState &t, uint32_t *lds, int tid,
                                            TraceStats *st) {
     const uint32_t lane = threadIdx.x;
@@ -310,7 +313,7 @@ __device__ __forceinline__ bool trace_step(const TraceCtx &c, const DRay &r, Tra
             t.hit.prim = int(prim & ~kSphereBit);
             t.hit.b1 = u;
             t.hit.b2 = v;
-            if (kAny) return true;
+            if (kAny || (kRuntimeAny && t.any)) return true;
         }
     }
     if (t.sp == 0) return true;
