@@ -1389,6 +1389,46 @@ __global__ __launch_bounds__(kBlock) PG_WAVES(kType == QLayered ? 0 : kMaterialW
     }
 }
 
+// The plain families (diffuse from family f0 on, conductor, dielectric) in one launch over their
+// queues laid end to end: in late bounces, where each family holds a few thousand paths, three
+// launches cost three latency tails. A wave meets at most the two boundaries between the queues.
+#ifndef PG_FUSED_MATERIAL
+#define PG_FUSED_MATERIAL 1
+#endif
+constexpr bool kFusedMaterial = PG_FUSED_MATERIAL != 0;
+template <bool kMedia>
+__global__ __launch_bounds__(kBlock) PG_WAVES(kMaterialWaves) void k_material_plain(const DevScene *__restrict__ sp, Params p, Bufs b,
+                                                                                 int *nextQueue, int *nextCount, int f0) {
+    __shared__ TileStage ts;
+    const DevScene &s = *sp;
+    const int nD = f0 <= QDiffuse ? b.counts[mat_count_index(QDiffuse)] : 0;
+    const int nC = b.counts[mat_count_index(QConductor)];
+    const int nE = b.counts[mat_count_index(QDielectric)];
+    const int n = nD + nC + nE;
+    tile_begin(ts);
+    for (int t0 = int(blockIdx.x) * kTile; t0 < n; t0 += int(gridDim.x) * kTile) {
+#pragma unroll 1
+        for (int k = 0; k < kAppendK; ++k) {
+            const int j = k * kBlock + int(threadIdx.x), i = t0 + j;
+            int slot = 0, q1 = -1, q2 = -1;
+            if (i < nD) {
+                slot = b.qm[QDiffuse * b.qcap + i];
+                material_vertex<QDiffuse, kMedia, false>(s, p, b, i, slot, &q1, &q2);
+            } else if (i < nD + nC) {
+                const int ii = i - nD;
+                slot = b.qm[QConductor * b.qcap + ii];
+                material_vertex<QConductor, kMedia, false>(s, p, b, ii, slot, &q1, &q2);
+            } else if (i < n) {
+                const int ii = i - nD - nC;
+                slot = b.qm[QDielectric * b.qcap + ii];
+                material_vertex<QDielectric, kMedia, false>(s, p, b, ii, slot, &q1, &q2);
+            }
+            tile_stage(ts, j, slot, q1, q2);
+        }
+        tile_flush(ts, b, nextQueue, nextCount);
+    }
+}
+
 // The deferred add() of a visible shadow record: beta * (f * Tr * L * k), and the rPath
 // average at sampling time for the spectral-MIS estimate.
 __device__ __forceinline__ void shadow_add(const Bufs &b, int slot, const SS &T, float k, float a) {
@@ -1953,13 +1993,22 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
                                  b.h1, b.d, p.box, dev.cus, st);
             });
         timer.Run(prgpu::KernMaterial, active, nullptr, [&] {
-            auto *kd = guided ? (media ? k_material<QDiffuse, true, true> : k_material<QDiffuse, false, true>)
-                              : (media ? k_material<QDiffuse, true> : k_material<QDiffuse, false>);
-            auto *kc = media ? k_material<QConductor, true> : k_material<QConductor, false>;
-            auto *ke = media ? k_material<QDielectric, true> : k_material<QDielectric, false>;
-            hipLaunchKernelGGL(kd, dim3(grid), dim3(kBlock), 0, st, ds, p, rb, nxt, b.counts + nxtIdx);
-            hipLaunchKernelGGL(kc, dim3(grid), dim3(kBlock), 0, st, ds, p, rb, nxt, b.counts + nxtIdx);
-            hipLaunchKernelGGL(ke, dim3(grid), dim3(kBlock), 0, st, ds, p, rb, nxt, b.counts + nxtIdx);
+            if (guided) {
+                auto *kd = media ? k_material<QDiffuse, true, true> : k_material<QDiffuse, false, true>;
+                hipLaunchKernelGGL(kd, dim3(grid), dim3(kBlock), 0, st, ds, p, rb, nxt, b.counts + nxtIdx);
+            }
+            if (kFusedMaterial) {
+                auto *kp = media ? k_material_plain<true> : k_material_plain<false>;
+                hipLaunchKernelGGL(kp, dim3(grid), dim3(kBlock), 0, st, ds, p, rb, nxt, b.counts + nxtIdx,
+                                   guided ? int(QConductor) : int(QDiffuse));
+            } else {
+                auto *kd = media ? k_material<QDiffuse, true> : k_material<QDiffuse, false>;
+                auto *kc = media ? k_material<QConductor, true> : k_material<QConductor, false>;
+                auto *ke = media ? k_material<QDielectric, true> : k_material<QDielectric, false>;
+                if (!guided) hipLaunchKernelGGL(kd, dim3(grid), dim3(kBlock), 0, st, ds, p, rb, nxt, b.counts + nxtIdx);
+                hipLaunchKernelGGL(kc, dim3(grid), dim3(kBlock), 0, st, ds, p, rb, nxt, b.counts + nxtIdx);
+                hipLaunchKernelGGL(ke, dim3(grid), dim3(kBlock), 0, st, ds, p, rb, nxt, b.counts + nxtIdx);
+            }
             if (pt.hasLayered) {
                 auto *kl = media ? k_material<QLayered, true> : k_material<QLayered, false>;
                 hipLaunchKernelGGL(kl, dim3(grid), dim3(kBlock), 0, st, ds, p, rb, nxt, b.counts + nxtIdx);
