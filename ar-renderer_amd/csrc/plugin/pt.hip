@@ -1197,48 +1197,50 @@ __global__ __launch_bounds__(kBlock) PG_WAVES(kSurfaceWaves) void k_surface(cons
 
 // Emission met by the paths of the emission queue, with its MIS weight (path.cpp:143-153,
 // 159-167): area light of the hit primitive, or the environment for an escaped path.
+// Emission met by one path of the emission queue, with its MIS weight.
+__device__ __forceinline__ void emit_vertex(const DevScene &s, const Bufs &b, int slot) {
+    const int4 e = b.qe[slot];
+    const int prim = e.w;
+    PathRegs r;
+    float4 o = b.o[slot], d = b.d[slot], l = b.lam[slot], lp = b.qeLpdf[slot];
+    r.ro = V3(o.x, o.y, o.z);
+    r.prevPdf = o.w;
+    r.rd = V3(d.x, d.y, d.z);
+    r.lam = {{l.x, l.y, l.z, l.w}, {lp.x, lp.y, lp.z, lp.w}};
+    r.depth = e.y;
+    r.flags = e.z;
+    r.beta = ld4(b.beta, slot);
+    r.rpath = ld4(b.rpath, slot);
+    r.L = ld4(b.L, slot);
+    r.Lh = ld4(b.Lh, slot);
+    float4 pp = b.prevP[slot], pn = b.prevN[slot];
+    r.prevP = V3(pp.x, pp.y, pp.z);
+    r.prevN = V3(pn.x, pn.y, pn.z);
+    if (prim < 0) {
+        for (int li = 0; li < s.infCount; ++li) {
+            int L = s.infinite[li];
+            if (s.lights[L].kind == prgpu::LightExEnvironment)
+                add_emitted(s, r, L, env_L(s, s.lights[L], r.rd, r.lam), r.rd);
+            else if (s.lights[L].kind == prgpu::LightExSun &&  // SunLight::Le (sky.cpp:92-95)
+                     dot(normalize(r.rd), f3(s.lights[L].direction)) >= s.lights[L].cosFalloffEnd)
+                add_emitted(s, r, L, spec_sample(s.spec, s.lights[L].spectrum, r.lam) * s.lights[L].scale, r.rd);
+        }
+    } else {
+        const prgpu::Primitive &pr = s.prims[prim];
+        const float4 hv = b.hit[slot];
+        Surf si;
+        interaction(pr, r.ro, r.rd, hv.x, hv.z, hv.w, &si);
+        add_emitted(s, r, pr.light, area_L(s, s.lights[pr.light], si.n, -r.rd, r.lam), r.rd);
+    }
+    st4(b.L, slot, r.L);
+    st4(b.Lh, slot, r.Lh);
+}
+
 __global__ __launch_bounds__(kBlock) void k_emit(const DevScene *__restrict__ sp, Bufs b) {
     const DevScene &s = *sp;
     const int n = b.counts[9];
     const int stride = int(gridDim.x * blockDim.x);
-    for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += stride) {
-        const int slot = b.qes[i];
-        const int4 e = b.qe[slot];
-        const int prim = e.w;
-        PathRegs r;
-        float4 o = b.o[slot], d = b.d[slot], l = b.lam[slot], lp = b.qeLpdf[slot];
-        r.ro = V3(o.x, o.y, o.z);
-        r.prevPdf = o.w;
-        r.rd = V3(d.x, d.y, d.z);
-        r.lam = {{l.x, l.y, l.z, l.w}, {lp.x, lp.y, lp.z, lp.w}};
-        r.depth = e.y;
-        r.flags = e.z;
-        r.beta = ld4(b.beta, slot);
-        r.rpath = ld4(b.rpath, slot);
-        r.L = ld4(b.L, slot);
-        r.Lh = ld4(b.Lh, slot);
-        float4 pp = b.prevP[slot], pn = b.prevN[slot];
-        r.prevP = V3(pp.x, pp.y, pp.z);
-        r.prevN = V3(pn.x, pn.y, pn.z);
-        if (prim < 0) {
-            for (int li = 0; li < s.infCount; ++li) {
-                int L = s.infinite[li];
-                if (s.lights[L].kind == prgpu::LightExEnvironment)
-                    add_emitted(s, r, L, env_L(s, s.lights[L], r.rd, r.lam), r.rd);
-                else if (s.lights[L].kind == prgpu::LightExSun &&  // SunLight::Le (sky.cpp:92-95)
-                         dot(normalize(r.rd), f3(s.lights[L].direction)) >= s.lights[L].cosFalloffEnd)
-                    add_emitted(s, r, L, spec_sample(s.spec, s.lights[L].spectrum, r.lam) * s.lights[L].scale, r.rd);
-            }
-        } else {
-            const prgpu::Primitive &pr = s.prims[prim];
-            const float4 hv = b.hit[slot];
-            Surf si;
-            interaction(pr, r.ro, r.rd, hv.x, hv.z, hv.w, &si);
-            add_emitted(s, r, pr.light, area_L(s, s.lights[pr.light], si.n, -r.rd, r.lam), r.rd);
-        }
-        st4(b.L, slot, r.L);
-        st4(b.Lh, slot, r.Lh);
-    }
+    for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += stride) emit_vertex(s, b, b.qes[i]);
 }
 
 // ---- stage 2: one BSDF family (path.cpp:208-235) ----------------------------------------------------
@@ -1429,6 +1431,10 @@ __global__ __launch_bounds__(kBlock) PG_WAVES(kMaterialWaves) void k_material_pl
     }
 }
 
+// Shadow rays run from the offset origin to the offset light point, d = target - origin: the
+// query's range is (0, 1 - 1e-4) in units of d (src/scene/interaction.h:43-47, scene.cpp:41-59).
+constexpr float kShadowTMax = 1 - 1e-4f;
+
 // The deferred add() of a visible shadow record: beta * (f * Tr * L * k), and the rPath
 // average at sampling time for the spectral-MIS estimate.
 __device__ __forceinline__ void shadow_add(const Bufs &b, int slot, const SS &T, float k, float a) {
@@ -1468,7 +1474,6 @@ __global__ __launch_bounds__(kTraceBlock) PG_WAVES(8) void k_trace_bounce(const 
     DRay r{};
     TravState t{};
     FetchState fc = fetch_begin(), fs = fetch_begin();
-    const float tMaxShadow = 1 - 1e-4f;
     while (true) {
         bool more;
         if (!fc.done) {
@@ -1487,7 +1492,7 @@ __global__ __launch_bounds__(kTraceBlock) PG_WAVES(8) void k_trace_bounce(const 
                 slot = b.sq[idx];
                 const float4 so = b.so[slot], sd = b.sd[slot];
                 r = DRay{so.x, so.y, so.z, sd.x, sd.y, sd.z};
-                trace_begin(t, r, tMaxShadow);
+                trace_begin(t, r, kShadowTMax);
                 t.any = true;
             } else {
                 slot = queue[idx];
@@ -1535,6 +1540,59 @@ __global__ __launch_bounds__(kTraceBlock) PG_WAVES(8) void k_trace_bounce(const 
 // Transmittance of the shadow records (src/scene/scene.cpp:41-59) and the deferred add().
 // Without media or interfaces a shadow ray is one any-hit query, traversed persistently as in
 // k_intersect; otherwise each ray walks its interface crossings to completion.
+// Transmittance of one shadow record (src/scene/scene.cpp:41-59): one any-hit query, or with
+// interfaces a walk through their crossings, with the media's transmittance along the way; then
+// the deferred add().
+template <bool kMedia, bool kInterfaces, bool kStats>
+__device__ __forceinline__ void shadow_walk(const DevScene &s, const Bufs &b, int slot, uint32_t *lds, int tid,
+                                            TraceStats *ts) {
+    float4 so = b.so[slot], sd = b.sd[slot], spt = b.sp[slot], sn = b.sn[slot];
+    V3 ro(so.x, so.y, so.z), rd(sd.x, sd.y, sd.z), pl(spt.x, spt.y, spt.z), nl(sn.x, sn.y, sn.z);
+    int medium = __float_as_int(so.w);
+    Lambda lam;
+    {
+        float4 l = b.lam[slot], lp = b.lpdf[slot];
+        lam = {{l.x, l.y, l.z, l.w}, {lp.x, lp.y, lp.z, lp.w}};
+    }
+    SS T(1.f);
+    for (int iter = 0;; ++iter) {
+        if (iter == 256) {
+            T = SS(0.f);
+            break;
+        }
+        DRay r{ro.x, ro.y, ro.z, rd.x, rd.y, rd.z};
+        DHit h;
+        if (!kInterfaces) {
+            if (dev_trace<true, kStats>(s.trace, r, kShadowTMax, &h, lds, tid, ts)) {
+                T = SS(0.f);
+            } else if (kMedia && medium >= 0) {
+                float l = len(rd);
+                T = T * medium_tr(s, medium, ro, rd / l, 1.f * l, lam);
+            }
+            break;
+        }
+        bool hit = dev_trace<false, kStats>(s.trace, r, kShadowTMax, &h, lds, tid, ts);
+        const prgpu::Primitive *pr = hit ? &s.prims[h.prim] : nullptr;
+        if (hit && !is_interface(s, *pr)) {
+            T = SS(0.f);
+            break;
+        }
+        if (kMedia && medium >= 0) {
+            float l = len(rd);
+            T = T * medium_tr(s, medium, ro, rd / l, (hit ? h.t : 1.f) * l, lam);
+        }
+        if (!T.nonzero() || !hit) break;
+        Surf si;
+        interaction(*pr, ro, rd, h.t, h.b1, h.b2, &si);
+        medium = dot(rd, si.n) > 0 ? pr->mediumOutside : pr->mediumInside;
+        ro = offset_origin(si.p, si.n, pl - si.p);
+        V3 target = offset_origin(pl, nl, ro - pl);
+        rd = target - ro;
+    }
+    if (!T.nonzero()) return;
+    shadow_add(b, slot, T, spt.w, sn.w);
+}
+
 template <bool kStats, bool kMedia, bool kInterfaces>
 __global__ __launch_bounds__(kTraceBlock) void k_shadow(const DevScene *__restrict__ sp, Bufs b, const int *count,
                                                         int *fetch, unsigned long long *counters) {
@@ -1544,7 +1602,6 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(const DevScene *__restri
     const int tid = int(blockIdx.x * blockDim.x + threadIdx.x);
     TraceStats ts;
     uint32_t rays = 0;
-    const float tMax = 1 - 1e-4f;
     if constexpr (!kMedia && !kInterfaces) {
         int idx = -1, slot = 0;
         bool started = false;
@@ -1556,7 +1613,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(const DevScene *__restri
                 slot = b.sq[idx];
                 float4 so = b.so[slot], sd = b.sd[slot];
                 r = DRay{so.x, so.y, so.z, sd.x, sd.y, sd.z};
-                trace_begin(t, r, tMax);
+                trace_begin(t, r, kShadowTMax);
                 started = true;
             }
             if (!trace_turn(idx >= 0, t.cur)) continue;
@@ -1574,52 +1631,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(const DevScene *__restri
         const int stride = int(gridDim.x * blockDim.x);
         for (int i = tid; i < n; i += stride) {
             ++rays;
-            const int slot = b.sq[i];
-            float4 so = b.so[slot], sd = b.sd[slot], spt = b.sp[slot], sn = b.sn[slot];
-            V3 ro(so.x, so.y, so.z), rd(sd.x, sd.y, sd.z), pl(spt.x, spt.y, spt.z), nl(sn.x, sn.y, sn.z);
-            int medium = __float_as_int(so.w);
-            Lambda lam;
-            {
-                float4 l = b.lam[slot], lp = b.lpdf[slot];
-                lam = {{l.x, l.y, l.z, l.w}, {lp.x, lp.y, lp.z, lp.w}};
-            }
-            SS T(1.f);
-            for (int iter = 0;; ++iter) {
-                if (iter == 256) {
-                    T = SS(0.f);
-                    break;
-                }
-                DRay r{ro.x, ro.y, ro.z, rd.x, rd.y, rd.z};
-                DHit h;
-                if (!kInterfaces) {
-                    if (dev_trace<true, kStats>(s.trace, r, tMax, &h, lds, tid, &ts)) {
-                        T = SS(0.f);
-                    } else if (kMedia && medium >= 0) {
-                        float l = len(rd);
-                        T = T * medium_tr(s, medium, ro, rd / l, 1.f * l, lam);
-                    }
-                    break;
-                }
-                bool hit = dev_trace<false, kStats>(s.trace, r, tMax, &h, lds, tid, &ts);
-                const prgpu::Primitive *pr = hit ? &s.prims[h.prim] : nullptr;
-                if (hit && !is_interface(s, *pr)) {
-                    T = SS(0.f);
-                    break;
-                }
-                if (kMedia && medium >= 0) {
-                    float l = len(rd);
-                    T = T * medium_tr(s, medium, ro, rd / l, (hit ? h.t : 1.f) * l, lam);
-                }
-                if (!T.nonzero() || !hit) break;
-                Surf si;
-                interaction(*pr, ro, rd, h.t, h.b1, h.b2, &si);
-                medium = dot(rd, si.n) > 0 ? pr->mediumOutside : pr->mediumInside;
-                ro = offset_origin(si.p, si.n, pl - si.p);
-                V3 target = offset_origin(pl, nl, ro - pl);
-                rd = target - ro;
-            }
-            if (!T.nonzero()) continue;
-            shadow_add(b, slot, T, spt.w, sn.w);
+            shadow_walk<kMedia, kInterfaces, kStats>(s, b, b.sq[i], lds, tid, &ts);
         }
     }
     if (kStats && rays) {

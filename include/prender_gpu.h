@@ -322,8 +322,9 @@ struct SampleRecordEx {
     uint32_t flags;      // bit 0: L holds NaN/Inf (the film drops the sample)
 };
 
-// Profiling counters of one kind of kernel launch (prgpu_profile_read). Surface: the surface and
-// emission stages; material: the three BSDF-family kernels; guide: guiding inference and training.
+// Profiling counters of one kind of kernel launch (prgpu_profile_read). Intersect: the closest hits
+// (and, without media or interfaces, the previous bounce's shadow rays); surface: the surface and
+// emission stages; material: the BSDF-family kernels; guide: guiding inference and training.
 enum KernelKindEx : int {
     KernCamera = 0, KernIntersect = 1, KernSurface = 2, KernShadow = 3, KernFinish = 4, KernGather = 5, KernGuide = 6,
     KernMaterial = 7, KernCount = 8

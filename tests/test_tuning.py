@@ -55,3 +55,14 @@ def test_missing_tuning_file_means_defaults(tmp_path):
                          capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     assert out.stdout.split("\n")[:2] == ["", ""]
+
+
+def test_committed_tuning_file_beside_the_library_is_valid():
+    """The autotuner's file shipped beside libprender_gpu.so is what the plugin reads by default."""
+    env = {k: v for k, v in os.environ.items() if k != "PRGPU_TUNING_FILE"}
+    out = subprocess.run([sys.executable, "-c", PROBE.format(pkg=os.path.join(ROOT, "ar-renderer_amd"))], env=env,
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    path, err = (out.stdout.split("\n") + ["", ""])[:2]
+    assert os.path.realpath(path) == os.path.realpath(os.path.join(ROOT, "ar-renderer_amd", "prgpu_tuning.txt"))
+    assert err == ""
