@@ -10,8 +10,9 @@ fastest setting to the tuning file the plugin reads when it first enumerates dev
   graph_rounds          bounce rounds replayed per captured HIP graph (0: eager launches)
 
 The objective is the wall time of render calls on a benchmark configuration (default: the
-1920x1080 terrain scene, configuration 2, 4 samples per pixel per call, rendered into the
-device-resident film). Coordinate descent: each key in turn is set to every candidate with the
+1920x1080 terrain scene, configuration 2, rendered into the
+device-resident film; 16 spp per call, four times the bench's step, so that a call spans several
+batches at the smaller queue capacities). Coordinate descent: each key in turn is set to every candidate with the
 others held at their current best, for --passes passes. Every candidate renders the same sample
 ranges, and the film of each candidate is checked against the first one's (a launch shape never
 changes a sample's path; a smaller queue capacity splits a call into more batches, which only
@@ -37,11 +38,13 @@ import prender_amd as pa  # noqa: E402
 CANDIDATES = {
     "trace_blocks_per_cu": [4, 5, 6, 7, 8],
     "shade_blocks_per_cu": [4, 8, 12, 16, 24, 32],
-    "path_budget": [1 << 22, 6 << 20, 1 << 23, 12 << 20, 1 << 24],
+    "path_budget": [1 << 22, 1 << 23, 1 << 24, 1 << 25],
     "graph_rounds": [0, 2, 4],
 }
-# (config, width, height, max depth, rr start, spp per call)
-WORKLOADS = {1: (1, 256, 256, 8, 3, 16), 2: (2, 1920, 1080, 16, 1, 4), 3: (3, 1280, 720, 256, 1, 4),
+# (config, width, height, max depth, rr start, spp per call). Configuration 2 is swept at 16 spp per
+# call (33 M samples): a call larger than the smaller queue capacities, so that the capacity's cost
+# (a batch's late, nearly empty bounces, paid once per batch) is part of the objective.
+WORKLOADS = {1: (1, 256, 256, 8, 3, 16), 2: (2, 1920, 1080, 16, 1, 16), 3: (3, 1280, 720, 256, 1, 4),
              4: (4, 1920, 1080, 32, 1, 4), 5: (5, 3840, 2160, 16, 1, 1)}
 
 

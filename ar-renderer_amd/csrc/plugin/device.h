@@ -53,7 +53,7 @@ struct DevBuf {
 struct Tuning {
     int traceBlocksPerCU = 8;                // persistent traversal grid per CU (1..8: LDS-bound)
     int shadeBlocksPerCU = 16;               // grid of the surface, emission and material kernels
-    int64_t pathBudget = int64_t(8) << 20;   // paths in flight per batch: sizes every queue
+    int64_t pathBudget = int64_t(32) << 20;  // paths in flight per batch: sizes every queue (~15 GB)
     int guideBatch = 1 << 18;                // training records per Adam step
     int guideSteps = 4;                      // Adam steps per batch of paths, at most
     int guideMinSteps = 1;                   // ... and at least, cycling over the records

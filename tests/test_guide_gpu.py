@@ -169,3 +169,98 @@ def test_guided_mean_matches_plain_at_high_spp(plugin, c4):
         assert abs(rel) < tol
         zs.append(abs(z))
     assert zs[1] < 5
+
+
+def _mixture_directions(out, rng):
+    """One direction per row from the decoded 8-lobe vMF mixture of the network's 41 outputs (the
+    decode of guide.h: lobe k's mean = normalised outputs 5k..5k+2, kappa = min(0.01 + softplus(o[5k+3]),
+    1000), weights = softmax of o[5k+4])."""
+    n = len(out)
+    logit = out[:, 4:40:5]
+    w = np.exp(logit - logit.max(1, keepdims=True))
+    w /= w.sum(1, keepdims=True)
+    mu = np.stack([out[:, 5 * k:5 * k + 3] for k in range(8)], 1)
+    ln = np.linalg.norm(mu, axis=2, keepdims=True)
+    mu = np.where(ln > 1e-6, mu / np.maximum(ln, 1e-30), np.array([0.0, 0.0, 1.0]))
+    kap = np.minimum(0.01 + np.logaddexp(0, out[:, 3:40:5]), 1000.0)
+    k = np.minimum((rng.random(n)[:, None] >= np.cumsum(w, 1)).sum(1), 7)
+    m, kp = mu[np.arange(n), k], kap[np.arange(n), k]
+    u1, u2 = rng.random(n), rng.random(n)
+    ct = np.clip(1 + np.log1p(u1 * np.expm1(-2 * kp)) / kp, -1, 1)  # vMF cos theta about the mean
+    return _around(m, ct, 2 * np.pi * u2)
+
+
+def _around(axis, ct, phi):
+    s = np.where(axis[:, 2:3] >= 0, 1.0, -1.0)[:, 0]
+    a = -1 / (s + axis[:, 2])
+    b = axis[:, 0] * axis[:, 1] * a
+    t = np.stack([1 + s * axis[:, 0] ** 2 * a, s * b, -s * axis[:, 0]], 1)
+    u = np.stack([b, s + axis[:, 1] ** 2 * a, -axis[:, 1]], 1)
+    st = np.sqrt(np.maximum(0, 1 - ct * ct))
+    return (st * np.cos(phi))[:, None] * t + (st * np.sin(phi))[:, None] * u + ct[:, None] * axis
+
+
+def _sunlit(plugin, scene, sun, origins, dirs):
+    """Whether the ray from each origin along dirs reaches a point the sun lights directly (the hit
+    face turned to the sun and nothing in between), with the plugin's own ray queries."""
+    rays = np.zeros(len(dirs), pa.RAY_DTYPE)
+    rays["o"] = (origins + 1e-3 * dirs).astype(np.float32)
+    rays["d"] = dirs.astype(np.float32)
+    rays["tMax"] = np.inf
+    h = plugin.trace(scene, rays)
+    hit = h["prim"] >= 0
+    q = scene.prims[h["prim"][hit]]
+    nh = np.cross(q["p1"] - q["p0"], q["p2"] - q["p0"]).astype(np.float64)
+    nh /= np.linalg.norm(nh, axis=1, keepdims=True)
+    nh *= -np.sign(np.sum(nh * dirs[hit], 1))[:, None]
+    ph = origins[hit] + dirs[hit] * h["t"][hit][:, None]
+    facing = nh @ sun > 0
+    s = np.zeros(int(facing.sum()), pa.RAY_DTYPE)
+    s["o"] = (ph[facing] + 1e-3 * nh[facing]).astype(np.float32)
+    s["d"] = sun.astype(np.float32)
+    s["tMax"] = np.inf
+    out = np.zeros(len(dirs), bool)
+    out[np.flatnonzero(hit)[np.flatnonzero(facing)[plugin.trace(scene, s, any_hit=True)["prim"] < 0]]] = True
+    return out
+
+
+def test_trained_guide_sends_samples_toward_the_sunlight(plugin, host):
+    """What the trained network gains over the sampling it is mixed with. The room is lit only by sun
+    through a window, so indirect light comes from the few sunlit patches. After a short training
+    render (8 calls of 16 spp at 160x90, the model trained on the GPU between calls), at the vertices
+    of the last batch's training records, directions drawn from the learned mixture reach a sunlit
+    point far more often than cosine-weighted directions about the same normals (the Lambertian
+    BSDF's proposal). Measured: 14.6% against 1.0%; the bar is 6% and five times cosine."""
+    import raygen
+    scene = host.generate(4, 160, 90, 0.25)
+    keys = ("guide_batch", "guide_min_steps", "guide_lr", "guide_bounces")
+    saved = {k: plugin.tune_get(k) for k in keys}
+    try:
+        for k, v in zip(keys, (65536, 32, 3e-3, 1)):
+            plugin.tune_set(k, v)
+        plugin.film_reset(160, 90)
+        for k in range(8):
+            f = pa.RENDER_GUIDE | pa.RENDER_GUIDE_TRAIN | (pa.RENDER_GUIDE_RESET if k == 0 else 0)
+            plugin.render_resident(scene, pa.render_params(spp=16, first_sample=16 * k, seed=1, max_depth=32,
+                                                           rr_start=1, flags=f))
+        R, produced = plugin.guide_records(1 << 20)
+        rng = np.random.default_rng(5)
+        sub = R[rng.choice(len(R), min(len(R), 20000), replace=False)]
+        out = plugin.guide_eval(sub[:, :9]).astype(np.float64)
+    finally:
+        for k, v in saved.items():
+            plugin.tune_set(k, v)
+    assert len(R) > 5000, f"only {len(R)} training records"
+    sun = np.array(scene.lights[scene.lights["kind"] == pa.LIGHT_DISTANT][0]["direction"], np.float64)
+    sun /= np.linalg.norm(sun)
+    lo, hi = raygen.scene_bounds(scene)
+    pos = lo + sub[:, 0:3].astype(np.float64) * (hi - lo)
+    nrm = sub[:, 3:6].astype(np.float64)
+    nrm *= np.sign(np.sum(nrm * sub[:, 6:9], 1))[:, None]  # the side the path arrived from
+    guided = _mixture_directions(out, rng)
+    cosine = _around(nrm, np.sqrt(1 - rng.random(len(sub))), 2 * np.pi * rng.random(len(sub)))
+    fg = _sunlit(plugin, scene, sun, pos, guided).mean()
+    fc = _sunlit(plugin, scene, sun, pos, cosine).mean()
+    print(f"trained guide: {len(R)} records ({produced} produced); directions reaching sunlit points: "
+          f"mixture {fg:.4f}, cosine {fc:.4f}", flush=True)
+    assert fg >= 0.06 and fg >= 5 * fc
