@@ -5,6 +5,7 @@
 #include <cstring>
 #include <atomic>
 #include <map>
+#include <stdexcept>
 #include <thread>
 
 namespace phost {
@@ -853,16 +854,12 @@ int SceneBuilder::AddEnvironmentMap(const float *rgb, int w, int h, float scale,
     // into the map's frame by the inverse of Rotate(rotation, +y).
     EnvMapDistribution dist;
     dist.Build(rgb, w, h);
-    if (rgbTable_ < 0) {
-        const std::vector<float> &t = RgbToSpectrumTableRecord();
-        rgbTable_ = int(knots.size());
-        knots.insert(knots.end(), t.begin(), t.end());
-    }
+    const int table = RgbTable();
     const int d65 = D65Illuminant();
     const float rad = (3.14159265358979323846f / 180.f) * rotationDeg;
     const int off = int(knots.size());
     const float header[prgpu::kEnvHeader] = {float(w), float(h), std::cos(rad), std::sin(rad), dist.margInt,
-                                             dist.avgY, float(d65), float(rgbTable_)};
+                                             dist.avgY, float(d65), float(table)};
     knots.insert(knots.end(), header, header + prgpu::kEnvHeader);
     knots.insert(knots.end(), rgb, rgb + size_t(w) * size_t(h) * 3);
     knots.insert(knots.end(), dist.func.begin(), dist.func.end());
@@ -878,6 +875,52 @@ int SceneBuilder::AddEnvironmentMap(const float *rgb, int w, int h, float scale,
     l.profileSize = int(knots.size()) - off;
     lights.push_back(l);
     return int(lights.size() - 1);
+}
+
+// Knots offsets that records hold as floats must be exact in a float (below 2^24).
+static int FloatOffset(size_t off) {
+    if (off >= (size_t(1) << 24)) throw std::runtime_error("scene too large: a texture record would start beyond 2^24 knots");
+    return int(off);
+}
+
+// The RGB-to-spectrum table record, added to knots once per scene.
+int SceneBuilder::RgbTable() {
+    if (rgbTable_ < 0) {
+        const std::vector<float> &t = RgbToSpectrumTableRecord();
+        rgbTable_ = FloatOffset(knots.size());
+        knots.insert(knots.end(), t.begin(), t.end());
+    }
+    return rgbTable_;
+}
+
+int SceneBuilder::AddImageTexture(const float *rgb, int w, int h, int wrap, bool flipV, const float uvScale[2],
+                                  const float uvOffset[2], const float mult[3]) {
+    const int table = RgbTable();
+    const int off = FloatOffset(knots.size());
+    const float header[prgpu::kTexHeader] = {float(w),    float(h),    float(wrap), flipV ? 1.f : 0.f,
+                                             uvScale[0],  uvScale[1],  uvOffset[0], uvOffset[1],
+                                             mult[0],     mult[1],     mult[2],     float(table)};
+    knots.insert(knots.end(), header, header + prgpu::kTexHeader);
+    knots.insert(knots.end(), rgb, rgb + size_t(w) * size_t(h) * 3);
+    return off;
+}
+
+int SceneBuilder::ImageSpectrum(int textureRecord) {
+    prgpu::SpectrumEx s{};
+    s.kind = prgpu::SpecImage;
+    s.scale = 1;
+    s.offset = textureRecord;
+    return AddSpectrum(s);
+}
+
+void SceneBuilder::SetShading(int material, int normalRecord, float strength, bool flipGreen, int bumpRecord,
+                              int channel, float bumpScale) {
+    const int off = int(knots.size());
+    const float r[prgpu::kShadeHeader] = {normalRecord >= 0 ? 1.f : 0.f, float(std::max(0, normalRecord)), strength,
+                                          flipGreen ? 1.f : 0.f,          bumpRecord >= 0 ? 1.f : 0.f,
+                                          float(std::max(0, bumpRecord)), float(channel), bumpScale};
+    knots.insert(knots.end(), r, r + prgpu::kShadeHeader);
+    materials[size_t(material)].shading = uint32_t(off + 1);
 }
 
 int SceneBuilder::AddHomogeneous(int sa, int ss, float scale, float g) {

@@ -139,6 +139,16 @@ class SceneBuilder {
     // Image environment (an equirectangular linear-RGB map, rows from the zenith down, w x h):
     // its record (prgpu::kEnvHeader) and, once per scene, the RGB-to-spectrum table go to knots.
     int AddEnvironmentMap(const float *rgb, int w, int h, float scale, float rotationDeg);
+    // Image texture record (prgpu::kTexHeader: linear RGB texels, top row first, w x h; wrap 0
+    // repeat, 1 clamp, 2 mirror; uv scale and offset; RGB multiplier) in knots: its offset.
+    int AddImageTexture(const float *rgb, int w, int h, int wrap, bool flipV, const float uvScale[2],
+                        const float uvOffset[2], const float mult[3]);
+    // An albedo spectrum that looks the texture record up at the hit (prgpu::SpecImage).
+    int ImageSpectrum(int textureRecord);
+    // Normal map (texture record or -1, strength, flip green) and bump map (height texture record or
+    // -1, channel -1 / 0 / 1 / 2, scale) of a material (prgpu::kShadeHeader).
+    void SetShading(int material, int normalRecord, float strength, bool flipGreen, int bumpRecord, int channel,
+                    float bumpScale);
     int AddHomogeneous(int sigmaA, int sigmaS, float scale, float g);
     int AddGrid(int sigmaA, int sigmaS, float scale, float g, int nx, int ny, int nz, const std::vector<float> &data,
                 F3 boundsMin, F3 boundsMax);
@@ -164,6 +174,7 @@ class SceneBuilder {
 
   private:
     int AddSpectrum(const prgpu::SpectrumEx &s);
+    int RgbTable();
     int d65_ = -1;
     int rgbTable_ = -1;  // knots offset of the RGB-to-spectrum table record, once added
     prgpu::SceneDescEx desc_{};

@@ -4,13 +4,15 @@
 // GPU exporter (src/integrators/gpu_path.cpp:104-456) builds from the loaded scene.
 //
 // What the exporter approximates or drops is approximated or dropped here, with the same warning
-// (checker textures keep both colours and the u scale; roughness textures become the default;
-// normal and bump maps are ignored). Distant, spot, IES and sun lights, image environments (PFM and
-// PPM maps) and the Preetham sky of sun-sky lights, which the v4 exporter skips, are kept: the
-// extension has them. Everything the renderer cannot express and the exporter has no fallback for (image and
-// noise colour textures, orthographic and realistic cameras, glTF / pbrt imports, noise media) is a
-// load error naming the item. Illuminant RGB spectra (sigmoid x D65) are tabulated every 5 nm, as
-// the exporter tabulates every spectrum.
+// (checker textures keep both colours and the u scale; roughness textures become the default).
+// Distant, spot, IES and sun lights, image environments, the Preetham sky of sun-sky lights, image
+// textures as a material's reflectance, and normal and bump maps (images in PFM, PPM or PNG), which
+// the v4 exporter averages or skips, are kept: the extension has them.
+// Everything the renderer cannot express and the exporter has no fallback for (image textures in
+// other slots, noise colour textures, orthographic and realistic cameras, glTF / pbrt imports, noise
+// media) is a load error naming the item. Illuminant RGB spectra (sigmoid x D65) are tabulated every
+// 5 nm, as the exporter tabulates every spectrum.
+#include "png_read.h"
 #include "prender_host.h"
 #include "scene_build.h"
 
@@ -275,10 +277,12 @@ std::string ReadFile(const fs::path &p) {
 }
 
 // Relative "file" values (and bare rig paths) are resolved against the file that names them.
-// Images for environment lights (src/core/image.cpp:31-101): PFM (linear float, "PF" RGB or "Pf"
-// grey, bottom row first, negative scale = little-endian) and binary PPM ("P6", 8-bit, decoded from
-// sRGB). Rows come out top row first, linear RGB. Other formats are a named error.
-bool ReadImageRGB(const std::string &path, int *w, int *h, std::vector<float> *rgb, std::string *err) {
+// Images for environment lights and image textures (src/core/image.cpp:31-101): PFM (linear
+// float, "PF" RGB or "Pf" grey, bottom row first, negative scale = little-endian), binary PPM ("P6",
+// 8-bit) and PNG (reduced to 8-bit RGB as stb_image does); 8-bit values are decoded from sRGB
+// unless the image is data ("linear"). Rows come out top row first, linear RGB. Other formats
+// (EXR, HDR, JPEG) are a named error.
+bool ReadImageRGB(const std::string &path, int *w, int *h, std::vector<float> *rgb, std::string *err, bool srgb8) {
     std::string ext = fs::path(path).extension().string();
     for (char &c : ext) c = char(std::tolower(static_cast<unsigned char>(c)));
     FILE *f = std::fopen(path.c_str(), "rb");
@@ -347,17 +351,32 @@ bool ReadImageRGB(const std::string &path, int *w, int *h, std::vector<float> *r
             ok = std::fread(data.data(), 1, data.size(), f) == data.size();
             if (!ok) *err = "truncated PPM";
             float lut[256];
-            for (int i = 0; i < 256; ++i) {  // SRGBDecode (src/core/color.h)
+            for (int i = 0; i < 256; ++i) {  // SRGBDecode (src/core/color.h), or linear (image.cpp:96)
                 float v = float(i) / 255.f;
-                lut[i] = v <= 0.04045f ? v / 12.92f : std::pow((v + 0.055f) / 1.055f, 2.4f);
+                lut[i] = !srgb8 ? v : (v <= 0.04045f ? v / 12.92f : std::pow((v + 0.055f) / 1.055f, 2.4f));
             }
             if (ok) {
                 rgb->resize(data.size());
                 for (size_t i = 0; i < data.size(); ++i) (*rgb)[i] = lut[data[i]];
             }
         }
+    } else if (ext == ".png") {
+        std::vector<unsigned char> file;
+        unsigned char buf[1 << 16];
+        for (size_t n; (n = std::fread(buf, 1, sizeof(buf), f)) > 0;) file.insert(file.end(), buf, buf + n);
+        std::vector<unsigned char> data;
+        ok = DecodePNG(file, w, h, &data, err);
+        if (ok) {
+            float lut[256];  // as for PPM: 8-bit values, sRGB-decoded or linear (image.cpp:96)
+            for (int i = 0; i < 256; ++i) {
+                float v = float(i) / 255.f;
+                lut[i] = !srgb8 ? v : (v <= 0.04045f ? v / 12.92f : std::pow((v + 0.055f) / 1.055f, 2.4f));
+            }
+            rgb->resize(data.size());
+            for (size_t i = 0; i < data.size(); ++i) (*rgb)[i] = lut[data[i]];
+        }
     } else {
-        *err = "image format '" + ext + "' is not read by this loader (PFM and PPM are)";
+        *err = "image format '" + ext + "' is not read by this loader (PFM, PPM and PNG are)";
     }
     std::fclose(f);
     return ok;
@@ -1146,14 +1165,61 @@ class Loader {
         return Spectrum(j, SpecType::Unbounded, "ior");
     }
 
-    // ---- textures (scene_loader.cpp:481-553, exporter gpu_path.cpp:161-184) ----
-    Tex SpecTex(const J &v, SpecType type, const std::string &what) {
+    // ---- textures (scene_loader.cpp:438-553, texture.h:61-124) ----
+    // An image texture's record (GetImage + ParseUVMapping, scene_loader.cpp:438-479): PFM / PPM /
+    // PNG, colorspace (8-bit images decoded from sRGB unless "linear"), wrap, flip_v, uv_scale, uv_offset,
+    // and the RGB multiplier; identical textures share one record.
+    int ImageRecord(const J &tx, const std::string &id, bool linearDefault, const float mult[3]) {
+        if (tx.has("gltf")) Fail("texture '" + id + "': glTF-embedded images are not read by this loader");
+        const std::string file = GetS(tx, "file", "");
+        if (file.empty()) Fail("texture '" + id + "': missing 'file'");
+        const std::string cs = GetS(tx, "colorspace", linearDefault ? "linear" : "srgb");
+        const bool srgb = cs == "srgb" || cs == "sRGB";
+        const std::string wrapS = GetS(tx, "wrap", "repeat");
+        const int wrap = wrapS == "clamp" ? 1 : (wrapS == "mirror" ? 2 : 0);
+        const bool flip = GetB(tx, "flip_v", true);
+        float sc[2] = {1, 1}, of[2] = {0, 0};
+        if (tx.has("uv_scale")) {
+            const J &s = tx.at("uv_scale");
+            if (s.isNum()) sc[0] = sc[1] = float(s.n);
+            else sc[0] = Num(s.a.at(0), "uv_scale"), sc[1] = Num(s.a.at(1), "uv_scale");
+        }
+        if (tx.has("uv_offset")) of[0] = Num(tx.at("uv_offset").a.at(0), "uv_offset"), of[1] = Num(tx.at("uv_offset").a.at(1), "uv_offset");
+        char key[512];
+        std::snprintf(key, sizeof key, "|%d|%d|%d|%a|%a|%a|%a|%a|%a|%a", int(srgb), wrap, int(flip), sc[0], sc[1], of[0],
+                      of[1], mult[0], mult[1], mult[2]);
+        const std::string k = file + key;
+        auto it = imageCache_.find(k);
+        if (it != imageCache_.end()) return it->second;
+        int w = 0, h = 0;
+        std::vector<float> img;
+        std::string err;
+        if (!ReadImageRGB(file, &w, &h, &img, &err, srgb)) Fail("texture '" + id + "': " + err);
+        const int rec = sb_.AddImageTexture(img.data(), w, h, wrap, flip, sc, of, mult);
+        imageCache_[k] = rec;
+        return rec;
+    }
+    static void RGB3(const J &j, float out[3], const std::string &what) {
+        if (j.isNum()) out[0] = out[1] = out[2] = float(j.n);
+        else if (j.isArr() && j.size() == 3)
+            for (int i = 0; i < 3; ++i) out[i] = Num(j.a[size_t(i)], what.c_str());
+        else Fail(what + ": expected a number or [r, g, b]");
+    }
+    // A colour input. Image textures are read for albedo slots that take one (imageOK: a
+    // material's reflectance); elsewhere, and inside checkers, they are a named error.
+    Tex SpecTex(const J &v, SpecType type, const std::string &what, bool imageOK = false) {
         Tex t;
         if (v.isObj() && v.has("texture")) {
             const std::string id = v.at("texture").s;
             if (!doc_.at("textures").has(id)) Fail(what + ": unknown texture '" + id + "'");
             const J &tx = doc_.at("textures").at(id);
             const std::string tt = GetS(tx, "type", "image");
+            if (tt == "image" && imageOK && type == SpecType::Albedo) {
+                float mult[3] = {1, 1, 1};
+                if (tx.has("multiplier")) RGB3(tx.at("multiplier"), mult, "texture '" + id + "': multiplier");
+                t.spec = sb_.ImageSpectrum(ImageRecord(tx, id, false, mult));
+                return t;
+            }
             if (tt == "checker") {
                 Tex a = SpecTex(tx.has("a") ? tx.at("a") : J::num(0.8), type, what);
                 Tex b = SpecTex(tx.has("b") ? tx.at("b") : J::num(0.2), type, what);
@@ -1166,9 +1232,10 @@ class Loader {
                 }
                 return t;
             }
-            if (tt == "constant") return SpecTex(tx.at("value"), type, what);
-            if (tt == "image" || tt == "noise")
-                Fail("texture '" + id + "': " + tt + " colour textures are not supported (checker and constant are)");
+            if (tt == "constant") return SpecTex(tx.at("value"), type, what, imageOK);
+            if (tt == "image")
+                Fail("texture '" + id + "': image textures are read as a material's reflectance only (" + what + ")");
+            if (tt == "noise") Fail("texture '" + id + "': noise colour textures are not supported (checker and constant are)");
             Fail("texture '" + id + "': unknown type '" + tt + "'");
         }
         t.spec = Spectrum(v, type, what);
@@ -1284,13 +1351,60 @@ class Loader {
         } else {
             J def = J::array();
             for (int i = 0; i < 3; ++i) def.a.push_back(J::num(0.9));
-            *refl = SpecTex(j.has("reflectance") ? j.at("reflectance") : def, SpecType::Albedo, "reflectance");
+            *refl = SpecTex(j.has("reflectance") ? j.at("reflectance") : def, SpecType::Albedo, "reflectance", true);
         }
         J r = j.has("roughness") ? j.at("roughness") : J::num(0.0);
         float ur = FloatTex(j.has("roughness_u") ? j.at("roughness_u") : r, 0.f, "roughness_u");
         float vr = FloatTex(j.has("roughness_v") ? j.at("roughness_v") : r, ur, "roughness_v");
         *ax = RoughnessToAlpha(ur);
         *ay = RoughnessToAlpha(vr);
+    }
+    // Normal and bump maps (scene_loader.cpp:732-752): a normal map is an image texture read as data
+    // (strength, flip_green); a bump map's height is a float texture (an image channel or the mean
+    // of its channels, times its multiplier) with a scale. A constant height has no gradient.
+    void ShadingMaps(const J &j, const std::string &id, int m) {
+        int normal = -1, bump = -1, channel = -1;
+        float strength = 1, bumpScale = 0.01f;
+        bool flipGreen = false;
+        const float one[3] = {1, 1, 1};
+        if (j.has("normal_map")) {
+            const J &nm = j.at("normal_map");
+            const std::string tid = nm.isStr() ? nm.s : GetS(nm, "texture", "");
+            if (!doc_.at("textures").has(tid)) Fail("material '" + id + "': unknown normal map texture '" + tid + "'");
+            J t = doc_.at("textures").at(tid);
+            t.set("colorspace") = J::str("linear");  // normal maps are data
+            if (GetS(t, "type", "image") != "image") Fail("material '" + id + "': a normal map is an image texture");
+            normal = ImageRecord(t, tid, true, one);
+            if (nm.isObj()) {
+                strength = GetF(nm, "strength", 1.f);
+                flipGreen = GetB(nm, "flip_green", false);
+            }
+        }
+        if (j.has("bump_map")) {
+            const J &bm = j.at("bump_map");
+            bumpScale = GetF(bm, "scale", 0.01f);
+            std::string tid, ch = "avg";
+            if (bm.has("texture")) {
+                tid = bm.at("texture").s;
+                ch = GetS(bm, "channel", "avg");
+            } else if (bm.has("height") && bm.at("height").isObj() && bm.at("height").has("texture")) {
+                tid = bm.at("height").at("texture").s;
+                ch = GetS(bm.at("height"), "channel", "avg");
+            } else if (!(bm.has("height") && bm.at("height").isNum())) {
+                Fail("material '" + id + "': bump_map needs a texture or a numeric height");
+            }
+            if (!tid.empty()) {
+                if (!doc_.at("textures").has(tid)) Fail("material '" + id + "': unknown bump map texture '" + tid + "'");
+                const J &t = doc_.at("textures").at(tid);
+                if (GetS(t, "type", "image") != "image")
+                    Fail("texture '" + tid + "': bump heights are read from image textures only");
+                channel = ch == "r" ? 0 : ch == "g" ? 1 : ch == "b" ? 2 : -1;
+                const float mu = GetF(t, "multiplier", 1.f);
+                const float mult[3] = {mu, mu, mu};
+                bump = ImageRecord(t, tid, true, mult);
+            }
+        }
+        if (normal >= 0 || bump >= 0) sb_.SetShading(m, normal, strength, flipGreen, bump, channel, bumpScale);
     }
     SceneBuilder::Coat ParseCoat(const J &j) {
         SceneBuilder::Coat c;
@@ -1313,7 +1427,7 @@ class Loader {
         const std::string type = GetS(j, "type", "diffuse");
         auto reflectance = [&]() {
             J r = j.has("reflectance") ? j.at("reflectance") : j.has("color") ? j.at("color") : J::num(0.5);
-            return SpecTex(r, SpecType::Albedo, "reflectance");
+            return SpecTex(r, SpecType::Albedo, "reflectance", true);
         };
         int m = -1;
         if (type == "diffuse") {
@@ -1369,7 +1483,7 @@ class Loader {
                 float rough = FloatTex(s.has("roughness") ? s.at("roughness") : J::num(0.3), 0.3f, "sheen.roughness");
                 sb_.SetSheen(m, col.spec, rough, GetF(s, "weight", 1.f));
             }
-            if (j.has("normal_map") || j.has("bump_map")) Warn("normal/bump maps are ignored on the GPU");
+            if (j.has("normal_map") || j.has("bump_map")) ShadingMaps(j, id, m);
         }
         if (j.has("medium")) materialInterior_[id] = Medium(j.at("medium").s);
         if (j.has("emission") && j.at("emission").k != J::Null) materialEmission_[id] = ParseEmission(j.at("emission"));
@@ -1681,7 +1795,7 @@ class Loader {
                 int w = 0, h = 0;
                 std::vector<float> img;
                 std::string err;
-                if (!ReadImageRGB(GetS(l, "file", ""), &w, &h, &img, &err)) Fail("light '" + id + "': " + err);
+                if (!ReadImageRGB(GetS(l, "file", ""), &w, &h, &img, &err, true)) Fail("light '" + id + "': " + err);
                 sb_.AddEnvironmentMap(img.data(), w, h, GetF(l, "intensity", 1.f) * scaleMul, GetF(l, "rotation", 0.f));
                 return;
             }
@@ -1882,6 +1996,7 @@ class Loader {
     std::vector<std::string> &warn_;
     std::string cameraName_;
     std::map<std::string, int> mediumCache_, materialCache_, materialInterior_;
+    std::map<std::string, int> imageCache_;  // image texture records in knots, by file and parameters
     std::map<std::string, Emission> materialEmission_;
     std::map<std::string, LMesh> meshCache_;
     std::vector<Object> objects_;

@@ -285,8 +285,37 @@ void Validate(const prgpu::SceneDescEx &sd) {
             bad("primitive references out of range");
     }
     auto specOK = [&](int s, bool optional) { return (optional && s < 0) || (s >= 0 && s < sd.spectrumCount); };
+    // An image texture record (kTexHeader) and its RGB-to-spectrum table in bounds.
+    auto texOK = [&](int64_t o) {
+        if (o < 0 || o + prgpu::kTexHeader > sd.knotCount) return false;
+        const float *r = sd.knots + o;
+        const int64_t w = int64_t(r[0]), h = int64_t(r[1]), tab = int64_t(r[11]);
+        if (!(w >= 1 && h >= 1 && float(w) == r[0] && float(h) == r[1] && w * h <= (int64_t(1) << 26) &&
+              (r[2] == 0 || r[2] == 1 || r[2] == 2) && (r[3] == 0 || r[3] == 1) &&
+              o + prgpu::kTexHeader + 3 * w * h <= sd.knotCount && float(tab) == r[11] && tab >= 0 && tab < sd.knotCount))
+            return false;
+        const int64_t n = int64_t(sd.knots[tab]);
+        return n >= 2 && n <= 256 && float(n) == sd.knots[tab] && tab + 1 + n + 3 * n * n * n * 3 <= sd.knotCount;
+    };
+    auto isImage = [&](int s) { return s >= 0 && s < sd.spectrumCount && sd.spectra[s].kind == prgpu::SpecImage; };
     for (int i = 0; i < sd.materialCount; ++i) {
         const prgpu::Material &m = sd.materials[i];
+        // Image textures only in the albedo slot, and not inside a checker.
+        if (isImage(m.albedoB) || isImage(m.eta) || isImage(m.k) || isImage(m.coatEta) || isImage(m.coatAlbedo) ||
+            isImage(m.sheenColor) || (isImage(m.albedo) && m.albedoB >= 0 && m.checkerScale != 0))
+            bad("image texture outside a material's albedo");
+        if (m.shading != 0) {
+            const int64_t o = int64_t(m.shading) - 1;
+            bool ok = o + prgpu::kShadeHeader <= sd.knotCount;
+            if (ok) {
+                const float *r = sd.knots + o;
+                ok = (r[0] == 0 || r[0] == 1) && (r[4] == 0 || r[4] == 1) && (r[3] == 0 || r[3] == 1) &&
+                     (r[0] == 0 || (float(int64_t(r[1])) == r[1] && texOK(int64_t(r[1])))) &&
+                     (r[4] == 0 || (float(int64_t(r[5])) == r[5] && texOK(int64_t(r[5])) &&
+                                    (r[6] == -1 || r[6] == 0 || r[6] == 1 || r[6] == 2)));
+            }
+            if (!ok) bad("shading perturbation record");
+        }
         if (m.type == prgpu::MatDiffuse && !specOK(m.albedo, false)) bad("diffuse reflectance");
         if (m.type == prgpu::MatDiffuse && m.albedoB >= sd.spectrumCount) bad("checker spectrum");
         if (m.type == prgpu::MatConductor && !((specOK(m.eta, false) && specOK(m.k, false)) || specOK(m.albedo, false)))
@@ -307,11 +336,12 @@ void Validate(const prgpu::SceneDescEx &sd) {
         if (s.kind == prgpu::SpecTable95 && (s.offset < 0 || s.offset + prgpu::kSpectrumSamples > sd.knotCount))
             bad("tabulated spectrum");
         if (s.kind == prgpu::SpecSellmeier && (s.count < 0 || s.count > 3)) bad("Sellmeier terms");
-        if (s.kind > prgpu::SpecTable95) bad("spectrum kind");
+        if (s.kind == prgpu::SpecImage && !texOK(s.offset)) bad("image texture record");
+        if (s.kind > prgpu::SpecImage) bad("spectrum kind");
     }
     for (int i = 0; i < sd.lightCount; ++i) {
         const prgpu::LightEx &l = sd.lights[i];
-        if (!specOK(l.spectrum, false)) bad("light spectrum");
+        if (!specOK(l.spectrum, false) || isImage(l.spectrum)) bad("light spectrum");
         if (l.kind == prgpu::LightExArea && (l.primitive < 0 || l.primitive >= sd.primCount)) bad("area light primitive");
         if (l.kind > prgpu::LightExSun) bad("light kind");
         if (l.kind == prgpu::LightExSun && !(l.cosFalloffEnd > -1 && l.cosFalloffEnd < 1)) bad("sun disk");
@@ -350,7 +380,8 @@ void Validate(const prgpu::SceneDescEx &sd) {
     }
     for (int i = 0; i < sd.mediumCount; ++i) {
         const prgpu::MediumEx &m = sd.media[i];
-        if (!specOK(m.sigmaA, false) || !specOK(m.sigmaS, false)) bad("medium spectra");
+        if (!specOK(m.sigmaA, false) || !specOK(m.sigmaS, false) || isImage(m.sigmaA) || isImage(m.sigmaS))
+            bad("medium spectra");
         if (m.kind == prgpu::MediumGrid &&
             (m.nx <= 0 || m.ny <= 0 || m.nz <= 0 || m.gridOffset < 0 ||
              m.gridOffset + int64_t(m.nx) * m.ny * m.nz > sd.gridCount))
@@ -1143,6 +1174,7 @@ __device__ __forceinline__ void surface_vertex(const DevScene &s, const Params &
                 } else {
                     Surf si;
                     interaction(pr, r.ro, r.rd, tHit, hv.z, hv.w, &si);
+                    perturb_shading(s, m, &si);
                     toMaterial = true;
                     mq = (is_coated(m) || has_sheen(m)) ? QLayered
                          : m.type == prgpu::MatDiffuse ? QDiffuse

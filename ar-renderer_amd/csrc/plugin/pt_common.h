@@ -364,6 +364,46 @@ PG_HD void rgb_sigmoid_coeffs(const float *table, const float rgbIn[3], float c[
                     lerp(dy, lerp(dx, at(zi + 1, yi, xi, i), at(zi + 1, yi, xi + 1, i)),
                          lerp(dx, at(zi + 1, yi + 1, xi, i), at(zi + 1, yi + 1, xi + 1, i))));
 }
+// ---- image textures (record layout: prgpu::kTexHeader) ------------------------------------------
+// ImageData::Lookup with the texture's UVMapping (src/materials/texture.cpp:8-36, texture.h:61-64):
+// bilinear RGB at the mapped (u, v), texel centres at half-integers, the wrap mode on both axes.
+PG_HD int tex_wrap(int v, int n, int mode) {
+    if (mode == 0) {
+        v %= n;
+        return v < 0 ? v + n : v;
+    }
+    if (mode == 1) return clampi(v, 0, n - 1);
+    const int period = 2 * n;
+    v %= period;
+    if (v < 0) v += period;
+    return v < n ? v : period - 1 - v;
+}
+PG_HD void tex_rgb(const float *rec, float u, float v, float out[3]) {
+    const int w = int(rec[0]), h = int(rec[1]), mode = int(rec[2]);
+    const float su = u * rec[4] + rec[6], sv = v * rec[5] + rec[7];
+    const float vv = rec[3] != 0 ? 1 - sv : sv;
+    // (coordinates beyond +-2^24 texels are clamped before the integer conversion)
+    const float x = clampf(su * float(w) - 0.5f, -16777216.f, 16777216.f);
+    const float y = clampf(vv * float(h) - 0.5f, -16777216.f, 16777216.f);
+    const int x0 = int(floorf(x)), y0 = int(floorf(y));
+    const float dx = x - float(x0), dy = y - float(y0);
+    const float *px = rec + prgpu::kTexHeader;
+    auto texel = [&](int xi, int yi) {
+        return px + 3 * (size_t(tex_wrap(yi, h, mode)) * size_t(w) + size_t(tex_wrap(xi, w, mode)));
+    };
+    const float *a = texel(x0, y0), *b = texel(x0 + 1, y0), *c = texel(x0, y0 + 1), *d = texel(x0 + 1, y0 + 1);
+    for (int i = 0; i < 3; ++i)
+        out[i] = (1 - dx) * (1 - dy) * a[i] + dx * (1 - dy) * b[i] + (1 - dx) * dy * c[i] + dx * dy * d[i];
+}
+// ImageFloatTexture::Evaluate (texture.h:111-116): one channel, or the mean of the three, times the
+// texture's multiplier.
+PG_HD float tex_float(const float *rec, int channel, float u, float v) {
+    float c[3];
+    tex_rgb(rec, u, v, c);
+    const float x = channel < 0 ? (c[0] + c[1] + c[2]) / 3 : c[channel];
+    return x * rec[8];
+}
+
 // The sigmoid polynomial at wavelength l (RGBSigmoidPolynomial, src/core/rgb2spec.h:13-27).
 PG_HD float sigmoid_poly(const float c[3], float l) {
     float x = fmaf(l, fmaf(l, c[0], c[1]), c[2]);

@@ -157,6 +157,7 @@ PG_HD bool has_bsdf(const DevScene &s, const prgpu::Primitive &p) {
 // ---- surface interactions (src/geometry/shape.cpp:63-85, 224-264) --------------------------------
 struct Surf {
     V3 p, n, ns, dpdus, wo;
+    V3 dpdv;  // the parameterisation's v derivative (bump mapping)
     float u, v;
 };
 
@@ -188,6 +189,7 @@ PG_HD void interaction(const prgpu::Primitive &pr, V3 ro, V3 rd, float tHit, flo
         if (len2(dpdu) < 1e-12f || len2(dpdv) < 1e-12f) coordinate_system(n, &dpdu, &dpdv);
         si->ns = si->n;
         si->dpdus = dpdu;
+        si->dpdv = dpdv;
         return;
     }
     V3 p0 = f3(pr.p0), p1 = f3(pr.p1), p2 = f3(pr.p2);
@@ -218,6 +220,7 @@ PG_HD void interaction(const prgpu::Primitive &pr, V3 ro, V3 rd, float tHit, flo
     si->n = n;
     si->ns = ns;
     si->dpdus = dpdu;
+    si->dpdv = dpdv;
     si->wo = -normalize(rd);
 }
 
@@ -944,6 +947,71 @@ struct LBSDF {
     PG_HD bool non_specular() const { return (flags() & (kDiffuse | kGlossy)) != 0; }
 };
 
+// A material's albedo spectrum at the hit (src/materials/material.cpp:64-117): a spectrum, or an
+// image texture (ImageSpectrumTexture as an albedo, texture.cpp:38-52: the multiplied bilinear
+// lookup, clamped to [0, 1], uplifted through the RGB-to-spectrum table, RGBAlbedoSample).
+PG_HD SS albedo_sample(const DevScene &s, int id, const Surf &si, const Lambda &lam) {
+    const prgpu::SpectrumEx &sp = s.spec.s[id];
+    if (sp.kind != prgpu::SpecImage) return spec_sample(s.spec, id, lam);
+    const float *rec = s.spec.knots + sp.offset;
+    float rgb[3], c[3];
+    tex_rgb(rec, si.u, si.v, rgb);
+    for (int i = 0; i < 3; ++i) rgb[i] = rgb[i] * rec[8 + i];
+    rgb_sigmoid_coeffs(s.spec.knots + int(rec[11]), rgb, c);
+    SS out;
+    for (int i = 0; i < 4; ++i) out.v[i] = sigmoid_poly(c, lam.l[i]);
+    return out;
+}
+
+// Material::PerturbShading (src/materials/material.cpp:16-58): a bump map's finite-difference
+// height gradient in uv tilts the shading normal, then a normal map's tangent-space normal
+// replaces it in the shading frame; the result is kept in the geometric normal's hemisphere and
+// the shading tangent re-orthogonalised. The record is at knots[m.shading - 1] (kShadeHeader).
+PG_HD void perturb_shading(const DevScene &s, const prgpu::Material &m, Surf *si) {
+    if (m.shading == 0) return;
+    const float *knots = s.spec.knots;
+    const float *r = knots + (m.shading - 1);
+    V3 ns = si->ns;
+    bool changed = false;
+    if (r[4] != 0) {
+        const float *hrec = knots + int(r[5]);
+        const int ch = int(r[6]);
+        const float du = 1.f / 2048, dv = 1.f / 2048;
+        const float h0 = tex_float(hrec, ch, si->u, si->v);
+        const float dhdu = (tex_float(hrec, ch, si->u + du, si->v) - h0) / du * r[7];
+        const float dhdv = (tex_float(hrec, ch, si->u, si->v + dv) - h0) / dv * r[7];
+        const V3 dpdu = si->dpdus + ns * dhdu, dpdv = si->dpdv + ns * dhdv;
+        V3 n = cross_d(dpdu, dpdv);
+        const float l = len(n);
+        if (l > 0 && is_finite(l)) {
+            n = n / l;
+            if (dot(n, ns) < 0) n = -n;
+            ns = n;
+            si->dpdus = dpdu;
+            changed = true;
+        }
+    }
+    if (r[0] != 0) {
+        float c[3];
+        tex_rgb(knots + int(r[1]), si->u, si->v, c);
+        V3 t(2 * c[0] - 1, 2 * c[1] - 1, 2 * c[2] - 1);
+        t.x *= r[2];
+        t.y *= r[3] != 0 ? -r[2] : r[2];
+        Frame f = Frame::from_xz(si->dpdus, ns);
+        if (dot(f.y, si->dpdv) < 0) f.y = -f.y;  // the bitangent follows the parameterisation's v
+        const V3 n = f.x * t.x + f.y * t.y + f.z * maxf(t.z, 1e-3f);
+        const float l = len(n);
+        if (l > 0 && is_finite(l)) {
+            ns = n / l;
+            changed = true;
+        }
+    }
+    if (!changed) return;
+    if (dot(ns, si->n) < 0) ns = -ns;
+    si->ns = ns;
+    si->dpdus = si->dpdus - ns * dot(si->dpdus, ns);
+}
+
 // src/materials/material.cpp:64-117 (textures: constant spectra and uv checkers).
 PG_HD int checker_pick(const prgpu::Material &m, float u, float v, int a) {
     if (m.albedoB >= 0 && m.checkerScale != 0) {
@@ -955,7 +1023,7 @@ PG_HD int checker_pick(const prgpu::Material &m, float u, float v, int a) {
 }
 
 PG_HD void diffuse_bx(const DevScene &s, const prgpu::Material &m, const Surf &si, const Lambda &lam, BxDF &bx) {
-    SS r = spec_sample(s.spec, checker_pick(m, si.u, si.v, m.albedo), lam);
+    SS r = albedo_sample(s, checker_pick(m, si.u, si.v, m.albedo), si, lam);
     for (int i = 0; i < 4; ++i) r.v[i] = clampf(r.v[i], 0.f, 1.f);
     bx.type = BxDiffuse;
     bx.a = r;
@@ -974,7 +1042,7 @@ PG_HD void conductor_bx(const DevScene &s, const prgpu::Material &m, const Surf 
             bx.b = bx.b / coatEta;
         }
     } else {
-        SS r = spec_sample(s.spec, checker_pick(m, si.u, si.v, m.albedo), lam);
+        SS r = albedo_sample(s, checker_pick(m, si.u, si.v, m.albedo), si, lam);
         for (int i = 0; i < 4; ++i) r.v[i] = clampf(r.v[i], 0.f, 0.9999f);
         bx.a = SS(1.f);
         bx.b = ss_sqrt(r) * 2.f / ss_sqrt(ss_clamp0(SS(1.f) - r));

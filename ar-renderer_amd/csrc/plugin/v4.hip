@@ -19,6 +19,7 @@ namespace pgpu {
 namespace {
 
 struct V4Scene {
+    std::vector<prgpu::Material> materials;  // the caller's, with the padding word cleared
     std::vector<prgpu::SpectrumEx> spectra;
     std::vector<float> knots;
     std::vector<prgpu::LightEx> lights;
@@ -114,7 +115,11 @@ void Export(const prgpu::SceneDesc &sd, V4Scene &out) {
     d.extVersion = prgpu::kExtVersion;
     d.prims = sd.prims;  // same 128-byte layout
     d.primCount = sd.primCount;
-    d.materials = sd.materials;  // same 88-byte layout; spectrum indices are kept
+    // Same 88-byte layout and spectrum indices; the v4 padding word is the extension's shading
+    // record index, which a v4 scene never has.
+    out.materials.assign(sd.materials, sd.materials + std::max(0, sd.materialCount));
+    for (prgpu::Material &m : out.materials) m.shading = 0;
+    d.materials = out.materials.data();
     d.materialCount = sd.materialCount;
     d.spectra = out.spectra.data();
     d.spectrumCount = int32_t(out.spectra.size());

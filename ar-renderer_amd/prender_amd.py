@@ -18,13 +18,13 @@ PLUGIN_PATH = os.path.join(HERE, "libprender_gpu.so")
 HOST_PATH = os.path.join(HERE, "libprender_host.so")
 
 API_VERSION = 4
-EXT_VERSION = 3
+EXT_VERSION = 4
 
 # ---- enums (include/prender_gpu.h) ---------------------------------------------------------------
 PRIM_TRIANGLE, PRIM_SPHERE = 0, 1
 MAT_DIFFUSE, MAT_CONDUCTOR, MAT_DIELECTRIC, MAT_BLACK, MAT_INTERFACE = 0, 1, 2, 3, 4
 MAT_COATED_DIFFUSE, MAT_COATED_CONDUCTOR = 5, 6
-SPEC_CONSTANT, SPEC_SIGMOID, SPEC_BLACKBODY, SPEC_PIECEWISE, SPEC_SELLMEIER, SPEC_CAUCHY, SPEC_TABLE95 = range(7)
+SPEC_CONSTANT, SPEC_SIGMOID, SPEC_BLACKBODY, SPEC_PIECEWISE, SPEC_SELLMEIER, SPEC_CAUCHY, SPEC_TABLE95, SPEC_IMAGE = range(8)
 LIGHT_AREA, LIGHT_POINT, LIGHT_DISTANT, LIGHT_ENV, LIGHT_SPOT, LIGHT_GONIO, LIGHT_SUN = 0, 1, 2, 3, 4, 5, 6
 MEDIUM_HOMOGENEOUS, MEDIUM_GRID = 0, 1
 FILTER_BOX, FILTER_GAUSSIAN, FILTER_BH = 0, 1, 2
@@ -42,7 +42,7 @@ PRIM_DTYPE = np.dtype({
 MATERIAL_DTYPE = np.dtype({
     "names": ["type", "albedo", "albedoB", "checkerScale", "eta", "k", "alphaX", "alphaY", "dispersive", "thin",
               "coatEta", "coatAlpha", "thickness", "coatG", "coatAlbedo", "coatDispersive", "coatMaxDepth",
-              "coatSamples", "sheenColor", "sheenRoughness", "sheenWeight", "pad"],
+              "coatSamples", "sheenColor", "sheenRoughness", "sheenWeight", "shading"],
     "formats": ["<u4", "<i4", "<i4", "<f4", "<i4", "<i4", "<f4", "<f4", "<u4", "<u4", "<i4", "<f4", "<f4", "<f4",
                 "<i4", "<u4", "<i4", "<i4", "<i4", "<f4", "<f4", "<u4"],
     "itemsize": 88})

@@ -79,7 +79,8 @@ struct Material {  // gpu_api.h:61-81
     int32_t coatMaxDepth, coatSamples;
     int32_t sheenColor;
     float sheenRoughness, sheenWeight;
-    uint32_t pad;
+    uint32_t shading;  // extension scenes: knots offset + 1 of the shading-perturbation record (normal / bump
+                       // map, kShadeHeader), 0 = none; v4 padding (the v4 entry point clears it)
 };
 
 struct Medium {  // gpu_api.h:84-87: homogeneous, sigma = spectrum * scale, Henyey-Greenstein g
@@ -165,7 +166,7 @@ struct DeviceInfo {  // gpu_api.h:156-162
 // ---------------------------------------------------------------------------------------------
 // Part 2: extension (no counterpart in the reference ABI; see the file comment).
 
-constexpr int kExtVersion = 3;
+constexpr int kExtVersion = 4;
 
 // Analytic spectrum: value(lambda) = scale * base(lambda). Mirrors the reference Spectrum
 // classes (src/core/spectrum.h:151-231, src/core/rgb2spec.h:13-74) so the device evaluates
@@ -178,7 +179,21 @@ enum SpectrumKind : uint32_t {
     SpecSellmeier = 4,  // base = sqrt(max(1, 1 + sum_i p[i] l2 / (l2 - p[3+i]))), l in um, count terms (<= 3)
     SpecCauchy = 5,     // base = p[0] + p[1] / l^2
     SpecTable95 = 6,    // base = linear interpolation of knots[offset .. offset+95) at 5 nm from 360 nm (v4 tables)
+    SpecImage = 7,      // an image texture: the record at knots[offset] (kTexHeader), looked up at the hit's uv and
+                        // uplifted as an albedo (RGBAlbedoSample); only in a material's albedo slot
 };
+
+// Record of an image texture in SceneDescEx::knots (ImageSpectrumTexture / ImageFloatTexture,
+// reference src/materials/texture.h:67-124, texture.cpp:8-52): [0] width w, [1] height h, [2] wrap
+// (0 repeat, 1 clamp, 2 mirror), [3] flip v (1: v = 1 - v before the lookup), [4] [5] uv scale,
+// [6] [7] uv offset, [8] [9] [10] RGB multiplier, [11] the knots offset of the RGB-to-spectrum table
+// record; then w x h linear RGB texels, top row first.
+constexpr int kTexHeader = 12;  // (record offsets held in floats are below 2^24)
+// Record of a material's shading perturbation (Material::PerturbShading, src/materials/
+// material.cpp:16-58), at knots[Material::shading - 1]: [0] 1 with a normal map, [1] its texture
+// record's knots offset, [2] strength, [3] flip green (1 / 0); [4] 1 with a bump map, [5] its height
+// texture record's knots offset, [6] the channel read (-1: the mean of r, g, b), [7] the bump scale.
+constexpr int kShadeHeader = 8;
 
 struct SpectrumEx {  // 48 bytes
     uint32_t kind;

@@ -70,6 +70,8 @@ def lib():
         L.or_sample_hg.argtypes = [vp, C.c_float, vp, vp, vp]
         L.or_filter.argtypes = [C.c_int, C.c_float, vp, C.c_int, vp, vp]
         L.or_light_sample_li.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp]
+        L.or_tex_lookup.argtypes = [vp, vp, C.c_int, vp]
+        L.or_perturb_shading.argtypes = [vp, vp, vp, C.c_int, vp]
         L.or_light_pdf_li.restype = C.c_float
         L.or_light_pdf_li.argtypes = [vp, C.c_int, vp, vp]
         L.or_light_power.restype = C.c_float

@@ -1474,6 +1474,122 @@ struct LightSample {
 
 struct Scene;
 
+// RGBToSpectrumTable::operator() (src/core/rgb2spec.cpp:278-304) over a table record (resolution n,
+// n z nodes, coefficients [max component][z][y][x][3]).
+static void rgb_table_coeffs(const float *table, const float rgb[3], float co[3]) {
+    float rgbc[3] = {clampv(rgb[0], 0.f, 1.f), clampv(rgb[1], 0.f, 1.f), clampv(rgb[2], 0.f, 1.f)};
+    if (rgbc[0] == rgbc[1] && rgbc[1] == rgbc[2]) {
+        float v = rgbc[0];
+        co[0] = co[1] = 0;
+        co[2] = v <= 0 ? -std::numeric_limits<float>::infinity()
+                       : (v >= 1 ? std::numeric_limits<float>::infinity() : (v - .5f) / std::sqrt(v * (1 - v)));
+        return;
+    }
+    const int n = int(table[0]);
+    const float *zNodes = table + 1, *cf = zNodes + n;
+    int maxc = (rgbc[0] > rgbc[1]) ? ((rgbc[0] > rgbc[2]) ? 0 : 2) : ((rgbc[1] > rgbc[2]) ? 1 : 2);
+    float z = rgbc[maxc];
+    float x = rgbc[(maxc + 1) % 3] * (n - 1) / z;
+    float y = rgbc[(maxc + 2) % 3] * (n - 1) / z;
+    int xi = std::min(int(x), n - 2), yi = std::min(int(y), n - 2);
+    int zi = find_interval(n, [&](int i) { return zNodes[i] < z; });
+    float dx = x - xi, dy = y - yi, dz = (z - zNodes[zi]) / (zNodes[zi + 1] - zNodes[zi]);
+    auto at = [&](int zz, int yy, int xx, int i) {
+        return cf[3 * (((size_t(maxc) * n + zz) * n + yy) * n + xx) + i];
+    };
+    for (int i = 0; i < 3; ++i)
+        co[i] = lerpf(dz,
+                      lerpf(dy, lerpf(dx, at(zi, yi, xi, i), at(zi, yi, xi + 1, i)),
+                            lerpf(dx, at(zi, yi + 1, xi, i), at(zi, yi + 1, xi + 1, i))),
+                      lerpf(dy, lerpf(dx, at(zi + 1, yi, xi, i), at(zi + 1, yi, xi + 1, i)),
+                            lerpf(dx, at(zi + 1, yi + 1, xi, i), at(zi + 1, yi + 1, xi + 1, i))));
+}
+
+// ---- image textures (src/materials/texture.cpp:8-52, texture.h:61-124) ----------------------------
+static int tex_wrap(int v, int n, int mode) {  // ImageData::Texel's wrap (texture.cpp:10-20)
+    switch (mode) {
+    case 0: v %= n; return v < 0 ? v + n : v;
+    case 1: return clampv(v, 0, n - 1);
+    default: {
+        int period = 2 * n;
+        v %= period;
+        if (v < 0) v += period;
+        return v < n ? v : period - 1 - v;
+    }
+    }
+}
+// ImageData::Lookup(UVMapping::Map(uv)) on a texture record (kTexHeader).
+static void tex_rgb(const float *rec, V2 uv, float out[3]) {
+    const int w = int(rec[0]), h = int(rec[1]), mode = int(rec[2]);
+    V2 st{uv.x * rec[4] + rec[6], uv.y * rec[5] + rec[7]};
+    float v = rec[3] != 0 ? 1 - st.y : st.y;
+    float x = st.x * w - 0.5f, y = v * h - 0.5f;
+    int x0 = int(std::floor(x)), y0 = int(std::floor(y));
+    float dx = x - x0, dy = y - y0;
+    auto texel = [&](int xi, int yi) {
+        return rec + prgpu::kTexHeader + 3 * (size_t(tex_wrap(yi, h, mode)) * size_t(w) + size_t(tex_wrap(xi, w, mode)));
+    };
+    const float *a = texel(x0, y0), *b = texel(x0 + 1, y0), *c = texel(x0, y0 + 1), *d = texel(x0 + 1, y0 + 1);
+    for (int i = 0; i < 3; ++i)
+        out[i] = (1 - dx) * (1 - dy) * a[i] + dx * (1 - dy) * b[i] + (1 - dx) * dy * c[i] + dx * dy * d[i];
+}
+static float tex_float(const float *rec, int channel, V2 uv) {  // ImageFloatTexture::Evaluate
+    float c[3];
+    tex_rgb(rec, uv, c);
+    float v = channel < 0 ? (c[0] + c[1] + c[2]) / 3 : c[channel];
+    return v * rec[8];
+}
+// Material::PerturbShading (src/materials/material.cpp:16-58) with a shading record (kShadeHeader,
+// texture records in knots): a bump map's finite-difference height gradient in uv tilts the
+// shading normal, a normal map's tangent-space normal replaces it in the shading frame; the result
+// is kept in the geometric normal's hemisphere and the shading tangent re-orthogonalised. Returns
+// whether the frame changed.
+static bool perturb_record(const float *knots, const float *r, V2 uv, V3 n, V3 dpdu, V3 dpdv, V3 *nsIo, V3 *dpdusIo) {
+    V3 ns = *nsIo;
+    bool changed = false;
+    if (r[4] != 0) {
+        const float *hrec = knots + int(r[5]);
+        const int ch = int(r[6]);
+        const float du = 1.f / 2048, dv = 1.f / 2048;
+        V2 cu = uv, cv = uv;
+        cu.x += du;
+        cv.y += dv;
+        float h0 = tex_float(hrec, ch, uv);
+        float dhdu = (tex_float(hrec, ch, cu) - h0) / du * r[7];
+        float dhdv = (tex_float(hrec, ch, cv) - h0) / dv * r[7];
+        V3 pu = dpdu + ns * dhdu, pv = dpdv + ns * dhdv;
+        V3 nb = cross_d(pu, pv);
+        float l = len(nb);
+        if (l > 0 && std::isfinite(l)) {
+            nb = nb / l;
+            if (dot(nb, ns) < 0) nb = -nb;
+            ns = nb;
+            *dpdusIo = pu;
+            changed = true;
+        }
+    }
+    if (r[0] != 0) {
+        float c[3];
+        tex_rgb(knots + int(r[1]), uv, c);
+        V3 t(2 * c[0] - 1, 2 * c[1] - 1, 2 * c[2] - 1);
+        t.x *= r[2];
+        t.y *= r[3] != 0 ? -r[2] : r[2];
+        Frame f = Frame::from_xz(*dpdusIo, ns);
+        if (dot(f.y, dpdv) < 0) f.y = -f.y;  // the bitangent follows the parameterisation's v
+        V3 nm = f.x * t.x + f.y * t.y + f.z * std::max(t.z, 1e-3f);
+        float l = len(nm);
+        if (l > 0 && std::isfinite(l)) {
+            ns = nm / l;
+            changed = true;
+        }
+    }
+    if (!changed) return false;
+    if (dot(ns, n) < 0) ns = -ns;
+    *nsIo = ns;
+    *dpdusIo = *dpdusIo - ns * dot(*dpdusIo, ns);
+    return true;
+}
+
 struct EnvImage;
 
 struct LightObj {
@@ -1726,33 +1842,9 @@ struct EnvImage {
         float st = std::sin(theta);
         return V3(st * std::cos(phi), std::cos(theta), st * std::sin(phi));
     }
-    void coeffs(float r, float g, float b, float co[3]) const {  // RGBToSpectrumTable::operator()
-        float rgbc[3] = {clampv(r, 0.f, 1.f), clampv(g, 0.f, 1.f), clampv(b, 0.f, 1.f)};
-        if (rgbc[0] == rgbc[1] && rgbc[1] == rgbc[2]) {
-            float v = rgbc[0];
-            co[0] = co[1] = 0;
-            co[2] = v <= 0 ? -std::numeric_limits<float>::infinity()
-                           : (v >= 1 ? std::numeric_limits<float>::infinity() : (v - .5f) / std::sqrt(v * (1 - v)));
-            return;
-        }
-        const int n = int(table[0]);
-        const float *zNodes = table.data() + 1, *cf = zNodes + n;
-        int maxc = (rgbc[0] > rgbc[1]) ? ((rgbc[0] > rgbc[2]) ? 0 : 2) : ((rgbc[1] > rgbc[2]) ? 1 : 2);
-        float z = rgbc[maxc];
-        float x = rgbc[(maxc + 1) % 3] * (n - 1) / z;
-        float y = rgbc[(maxc + 2) % 3] * (n - 1) / z;
-        int xi = std::min(int(x), n - 2), yi = std::min(int(y), n - 2);
-        int zi = find_interval(n, [&](int i) { return zNodes[i] < z; });
-        float dx = x - xi, dy = y - yi, dz = (z - zNodes[zi]) / (zNodes[zi + 1] - zNodes[zi]);
-        auto at = [&](int zz, int yy, int xx, int i) {
-            return cf[3 * (((size_t(maxc) * n + zz) * n + yy) * n + xx) + i];
-        };
-        for (int i = 0; i < 3; ++i)
-            co[i] = lerpf(dz,
-                          lerpf(dy, lerpf(dx, at(zi, yi, xi, i), at(zi, yi, xi + 1, i)),
-                                lerpf(dx, at(zi, yi + 1, xi, i), at(zi, yi + 1, xi + 1, i))),
-                          lerpf(dy, lerpf(dx, at(zi + 1, yi, xi, i), at(zi + 1, yi, xi + 1, i)),
-                                lerpf(dx, at(zi + 1, yi + 1, xi, i), at(zi + 1, yi + 1, xi + 1, i))));
+    void coeffs(float r, float g, float b, float co[3]) const {
+        const float rgb[3] = {r, g, b};
+        rgb_table_coeffs(table.data(), rgb, co);
     }
     SS lookup(const SpectrumTable &spec, float scale, V3 wWorld, const Lambda &lam) const {  // light.cpp:249-261
         float u, v;
@@ -2309,6 +2401,29 @@ struct Scene {
         return SS(0.f);
     }
 
+    // A material's albedo at the hit: a spectrum, or an image texture as an albedo
+    // (ImageSpectrumTexture::Evaluate with RGBAlbedoSample, texture.cpp:38-52).
+    SS albedo(int id, const SurfHit &si, const Lambda &lam) const {
+        const SpectrumEx &s = spec.specs[size_t(id)];
+        if (s.kind != prgpu::SpecImage) return spec.sample(id, lam);
+        const float *rec = spec.knots.data() + s.offset;
+        float c[3], co[3];
+        tex_rgb(rec, si.uv, c);
+        for (int i = 0; i < 3; ++i) c[i] = c[i] * rec[8 + i];
+        rgb_table_coeffs(spec.knots.data() + int(rec[11]), c, co);
+        SS r;
+        for (int i = 0; i < 4; ++i) r.v[i] = sigmoid_s(std::fma(lam.l[i], std::fma(lam.l[i], co[0], co[1]), co[2]));
+        return r;
+    }
+    // Material::PerturbShading with the material's record (knots[m.shading - 1]); EvaluateSurface
+    // (material.cpp:60-66) then keeps the perturbed shading normal and tangent in the interaction.
+    void perturb_shading(SurfHit &si) const {
+        const Material &m = mats[size_t(prims[size_t(si.prim)].material)];
+        if (m.shading == 0) return;
+        perturb_record(spec.knots.data(), spec.knots.data() + (m.shading - 1), si.uv, si.n, si.dpdu, si.dpdv, &si.ns,
+                       &si.dpdus);
+    }
+
     // ---- materials (src/materials/material.cpp:5-14, 64-166) ----
     int checker(const prgpu::Material &m, const SurfHit &si, int sid) const {  // CheckerSpectrumTexture (texture.h:126-134)
         if (m.albedoB >= 0 && m.checkerScale != 0) {
@@ -2319,7 +2434,7 @@ struct Scene {
         return sid;
     }
     void diffuse_bx(const prgpu::Material &m, const SurfHit &si, const Lambda &lam, BxDF &bx) const {
-        SS r = spec.sample(checker(m, si, m.albedo), lam);
+        SS r = albedo(checker(m, si, m.albedo), si, lam);
         for (int i = 0; i < 4; ++i) r.v[i] = clampv(r.v[i], 0.f, 1.f);
         bx.type = BxDiffuse;
         bx.R = r;
@@ -2336,7 +2451,7 @@ struct Scene {
                 bx.k = bx.k / coatEta;
             }
         } else {
-            SS r = spec.sample(checker(m, si, m.albedo), lam);
+            SS r = albedo(checker(m, si, m.albedo), si, lam);
             for (int i = 0; i < 4; ++i) r.v[i] = clampv(r.v[i], 0.f, 0.9999f);
             bx.eta = SS(1.f);
             bx.k = ss_sqrt(r) * 2.f / ss_sqrt(ss_clamp0(SS(1.f) - r));
@@ -2538,6 +2653,7 @@ static SS li(const Scene &sc, const Settings &st, Ray ray, Lambda &lam, Sampler 
         }
 
         if (!sc.has_bsdf(si.prim)) break;
+        sc.perturb_shading(si);
         BSDF bsdf = sc.eval_surface(si, lam);
         if (depth++ >= maxDepth) break;
 
@@ -3101,6 +3217,24 @@ float or_light_pdf_li(void *h, int light, const float *p, const float *wi) {
     return sc.pdf_li(light, V3(p[0], p[1], p[2]), V3(wi[0], wi[1], wi[2]));
 }
 float or_light_power(void *h, int light) { return static_cast<Scene *>(h)->light_power(light); }
+
+// Texture probes: the lookup of a texture record at n uv points (out n x RGB), and the shading
+// perturbation of a shading record for n contexts (p, uv, n, ns, dpdus, dpdu, dpdv: 20 floats;
+// out: ns, dpdus, changed: 7 floats).
+void or_tex_lookup(const float *rec, const float *uv, int n, float *out) {
+    for (int i = 0; i < n; ++i) tex_rgb(rec, V2{uv[2 * i], uv[2 * i + 1]}, out + 3 * size_t(i));
+}
+void or_perturb_shading(const float *knots, const float *shade, const float *ctx, int n, float *out) {
+    for (int i = 0; i < n; ++i) {
+        const float *c = ctx + 20 * size_t(i);
+        V3 ns(c[8], c[9], c[10]), dpdus(c[11], c[12], c[13]);
+        bool changed = perturb_record(knots, shade, V2{c[3], c[4]}, V3(c[5], c[6], c[7]), V3(c[14], c[15], c[16]),
+                                      V3(c[17], c[18], c[19]), &ns, &dpdus);
+        float *o = out + 7 * size_t(i);
+        o[0] = ns.x, o[1] = ns.y, o[2] = ns.z, o[3] = dpdus.x, o[4] = dpdus.y, o[5] = dpdus.z;
+        o[6] = changed ? 1.f : 0.f;
+    }
+}
 
 // Medium probes: transmittance and distance sampling along a ray.
 void or_medium_tr(void *h, int medium, const float *o, const float *d, float tMax, const float *lam, const float *lpdf,

@@ -18,6 +18,8 @@
 #include "lights/light.h"
 #include "lights/light_bvh.h"
 #include "materials/bxdf.h"
+#include "materials/material.h"
+#include "materials/texture.h"
 #include "scene/scene.h"
 
 #include "../include/prender_gpu.h"
@@ -172,6 +174,71 @@ void rf_envmap_sample(const float *rgb, int w, int h, const float *u, int n, flo
         float *o = out + 5 * size_t(i);
         o[0] = s.x, o[1] = s.y, o[2] = pdf, o[3] = d.PDF(s);
         o[4] = light.PDF_Li(LightSampleContext(), EquirectToDir(s));
+    }
+}
+
+// The reference's image and mapping from a texture record in this build's layout (kTexHeader:
+// w, h, wrap, flip v, uv scale, uv offset, multiplier, table offset; texels top row first).
+static std::shared_ptr<const ImageData> RecordImage(const float *rec, UVMapping *map) {
+    const int w = int(rec[0]), h = int(rec[1]);
+    Image img(w, h);
+    const float *px = rec + prgpu::kTexHeader;
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            const float *c = px + 3 * (size_t(y) * w + x);
+            img.Set(x, y, RGB(c[0], c[1], c[2]));
+        }
+    map->scale = Vec2f(rec[4], rec[5]);
+    map->offset = Vec2f(rec[6], rec[7]);
+    return std::make_shared<const ImageData>(std::move(img), WrapMode(int(rec[2])), rec[3] != 0);
+}
+
+// ImageData::Lookup(UVMapping::Map(uv)) (texture.cpp:22-36) at n points: out = n x RGB.
+void rf_tex_lookup(const float *rec, const float *uv, int n, float *out) {
+    UVMapping map;
+    auto img = RecordImage(rec, &map);
+    for (int i = 0; i < n; ++i) {
+        RGB c = img->Lookup(map.Map(Vec2f(uv[2 * i], uv[2 * i + 1])));
+        out[3 * i] = c.r, out[3 * i + 1] = c.g, out[3 * i + 2] = c.b;
+    }
+}
+
+// Material::PerturbShading (material.cpp:16-58) of a diffuse material carrying the normal and bump
+// maps of a shading record in this build's layout (kShadeHeader, texture records in knots). ctx
+// per case: p, uv, n, ns, dpdus, dpdu, dpdv (20 floats); out per case: ns, dpdus, changed (7).
+void rf_perturb_shading(const float *knots, const float *shade, const float *ctx, int n, float *out) {
+    // (a constant reflectance: the RGB uplift lives in rgb2spec.cpp, which is not built here)
+    DiffuseMaterial m(std::make_shared<ConstantSpectrumTexture>(std::make_shared<ConstantSpectrum>(0.5f), RGB(0.5f, 0.5f, 0.5f)));
+    if (shade[0] != 0) {
+        NormalMapParams np;
+        np.image = RecordImage(knots + int(shade[1]), &np.mapping);
+        np.strength = shade[2];
+        np.flipGreen = shade[3] != 0;
+        m.SetNormalMap(np);
+    }
+    if (shade[4] != 0) {
+        const float *hrec = knots + int(shade[5]);
+        UVMapping map;
+        auto img = RecordImage(hrec, &map);
+        BumpMapParams bp;
+        bp.height = std::make_shared<ImageFloatTexture>(img, map, int(shade[6]), hrec[8]);
+        bp.scale = shade[7];
+        m.SetBumpMap(bp);
+    }
+    for (int i = 0; i < n; ++i) {
+        const float *c = ctx + 20 * size_t(i);
+        MaterialEvalContext e;
+        e.p = Vec3f(c[0], c[1], c[2]);
+        e.uv = Vec2f(c[3], c[4]);
+        e.n = Vec3f(c[5], c[6], c[7]);
+        e.ns = Vec3f(c[8], c[9], c[10]);
+        e.dpdus = Vec3f(c[11], c[12], c[13]);
+        e.dpdu = Vec3f(c[14], c[15], c[16]);
+        e.dpdv = Vec3f(c[17], c[18], c[19]);
+        const bool changed = m.PerturbShading(e);
+        float *o = out + 7 * size_t(i);
+        o[0] = e.ns.x, o[1] = e.ns.y, o[2] = e.ns.z, o[3] = e.dpdus.x, o[4] = e.dpdus.y, o[5] = e.dpdus.z;
+        o[6] = changed ? 1.f : 0.f;
     }
 }
 

@@ -42,6 +42,8 @@ def load_ref():
         ("rf_sun_radiance", None, [vp, C.c_float, C.c_float, vp]),
         ("rf_preetham_sky", None, [vp, C.c_float, C.c_float, C.c_int, C.c_int, vp]),
         ("rf_envmap_sample", None, [vp, C.c_int, C.c_int, vp, C.c_int, vp]),
+        ("rf_tex_lookup", None, [vp, vp, C.c_int, vp]),
+        ("rf_perturb_shading", None, [vp, vp, vp, C.c_int, vp]),
         ("rf_trace_counts", None, [vp, vp, C.c_int64, vp, vp]),
         ("rf_bxdf", None, [vp, C.c_int, vp, vp, C.c_float, vp, vp]),
         ("rf_fr_dielectric", C.c_float, [C.c_float, C.c_float]),
@@ -329,10 +331,76 @@ def main():
     envmap_sampling(L, out)
     thin_lens_camera(L, out, host)
     c2_light_bvh(L, out, host)
+    texture_lookup(L, out)
+    shading_perturbation(L, out)
 
     path = os.path.join(HERE, "ref_components.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")
+
+
+def texture_record(rgb, wrap, flip, scale, offset, mult=(1, 1, 1)):
+    """A texture record in the extension's layout (kTexHeader): header, then texels top row first."""
+    h, w, _ = rgb.shape
+    head = [w, h, wrap, flip, scale[0], scale[1], offset[0], offset[1], mult[0], mult[1], mult[2], 0]
+    return f32(np.concatenate([f32(head), f32(rgb).reshape(-1)]))
+
+
+def texture_lookup(L, out):
+    """ImageData::Lookup with UVMapping (src/materials/texture.cpp:8-36) on a 3 x 2 image for every
+    wrap mode (repeat, clamp, mirror), with and without the v flip, under two uv mappings, at 300 uv
+    points in [-1.5, 2.5]^2 and at texel centres and edges."""
+    rng = np.random.default_rng(4409)
+    img = f32(rng.random((2, 3, 3)))
+    uv = f32(rng.uniform(-1.5, 2.5, (300, 2)))
+    uv[:8] = [[1 / 6, 0.25], [0.5, 0.75], [0, 0], [1, 1], [1 / 3, 0.5], [2 / 3, 0.5], [-1 / 6, -0.25], [0.999, 0.001]]
+    cases, recs, refs = [], [], []
+    for wrap in (0, 1, 2):
+        for flip in (0, 1):
+            for scale, offset in (((1, 1), (0, 0)), ((2.5, -1.5), (0.3, -0.7))):
+                rec = texture_record(img, wrap, flip, scale, offset)
+                res = np.zeros((len(uv), 3), np.float32)
+                L.rf_tex_lookup(ptr(rec), ptr(uv), len(uv), ptr(res))
+                cases.append([wrap, flip, *scale, *offset])
+                recs.append(rec)
+                refs.append(res)
+    out.update(tex_cases=f32(cases), tex_recs=np.stack(recs), tex_uv=uv, tex_ref=np.stack(refs))
+
+
+def shading_perturbation(L, out):
+    """Material::PerturbShading (src/materials/material.cpp:16-58) for normal maps (strength 1 and 2.5,
+    green flipped or not), bump maps (the mean of the channels, one channel, a multiplier, two scales)
+    and both together, on 120 random shading contexts (uv in [0, 1]^2, random geometric and shading
+    normals and parameterisation derivatives), including shading normals opposite the geometric one."""
+    rng = np.random.default_rng(7717)
+    nimg = f32(np.clip(rng.normal([0.5, 0.5, 0.85], [0.2, 0.2, 0.1], (3, 4, 3)), 0, 1))
+    himg = f32(rng.random((4, 5, 3)))
+    nrec = texture_record(nimg, 0, 1, (1.0, 1.0), (0.0, 0.0))
+    hrec = texture_record(himg, 2, 0, (1.5, 0.75), (0.1, 0.2), (0.7, 0.7, 0.7))
+    knots = np.concatenate([nrec, hrec])
+    no, ho = 0.0, float(len(nrec))
+    shades = f32([[1, no, 1.0, 0, 0, 0, -1, 0.0], [1, no, 2.5, 1, 0, 0, -1, 0.0],
+                  [0, 0, 1.0, 0, 1, ho, -1, 0.01], [0, 0, 1.0, 0, 1, ho, 1, 0.2],
+                  [1, no, 1.0, 0, 1, ho, 2, 0.05], [1, no, 0.5, 1, 1, ho, -1, 0.5]])
+    n = 120
+    ctx = np.zeros((n, 20), np.float32)
+    ctx[:, 0:3] = rng.normal(size=(n, 3))
+    ctx[:, 3:5] = rng.random((n, 2))
+    g = rng.normal(size=(n, 3))
+    g /= np.linalg.norm(g, axis=1, keepdims=True)
+    ns = g + 0.3 * rng.normal(size=(n, 3))
+    ns /= np.linalg.norm(ns, axis=1, keepdims=True)
+    ns[::7] *= -1
+    du = np.cross(g, rng.normal(size=(n, 3))) * rng.uniform(0.5, 3, (n, 1))
+    dv = np.cross(g, du) / np.linalg.norm(du, axis=1, keepdims=True) * rng.uniform(0.5, 3, (n, 1))
+    dv[::5] *= -1
+    ctx[:, 5:8], ctx[:, 8:11], ctx[:, 11:14], ctx[:, 14:17], ctx[:, 17:20] = g, ns, du, du, dv
+    refs = []
+    for sh in shades:
+        res = np.zeros((n, 7), np.float32)
+        L.rf_perturb_shading(ptr(knots), ptr(sh), ptr(ctx), n, ptr(res))
+        refs.append(res)
+    out.update(shade_knots=knots, shade_recs=shades, shade_ctx=ctx, shade_ref=np.stack(refs))
 
 
 def thin_lens_camera(L, out, host):

@@ -1,5 +1,5 @@
 """Image environment lights (src/lights/light.cpp:215-297) and the sun-sky's Preetham map, on the
-CPU: the loader's PFM / PPM reading and its record in the scene's knots; the oracle's sampling
+CPU: the loader's PFM / PPM / PNG reading and its record in the scene's knots; the oracle's sampling
 against the host's Distribution2D tables (themselves pinned bit for bit against the reference's);
 and what the mathematics fixes for the path loop under an image environment:
 - a pixel that sees a constant map directly gets the map's colour as an RGB illuminant: for white,
@@ -62,9 +62,13 @@ def test_pfm_and_ppm_maps_load_into_the_record(host, tmpdir):
 
 
 def test_unreadable_maps_are_named_errors(host, tmpdir):
+    with open(f"{tmpdir}/bad.jpg", "wb") as f:
+        f.write(b"\xff\xd8\xff....")
+    with pytest.raises(RuntimeError, match="not read by this loader"):
+        es.load(host, tmpdir, "jpg", es.env_doc(f"{tmpdir}/bad.jpg", objects=False))
     with open(f"{tmpdir}/bad.png", "wb") as f:
         f.write(b"\x89PNG....")
-    with pytest.raises(RuntimeError, match="not read by this loader"):
+    with pytest.raises(RuntimeError, match="not a PNG file"):
         es.load(host, tmpdir, "png", es.env_doc(f"{tmpdir}/bad.png", objects=False))
     with open(f"{tmpdir}/short.pfm", "wb") as f:
         f.write(b"PF\n4 4\n-1.0\n" + b"\0" * 20)

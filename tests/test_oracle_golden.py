@@ -4,7 +4,8 @@ tests/golden/ref_components.npz holds inputs and outputs of the reference's RNG 
 (src/integrators/integrator.h:29-38), SampledWavelengths::SampleVisible and the CIE fits
 (src/core/spectrum.*), ToXYZ, the analytic spectra and named metal/glass data, the BxDFs
 (src/materials/bxdf.*), Fresnel terms, Henyey-Greenstein, the pixel filters (src/core/film.*),
-and, on the configuration-1 scene, the reference BVH (src/accel/bvh.cpp), light BVH
+image textures and shading perturbation (src/materials/texture.cpp, material.cpp), and, on the
+configuration-1 scene, the reference BVH (src/accel/bvh.cpp), light BVH
 (src/lights/light_bvh.*), area-light sampling and the perspective camera. They were written by
 tests/golden/make_goldens.py from oracle/_ref/libprref.so (the reference sources compiled as
 they are). CPU only.
@@ -295,3 +296,26 @@ def test_spot_and_goniometric_lights_match_reference(host, name, prefix, kind, c
             got = L.or_light_pdf_li(o.h, k, p(f32(P[i])), p(f32(ref[4:7])))
             np.testing.assert_allclose(got, G[f"{prefix}_lpdf"][i], rtol=2e-5, err_msg=f"pdf {i}")
     assert lit > 50  # the new light type is sampled inside and across its profile
+
+
+def test_image_texture_lookup_bit_exact():
+    """ImageData::Lookup with the texture's UVMapping (src/materials/texture.cpp:8-36): every wrap
+    mode, with and without the v flip, two uv mappings, texel centres and edges and points far
+    outside [0, 1]^2 -- the record's texels read back bit for bit as the reference reads its Image."""
+    uv = G["tex_uv"]
+    for case, rec, ref in zip(G["tex_cases"], G["tex_recs"], G["tex_ref"]):
+        out = np.zeros_like(ref)
+        L.or_tex_lookup(p(rec), p(uv), len(uv), p(out))
+        assert np.array_equal(out.view(np.uint32), ref.view(np.uint32)), f"lookup {case}"
+
+
+def test_shading_perturbation_bit_exact():
+    """Material::PerturbShading (src/materials/material.cpp:16-58): normal maps (strength, flipped
+    green), bump maps (mean or one channel, multiplier, scale) and both together, on random shading
+    contexts -- the perturbed shading normal and tangent, and whether the frame changed."""
+    knots, ctx = G["shade_knots"], G["shade_ctx"]
+    for k, (sh, ref) in enumerate(zip(G["shade_recs"], G["shade_ref"])):
+        out = np.zeros_like(ref)
+        L.or_perturb_shading(p(knots), p(sh), p(ctx), len(ctx), p(out))
+        assert np.array_equal(out.view(np.uint32), ref.view(np.uint32)), f"shading record {k}"
+        assert ref[:, 6].all()

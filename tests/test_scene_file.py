@@ -566,7 +566,9 @@ BASE = {"materials": {"m": {"type": "diffuse"}}, "geometry": {"g": {"type": "sph
     ({"objects": [{"geometry": "g"}]}, "missing 'material'"),
     ({"active_rigs": ["nope"]}, "unknown light rig 'nope'"),
     ({"materials": {"m": {"type": "diffuse", "reflectance": {"texture": "t"}}},
-      "textures": {"t": {"type": "image", "file": "x.png"}}}, "image colour textures are not supported"),
+      "textures": {"t": {"type": "image", "file": "x.png"}}}, "texture 't': cannot open"),
+    ({"materials": {"m": {"type": "diffuse", "reflectance": {"texture": "t"}}},
+      "textures": {"t": {"type": "noise"}}}, "noise colour textures are not supported"),
     ({"materials": {"m": {"type": "plastic"}}}, "unknown type 'plastic'"),
     ({"materials": {"m": {"type": "conductor", "metal": "Pt"}}}, "unknown metal 'Pt'"),
     ({"materials": {"m": {"type": "dielectric", "ior": "unobtainium"}}}, "unknown IOR 'unobtainium'"),
@@ -653,6 +655,17 @@ def test_reference_diamonds(host):
     rig = json.loads(strip_comments(open(os.path.join(root, "rigs", "jeweller.json")).read()))
     rects = sum(1 for l in rig["lights"].values() if l["type"] == "rect")
     assert len(s.prims) == expected + 2 * rects
-    assert any("bump" in w for w in s.warnings)
+    # the silk's bump map is the scene's 512x512 8-bit grey PNG, read as linear data, tiled 35 times
+    shaded = [int(m["shading"]) for m in s.materials if m["shading"] > 0]
+    assert len(shaded) == 1
+    rec = s.knots[shaded[0] - 1:shaded[0] + 7]
+    assert (rec[0], rec[4], rec[6]) == (0, 1, -1) and rec[7] == pytest.approx(0.01)
+    bump = s.knots[int(rec[5]):]
+    np.testing.assert_array_equal(bump[:8], [512, 512, 0, 1, 35, 35, 0, 0])
+    texels = bump[12:12 + 512 * 512 * 3].reshape(512, 512, 3)
+    np.testing.assert_array_equal(texels[..., 0], texels[..., 2])
+    np.testing.assert_array_equal(np.round(texels * 255) / np.float32(255), texels)
+    assert 0.1 < texels.mean() < 0.9 and texels.min() == 0 and texels.max() == 1
+    assert not any("bump" in w for w in s.warnings)
     both = host.load_scene(os.path.join(root, "scene.prscene.json"), rigs=["sparkle", "window"])
     assert (both.lights["kind"] == pa.LIGHT_ENV).sum() == 1
