@@ -8,6 +8,7 @@ fastest setting to the tuning file the plugin reads when it first enumerates dev
   shade_blocks_per_cu   blocks per CU of the surface, emission and material kernels
   path_budget           paths in flight per batch: the capacity of every ray and material queue
   graph_rounds          bounce rounds replayed per captured HIP graph (0: eager launches)
+  material_sort         sort the material queues by material and medium each bounce (0 / 1)
 
 The objective is the wall time of render calls on a benchmark configuration (default: the
 1920x1080 terrain scene, configuration 2, rendered into the
@@ -15,7 +16,8 @@ device-resident film; 16 spp per call, four times the bench's step, so that a ca
 batches at the smaller queue capacities). Coordinate descent: each key in turn is set to every candidate with the
 others held at their current best, for --passes passes. Every candidate renders the same sample
 ranges, and the film of each candidate is checked against the first one's (a launch shape never
-changes a sample's path; a smaller queue capacity splits a call into more batches, which only
+changes a sample's path, nor does the order the material stage takes its queues in; a smaller
+queue capacity splits a call into more batches, which only
 regroups the FP64 film additions: relative 1e-9), so a setting that rendered differently is never
 chosen.
 
@@ -40,6 +42,7 @@ CANDIDATES = {
     "shade_blocks_per_cu": [4, 8, 12, 16, 24, 32],
     "path_budget": [1 << 22, 1 << 23, 1 << 24, 1 << 25],
     "graph_rounds": [0, 2, 4],
+    "material_sort": [0, 1],
 }
 # (config, width, height, max depth, rr start, spp per call). Configuration 2 is swept at 16 spp per
 # call (33 M samples): a call larger than the smaller queue capacities, so that the capacity's cost

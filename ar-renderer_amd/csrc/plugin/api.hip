@@ -64,7 +64,7 @@ const TuneKey kTuneKeys[] = {
     {"guide_steps", 1, 64, true},                      {"guide_min_steps", 0, 64, true},
     {"guide_lr", 1e-6, 1.0, false},                    {"guide_bounces", 0, 64, true},
     {"guide_blend", 0.0, 1.0, false},                  {"graph_rounds", 0, 16, true},
-    {"guide_replay", 0, 1 << 24, true},
+    {"guide_replay", 0, 1 << 24, true},                {"material_sort", 0, 1, true},
 };
 
 bool TuneSet(Tuning &t, const std::string &key, double v) {
@@ -81,6 +81,7 @@ bool TuneSet(Tuning &t, const std::string &key, double v) {
         else if (key == "guide_bounces") t.guideBounces = int(v);
         else if (key == "guide_blend") t.guideBlend = float(v);
         else if (key == "guide_replay") t.guideReplay = int(v);
+        else if (key == "material_sort") t.materialSort = int(v);
         else if (key == "graph_rounds") {
             if (int(v) % 2) return false;  // a replay must end on the queue it started from
             t.graphRounds = int(v);
@@ -102,6 +103,7 @@ bool TuneGet(const Tuning &t, const std::string &key, double *v) {
     else if (key == "guide_blend") *v = t.guideBlend;
     else if (key == "graph_rounds") *v = t.graphRounds;
     else if (key == "guide_replay") *v = t.guideReplay;
+    else if (key == "material_sort") *v = t.materialSort;
     else return false;
     return true;
 }

@@ -62,6 +62,7 @@ struct Tuning {
     float guideBlend = 0.5f;                 // weight of the MIS-proposal term in the training loss
     int guideReplay = 1 << 20;               // training records kept across batches (0: the batch's only)
     int graphRounds = 0;                     // bounce rounds per captured HIP graph (even; 0: none)
+    int materialSort = 0;                    // sort the material queues by material and medium each bounce
 };
 
 struct PTScene;     // path-tracing data (pt.hip)

@@ -217,12 +217,13 @@ struct PTScene {
     float powerInt = 0;
     float worldR = 1;
     int hasInterfaces = 0, hasMedia = 0, hasLayered = 0;  // hasLayered: coated or sheen materials
+    int sortable = 0;  // more than one material or medium: the material sort can group anything
     float cieYInt = 1;
     // host copies for inspection
     std::vector<LNode> hostNodes;
     // batch buffers
     size_t cap = 0;
-    DevBuf o, d, rng, lam, lpdf, beta, rpath, L, Lh, prevP, prevN, st, hit, q0, q1, qm, pfilm, xyz;
+    DevBuf o, d, rng, lam, lpdf, beta, rpath, L, Lh, prevP, prevN, st, hit, q0, q1, qm, qk, qms, sortBins, pfilm, xyz;
     DevBuf h0, h1, h2, h3, qe, qeLpdf, qes;
     DevBuf sceneBuf;  // the DevScene the kernels read through a pointer
     DevBuf so, sd, sp, sn, sf, sL, sb, sq;  // shadow records and the shadow queue
@@ -435,6 +436,7 @@ void EnsurePTScene(Device &dev, const prgpu::SceneDescEx &sd, const float *sheen
     pt->cieYInt = host_cie_y_integral();
     pt->sheen.upload(sheenTable ? sheenTable : HostSheenTable().data(), size_t(prgpu::kSheenAlpha) * prgpu::kSheenMu, s);
     pt->hasLayered = 0;
+    pt->sortable = (sd.materialCount > 1 || sd.mediumCount > 1) ? 1 : 0;
     for (int i = 0; i < sd.materialCount; ++i)
         pt->hasLayered |= (is_coated(sd.materials[i]) || has_sheen(sd.materials[i])) ? 1 : 0;
 
@@ -701,10 +703,15 @@ struct Bufs {
     float4 *h0, *h1, *h2, *h3;  // surface hit records for the material stage
     int *q0, *q1;
     int *qm;     // material queues, QCount x qcap
+    int *qk;     // with the material sort: each material-queue entry's sort key (its material, or for a
+                 // medium scattering vertex its medium), written beside it; null without the sort
+    int *qms;    // the material queues sorted by key, which the material stage then reads
+    int *sortTotals, *sortCursor;  // per family and key bin: entries, then each bin's next position
     int qcap;
     int *counts;  // [0] queue A, [1] queue B, [2] and [11] shadow queues (alternating bounces when the
                   // shadow rays are traced with the next bounce's closest hits), [3..5] plain
-                  // material queues, [9] emission queue, [10] coated / sheen material queue,
+                  // material queues, [6] medium scattering vertices, [9] emission queue, [10]
+                  // coated / sheen material queue,
                   // [16..16+P) closest-hit fetch counters, [32..32+P) shadow fetch counters
                   // (P = kFetchParts)
     float2 *pfilm;
@@ -761,15 +768,17 @@ __device__ __forceinline__ int wave_append(int *counter) {
 constexpr int kAppendK = PG_APPEND_K;
 constexpr int kTile = kAppendK * kBlock;
 static_assert(kTile <= 2048, "tile positions are 11 bits");
-// Queue ids: the material families 0..3, then the next bounce, emission and shadow queues.
-enum QueueId : int { QNext = 4, QEmit = 5, QShadow = 6, QIds = 7 };
-constexpr uint32_t kNoQueue = 7;
+// Queue ids: the material families 0..4 (the last: medium scattering vertices), then the next
+// bounce, emission and shadow queues.
+enum QueueId : int { QNext = 5, QEmit = 6, QShadow = 7, QIds = 8 };
+constexpr uint32_t kNoQueue = 15;
 
 struct TileStage {
     int cnt[8];             // entries of the current tile per queue
     int base[8];            // each queue's reserved range for the tile
     int slot[kTile];        // the item's path slot
-    uint32_t code[kTile];   // two entries of 14 bits: queue id << 11 | position in the tile's range
+    uint32_t code[kTile];   // two entries of 15 bits: queue id << 11 | position in the tile's range
+    int key[kTile];         // the material-sort key of the first entry (material queues only)
 };
 
 __device__ __forceinline__ void tile_begin(TileStage &t) {
@@ -796,19 +805,21 @@ __device__ __forceinline__ int lds_append(int *cnt, int q) {
     return pos;
 }
 
-__device__ __forceinline__ void tile_stage(TileStage &t, int j, int slot, int q1, int q2) {
+__device__ __forceinline__ void tile_stage(TileStage &t, int j, int slot, int q1, int q2, int key = 0) {
     const int p1 = lds_append(t.cnt, q1), p2 = lds_append(t.cnt, q2);
     const uint32_t c1 = q1 < 0 ? (kNoQueue << 11) : (uint32_t(q1) << 11) | uint32_t(p1);
     const uint32_t c2 = q2 < 0 ? (kNoQueue << 11) : (uint32_t(q2) << 11) | uint32_t(p2);
     t.slot[j] = slot;
     t.code[j] = c1 | (c2 << 16);
+    t.key[j] = key;
 }
 
 __device__ __forceinline__ int *queue_ptr(const Bufs &b, int *next, int q) {
     return q < QNext ? b.qm + q * b.qcap : (q == QNext ? next : (q == QEmit ? b.qes : b.sq));
 }
-// Index in Bufs::counts of material family q's queue length (the layered family's is 10).
-__device__ __host__ constexpr int mat_count_index(int q) { return q == 3 ? 10 : 3 + q; }
+// Index in Bufs::counts of material family q's queue length (the layered family's is 10, the
+// medium scattering vertices' 6).
+__device__ __host__ constexpr int mat_count_index(int q) { return q == 3 ? 10 : (q == 4 ? 6 : 3 + q); }
 __device__ __forceinline__ int *count_ptr(const Bufs &b, int *nextCount, int q) {
     return q < QNext ? b.counts + mat_count_index(q) : (q == QNext ? nextCount : (q == QEmit ? b.counts + 9 : b.counts + b.shadowIdx));
 }
@@ -829,9 +840,14 @@ __device__ __forceinline__ void tile_flush(TileStage &t, const Bufs &b, int *nex
         const int slot = t.slot[j];
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-            const uint32_t ce = (code >> (16 * e)) & 0x3fffu;
+            const uint32_t ce = (code >> (16 * e)) & 0x7fffu;
             const int q = int(ce >> 11);
-            if (q != int(kNoQueue)) queue_ptr(b, next, q)[t.base[q] + int(ce & 0x7ffu)] = slot;
+            if (q != int(kNoQueue)) {
+                const int pos = t.base[q] + int(ce & 0x7ffu);
+                queue_ptr(b, next, q)[pos] = slot;
+                // a material queue entry (only an item's first) keeps its sort key beside it
+                if (e == 0 && q < int(QNext) && b.qk) b.qk[size_t(q) * b.qcap + pos] = t.key[j];
+            }
         }
     }
 }
@@ -1046,18 +1062,22 @@ __device__ __forceinline__ void add_emitted(const DevScene &s, PathRegs &r, int 
 }
 
 // ---- stage 1 of a bounce: everything up to the BSDF (path.cpp:155-210) ----------------------------
-// Medium sampling (and, for scattering events, the whole medium vertex), escape and area
-// emission, interfaces, the depth limit. Paths that continue at a surface are compacted into one
-// queue per BSDF family together with their hit record; paths that meet emission (an emitting
-// primitive, or escape with an environment light) are appended to the emission queue, whose
-// MIS-weighted contributions k_emit adds before the BSDF stage changes the throughput.
-enum MatQueue : int { QDiffuse = 0, QConductor = 1, QDielectric = 2, QLayered = 3, QCount = 4 };
+// Medium distance sampling, escape and area emission, interfaces, the depth limit. Paths that
+// continue at a surface are compacted into one queue per BSDF family together with their hit
+// record, and paths that scatter in a medium into the phase-function family with the scattering
+// point: the material stage runs both kinds of vertex (next-event estimation, then the direction
+// sample), so this stage carries no light-sampling code and keeps its registers for the hit
+// records' gathers. Paths that meet emission (an emitting primitive, or escape with an
+// environment light) are appended to the emission queue, whose MIS-weighted contributions k_emit
+// adds before the BSDF stage changes the throughput.
+enum MatQueue : int { QDiffuse = 0, QConductor = 1, QDielectric = 2, QLayered = 3, QPhase = 4, QCount = 5 };
 static_assert(int(QCount) == int(QNext), "the next-bounce queue id follows the material families");
 
 // One path's surface stage; *q1 is the queue the path continues in (a material family, QNext:
 // straight to the next bounce, -1: ended) and *q2 the emission or shadow queue (or -1).
 template <bool kMedia, bool kInterfaces>
-__device__ __forceinline__ void surface_vertex(const DevScene &s, const Params &p, const Bufs &b, int slot, int *q1, int *q2) {
+__device__ __forceinline__ void surface_vertex(const DevScene &s, const Params &p, const Bufs &b, int slot, int *q1, int *q2,
+                                               int *key) {
     PathRegs r;
     float4 o = b.o[slot], d = b.d[slot];
     r.ro = V3(o.x, o.y, o.z);
@@ -1093,16 +1113,12 @@ __device__ __forceinline__ void surface_vertex(const DevScene &s, const Params &
     int mq = 0;
 
     if (kMedia && r.medium >= 0) {
-        // The medium vertex needs the whole path state (throughput, MIS ratios, radiance).
+        // Distance sampling (path.cpp:159-191): the throughput and the spectral-MIS ratios take
+        // the sample's weights; a scattering event moves the path to the phase-function queue.
         float4 l = b.lam[slot];
         r.lam.l[0] = l.x, r.lam.l[1] = l.y, r.lam.l[2] = l.z, r.lam.l[3] = l.w;
         r.beta = ld4(b.beta, slot);
         r.rpath = ld4(b.rpath, slot);
-        r.L = ld4(b.L, slot);
-        r.Lh = ld4(b.Lh, slot);
-        float4 pp = b.prevP[slot], pn = b.prevN[slot];
-        r.prevP = V3(pp.x, pp.y, pp.z);
-        r.prevN = V3(pn.x, pn.y, pn.z);
         r.rng = load_rng(b, slot);
         DistSample ds = medium_sample(s, r.medium, r.ro, r.rd, hit ? tHit : kInfF, r.rng.uniform(), r.lam);
         r.beta = r.beta * ds.weight;
@@ -1115,33 +1131,16 @@ __device__ __forceinline__ void surface_vertex(const DevScene &s, const Params &
             if (r.depth++ >= p.maxDepth) {
                 alive = false;
             } else {
-                V3 mp = r.ro + r.rd * ds.t;
-                V3 mwo = -r.rd;
-                float g = clampf(s.media[r.medium].g, -0.99f, 0.99f);
-                if (sample_ld<kMedia>(s, b, slot, r, mp, V3(), mwo, static_cast<const BSDF *>(nullptr), g, V3(), false, -1, -1))
-                *q2 = QShadow;
-                float phasePdf;
-                V3 wi = sample_hg(mwo, g, get2(r.rng), &phasePdf);
-                if (phasePdf == 0) {
-                    alive = false;
-                } else {
-                    r.prevPdf = phasePdf;
-                    r.prevP = mp;
-                    r.prevN = V3();
-                    r.flags &= ~kSpecularFlag;
-                    r.ro = mp;
-                    r.rd = wi;
-                    if (roulette(p.rr, p.rrStart, r)) alive = false;
-                }
+                const V3 mp = r.ro + r.rd * ds.t;
+                b.h0[slot] = make_float4(mp.x, mp.y, mp.z, 0.f);
+                toMaterial = true;
+                mq = QPhase;
+                *key = r.medium;
             }
         }
         b.rng[slot] = make_ulonglong2(r.rng.state, r.rng.inc);
         st4(b.beta, slot, r.beta);
         st4(b.rpath, slot, r.rpath);
-        st4(b.L, slot, r.L);
-        st4(b.Lh, slot, r.Lh);
-        b.prevP[slot] = make_float4(r.prevP.x, r.prevP.y, r.prevP.z, 0.f);
-        b.prevN[slot] = make_float4(r.prevN.x, r.prevN.y, r.prevN.z, 0.f);
     }
 
     const int depthIn = r.depth, flagsIn = r.flags;
@@ -1176,6 +1175,7 @@ __device__ __forceinline__ void surface_vertex(const DevScene &s, const Params &
                     interaction(pr, r.ro, r.rd, tHit, hv.z, hv.w, &si);
                     perturb_shading(s, m, &si);
                     toMaterial = true;
+                    *key = pr.material;
                     mq = (is_coated(m) || has_sheen(m)) ? QLayered
                          : m.type == prgpu::MatDiffuse ? QDiffuse
                          : (m.type == prgpu::MatConductor ? QConductor : QDielectric);
@@ -1216,12 +1216,12 @@ __global__ __launch_bounds__(kBlock) PG_WAVES(kSurfaceWaves) void k_surface(cons
 #pragma unroll 1
         for (int k = 0; k < kAppendK; ++k) {
             const int j = k * kBlock + int(threadIdx.x), i = t0 + j;
-            int slot = 0, q1 = -1, q2 = -1;
+            int slot = 0, q1 = -1, q2 = -1, key = 0;
             if (i < n) {
                 slot = queue[i];
-                surface_vertex<kMedia, kInterfaces>(s, p, b, slot, &q1, &q2);
+                surface_vertex<kMedia, kInterfaces>(s, p, b, slot, &q1, &q2, &key);
             }
-            tile_stage(ts, j, slot, q1, q2);
+            tile_stage(ts, j, slot, q1, q2, key);
         }
         tile_flush(ts, b, nextQueue, nextCount);
     }
@@ -1400,6 +1400,52 @@ __device__ __forceinline__ void material_vertex(const DevScene &s, const Params 
     }
 }
 
+// A medium scattering vertex (path.cpp:166-190): next-event estimation with the Henyey-Greenstein
+// phase function as the scattering density, then the phase-function direction sample and Russian
+// roulette, drawing from the path's stream in that order. k_surface has applied the distance
+// sample's weights, counted the vertex against the depth limit and left the scattering point in
+// the hit record.
+__device__ __forceinline__ void phase_vertex(const DevScene &s, const Params &p, const Bufs &b, int slot, int *q1, int *q2) {
+    PathRegs r;
+    const float4 d = b.d[slot];
+    r.rd = V3(d.x, d.y, d.z);
+    r.etaScale = d.w;
+    r.rng = load_rng(b, slot);
+    const float4 l = b.lam[slot], lp = b.lpdf[slot];
+    r.lam = {{l.x, l.y, l.z, l.w}, {lp.x, lp.y, lp.z, lp.w}};
+    r.beta = ld4(b.beta, slot);
+    r.rpath = ld4(b.rpath, slot);
+    const int4 st = b.st[slot];
+    r.depth = st.x;
+    r.flags = st.y;
+    r.medium = st.z;
+    const float4 h0 = b.h0[slot];
+    const V3 mp(h0.x, h0.y, h0.z), mwo = -r.rd;
+    const float g = clampf(s.media[r.medium].g, -0.99f, 0.99f);
+    if (sample_ld<true>(s, b, slot, r, mp, V3(), mwo, static_cast<const BSDF *>(nullptr), g, V3(), false, -1, -1))
+        *q2 = QShadow;
+    float phasePdf;
+    const V3 wi = sample_hg(mwo, g, get2(r.rng), &phasePdf);
+    bool alive = phasePdf != 0;
+    if (alive) {
+        r.prevPdf = phasePdf;
+        r.flags &= ~kSpecularFlag;
+        r.ro = mp;
+        r.rd = wi;
+        if (roulette(p.rr, p.rrStart, r)) alive = false;
+    }
+    b.rng[slot] = make_ulonglong2(r.rng.state, r.rng.inc);
+    if (alive) {
+        b.o[slot] = make_float4(r.ro.x, r.ro.y, r.ro.z, r.prevPdf);
+        b.d[slot] = make_float4(r.rd.x, r.rd.y, r.rd.z, r.etaScale);
+        st4(b.beta, slot, r.beta);
+        b.prevP[slot] = make_float4(mp.x, mp.y, mp.z, 0.f);
+        b.prevN[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        b.st[slot] = make_int4(r.depth, r.flags, r.medium, st.w);
+        *q1 = QNext;
+    }
+}
+
 template <int kType, bool kMedia, bool kGuided = false>
 __global__ __launch_bounds__(kBlock) PG_WAVES(kType == QLayered ? 0 : kMaterialWaves) void k_material(const DevScene *__restrict__ sp, Params p, Bufs b, int *nextQueue,
                                                      int *nextCount) {
@@ -1415,7 +1461,8 @@ __global__ __launch_bounds__(kBlock) PG_WAVES(kType == QLayered ? 0 : kMaterialW
             int slot = 0, q1 = -1, q2 = -1;
             if (i < n) {
                 slot = queue[i];
-                material_vertex<kType, kMedia, kGuided>(s, p, b, i, slot, &q1, &q2);
+                if constexpr (kType == QPhase) phase_vertex(s, p, b, slot, &q1, &q2);
+                else material_vertex<kType, kMedia, kGuided>(s, p, b, i, slot, &q1, &q2);
             }
             tile_stage(ts, j, slot, q1, q2);
         }
@@ -1423,9 +1470,10 @@ __global__ __launch_bounds__(kBlock) PG_WAVES(kType == QLayered ? 0 : kMaterialW
     }
 }
 
-// The plain families (diffuse from family f0 on, conductor, dielectric) in one launch over their
-// queues laid end to end: in late bounces, where each family holds a few thousand paths, three
-// launches cost three latency tails. A wave meets at most the two boundaries between the queues.
+// The plain families (diffuse from family f0 on, conductor, dielectric and, in scenes with media,
+// the medium scattering vertices) in one launch over their queues laid end to end: in late
+// bounces, where each family holds a few thousand paths, separate launches cost a latency tail
+// each. A wave meets at most the boundaries between the queues.
 #ifndef PG_FUSED_MATERIAL
 #define PG_FUSED_MATERIAL 1
 #endif
@@ -1438,7 +1486,8 @@ __global__ __launch_bounds__(kBlock) PG_WAVES(kMaterialWaves) void k_material_pl
     const int nD = f0 <= QDiffuse ? b.counts[mat_count_index(QDiffuse)] : 0;
     const int nC = b.counts[mat_count_index(QConductor)];
     const int nE = b.counts[mat_count_index(QDielectric)];
-    const int n = nD + nC + nE;
+    const int nP = kMedia ? b.counts[mat_count_index(QPhase)] : 0;
+    const int nDCE = nD + nC + nE, n = nDCE + nP;
     tile_begin(ts);
     for (int t0 = int(blockIdx.x) * kTile; t0 < n; t0 += int(gridDim.x) * kTile) {
 #pragma unroll 1
@@ -1452,14 +1501,105 @@ __global__ __launch_bounds__(kBlock) PG_WAVES(kMaterialWaves) void k_material_pl
                 const int ii = i - nD;
                 slot = b.qm[QConductor * b.qcap + ii];
                 material_vertex<QConductor, kMedia, false>(s, p, b, ii, slot, &q1, &q2);
-            } else if (i < n) {
+            } else if (i < nDCE) {
                 const int ii = i - nD - nC;
                 slot = b.qm[QDielectric * b.qcap + ii];
                 material_vertex<QDielectric, kMedia, false>(s, p, b, ii, slot, &q1, &q2);
+            } else if (kMedia && i < n) {
+                slot = b.qm[QPhase * b.qcap + (i - nDCE)];
+                phase_vertex(s, p, b, slot, &q1, &q2);
             }
             tile_stage(ts, j, slot, q1, q2);
         }
         tile_flush(ts, b, nextQueue, nextCount);
+    }
+}
+
+// ---- the material queues sorted by material and medium ---------------------------------------
+// Between the surface and material stages each material queue is sorted by its entries' keys (the
+// material, or for the medium scattering vertices the medium), so that a wave of the material
+// stage holds one material's paths: a material's branches (smooth or rough lobes, IOR models,
+// spectrum kinds, the phase function of one medium) then run once per wave instead of once per
+// material present in it. A counting sort: k_sort_count adds each block's key counts (per family
+// and key bin; a key is folded into kSortBins bins) into totals, k_sort_scan turns the totals into
+// each bin's first position within its family's queue, and k_sort_scatter, over the same slices as
+// k_sort_count, reserves each block's range per bin with one atomic and places the entries. Every
+// count is aggregated over a wave (one LDS atomic per distinct key in the wave) and over a block
+// (one device atomic per bin the block holds). Order within a bin follows the atomics and does not
+// affect any path's result.
+constexpr int kSortBins = kBlock;
+constexpr int kSortSlots = int(QCount) * kSortBins;
+
+// The slice [*lo, *hi) of family f's queue that block `blk` of `nblk` sorts.
+__device__ __forceinline__ void sort_slice(const Bufs &b, int f, int blk, int nblk, int *n, int *lo, int *hi) {
+    *n = b.counts[mat_count_index(f)];
+    const int per = (*n + nblk - 1) / nblk;
+    *lo = min(*n, blk * per);
+    *hi = min(*n, *lo + per);
+}
+
+__device__ __forceinline__ void sort_block_counts(const Bufs &b, int *hist) {
+    for (int i = int(threadIdx.x); i < kSortSlots; i += kBlock) hist[i] = 0;
+    __syncthreads();
+    for (int f = 0; f < int(QCount); ++f) {
+        int n, lo, hi;
+        sort_slice(b, f, int(blockIdx.x), int(gridDim.x), &n, &lo, &hi);
+        const int *keys = b.qk + size_t(f) * b.qcap;
+        for (int i0 = lo; i0 < hi; i0 += kBlock) {
+            const int i = i0 + int(threadIdx.x);
+            const int bin = i < hi ? f * kSortBins + (keys[i] & (kSortBins - 1)) : -1;
+            (void)lds_append(hist, bin);
+        }
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kBlock) void k_sort_count(Bufs b) {
+    __shared__ int hist[kSortSlots];
+    sort_block_counts(b, hist);
+    for (int i = int(threadIdx.x); i < kSortSlots; i += kBlock)
+        if (hist[i]) atomicAdd(&b.sortTotals[i], hist[i]);
+}
+
+// One block: per family, the exclusive scan of its bins' totals gives each bin's first position;
+// the totals are cleared for the next bounce.
+__global__ __launch_bounds__(kBlock) void k_sort_scan(Bufs b) {
+    static_assert(kSortBins == kBlock, "one thread per bin");
+    __shared__ int sc[kSortBins];
+    const int t = int(threadIdx.x);
+    for (int f = 0; f < int(QCount); ++f) {
+        const int i = f * kSortBins + t;
+        const int v = b.sortTotals[i];
+        sc[t] = v;
+        __syncthreads();
+        for (int off = 1; off < kSortBins; off <<= 1) {
+            const int add = t >= off ? sc[t - off] : 0;
+            __syncthreads();
+            sc[t] += add;
+            __syncthreads();
+        }
+        b.sortCursor[i] = sc[t] - v;
+        b.sortTotals[i] = 0;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sort_scatter(Bufs b) {
+    __shared__ int hist[kSortSlots];
+    sort_block_counts(b, hist);
+    for (int i = int(threadIdx.x); i < kSortSlots; i += kBlock)
+        if (hist[i]) hist[i] = atomicAdd(&b.sortCursor[i], hist[i]);
+    __syncthreads();
+    for (int f = 0; f < int(QCount); ++f) {
+        int n, lo, hi;
+        sort_slice(b, f, int(blockIdx.x), int(gridDim.x), &n, &lo, &hi);
+        const size_t off = size_t(f) * b.qcap;
+        for (int i0 = lo; i0 < hi; i0 += kBlock) {
+            const int i = i0 + int(threadIdx.x);
+            const int bin = i < hi ? f * kSortBins + (b.qk[off + i] & (kSortBins - 1)) : -1;
+            const int pos = lds_append(hist, bin);
+            if (bin >= 0) b.qms[off + pos] = b.qm[off + i];
+        }
     }
 }
 
@@ -1830,6 +1970,12 @@ void EnsureBatch(PTScene &pt, size_t n) {
     pt.qes.ensure(n * sizeof(int));
     pt.sq.ensure(n * sizeof(int));
     pt.qm.ensure(n * QCount * sizeof(int));
+    pt.qk.ensure(n * QCount * sizeof(int));
+    pt.qms.ensure(n * QCount * sizeof(int));
+    if (!pt.sortBins.p) {  // the totals start at zero; k_sort_scan clears them after each use
+        pt.sortBins.ensure(2 * kSortSlots * sizeof(int));
+        PG_CHECK(hipMemset(pt.sortBins.p, 0, pt.sortBins.bytes));
+    }
     f4(pt.so); f4(pt.sd); f4(pt.sp); f4(pt.sn); f4(pt.sf); f4(pt.sL); f4(pt.sb);
     pt.rng.ensure(n * sizeof(ulonglong2));
     pt.st.ensure(n * sizeof(int4));
@@ -1848,6 +1994,10 @@ Bufs MakeBufs(PTScene &pt) {
     b.hit = pt.hit.as<float4>(); b.q0 = pt.q0.as<int>(); b.q1 = pt.q1.as<int>(); b.counts = pt.counts.as<int>();
     b.h0 = pt.h0.as<float4>(); b.h1 = pt.h1.as<float4>(); b.h2 = pt.h2.as<float4>(); b.h3 = pt.h3.as<float4>();
     b.qm = pt.qm.as<int>();
+    b.qk = nullptr;  // set by a batch that sorts the material queues
+    b.qms = pt.qms.as<int>();
+    b.sortTotals = pt.sortBins.as<int>();
+    b.sortCursor = b.sortTotals + kSortSlots;
     b.qes = pt.qes.as<int>();
     b.sq = pt.sq.as<int>();
     b.shadowIdx = 2;
@@ -2001,6 +2151,9 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
     // the next one, whose curIdx is the other, consumes them. After the last bounce a round over the
     // empty path queue traces the last shadow rays.
     const bool fused = kFusedShadow && !media && !interfaces;
+    const bool sorted = dev.tune.materialSort != 0 && pt.sortable != 0;
+    if (sorted) b.qk = pt.qk.as<int>();
+    const int sortGrid = std::max(1, std::min(shadeGrid, dev.cus * 4));
     int pendingShadow = 0;  // shadow rays the next round traces (profiled rounds: read back)
     auto round = [&](int curIdx, int bounce, int tgrid, int grid) {
         const int nxtIdx = 1 - curIdx;
@@ -2030,32 +2183,42 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
             hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, st, ds, p, rb, cur, b.counts + curIdx, nxt,
                                b.counts + nxtIdx);
             if (s.emitters) hipLaunchKernelGGL(k_emit, dim3(grid), dim3(kBlock), 0, st, ds, rb);
+            if (sorted) {
+                hipLaunchKernelGGL(k_sort_count, dim3(sortGrid), dim3(kBlock), 0, st, rb);
+                hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(kBlock), 0, st, rb);
+                hipLaunchKernelGGL(k_sort_scatter, dim3(sortGrid), dim3(kBlock), 0, st, rb);
+            }
         });
+        // The material stage reads the queues sorted by material and medium when the batch sorts.
+        Bufs mb = rb;
+        if (sorted) mb.qm = b.qms;
         if (guided)
             timer.Run(prgpu::KernGuide, 0, nullptr, [&] {
-                dev.guide->Infer(b.qm + size_t(QDiffuse) * b.qcap, b.counts + mat_count_index(QDiffuse), active, b.h0,
+                dev.guide->Infer(mb.qm + size_t(QDiffuse) * b.qcap, b.counts + mat_count_index(QDiffuse), active, b.h0,
                                  b.h1, b.d, p.box, dev.cus, st);
             });
         timer.Run(prgpu::KernMaterial, active, nullptr, [&] {
             if (guided) {
                 auto *kd = media ? k_material<QDiffuse, true, true> : k_material<QDiffuse, false, true>;
-                hipLaunchKernelGGL(kd, dim3(grid), dim3(kBlock), 0, st, ds, p, rb, nxt, b.counts + nxtIdx);
+                hipLaunchKernelGGL(kd, dim3(grid), dim3(kBlock), 0, st, ds, p, mb, nxt, b.counts + nxtIdx);
             }
             if (kFusedMaterial) {
                 auto *kp = media ? k_material_plain<true> : k_material_plain<false>;
-                hipLaunchKernelGGL(kp, dim3(grid), dim3(kBlock), 0, st, ds, p, rb, nxt, b.counts + nxtIdx,
+                hipLaunchKernelGGL(kp, dim3(grid), dim3(kBlock), 0, st, ds, p, mb, nxt, b.counts + nxtIdx,
                                    guided ? int(QConductor) : int(QDiffuse));
             } else {
                 auto *kd = media ? k_material<QDiffuse, true> : k_material<QDiffuse, false>;
                 auto *kc = media ? k_material<QConductor, true> : k_material<QConductor, false>;
                 auto *ke = media ? k_material<QDielectric, true> : k_material<QDielectric, false>;
-                if (!guided) hipLaunchKernelGGL(kd, dim3(grid), dim3(kBlock), 0, st, ds, p, rb, nxt, b.counts + nxtIdx);
-                hipLaunchKernelGGL(kc, dim3(grid), dim3(kBlock), 0, st, ds, p, rb, nxt, b.counts + nxtIdx);
-                hipLaunchKernelGGL(ke, dim3(grid), dim3(kBlock), 0, st, ds, p, rb, nxt, b.counts + nxtIdx);
+                if (!guided) hipLaunchKernelGGL(kd, dim3(grid), dim3(kBlock), 0, st, ds, p, mb, nxt, b.counts + nxtIdx);
+                hipLaunchKernelGGL(kc, dim3(grid), dim3(kBlock), 0, st, ds, p, mb, nxt, b.counts + nxtIdx);
+                hipLaunchKernelGGL(ke, dim3(grid), dim3(kBlock), 0, st, ds, p, mb, nxt, b.counts + nxtIdx);
+                auto *kph = k_material<QPhase, true>;
+                if (media) hipLaunchKernelGGL(kph, dim3(grid), dim3(kBlock), 0, st, ds, p, mb, nxt, b.counts + nxtIdx);
             }
             if (pt.hasLayered) {
                 auto *kl = media ? k_material<QLayered, true> : k_material<QLayered, false>;
-                hipLaunchKernelGGL(kl, dim3(grid), dim3(kBlock), 0, st, ds, p, rb, nxt, b.counts + nxtIdx);
+                hipLaunchKernelGGL(kl, dim3(grid), dim3(kBlock), 0, st, ds, p, mb, nxt, b.counts + nxtIdx);
             }
         });
         if (!fused)

@@ -339,7 +339,8 @@ struct SampleRecordEx {
 
 // Profiling counters of one kind of kernel launch (prgpu_profile_read). Intersect: the closest hits
 // (and, without media or interfaces, the previous bounce's shadow rays); surface: the surface and
-// emission stages; material: the BSDF-family kernels; guide: guiding inference and training.
+// emission stages and the sort of the material queues (tuning key material_sort); material: the
+// BSDF-family and medium-scattering kernels; guide: guiding inference and training.
 enum KernelKindEx : int {
     KernCamera = 0, KernIntersect = 1, KernSurface = 2, KernShadow = 3, KernFinish = 4, KernGather = 5, KernGuide = 6,
     KernMaterial = 7, KernCount = 8
@@ -411,7 +412,9 @@ PRGPU_API int prgpu_profile_read(int deviceIndex, prgpu::ProfileStatsEx *out, in
 // steps per batch of paths), guide_lr, guide_bounces (bounces that sample the guide, 0 = all),
 // guide_blend (weight of the MIS-proposal term in the training loss, the rest fits the mixture),
 // graph_rounds (bounce rounds replayed per captured HIP graph, even, 0 = eager launches),
-// guide_replay (training records kept across batches in a ring, 0 = train on each batch's own).
+// guide_replay (training records kept across batches in a ring, 0 = train on each batch's own),
+// material_sort (1: each bounce sorts the material queues by material, and the medium scattering
+// vertices by medium, before the material stage; 0: family order only).
 // Defaults come from the tuning file the autotuner writes ($PRGPU_TUNING_FILE, else
 // prgpu_tuning.txt beside the library: "key value" lines), read when devices are first
 // enumerated. set / get return 0, or 1 for an unknown key, a value out of range or no device.
