@@ -253,7 +253,15 @@ PH_API void ph_resolve_srgb(const double *xyz, int64_t pixels, double scale, flo
     for (int64_t i = 0; i < pixels; ++i) phost::XYZToLinearSRGB(xyz + 3 * i, scale, rgb + 3 * i);
 }
 
-PH_API void ph_fit_sigmoid(const float *rgb, float *coeffs) { phost::FitRGBSigmoid(rgb, coeffs); }
+PH_API void ph_fit_sigmoid(const float *rgb, float *coeffs) { phost::RgbTableCoeffs(rgb, coeffs); }
+PH_API int ph_rgb_fit_tables(double *rgbw, double *toXyz, double *white) {
+    return phost::RgbFitTables(rgbw, toXyz, white);
+}
+PH_API int ph_rgb_table_record(float *out, int n) {
+    const std::vector<float> &t = phost::RgbToSpectrumTableRecord();
+    if (out && n >= int(t.size())) std::copy(t.begin(), t.end(), out);
+    return int(t.size());
+}
 
 PH_API int ph_envmap_sample(const float *rgb, int width, int height, const float *u, int n, float *out, float *avgY) {
     if (!rgb || width <= 0 || height <= 0 || n < 0 || (n > 0 && (!u || !out))) return 1;

@@ -369,28 +369,6 @@ void GaussNewton(const FitData &t, const double rgb[3], double c[3], int iters) 
 }
 }  // namespace
 
-void FitRGBSigmoid(const float rgbIn[3], float out[3]) {
-    double rgb[3] = {std::clamp(double(rgbIn[0]), 0.0, 1.0), std::clamp(double(rgbIn[1]), 0.0, 1.0),
-                     std::clamp(double(rgbIn[2]), 0.0, 1.0)};
-    const FitData &t = Fit();
-    // Continuation from the grey of equal luminance towards the target keeps Gauss-Newton in
-    // its basin, as the reference's table walk from neighbouring nodes does.
-    double grey = (rgb[0] + rgb[1] + rgb[2]) / 3;
-    double c[3] = {0, 0, (grey - 0.5) / std::sqrt(std::max(1e-12, grey * (1 - grey)))};
-    const int steps = 8;
-    for (int s = 1; s <= steps; ++s) {
-        double a = double(s) / steps, tgt[3];
-        for (int k = 0; k < 3; ++k) tgt[k] = grey + a * (rgb[k] - grey);
-        GaussNewton(t, tgt, c, s == steps ? 30 : 8);
-    }
-    // Coefficients over normalised wavelength -> over nanometres (rgb2spec.cpp:264-268).
-    const double c0 = 360.0, c1 = 1.0 / (830.0 - 360.0);
-    double A = c[0], B = c[1], C = c[2];
-    out[0] = float(A * c1 * c1);
-    out[1] = float(B * c1 - 2 * A * c0 * c1 * c1);
-    out[2] = float(C - B * c0 * c1 + A * c0 * c0 * c1 * c1);
-}
-
 // The table's nodes, as the reference's generator places and fits them (rgb2spec.cpp:243-276):
 // z nodes smoothstep(smoothstep(k / (res - 1))); for maximum component l and the other two at
 // x, y in [0, 1] of it, each (l, y, x) column of z is fitted from z index res / 5 upwards and then
@@ -442,6 +420,59 @@ const std::vector<float> &RgbToSpectrumTableRecord() {
     return rec;
 }
 
+int RgbFitTables(double *rgbw, double *toXyz, double *white) {
+    const FitData &t = Fit();
+    for (int k = 0; k < 3; ++k) {
+        if (rgbw)
+            for (int i = 0; i < kFine; ++i) rgbw[k * kFine + i] = t.rgbw[k][i];
+        if (white) white[k] = t.white[k];
+        if (toXyz)
+            for (int j = 0; j < 3; ++j) toXyz[3 * k + j] = t.toXyz[k][j];
+    }
+    return kFine;
+}
+
+// RGBToSpectrumTable::operator() (src/core/rgb2spec.cpp:278-304): the colour clamped to [0, 1];
+// grey in closed form; otherwise the maximum component picks the table, the other two give its
+// (x, y) cell at (res - 1) / max, FindInterval over the z nodes gives the z cell, and the node
+// coefficients are interpolated trilinearly.
+void RgbTableCoeffs(const float rgbIn[3], float c[3]) {
+    float rgb[3];
+    for (int k = 0; k < 3; ++k) rgb[k] = std::clamp(rgbIn[k], 0.f, 1.f);
+    if (rgb[0] == rgb[1] && rgb[1] == rgb[2]) {
+        const float v = rgb[0];
+        c[0] = c[1] = 0;
+        c[2] = v <= 0 ? -INFINITY : (v >= 1 ? INFINITY : (v - .5f) / std::sqrt(v * (1 - v)));
+        return;
+    }
+    const std::vector<float> &t = RgbToSpectrumTableRecord();
+    const int n = int(t[0]);
+    const float *zn = t.data() + 1, *co = zn + n;
+    const int maxc = (rgb[0] > rgb[1]) ? ((rgb[0] > rgb[2]) ? 0 : 2) : ((rgb[1] > rgb[2]) ? 1 : 2);
+    const float z = rgb[maxc];
+    const float x = rgb[(maxc + 1) % 3] * float(n - 1) / z;
+    const float y = rgb[(maxc + 2) % 3] * float(n - 1) / z;
+    const int xi = std::min(int(x), n - 2), yi = std::min(int(y), n - 2);
+    int first = 1, len = n - 2;  // FindInterval(n, z nodes[i] < z)
+    while (len > 0) {
+        const int half = len >> 1, middle = first + half;
+        const bool below = zn[middle] < z;
+        first = below ? middle + 1 : first;
+        len = below ? len - (half + 1) : half;
+    }
+    const int zi = std::clamp(first - 1, 0, n - 2);
+    const float dx = x - float(xi), dy = y - float(yi), dz = (z - zn[zi]) / (zn[zi + 1] - zn[zi]);
+    auto at = [&](int zz, int yy, int xx, int i) {
+        return co[3 * (((size_t(maxc) * size_t(n) + size_t(zz)) * size_t(n) + size_t(yy)) * size_t(n) + size_t(xx)) + size_t(i)];
+    };
+    auto lerp = [](float a, float u, float v) { return (1 - a) * u + a * v; };
+    for (int i = 0; i < 3; ++i)
+        c[i] = lerp(dz, lerp(dy, lerp(dx, at(zi, yi, xi, i), at(zi, yi, xi + 1, i)),
+                             lerp(dx, at(zi, yi + 1, xi, i), at(zi, yi + 1, xi + 1, i))),
+                    lerp(dy, lerp(dx, at(zi + 1, yi, xi, i), at(zi + 1, yi, xi + 1, i)),
+                         lerp(dx, at(zi + 1, yi + 1, xi, i), at(zi + 1, yi + 1, xi + 1, i))));
+}
+
 // ---------------------------------------------------------------------------------------------
 // SceneBuilder
 
@@ -468,30 +499,20 @@ int SceneBuilder::Sigmoid(float c0, float c1, float c2, float scale) {
     return AddSpectrum(s);
 }
 
-int SceneBuilder::RGBAlbedo(float r, float g, float b) {
-    // Grey values map to the closed form of src/core/rgb2spec.cpp:280-285.
-    if (r == g && g == b) {
-        float v = std::clamp(r, 0.f, 1.f);
-        if (v <= 0) return Sigmoid(0, 0, -INFINITY);
-        if (v >= 1) return Sigmoid(0, 0, INFINITY);
-        return Sigmoid(0, 0, (v - .5f) / std::sqrt(v * (1 - v)));
-    }
-    float rgb[3] = {r, g, b}, c[3];
-    FitRGBSigmoid(rgb, c);
+int SceneBuilder::RGBAlbedo(float r, float g, float b) {  // RGBAlbedoSpectrum (rgb2spec.cpp:306)
+    const float rgb[3] = {r, g, b};
+    float c[3];
+    RgbTableCoeffs(rgb, c);
     return Sigmoid(c[0], c[1], c[2]);
 }
 
-int SceneBuilder::RGBUnbounded(float r, float g, float b) {  // rgb2spec.cpp:309-313
-    float m = std::max(0.f, std::max(r, std::max(g, b)));
-    float scale = 2 * m;
-    if (!(scale > 0)) return Sigmoid(0, 0, -INFINITY, 0.f);
-    float rgb[3] = {r / scale, g / scale, b / scale};
-    if (rgb[0] == rgb[1] && rgb[1] == rgb[2]) {
-        float v = rgb[0];
-        return Sigmoid(0, 0, (v - .5f) / std::sqrt(v * (1 - v)), scale);
-    }
+int SceneBuilder::RGBUnbounded(float r, float g, float b) {  // RGBUnboundedSpectrum (rgb2spec.cpp:309-313)
+    const float m = std::max(0.f, std::max(r, std::max(g, b)));
+    const float scale = 2 * m;
+    const float inv = scale > 0 ? 1 / scale : 0.f;
+    const float rgb[3] = {r * inv, g * inv, b * inv};
     float c[3];
-    FitRGBSigmoid(rgb, c);
+    RgbTableCoeffs(rgb, c);
     return Sigmoid(c[0], c[1], c[2], scale);
 }
 

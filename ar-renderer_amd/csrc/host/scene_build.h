@@ -181,9 +181,15 @@ class SceneBuilder {
 };
 
 float SpectrumValue(const prgpu::SpectrumEx &s, const float *knots, float lambda);
-// Fits sigmoid-polynomial coefficients (c0 l^2 + c1 l + c2 inside S) reproducing an sRGB
-// reflectance (Jakob & Hanika 2019, as src/core/rgb2spec.cpp:35-165 fits its table nodes).
-void FitRGBSigmoid(const float rgb[3], float coeffs[3]);
+// Sigmoid-polynomial coefficients (c0 l^2 + c1 l + c2 inside S, l in nm) of an sRGB colour by
+// the RGB-to-spectrum table, as the reference's RGBToSpectrumTable::operator() looks them up
+// (src/core/rgb2spec.cpp:278-304): every RGB reflectance, unbounded and illuminant spectrum.
+void RgbTableCoeffs(const float rgb[3], float coeffs[3]);
+// The table fit's inputs (src/core/rgb2spec.cpp:39-74): per fine wavelength (2.5 nm over
+// [360, 830]) the spectrum-to-linear-sRGB weights (D65, white adaptation, trapezoid weight, 1 /
+// CIE Y integral), 3 x n; the sRGB-to-XYZ matrix (9) and its white (3). Returns n; null outputs
+// are skipped.
+int RgbFitTables(double *rgbw, double *toXyz, double *white);
 // Linear sRGB of a film XYZ value, with the renderer's spectral white adaptation (src/core/film.cpp:170-175).
 void XYZToLinearSRGB(const double xyz[3], double scale, float rgb[3]);
 // Linear sRGB of an XYZ value by the sRGB matrix alone (IEC 61966-2-1, D65 white).

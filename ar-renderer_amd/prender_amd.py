@@ -303,6 +303,8 @@ class Host:
                                 C.c_char_p, C.c_int]
         L.ph_resolve_srgb.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.c_void_p]
         L.ph_fit_sigmoid.argtypes = [C.c_void_p, C.c_void_p]
+        L.ph_rgb_fit_tables.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ph_rgb_table_record.argtypes = [C.c_void_p, C.c_int]
         L.ph_preetham_sky.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_int, C.c_int, C.c_void_p]
         L.ph_envmap_sample.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.ph_spectrum_value.restype = C.c_float
@@ -380,6 +382,21 @@ class Host:
         rgb = np.ascontiguousarray(rgb, dtype=np.float32)
         out = np.zeros(3, np.float32)
         self.lib.ph_fit_sigmoid(rgb.ctypes.data, out.ctypes.data)
+        return out
+
+    def rgb_fit_tables(self):
+        """The RGB-to-spectrum fit's inputs: (3, n) spectrum-to-linear-sRGB weights, the (3, 3)
+        sRGB-to-XYZ matrix and its white."""
+        n = self.lib.ph_rgb_fit_tables(None, None, None)
+        w, m, white = np.zeros((3, n)), np.zeros((3, 3)), np.zeros(3)
+        self.lib.ph_rgb_fit_tables(w.ctypes.data, m.ctypes.data, white.ctypes.data)
+        return w, m, white
+
+    def rgb_table_record(self):
+        """The RGB-to-spectrum table record: [resolution, z nodes..., coefficients...]."""
+        n = self.lib.ph_rgb_table_record(None, 0)
+        out = np.zeros(n, np.float32)
+        self.lib.ph_rgb_table_record(out.ctypes.data, n)
         return out
 
     def preetham_sky(self, sun_dir, turbidity, ground_albedo, width, height):

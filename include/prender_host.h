@@ -103,8 +103,17 @@ PH_API int ph_checkpoint_read(const char *path, const prgpu::SceneDescEx *scene,
 // Film helpers -------------------------------------------------------------------------------------
 // Linear sRGB (the film's output colour space) of an FP64 XYZ film scaled by `scale` (src/core/film.cpp:170-175).
 PH_API void ph_resolve_srgb(const double *xyz, int64_t pixels, double scale, float *rgb);
-// Sigmoid-polynomial coefficients of an sRGB reflectance (RGB uplift used by the generators).
+// Sigmoid-polynomial coefficients of an sRGB reflectance: the RGB-to-spectrum table lookup every
+// RGB spectrum of the generators and the loader goes through (RGBToSpectrumTable::operator(),
+// src/core/rgb2spec.cpp:278-304).
 PH_API void ph_fit_sigmoid(const float *rgb, float *coeffs);
+// The table fit's inputs (the reference's fit tables, src/core/rgb2spec.cpp:39-74): 3 x n
+// spectrum-to-linear-sRGB weights over n fine wavelengths, the sRGB-to-XYZ matrix (9) and its white
+// (3). Returns n (189); null outputs are skipped.
+PH_API int ph_rgb_fit_tables(double *rgbw, double *toXyz, double *white);
+// The RGB-to-spectrum table as its knots record (resolution, z nodes, coefficients); returns its
+// length and copies it into out when n is at least that.
+PH_API int ph_rgb_table_record(float *out, int n);
 // Preetham daylight map of a sun_sky light: width x height linear-sRGB texels (kcd/m^2),
 // equirectangular with v = 0 at the zenith, ground below the horizon (replaces MakePreethamSky,
 // src/lights/light.h:265-268, src/lights/sky.cpp:146-208). 0 on success, 1 on bad arguments.

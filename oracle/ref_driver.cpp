@@ -6,6 +6,7 @@
 // tests/golden/ can record the reference's outputs and the tests can check the oracle against
 // them. Nothing here is shipped or called by the product.
 #include "cameras/camera.h"
+#include "core/color.h"
 #include "core/film.h"
 #include "core/math.h"
 #include "core/rgb2spec.h"
@@ -124,6 +125,33 @@ void rf_cie(float l, float *xyz) {
     xyz[2] = CIE_Z(l);
 }
 float rf_cie_y_integral() { return CIE_Y_Integral(); }
+
+// The RGB-to-spectrum fit's inputs as the reference defines them (src/core/rgb2spec.cpp:39-74),
+// from its own colour matrices, white adaptation, D65 and CIE functions: per fine wavelength (189
+// over [360, 830], trapezoid weights) the spectrum-to-linear-sRGB weights, then the sRGB-to-XYZ
+// matrix and its white.
+void rf_rgb_fit_tables(double *rgbw, double *toXyz, double *white) {
+    const int n = 189;
+    const Mat3 toRGB = XYZToRGBMatrix(ColorSpaceId::sRGB) * SpectralWhiteAdaptation();
+    const Mat3 &xyz = RGBToXYZMatrix(ColorSpaceId::sRGB);
+    const Spectrum &d65 = StdIlluminantD65();
+    const double h = (LambdaMax - LambdaMin) / (n - 1), norm = 1.0 / CIE_Y_Integral();
+    for (int i = 0; i < n; ++i) {
+        const double l = LambdaMin + i * h;
+        const double w = (i == 0 || i == n - 1) ? 0.5 * h : h;
+        const double I = d65(float(l));
+        const double c[3] = {CIE_X(float(l)), CIE_Y(float(l)), CIE_Z(float(l))};
+        for (int k = 0; k < 3; ++k) {
+            double s = 0;
+            for (int j = 0; j < 3; ++j) s += toRGB.m[k][j] * c[j];
+            rgbw[k * n + i] = s * I * w * norm;
+        }
+    }
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) toXyz[3 * i + j] = xyz.m[i][j];
+        white[i] = double(xyz.m[i][0]) + xyz.m[i][1] + xyz.m[i][2];
+    }
+}
 
 // XYZ of radiance L carried by SampleVisible(u) (secondary wavelengths terminated when asked).
 void rf_to_xyz(const float *L, float u, int terminate, float *xyz) {

@@ -32,6 +32,7 @@ def load_ref():
         ("rf_sample_visible", None, [C.c_float, vp, vp]),
         ("rf_cie", None, [C.c_float, vp]),
         ("rf_cie_y_integral", C.c_float, []),
+        ("rf_rgb_fit_tables", None, [vp, vp, vp]),
         ("rf_to_xyz", None, [vp, C.c_float, C.c_int, vp]),
         ("rf_spectrum_eval", C.c_float, [vp, vp, C.c_float]),
         ("rf_spectrum_y", C.c_float, [vp, vp]),
@@ -333,10 +334,19 @@ def main():
     c2_light_bvh(L, out, host)
     texture_lookup(L, out)
     shading_perturbation(L, out)
+    rgb_fit_tables(L, out)
 
     path = os.path.join(HERE, "ref_components.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")
+
+
+def rgb_fit_tables(L, out):
+    """The RGB-to-spectrum fit's inputs from the reference's own colour and spectrum code
+    (src/core/rgb2spec.cpp:39-74): the objective every table node solves."""
+    w, m, white = np.zeros(3 * 189), np.zeros(9), np.zeros(3)
+    L.rf_rgb_fit_tables(ptr(w), ptr(m), ptr(white))
+    out["fit_rgbw"], out["fit_to_xyz"], out["fit_white"] = w.reshape(3, 189), m.reshape(3, 3), white
 
 
 def texture_record(rgb, wrap, flip, scale, offset, mult=(1, 1, 1)):

@@ -79,6 +79,77 @@ def test_blackbody_lights_have_unit_luminance_shape(host, c2_small):
         np.testing.assert_allclose(ratio, ratio[0], rtol=2e-5)  # same shape; the scale is the unit-Y factor
 
 
+def _lab(rgb, to_xyz, white):
+    """CIELAB of linear-sRGB rows under the fit's matrix and white (the fit's error measure)."""
+    xyz = rgb @ to_xyz.T / white
+    d = 6 / 29
+    f = np.where(xyz > d ** 3, np.cbrt(xyz), xyz / (3 * d * d) + 4 / 29)
+    return np.stack([116 * f[:, 1] - 16, 500 * (f[:, 0] - f[:, 1]), 200 * (f[:, 1] - f[:, 2])], 1)
+
+
+def _table(host):
+    rec = host.rgb_table_record()
+    n = int(rec[0])
+    assert n == 64 and rec.size == 1 + n + 3 * n ** 3 * 3
+    return n, rec[1:1 + n], rec[1 + n:].reshape(3, n, n, n, 3)
+
+
+def test_rgb_fit_inputs_equal_the_reference(host):
+    """The objective the RGB-to-spectrum table solves: per fine wavelength the spectrum-to-sRGB
+    weights (D65, the spectral white adaptation, the sRGB matrix, trapezoid weights, 1 / CIE Y
+    integral), the sRGB-to-XYZ matrix and its white. The goldens are computed from the reference's
+    own compiled colour and spectrum code (src/core/rgb2spec.cpp:39-74): bit for bit."""
+    w, m, white = host.rgb_fit_tables()
+    assert np.array_equal(w, G["fit_rgbw"]) and np.array_equal(m, G["fit_to_xyz"]) and np.array_equal(white, G["fit_white"])
+
+
+def test_rgb_table_nodes_solve_the_reference_objective(host):
+    """Each table node's coefficients (in nm, as stored) reproduce the node's colour under the
+    reference's fit inputs: CIELAB residual below 1e-3 for more than 99.9 % of the nodes from the
+    brightness where the table's walk starts (z node res / 5) upwards; the others, above 2e-3, are
+    the saturated edges (another component at most 2 / 63 of the maximum) and the corner by white,
+    which a sigmoid reaches only in the limit. Three equations in three unknowns: where the residual
+    vanishes the coefficients are the objective's root, the one the reference's own walk (the same
+    start, the same steps) converges to."""
+    n, zn, co = _table(host)
+    w, to_xyz, white = G["fit_rgbw"], G["fit_to_xyz"], G["fit_white"]
+    lam = 360.0 + 2.5 * np.arange(w.shape[1])
+    rng = np.random.default_rng(7)
+    l, k, j, i = rng.integers(3, size=40000), rng.integers(n // 5, n, 40000), rng.integers(n, size=40000), \
+        rng.integers(n, size=40000)
+    c = co[l, k, j, i].astype(np.float64)
+    v = (c[:, :1] * lam + c[:, 1:2]) * lam + c[:, 2:3]
+    rgb_fit = (0.5 * v / np.sqrt(1 + v * v) + 0.5) @ w.T
+    z = zn[k].astype(np.float64)
+    target = np.zeros((len(z), 3))
+    rows = np.arange(len(z))
+    target[rows, l], target[rows, (l + 1) % 3], target[rows, (l + 2) % 3] = z, i / (n - 1) * z, j / (n - 1) * z
+    r = np.linalg.norm(_lab(target, to_xyz, white) - _lab(rgb_fit, to_xyz, white), axis=1)
+    print(f"table nodes from z node {n // 5}: {np.mean(r < 1e-3):.5f} with CIELAB residual < 1e-3 "
+          f"(median {np.median(r):.2e})", flush=True)
+    assert np.mean(r < 1e-3) > 0.999
+    lo = np.minimum(i, j)
+    assert np.all(((lo <= 2) | ((k >= 54) & (lo >= 58)))[r >= 2e-3])
+
+
+def test_rgb_lookup_at_node_colours_returns_the_nodes(host):
+    """RGBToSpectrumTable::operator() at a node's own colour (z exactly a z node, the other two at
+    i / (res - 1) and j / (res - 1) of it) interpolates back to that node: the maximum component
+    picks the table, x and y index its columns in that order, FindInterval the z node."""
+    n, zn, co = _table(host)
+    rng = np.random.default_rng(8)
+    lam = np.linspace(360, 830, 48)
+    for _ in range(300):
+        l, k, j, i = int(rng.integers(3)), int(rng.integers(1, n)), int(rng.integers(n - 1)), int(rng.integers(n - 1))
+        z = zn[k]
+        rgb = np.zeros(3, np.float32)
+        rgb[l], rgb[(l + 1) % 3], rgb[(l + 2) % 3] = z, np.float32(i / (n - 1)) * z, np.float32(j / (n - 1)) * z
+        c = host.fit_sigmoid(rgb).astype(np.float64)
+        ref = co[l, k, j, i].astype(np.float64)
+        sig = lambda q: 0.5 * ((q[0] * lam + q[1]) * lam + q[2]) / np.sqrt(1 + ((q[0] * lam + q[1]) * lam + q[2]) ** 2) + 0.5  # noqa: E731
+        np.testing.assert_allclose(sig(c), sig(ref), atol=2e-4, err_msg=f"node {(l, k, j, i)}")
+
+
 def test_sigmoid_fit_grey_and_round_trip(host):
     c = host.fit_sigmoid([0.5, 0.5, 0.5])
     row = np.zeros(1, pa.SPECTRUM_DTYPE)
