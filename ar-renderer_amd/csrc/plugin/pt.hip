@@ -1896,16 +1896,33 @@ __global__ __launch_bounds__(kBlock) void k_finish(const DevScene *__restrict__ 
     b.xyz[slot] = out;
 }
 
-__device__ __forceinline__ float filter_1d(const DevScene &s, float x) {
+// The reconstruction filter's 1D profile with its per-launch constants hoisted.
+struct FilterK {
+    int type;
+    float r, invDiam, inv2s2, tail;  // radius, 1 / (2 r), 1 / (2 sigma^2), the Gaussian's value at r
+};
+__device__ __forceinline__ FilterK filter_k(const DevScene &s) {
+    FilterK k;
+    k.type = s.filterType;
+    k.r = s.filterRadius;
+    k.invDiam = 1 / (2 * s.filterRadius);
+    k.inv2s2 = 1 / (2 * s.filterSigma * s.filterSigma);
+    k.tail = expf(-s.filterRadius * s.filterRadius * k.inv2s2);
+    return k;
+}
+__device__ __forceinline__ float filter_1d(const FilterK &k, float x) {
     x = fabsf(x);
-    if (x > s.filterRadius) return 0;
-    if (s.filterType == prgpu::FilterBox) return 1;
-    if (s.filterType == prgpu::FilterGaussian)
-        return maxf(0.f, expf(-x * x / (2 * s.filterSigma * s.filterSigma)) -
-                             expf(-s.filterRadius * s.filterRadius / (2 * s.filterSigma * s.filterSigma)));
-    float t = 0.5f + x / (2 * s.filterRadius);
+    if (x > k.r) return 0;
+    if (k.type == prgpu::FilterBox) return 1;
+    if (k.type == prgpu::FilterGaussian) return maxf(0.f, expf(-x * x * k.inv2s2) - k.tail);
+    // Blackman-Harris, a0 - a1 cos(2 pi t) + a2 cos(4 pi t) - a3 cos(6 pi t) (film.cpp:9-39), with
+    // one cosine: v_cos_f32 takes its argument in revolutions, and cos 2x = 2c^2 - 1, cos 3x =
+    // 4c^3 - 3c. Every pixel evaluates the profile for each of its neighbours' samples in every
+    // pass, so three library cosines made the gather VALU-bound.
+    const float t = 0.5f + x * k.invDiam;
     const float a0 = 0.35875f, a1 = 0.48829f, a2 = 0.14128f, a3 = 0.01168f;
-    return a0 - a1 * cosf(2 * kPi * t) + a2 * cosf(4 * kPi * t) - a3 * cosf(6 * kPi * t);
+    const float c = __builtin_amdgcn_cosf(t), c2 = c * c;
+    return a0 - a1 * c + a2 * (2 * c2 - 1) - a3 * (4 * c2 - 3) * c;
 }
 
 // Film: each output pixel gathers the samples whose filter footprint covers it (the CPU film's
@@ -1919,6 +1936,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(const DevScene *__restrict__ 
     int y = pix / s.width, x = pix - y * s.width;
     const int sh = npix / sw;
     const float r = s.filterRadius;
+    const FilterK fk = filter_k(s);
     double accX = 0, accY = 0, accZ = 0, sqX = 0, sqY = 0, sqZ = 0;
     for (int pass = 0; pass < passes; ++pass) {
         double eX = 0, eY = 0, eZ = 0;
@@ -1933,7 +1951,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(const DevScene *__restrict__ 
                 int x0 = max(0, int(ceilf(pf.x - r - 0.5f))), x1 = min(s.width - 1, int(floorf(pf.x + r - 0.5f)));
                 int y0 = max(0, int(ceilf(pf.y - r - 0.5f))), y1 = min(s.height - 1, int(floorf(pf.y + r - 0.5f)));
                 if (x < x0 || x > x1 || y < y0 || y > y1) continue;
-                float w = filter_1d(s, pf.x - (x + 0.5f)) * filter_1d(s, pf.y - (y + 0.5f)) * s.filterNorm;
+                float w = filter_1d(fk, pf.x - (x + 0.5f)) * filter_1d(fk, pf.y - (y + 0.5f)) * s.filterNorm;
                 if (w == 0) continue;
                 eX += double(v.x) * w;
                 eY += double(v.y) * w;
