@@ -1976,6 +1976,92 @@ __global__ __launch_bounds__(kBlock) void k_gather(const DevScene *__restrict__ 
     }
 }
 
+// The same gather over 16 x 16 pixel tiles: per pass the tile's samples and their apron (reach
+// samples on each side) are staged in LDS once, each with its value and its separable weights
+// for the 2 reach + 1 pixel columns and rows it can reach (zero outside its footprint and for an
+// empty sample), and every pixel then sums its neighbours in the order and with the products of
+// k_gather, so the film is the same bit for bit. A sample is loaded and its filter evaluated once
+// per pass instead of once per pixel that reaches it: (2 reach + 1)^2 times fewer loads and
+// 2 reach + 1 times fewer filter evaluations.
+constexpr int kGatherTile = 16, kGatherMaxReach = 3;
+constexpr int kGatherApron = kGatherTile + 2 * kGatherMaxReach, kGatherTaps = 2 * kGatherMaxReach + 1;
+static_assert(kGatherTile * kGatherTile == kBlock, "one thread per tile pixel");
+
+__global__ __launch_bounds__(kBlock) void k_gather_tiled(const DevScene *__restrict__ sp, Bufs b, int passes, int npix,
+                                                         int sw, int reach, double *sum, double *sumsq) {
+    const DevScene &s = *sp;
+    __shared__ float sv[3][kGatherApron * kGatherApron];
+    __shared__ float swx[kGatherTaps][kGatherApron * kGatherApron], swy[kGatherTaps][kGatherApron * kGatherApron];
+    const int tx = int(threadIdx.x) % kGatherTile, ty = int(threadIdx.x) / kGatherTile;
+    const int X0 = int(blockIdx.x) * kGatherTile, Y0 = int(blockIdx.y) * kGatherTile;
+    const int x = X0 + tx, y = Y0 + ty;
+    const bool inside = x < s.width && y < s.height;
+    const int sh = npix / sw, side = kGatherTile + 2 * reach, taps = 2 * reach + 1;
+    const float r = s.filterRadius;
+    const FilterK fk = filter_k(s);
+    double accX = 0, accY = 0, accZ = 0, sqX = 0, sqY = 0, sqZ = 0;
+    for (int pass = 0; pass < passes; ++pass) {
+        __syncthreads();  // the previous pass's sums are done with the tile
+        for (int k = int(threadIdx.x); k < side * side; k += kBlock) {
+            const int ay = k / side, ax = k - ay * side;
+            const int sx = X0 + s.margin - reach + ax, sy = Y0 + s.margin - reach + ay;
+            float v[3] = {0.f, 0.f, 0.f};
+            bool live = false;
+            float2 pf = make_float2(0.f, 0.f);
+            int x0 = 0, x1 = -1, y0 = 0, y1 = -1;
+            if (sx >= 0 && sx < sw && sy >= 0 && sy < sh) {
+                const int slot = pass * npix + sy * sw + sx;
+                const float4 vv = b.xyz[slot];
+                if (vv.w != 0.f) {
+                    live = true;
+                    v[0] = vv.x, v[1] = vv.y, v[2] = vv.z;
+                    pf = b.pfilm[slot];
+                    x0 = max(0, int(ceilf(pf.x - r - 0.5f)));
+                    x1 = min(s.width - 1, int(floorf(pf.x + r - 0.5f)));
+                    y0 = max(0, int(ceilf(pf.y - r - 0.5f)));
+                    y1 = min(s.height - 1, int(floorf(pf.y + r - 0.5f)));
+                }
+            }
+            for (int c = 0; c < 3; ++c) sv[c][k] = v[c];
+            // tap e: the pixel e - reach columns (rows) from this sample's own
+            for (int e = 0; e < taps; ++e) {
+                const int px = sx - s.margin + e - reach, py = sy - s.margin + e - reach;
+                swx[e][k] = live && px >= x0 && px <= x1 ? filter_1d(fk, pf.x - (px + 0.5f)) : 0.f;
+                swy[e][k] = live && py >= y0 && py <= y1 ? filter_1d(fk, pf.y - (py + 0.5f)) : 0.f;
+            }
+        }
+        __syncthreads();
+        if (!inside) continue;
+        double eX = 0, eY = 0, eZ = 0;
+        for (int oy = -reach; oy <= reach; ++oy) {
+            for (int ox = -reach; ox <= reach; ++ox) {
+                const int k = (ty + reach + oy) * side + (tx + reach + ox);
+                const float w = swx[reach - ox][k] * swy[reach - oy][k] * s.filterNorm;
+                if (w == 0) continue;
+                eX += double(sv[0][k]) * w;
+                eY += double(sv[1][k]) * w;
+                eZ += double(sv[2][k]) * w;
+            }
+        }
+        accX += eX;
+        accY += eY;
+        accZ += eZ;
+        sqX += eX * eX;
+        sqY += eY * eY;
+        sqZ += eZ * eZ;
+    }
+    if (!inside) return;
+    const size_t idx = 3 * (size_t(y) * size_t(s.width) + size_t(x));
+    sum[idx] += accX;
+    sum[idx + 1] += accY;
+    sum[idx + 2] += accZ;
+    if (sumsq) {
+        sumsq[idx] += sqX;
+        sumsq[idx + 1] += sqY;
+        sumsq[idx + 2] += sqZ;
+    }
+}
+
 // =============================================================================================
 // Host driver
 
@@ -2416,8 +2502,13 @@ void RenderExImpl(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::Render
         });
         int pixels = s.width * s.height;
         timer.Run(prgpu::KernGather, int64_t(pixels) * passes, nullptr, [&] {
-            hipLaunchKernelGGL(k_gather, dim3((pixels + kBlock - 1) / kBlock), dim3(kBlock), 0, dev.stream, ds, b,
-                               passes, npix, sw, reach, dsum, dsq);
+            if (reach <= kGatherMaxReach)
+                hipLaunchKernelGGL(k_gather_tiled,
+                                   dim3((s.width + kGatherTile - 1) / kGatherTile, (s.height + kGatherTile - 1) / kGatherTile),
+                                   dim3(kBlock), 0, dev.stream, ds, b, passes, npix, sw, reach, dsum, dsq);
+            else
+                hipLaunchKernelGGL(k_gather, dim3((pixels + kBlock - 1) / kBlock), dim3(kBlock), 0, dev.stream, ds, b,
+                                   passes, npix, sw, reach, dsum, dsq);
         });
         PG_CHECK(hipGetLastError());
         if (p.train) {
