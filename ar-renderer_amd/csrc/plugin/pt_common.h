@@ -271,11 +271,15 @@ PG_HD uint64_t murmur_words(const uint64_t *w, int nWords, bool tail4, uint32_t 
 PG_HD uint64_t fbits(float a, float b) { return uint64_t(__float_as_uint(a)) | (uint64_t(__float_as_uint(b)) << 32); }
 
 // ---- spectra (src/core/spectrum.cpp, src/core/rgb2spec.h) --------------------------------------------
+// Planck's law in double precision as the CPU evaluates it (spectrum.cpp:116-122), with l^5 as four
+// multiplications instead of pow: within 2 ulp of the double power, so the float result is the
+// CPU's, and the light's spectrum, evaluated at every next-event estimate, no longer pays a
+// double-precision pow.
 PG_HD float planck(float lambdaNm, float T) {
     if (T <= 0) return 0;
     const double c = 299792458., h = 6.62606957e-34, kb = 1.3806488e-23;
-    double l = lambdaNm * 1e-9;
-    return float((2 * h * c * c) / (pow(l, 5.0) * (exp((h * c) / (l * kb * T)) - 1)));
+    const double l = lambdaNm * 1e-9, l2 = l * l;
+    return float((2 * h * c * c) / (l2 * l2 * l * (exp((h * c) / (l * kb * T)) - 1)));
 }
 
 // ---- image environment light (record layout: prgpu::kEnvHeader) ----------------------------------
