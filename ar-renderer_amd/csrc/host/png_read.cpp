@@ -6,7 +6,7 @@
 // expanded, sub-byte grey is scaled to 0..255 by bit replication (1 bit x255, 2 bits x85,
 // 4 bits x17) and alpha is dropped. Chunk CRCs are not checked and ancillary chunks (gAMA, tRNS,
 // ...) are skipped, as stb_image does when asked for RGB.
-#include "png_read.h"
+#include "image_codecs.h"
 
 #include <zlib.h>
 

@@ -6,13 +6,13 @@
 // What the exporter approximates or drops is approximated or dropped here, with the same warning
 // (checker textures keep both colours and the u scale; roughness textures become the default).
 // Distant, spot, IES and sun lights, image environments, the Preetham sky of sun-sky lights, image
-// textures as a material's reflectance, and normal and bump maps (images in PFM, PPM or PNG), which
-// the v4 exporter averages or skips, are kept: the extension has them.
+// textures as a material's reflectance, and normal and bump maps (images in PFM, PPM, PNG, HDR or
+// EXR), which the v4 exporter averages or skips, are kept: the extension has them.
 // Everything the renderer cannot express and the exporter has no fallback for (image textures in
 // other slots, noise colour textures, orthographic and realistic cameras, glTF / pbrt imports, noise
 // media) is a load error naming the item. Illuminant RGB spectra (sigmoid x D65) are tabulated every
 // 5 nm, as the exporter tabulates every spectrum.
-#include "png_read.h"
+#include "image_codecs.h"
 #include "prender_host.h"
 #include "scene_build.h"
 
@@ -279,9 +279,10 @@ std::string ReadFile(const fs::path &p) {
 // Relative "file" values (and bare rig paths) are resolved against the file that names them.
 // Images for environment lights and image textures (src/core/image.cpp:31-101): PFM (linear
 // float, "PF" RGB or "Pf" grey, bottom row first, negative scale = little-endian), binary PPM ("P6",
-// 8-bit) and PNG (reduced to 8-bit RGB as stb_image does); 8-bit values are decoded from sRGB
-// unless the image is data ("linear"). Rows come out top row first, linear RGB. Other formats
-// (EXR, HDR, JPEG) are a named error.
+// 8-bit), PNG (reduced to 8-bit RGB as stb_image does), Radiance RGBE (.hdr) and OpenEXR (.exr,
+// single-part scanline files); 8-bit values are decoded from sRGB unless the image is data
+// ("linear"), float formats are linear. Rows come out top row first, linear RGB. Other formats
+// (JPEG, BMP, TGA, ...) are a named error.
 bool ReadImageRGB(const std::string &path, int *w, int *h, std::vector<float> *rgb, std::string *err, bool srgb8) {
     std::string ext = fs::path(path).extension().string();
     for (char &c : ext) c = char(std::tolower(static_cast<unsigned char>(c)));
@@ -360,10 +361,15 @@ bool ReadImageRGB(const std::string &path, int *w, int *h, std::vector<float> *r
                 for (size_t i = 0; i < data.size(); ++i) (*rgb)[i] = lut[data[i]];
             }
         }
-    } else if (ext == ".png") {
+    } else if (ext == ".png" || ext == ".hdr" || ext == ".exr") {
         std::vector<unsigned char> file;
         unsigned char buf[1 << 16];
         for (size_t n; (n = std::fread(buf, 1, sizeof(buf), f)) > 0;) file.insert(file.end(), buf, buf + n);
+        if (ext != ".png") {  // float formats: linear whatever the colorspace says (image.cpp:76-89)
+            ok = ext == ".hdr" ? DecodeHDR(file, w, h, rgb, err) : DecodeEXR(file, w, h, rgb, err);
+            std::fclose(f);
+            return ok;
+        }
         std::vector<unsigned char> data;
         ok = DecodePNG(file, w, h, &data, err);
         if (ok) {
@@ -376,7 +382,7 @@ bool ReadImageRGB(const std::string &path, int *w, int *h, std::vector<float> *r
             for (size_t i = 0; i < data.size(); ++i) (*rgb)[i] = lut[data[i]];
         }
     } else {
-        *err = "image format '" + ext + "' is not read by this loader (PFM, PPM and PNG are)";
+        *err = "image format '" + ext + "' is not read by this loader (PFM, PPM, PNG, HDR and EXR are)";
     }
     std::fclose(f);
     return ok;

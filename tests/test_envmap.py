@@ -1,5 +1,5 @@
 """Image environment lights (src/lights/light.cpp:215-297) and the sun-sky's Preetham map, on the
-CPU: the loader's PFM / PPM / PNG reading and its record in the scene's knots; the oracle's sampling
+CPU: the loader's PFM / PPM / PNG / Radiance HDR / OpenEXR reading and its record in the scene's knots; the oracle's sampling
 against the host's Distribution2D tables (themselves pinned bit for bit against the reference's);
 and what the mathematics fixes for the path loop under an image environment:
 - a pixel that sees a constant map directly gets the map's colour as an RGB illuminant: for white,
@@ -15,6 +15,7 @@ import envscenes as es
 import oracle as orc
 import pathscenes as ps
 import prender_amd as pa
+import texscenes as ts
 
 # linear sRGB -> XYZ (IEC 61966-2-1, D65)
 SRGB_TO_XYZ = np.array([[0.4124564, 0.3575761, 0.1804375], [0.2126729, 0.7151522, 0.0721750],
@@ -159,3 +160,69 @@ def test_sun_sky_adds_the_preetham_map(host, tmpdir):
     sky = host.preetham_sky(sun, 3.0, 0.3, 64, 32)
     np.testing.assert_array_equal(rec[8:8 + 64 * 32 * 3], sky.reshape(-1))
     assert not any("sky" in w and "skipped" in w for w in s.warnings)
+
+
+def _map_texels(host, tmpdir, name, path):
+    s = es.load(host, tmpdir, name, es.env_doc(path, objects=False))
+    L = s.lights[_env_light(s)]
+    rec = s.knots[L["profileOffset"]:]
+    w, h = int(rec[0]), int(rec[1])
+    return rec[8:8 + w * h * 3].reshape(h, w, 3)
+
+
+@pytest.mark.parametrize("size,rle", [((5, 11), True), ((4, 6), True), ((3, 9), False)])
+def test_radiance_hdr_maps(host, tmpdir, size, rle):
+    """Radiance RGBE written by the format's definition (run-length coded scanlines, or flat; a
+    scanline narrower than 8 texels is always flat): each texel is c * 2^(e - 136), and that is
+    the map's colour within RGBE's 8-bit mantissas under the texel's shared exponent."""
+    h, w = size
+    rgb = (np.random.default_rng(w).random((h, w, 3)) ** 3 * 50).astype(np.float32)
+    rgb[0, 0] = 0
+    rgbe = ts.write_hdr(f"{tmpdir}/m{w}.hdr", rgb, rle=rle)
+    got = _map_texels(host, tmpdir, f"hdr{w}", f"{tmpdir}/m{w}.hdr")
+    e = rgbe[..., 3].astype(np.int64)
+    want = np.where(e[..., None] > 0, rgbe[..., :3] * np.ldexp(np.float32(1), e - 136)[..., None], 0).astype(np.float32)
+    np.testing.assert_array_equal(got, want)
+    # the shared exponent quantises every component to 1/128 of the texel's largest one
+    assert np.all(np.abs(got - rgb) <= rgb.max(axis=2, keepdims=True) / 128 + 1e-30)
+
+
+@pytest.mark.parametrize("compression", [0, 1, 2, 3])
+@pytest.mark.parametrize("dtype,layout", [(np.float16, "RGB"), (np.float32, "RGBA"), (np.uint32, "RGB"),
+                                          (np.float16, "Y")])
+def test_openexr_maps(host, tmpdir, compression, dtype, layout):
+    """Single-part scanline OpenEXR written by the file layout (channels in name order, NONE /
+    RLE / ZIPS / ZIP with the byte predictor and interleave, 16-line ZIP blocks, a data window
+    away from the origin, bottom-first chunk order for half the cases): the map's texels are the
+    R, G, B channels (a lone channel as grey), exactly."""
+    rng = np.random.default_rng(compression * 10 + len(layout))
+    h, w = 37, 13
+    base = rng.random((h, w)) * 4
+    base[5:20] = 1.5  # runs for RLE
+    if dtype == np.uint32:
+        chans = {c: (base * (k + 1) * 1000).astype(np.uint32) for k, c in enumerate(layout)}
+    else:
+        chans = {c: (base * (k + 1)).astype(dtype) for k, c in enumerate(layout)}
+    name = f"e{compression}{np.dtype(dtype).char}{layout}"
+    ts.write_exr(f"{tmpdir}/{name}.exr", chans, compression, window=(3, -2), decreasing=compression % 2 == 1)
+    got = _map_texels(host, tmpdir, name, f"{tmpdir}/{name}.exr")
+    want = np.stack([chans["Y" if layout == "Y" else c].astype(np.float32) for c in "RGB"], axis=2)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_openexr_and_hdr_errors(host, tmpdir):
+    chans = {c: np.ones((4, 4), np.float16) for c in "RGB"}
+    ts.write_exr(f"{tmpdir}/piz.exr", chans, 4)
+    with pytest.raises(RuntimeError, match="unsupported OpenEXR compression 4"):
+        es.load(host, tmpdir, "piz", es.env_doc(f"{tmpdir}/piz.exr", objects=False))
+    ts.write_exr(f"{tmpdir}/uv.exr", {"U": np.ones((4, 4), np.float16), "V": np.ones((4, 4), np.float16)})
+    with pytest.raises(RuntimeError, match="without R, G and B"):
+        es.load(host, tmpdir, "uv", es.env_doc(f"{tmpdir}/uv.exr", objects=False))
+    data = bytearray(open(f"{tmpdir}/uv.exr", "rb").read())
+    data[5] |= 0x02  # the tiled flag (bit 9 of the version field)
+    open(f"{tmpdir}/tiled.exr", "wb").write(bytes(data))
+    with pytest.raises(RuntimeError, match="not tiled"):
+        es.load(host, tmpdir, "tiled", es.env_doc(f"{tmpdir}/tiled.exr", objects=False))
+    open(f"{tmpdir}/bad.hdr", "wb").write(b"#?RADIANCE\n\n+Y 4 +X 4\n" + bytes(64))
+    with pytest.raises(RuntimeError, match="layout"):
+        es.load(host, tmpdir, "badhdr", es.env_doc(f"{tmpdir}/bad.hdr", objects=False))

@@ -162,3 +162,113 @@ def png_expected_rgb8(samples, colour_type, depth, palette=None):
     if colour_type in (0, 4):
         return np.repeat(s[..., :1], 3, axis=2)
     return s[..., :3]
+
+
+def write_hdr(path, rgb, rle=True):
+    """Radiance RGBE (Ward): per texel the shared exponent e = frexp(max) + 128 and mantissas
+    floor(c / 2^(e - 136)); scanlines run-length coded per component (runs of equal bytes as
+    128 + n, literals as n) or flat. Returns the RGBE bytes (h, w, 4)."""
+    h, w, _ = rgb.shape
+    m = rgb.max(axis=2)
+    mant, ex = np.frexp(m)
+    e = np.where(m > 1e-32, ex + 128, 0).astype(np.int64)
+    scale = np.where(e > 0, np.ldexp(1.0, -(e - 136)), 0.0)
+    rgbe = np.zeros((h, w, 4), np.uint8)
+    rgbe[..., :3] = np.clip(np.floor(rgb * scale[..., None]), 0, 255).astype(np.uint8)
+    rgbe[..., 3] = e.astype(np.uint8)
+    out = bytearray(b"#?RADIANCE\n# test map\nFORMAT=32-bit_rle_rgbe\n\n" + f"-Y {h} +X {w}\n".encode())
+    for y in range(h):
+        if not rle or w < 8 or w > 32767:  # the format codes such scanlines flat
+            out += rgbe[y].tobytes()
+            continue
+        out += bytes([2, 2, w >> 8, w & 255])
+        for c in range(4):
+            row, x = rgbe[y, :, c], 0
+            while x < w:
+                run = 1
+                while x + run < w and run < 127 and row[x + run] == row[x]:
+                    run += 1
+                if run >= 3:
+                    out += bytes([128 + run, row[x]])
+                    x += run
+                else:
+                    n = 1
+                    while x + n < w and n < 128 and not (x + n + 2 < w and row[x + n] == row[x + n + 1] == row[x + n + 2]):
+                        n += 1
+                    out += bytes([n]) + row[x:x + n].tobytes()
+                    x += n
+    with open(path, "wb") as f:
+        f.write(out)
+    return rgbe
+
+
+def write_exr(path, channels, compression=0, window=(0, 0), decreasing=False):
+    """A single-part scanline OpenEXR file by the file layout: channels {name: (h, w) array of
+    float16, float32 or uint32}; compression 0 NONE, 1 RLE, 2 ZIPS, 3 ZIP (RLE and ZIP after the
+    byte predictor and the even / odd interleave); data window at `window`; decreasing: chunks
+    stored bottom first."""
+    import struct
+    import zlib
+    names = sorted(channels)
+    h, w = channels[names[0]].shape
+    ptype = {np.dtype(np.uint32): 0, np.dtype(np.float16): 1, np.dtype(np.float32): 2}
+
+    def attr(name, typ, data):
+        return name.encode() + b"\0" + typ.encode() + b"\0" + struct.pack("<I", len(data)) + data
+
+    chl = b"".join(n.encode() + b"\0" + struct.pack("<iB3xii", ptype[channels[n].dtype], 0, 1, 1) for n in names) + b"\0"
+    x0, y0 = window
+    box = struct.pack("<iiii", x0, y0, x0 + w - 1, y0 + h - 1)
+    head = (attr("channels", "chlist", chl) + attr("compression", "compression", bytes([compression])) +
+            attr("dataWindow", "box2i", box) + attr("displayWindow", "box2i", box) +
+            attr("lineOrder", "lineOrder", bytes([1 if decreasing else 0])) +
+            attr("pixelAspectRatio", "float", struct.pack("<f", 1.0)) +
+            attr("screenWindowCenter", "v2f", struct.pack("<ff", 0, 0)) +
+            attr("screenWindowWidth", "float", struct.pack("<f", 1.0)) + b"\0")
+    lines = 16 if compression == 3 else 1
+
+    def predict(raw):
+        t = np.frombuffer(raw, np.uint8)
+        inter = np.concatenate([t[0::2], t[1::2]]).astype(np.int64)
+        d = inter.copy()
+        d[1:] = (inter[1:] - inter[:-1] + 128) % 256
+        return d.astype(np.uint8).tobytes()
+
+    def rle(b):
+        out, i = bytearray(), 0
+        while i < len(b):
+            run = 1
+            while i + run < len(b) and run < 128 and b[i + run] == b[i]:
+                run += 1
+            if run >= 3:
+                out += struct.pack("<b", run - 1) + b[i:i + 1]
+                i += run
+            else:
+                j = i
+                while j < len(b) and j - i < 127 and not (j + 2 < len(b) and b[j] == b[j + 1] == b[j + 2]):
+                    j += 1
+                out += struct.pack("<b", -(j - i)) + b[i:j]
+                i = j
+        return bytes(out)
+
+    chunks = []
+    for c0 in range(0, h, lines):
+        raw = b"".join(channels[n][r].astype(channels[n].dtype.newbyteorder("<")).tobytes()
+                       for r in range(c0, min(h, c0 + lines)) for n in names)
+        if compression == 1:
+            data = rle(predict(raw))
+        elif compression in (2, 3):
+            data = zlib.compress(predict(raw))
+        else:
+            data = raw
+        if len(data) >= len(raw):
+            data = raw
+        chunks.append(struct.pack("<ii", y0 + c0, len(data)) + data)
+    order = list(range(len(chunks)))[::-1] if decreasing else list(range(len(chunks)))
+    base = 8 + len(head) + 8 * len(chunks)
+    offsets, body = [0] * len(chunks), b""
+    for k in order:
+        offsets[k] = base + len(body)
+        body += chunks[k]
+    with open(path, "wb") as f:
+        f.write(struct.pack("<II", 20000630, 2) + head + struct.pack(f"<{len(chunks)}Q", *offsets) + body)
