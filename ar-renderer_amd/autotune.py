@@ -9,6 +9,7 @@ fastest setting to the tuning file the plugin reads when it first enumerates dev
   path_budget           paths in flight per batch: the capacity of every ray and material queue
   graph_rounds          bounce rounds replayed per captured HIP graph (0: eager launches)
   material_sort         sort the material queues by material and medium each bounce (0 / 1)
+  ray_sort              sort the next bounce's rays by direction octant and origin cell (0 / 1)
 
 The objective is the wall time of render calls on a benchmark configuration (default: the
 1920x1080 terrain scene, configuration 2, rendered into the
@@ -43,6 +44,7 @@ CANDIDATES = {
     "path_budget": [1 << 22, 1 << 23, 1 << 24, 1 << 25],
     "graph_rounds": [0, 2, 4],
     "material_sort": [0, 1],
+    "ray_sort": [0, 1],
 }
 # (config, width, height, max depth, rr start, spp per call). Configuration 2 is swept at 16 spp per
 # call (33 M samples): a call larger than the smaller queue capacities, so that the capacity's cost

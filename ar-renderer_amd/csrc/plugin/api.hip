@@ -65,6 +65,7 @@ const TuneKey kTuneKeys[] = {
     {"guide_lr", 1e-6, 1.0, false},                    {"guide_bounces", 0, 64, true},
     {"guide_blend", 0.0, 1.0, false},                  {"graph_rounds", 0, 16, true},
     {"guide_replay", 0, 1 << 24, true},                {"material_sort", 0, 1, true},
+    {"ray_sort", 0, 1, true},
 };
 
 bool TuneSet(Tuning &t, const std::string &key, double v) {
@@ -82,6 +83,7 @@ bool TuneSet(Tuning &t, const std::string &key, double v) {
         else if (key == "guide_blend") t.guideBlend = float(v);
         else if (key == "guide_replay") t.guideReplay = int(v);
         else if (key == "material_sort") t.materialSort = int(v);
+        else if (key == "ray_sort") t.raySort = int(v);
         else if (key == "graph_rounds") {
             if (int(v) % 2) return false;  // a replay must end on the queue it started from
             t.graphRounds = int(v);
@@ -104,6 +106,7 @@ bool TuneGet(const Tuning &t, const std::string &key, double *v) {
     else if (key == "graph_rounds") *v = t.graphRounds;
     else if (key == "guide_replay") *v = t.guideReplay;
     else if (key == "material_sort") *v = t.materialSort;
+    else if (key == "ray_sort") *v = t.raySort;
     else return false;
     return true;
 }

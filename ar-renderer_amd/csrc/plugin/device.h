@@ -63,6 +63,7 @@ struct Tuning {
     int guideReplay = 1 << 20;               // training records kept across batches (0: the batch's only)
     int graphRounds = 0;                     // bounce rounds per captured HIP graph (even; 0: none)
     int materialSort = 0;                    // sort the material queues by material and medium each bounce
+    int raySort = 0;                         // sort the next bounce's rays by direction octant and origin cell
 };
 
 struct PTScene;     // path-tracing data (pt.hip)

@@ -224,6 +224,7 @@ struct PTScene {
     // batch buffers
     size_t cap = 0;
     DevBuf o, d, rng, lam, lpdf, beta, rpath, L, Lh, prevP, prevN, st, hit, q0, q1, qm, qk, qms, sortBins, pfilm, xyz;
+    DevBuf nk, qr, raySortBins;
     DevBuf h0, h1, h2, h3, qe, qeLpdf, qes;
     DevBuf sceneBuf;  // the DevScene the kernels read through a pointer
     DevBuf so, sd, sp, sn, sf, sL, sb, sq;  // shadow records and the shadow queue
@@ -707,6 +708,9 @@ struct Bufs {
                  // medium scattering vertex its medium), written beside it; null without the sort
     int *qms;    // the material queues sorted by key, which the material stage then reads
     int *sortTotals, *sortCursor;  // per family and key bin: entries, then each bin's next position
+    int *nk;     // with the ray sort: each next-queue entry's key (direction octant, origin cell); null without
+    int *qr;     // the next queue sorted by that key, which the next bounce traces
+    int *raySortTotals, *raySortCursor;
     int qcap;
     int *counts;  // [0] queue A, [1] queue B, [2] and [11] shadow queues (alternating bounces when the
                   // shadow rays are traced with the next bounce's closest hits), [3..5] plain
@@ -845,8 +849,9 @@ __device__ __forceinline__ void tile_flush(TileStage &t, const Bufs &b, int *nex
             if (q != int(kNoQueue)) {
                 const int pos = t.base[q] + int(ce & 0x7ffu);
                 queue_ptr(b, next, q)[pos] = slot;
-                // a material queue entry (only an item's first) keeps its sort key beside it
+                // a material or next-queue entry (only an item's first) keeps its sort key beside it
                 if (e == 0 && q < int(QNext) && b.qk) b.qk[size_t(q) * b.qcap + pos] = t.key[j];
+                if (e == 0 && q == int(QNext) && b.nk) b.nk[pos] = t.key[j];
             }
         }
     }
@@ -1061,6 +1066,21 @@ __device__ __forceinline__ void add_emitted(const DevScene &s, PathRegs &r, int 
     }
 }
 
+// Sort key of a ray for the traversal (tuning key ray_sort): its direction octant, then the Morton
+// code of its origin's cell in an 8 x 8 x 8 grid over the scene's bounds (3 + 9 bits).
+constexpr int kRaySortBins = 1 << 12;
+__device__ __forceinline__ int ray_key(const Params &p, V3 o, V3 d) {
+    auto cell = [&](float v, int a) {
+        const float c = (v - p.box.lo[a]) * p.box.inv[a] * 8.f;
+        return c >= 0.f ? (c < 8.f ? int(c) : 7) : 0;
+    };
+    const int cx = cell(o.x, 0), cy = cell(o.y, 1), cz = cell(o.z, 2);
+    int m = 0;
+    for (int k = 0; k < 3; ++k)
+        m |= (((cx >> k) & 1) << (3 * k)) | (((cy >> k) & 1) << (3 * k + 1)) | (((cz >> k) & 1) << (3 * k + 2));
+    return ((d.x < 0 ? 1 : 0) | (d.y < 0 ? 2 : 0) | (d.z < 0 ? 4 : 0)) << 9 | m;
+}
+
 // ---- stage 1 of a bounce: everything up to the BSDF (path.cpp:155-210) ----------------------------
 // Medium distance sampling, escape and area emission, interfaces, the depth limit. Paths that
 // continue at a surface are compacted into one queue per BSDF family together with their hit
@@ -1203,6 +1223,7 @@ __device__ __forceinline__ void surface_vertex(const DevScene &s, const Params &
         b.d[slot] = make_float4(r.rd.x, r.rd.y, r.rd.z, r.etaScale);
     }
     *q1 = toMaterial ? mq : (alive ? int(QNext) : -1);
+    if (!toMaterial && alive && b.nk) *key = ray_key(p, r.ro, r.rd);
 }
 
 template <bool kMedia, bool kInterfaces>
@@ -1282,7 +1303,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(const DevScene *__restrict__ sp
 // continues, *q2 QShadow when it made a shadow ray.
 template <int kType, bool kMedia, bool kGuided>
 __device__ __forceinline__ void material_vertex(const DevScene &s, const Params &p, const Bufs &b, int i, int slot, int *q1,
-                                                int *q2) {
+                                                int *q2, int *key) {
     PathRegs r;
     float4 d = b.d[slot];
     r.rd = V3(d.x, d.y, d.z);
@@ -1397,6 +1418,7 @@ __device__ __forceinline__ void material_vertex(const DevScene &s, const Params 
         b.prevN[slot] = make_float4(si.n.x, si.n.y, si.n.z, 0.f);
         b.st[slot] = make_int4(r.depth, r.flags, r.medium, st.w);
         *q1 = QNext;
+        if (b.nk) *key = ray_key(p, r.ro, r.rd);
     }
 }
 
@@ -1405,7 +1427,8 @@ __device__ __forceinline__ void material_vertex(const DevScene &s, const Params 
 // roulette, drawing from the path's stream in that order. k_surface has applied the distance
 // sample's weights, counted the vertex against the depth limit and left the scattering point in
 // the hit record.
-__device__ __forceinline__ void phase_vertex(const DevScene &s, const Params &p, const Bufs &b, int slot, int *q1, int *q2) {
+__device__ __forceinline__ void phase_vertex(const DevScene &s, const Params &p, const Bufs &b, int slot, int *q1, int *q2,
+                                             int *key) {
     PathRegs r;
     const float4 d = b.d[slot];
     r.rd = V3(d.x, d.y, d.z);
@@ -1443,6 +1466,7 @@ __device__ __forceinline__ void phase_vertex(const DevScene &s, const Params &p,
         b.prevN[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
         b.st[slot] = make_int4(r.depth, r.flags, r.medium, st.w);
         *q1 = QNext;
+        if (b.nk) *key = ray_key(p, r.ro, r.rd);
     }
 }
 
@@ -1458,13 +1482,13 @@ __global__ __launch_bounds__(kBlock) PG_WAVES(kType == QLayered ? 0 : kMaterialW
 #pragma unroll 1
         for (int k = 0; k < kAppendK; ++k) {
             const int j = k * kBlock + int(threadIdx.x), i = t0 + j;
-            int slot = 0, q1 = -1, q2 = -1;
+            int slot = 0, q1 = -1, q2 = -1, key = 0;
             if (i < n) {
                 slot = queue[i];
-                if constexpr (kType == QPhase) phase_vertex(s, p, b, slot, &q1, &q2);
-                else material_vertex<kType, kMedia, kGuided>(s, p, b, i, slot, &q1, &q2);
+                if constexpr (kType == QPhase) phase_vertex(s, p, b, slot, &q1, &q2, &key);
+                else material_vertex<kType, kMedia, kGuided>(s, p, b, i, slot, &q1, &q2, &key);
             }
-            tile_stage(ts, j, slot, q1, q2);
+            tile_stage(ts, j, slot, q1, q2, key);
         }
         tile_flush(ts, b, nextQueue, nextCount);
     }
@@ -1493,23 +1517,23 @@ __global__ __launch_bounds__(kBlock) PG_WAVES(kMaterialWaves) void k_material_pl
 #pragma unroll 1
         for (int k = 0; k < kAppendK; ++k) {
             const int j = k * kBlock + int(threadIdx.x), i = t0 + j;
-            int slot = 0, q1 = -1, q2 = -1;
+            int slot = 0, q1 = -1, q2 = -1, key = 0;
             if (i < nD) {
                 slot = b.qm[QDiffuse * b.qcap + i];
-                material_vertex<QDiffuse, kMedia, false>(s, p, b, i, slot, &q1, &q2);
+                material_vertex<QDiffuse, kMedia, false>(s, p, b, i, slot, &q1, &q2, &key);
             } else if (i < nD + nC) {
                 const int ii = i - nD;
                 slot = b.qm[QConductor * b.qcap + ii];
-                material_vertex<QConductor, kMedia, false>(s, p, b, ii, slot, &q1, &q2);
+                material_vertex<QConductor, kMedia, false>(s, p, b, ii, slot, &q1, &q2, &key);
             } else if (i < nDCE) {
                 const int ii = i - nD - nC;
                 slot = b.qm[QDielectric * b.qcap + ii];
-                material_vertex<QDielectric, kMedia, false>(s, p, b, ii, slot, &q1, &q2);
+                material_vertex<QDielectric, kMedia, false>(s, p, b, ii, slot, &q1, &q2, &key);
             } else if (kMedia && i < n) {
                 slot = b.qm[QPhase * b.qcap + (i - nDCE)];
-                phase_vertex(s, p, b, slot, &q1, &q2);
+                phase_vertex(s, p, b, slot, &q1, &q2, &key);
             }
-            tile_stage(ts, j, slot, q1, q2);
+            tile_stage(ts, j, slot, q1, q2, key);
         }
         tile_flush(ts, b, nextQueue, nextCount);
     }
@@ -1600,6 +1624,71 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(Bufs b) {
             const int pos = lds_append(hist, bin);
             if (bin >= 0) b.qms[off + pos] = b.qm[off + i];
         }
+    }
+}
+
+// The same counting sort for the next bounce's path queue by ray key (tuning key ray_sort), so
+// that a traversal wave takes rays of one direction octant from one region of the scene.
+__device__ __forceinline__ void ray_sort_counts(const Bufs &b, int n, int *hist) {
+    for (int i = int(threadIdx.x); i < kRaySortBins; i += kBlock) hist[i] = 0;
+    __syncthreads();
+    const int per = (n + int(gridDim.x) - 1) / int(gridDim.x);
+    const int lo = min(n, int(blockIdx.x) * per), hi = min(n, lo + per);
+    for (int i0 = lo; i0 < hi; i0 += kBlock) {
+        const int i = i0 + int(threadIdx.x);
+        (void)lds_append(hist, i < hi ? (b.nk[i] & (kRaySortBins - 1)) : -1);
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kBlock) void k_ray_sort_count(Bufs b, const int *count) {
+    __shared__ int hist[kRaySortBins];
+    ray_sort_counts(b, *count, hist);
+    for (int i = int(threadIdx.x); i < kRaySortBins; i += kBlock)
+        if (hist[i]) atomicAdd(&b.raySortTotals[i], hist[i]);
+}
+
+// One block: the exclusive scan of the bin totals (each thread sums a run of consecutive bins, the
+// runs are scanned in LDS); the totals are cleared for the next bounce.
+__global__ __launch_bounds__(kBlock) void k_ray_sort_scan(Bufs b) {
+    constexpr int kRun = kRaySortBins / kBlock;
+    __shared__ int sc[kBlock];
+    const int t = int(threadIdx.x);
+    int v[kRun], run = 0;
+    for (int k = 0; k < kRun; ++k) {
+        v[k] = b.raySortTotals[t * kRun + k];
+        run += v[k];
+    }
+    sc[t] = run;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        const int add = t >= off ? sc[t - off] : 0;
+        __syncthreads();
+        sc[t] += add;
+        __syncthreads();
+    }
+    int pos = sc[t] - run;
+    for (int k = 0; k < kRun; ++k) {
+        b.raySortCursor[t * kRun + k] = pos;
+        pos += v[k];
+        b.raySortTotals[t * kRun + k] = 0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_ray_sort_scatter(Bufs b, const int *count, const int *src) {
+    __shared__ int hist[kRaySortBins];
+    const int n = *count;
+    ray_sort_counts(b, n, hist);
+    for (int i = int(threadIdx.x); i < kRaySortBins; i += kBlock)
+        if (hist[i]) hist[i] = atomicAdd(&b.raySortCursor[i], hist[i]);
+    __syncthreads();
+    const int per = (n + int(gridDim.x) - 1) / int(gridDim.x);
+    const int lo = min(n, int(blockIdx.x) * per), hi = min(n, lo + per);
+    for (int i0 = lo; i0 < hi; i0 += kBlock) {
+        const int i = i0 + int(threadIdx.x);
+        const int bin = i < hi ? (b.nk[i] & (kRaySortBins - 1)) : -1;
+        const int pos = lds_append(hist, bin);
+        if (bin >= 0) b.qr[pos] = src[i];
     }
 }
 
@@ -2080,6 +2169,12 @@ void EnsureBatch(PTScene &pt, size_t n) {
         pt.sortBins.ensure(2 * kSortSlots * sizeof(int));
         PG_CHECK(hipMemset(pt.sortBins.p, 0, pt.sortBins.bytes));
     }
+    pt.nk.ensure(n * sizeof(int));
+    pt.qr.ensure(n * sizeof(int));
+    if (!pt.raySortBins.p) {  // likewise for k_ray_sort_scan
+        pt.raySortBins.ensure(2 * kRaySortBins * sizeof(int));
+        PG_CHECK(hipMemset(pt.raySortBins.p, 0, pt.raySortBins.bytes));
+    }
     f4(pt.so); f4(pt.sd); f4(pt.sp); f4(pt.sn); f4(pt.sf); f4(pt.sL); f4(pt.sb);
     pt.rng.ensure(n * sizeof(ulonglong2));
     pt.st.ensure(n * sizeof(int4));
@@ -2102,6 +2197,10 @@ Bufs MakeBufs(PTScene &pt) {
     b.qms = pt.qms.as<int>();
     b.sortTotals = pt.sortBins.as<int>();
     b.sortCursor = b.sortTotals + kSortSlots;
+    b.nk = nullptr;  // set by a batch that sorts its rays
+    b.qr = pt.qr.as<int>();
+    b.raySortTotals = pt.raySortBins.as<int>();
+    b.raySortCursor = b.raySortTotals + kRaySortBins;
     b.qes = pt.qes.as<int>();
     b.sq = pt.sq.as<int>();
     b.shadowIdx = 2;
@@ -2257,11 +2356,15 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
     const bool fused = kFusedShadow && !media && !interfaces;
     const bool sorted = dev.tune.materialSort != 0 && pt.sortable != 0;
     if (sorted) b.qk = pt.qk.as<int>();
+    // The ray sort reorders the next queue into its own buffer, which the next bounce then traces
+    // (not with captured bounce graphs, whose rounds fix their queues).
+    const bool raySorted = dev.tune.raySort != 0 && dev.tune.graphRounds == 0;
+    if (raySorted) b.nk = pt.nk.as<int>();
     const int sortGrid = std::max(1, std::min(shadeGrid, dev.cus * 4));
     int pendingShadow = 0;  // shadow rays the next round traces (profiled rounds: read back)
     auto round = [&](int curIdx, int bounce, int tgrid, int grid) {
         const int nxtIdx = 1 - curIdx;
-        int *cur = curIdx ? b.q1 : b.q0, *nxt = curIdx ? b.q0 : b.q1;
+        int *cur = raySorted && bounce > 0 ? b.qr : (curIdx ? b.q1 : b.q0), *nxt = curIdx ? b.q0 : b.q1;
         Bufs rb = b;
         const int sIn = shadow_count_index(1 - curIdx);
         rb.shadowIdx = fused ? shadow_count_index(curIdx) : 2;
@@ -2323,6 +2426,11 @@ void TraceBatch(Device &dev, const DevScene &s, const DevScene *ds, Params p, in
             if (pt.hasLayered) {
                 auto *kl = media ? k_material<QLayered, true> : k_material<QLayered, false>;
                 hipLaunchKernelGGL(kl, dim3(grid), dim3(kBlock), 0, st, ds, p, mb, nxt, b.counts + nxtIdx);
+            }
+            if (raySorted) {  // the next bounce traces the queue sorted by ray key
+                hipLaunchKernelGGL(k_ray_sort_count, dim3(sortGrid), dim3(kBlock), 0, st, rb, b.counts + nxtIdx);
+                hipLaunchKernelGGL(k_ray_sort_scan, dim3(1), dim3(kBlock), 0, st, rb);
+                hipLaunchKernelGGL(k_ray_sort_scatter, dim3(sortGrid), dim3(kBlock), 0, st, rb, b.counts + nxtIdx, nxt);
             }
         });
         if (!fused)
@@ -2468,11 +2576,11 @@ void RenderExImpl(Device &dev, const prgpu::SceneDescEx &sd, const prgpu::Render
         if (p.train)
             dev.guide->EnsureTrain(std::max(size_t(dev.tune.guideBatch) * size_t(dev.tune.guideSteps),
                                             size_t(dev.tune.guideReplay)));
-        for (int a = 0; a < 3; ++a) {
-            p.box.lo[a] = dev.sceneLo[a];
-            const float ext = dev.sceneHi[a] - dev.sceneLo[a];
-            p.box.inv[a] = ext > 0 ? 1.f / ext : 0.f;
-        }
+    }
+    for (int a = 0; a < 3; ++a) {  // the scene's bounds: guiding's position encoding, the ray sort's cells
+        p.box.lo[a] = dev.sceneLo[a];
+        const float ext = dev.sceneHi[a] - dev.sceneLo[a];
+        p.box.inv[a] = ext > 0 ? 1.f / ext : 0.f;
     }
     for (int done = 0; done < rp.spp; done += per) {
         int passes = std::min(per, rp.spp - done);

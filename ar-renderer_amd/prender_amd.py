@@ -566,7 +566,7 @@ class Plugin:
 
     TUNE_KEYS = ("trace_blocks_per_cu", "shade_blocks_per_cu", "path_budget", "guide_batch", "guide_steps",
                  "guide_min_steps", "guide_lr", "guide_bounces", "guide_blend", "graph_rounds",
-                 "guide_replay", "material_sort")
+                 "guide_replay", "material_sort", "ray_sort")
 
     def tune_set(self, key, value, device=0):
         self.lib.prgpu_tune_set.argtypes = [C.c_int, C.c_char_p, C.c_double]

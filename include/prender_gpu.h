@@ -414,7 +414,9 @@ PRGPU_API int prgpu_profile_read(int deviceIndex, prgpu::ProfileStatsEx *out, in
 // graph_rounds (bounce rounds replayed per captured HIP graph, even, 0 = eager launches),
 // guide_replay (training records kept across batches in a ring, 0 = train on each batch's own),
 // material_sort (1: each bounce sorts the material queues by material, and the medium scattering
-// vertices by medium, before the material stage; 0: family order only).
+// vertices by medium, before the material stage; 0: family order only), ray_sort (1: each bounce
+// sorts the next bounce's rays by direction octant and origin cell; 0: the order they leave the
+// material stage in).
 // Defaults come from the tuning file the autotuner writes ($PRGPU_TUNING_FILE, else
 // prgpu_tuning.txt beside the library: "key value" lines), read when devices are first
 // enumerated. set / get return 0, or 1 for an unknown key, a value out of range or no device.

@@ -36,14 +36,14 @@ def _probe(tmp_path, text):
 
 def test_valid_tuning_file_is_read(tmp_path):
     f, path, err = _probe(tmp_path, "# written by the autotuner\ntrace_blocks_per_cu 6\nshade_blocks_per_cu 12  # comment\n"
-                                    "\npath_budget 4194304\nguide_lr 0.003\ngraph_rounds 4\nmaterial_sort 1\n")
+                                    "\npath_budget 4194304\nguide_lr 0.003\ngraph_rounds 4\nmaterial_sort 1\nray_sort 1\n")
     assert path == f
     assert err == ""
 
 
 def test_invalid_tuning_lines_are_named(tmp_path):
     for bad in ("no_such_key 3", "trace_blocks_per_cu 9", "graph_rounds 3", "guide_batch 1000.5", "guide_lr 0.1 extra",
-                "shade_blocks_per_cu", "material_sort 2"):
+                "shade_blocks_per_cu", "material_sort 2", "ray_sort -1"):
         f, path, err = _probe(tmp_path, "trace_blocks_per_cu 4\n" + bad + "\n")
         assert path == f
         assert err.endswith(":2: invalid tuning line"), (bad, err)

@@ -3,7 +3,8 @@
 The path of every camera sample is a pure function of (pixel, sample index, seed), so the grids of
 the traversal and shading kernels and replaying captured bounce graphs instead of launching eagerly
 must give the film of the default settings bit for bit, and so must sorting the material queues by
-material and medium (the order the material stage takes its paths in), and a smaller queue capacity (paths in
+material and medium (the order the material stage takes its paths in) and sorting the next
+bounce's rays by direction and origin (the order they are traced in), and a smaller queue capacity (paths in
 flight per batch, which splits a call into batches) the same film up to the grouping of its FP64
 sums. Captured graphs are replayed many times per call
 (one replay per graph_rounds bounces), with the eager renders of the same sample ranges between
@@ -46,7 +47,7 @@ def test_graph_replays_equal_eager_launches(plugin, host, defaults):
 
 
 @pytest.mark.parametrize("key,value", [("trace_blocks_per_cu", 3), ("shade_blocks_per_cu", 5),
-                                       ("path_budget", 65536), ("material_sort", 1)])
+                                       ("path_budget", 65536), ("material_sort", 1), ("ray_sort", 1)])
 def test_launch_shape_and_queue_capacity_keep_the_film(plugin, host, defaults, key, value):
     s = host.generate(2, 160, 90)
     ref = _film(plugin, s, 0, spp=16)
@@ -60,27 +61,29 @@ def test_launch_shape_and_queue_capacity_keep_the_film(plugin, host, defaults, k
         assert np.array_equal(film, ref)
 
 
-def test_material_sort_keeps_the_film_with_media_coats_and_guiding(plugin, host, defaults):
-    """The sort on scenes where every material queue is in use: the sample scene's glass, metals,
-    coats, sheen and subsurface media (the phase-function queue sorted by medium), and the room
-    with its learned guide (the guided diffuse queue is inferred and shaded in sorted order)."""
+@pytest.mark.parametrize("key", ["material_sort", "ray_sort"])
+def test_sorts_keep_the_film_with_media_coats_and_guiding(plugin, host, defaults, key):
+    """The material sort and the ray sort on scenes where every queue is in use: the sample
+    scene's glass, metals, coats, sheen and subsurface media (the phase-function queue sorted by
+    medium; interface crossings re-enter the path queue from the surface stage), and the room with
+    its learned guide (the guided diffuse queue is inferred and shaded in sorted order)."""
     g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sample_scene.npz"))
     for name in ("studio", "studio_fog"):
         scene = pa.flat_scene(g, name)
         prm = pa.render_params(spp=2, first_sample=5, seed=1, max_depth=scene.defaults["maxDepth"], rr_start=1)
         films = []
         for srt in (0, 1):
-            plugin.tune_set("material_sort", srt)
+            plugin.tune_set(key, srt)
             films.append(plugin.render_ex(scene, prm, variance=False)[0])
-        assert films[0].any() and np.array_equal(films[0], films[1]), f"{name}: the sort changed the film"
+        assert films[0].any() and np.array_equal(films[0], films[1]), f"{name}: {key} changed the film"
     room = host.generate(4, 96, 54, 0.25)
     films = []
     for srt in (0, 1):
-        plugin.tune_set("material_sort", srt)
+        plugin.tune_set(key, srt)
         prm = pa.render_params(spp=4, first_sample=0, seed=1, max_depth=8, flags=pa.RENDER_GUIDE | pa.RENDER_GUIDE_RESET)
         films.append(plugin.render_ex(room, prm, variance=False)[0])
-    assert films[0].any() and np.array_equal(films[0], films[1]), "guided room: the sort changed the film"
-    print("material sort: sample scene (studio, fog) and guided room films equal the unsorted ones", flush=True)
+    assert films[0].any() and np.array_equal(films[0], films[1]), f"guided room: {key} changed the film"
+    print(f"{key}: sample scene (studio, fog) and guided room films equal the unsorted ones", flush=True)
 
 
 def test_out_of_range_values_are_refused(plugin, defaults):
