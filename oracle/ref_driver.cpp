@@ -25,6 +25,12 @@
 
 #include "../include/prender_gpu.h"
 
+// The reference's image reader decodes PNG with stb_image asking for 3 channels
+// (src/core/image.cpp:90); the same vendored decoder, built here for the PNG golden vectors.
+#define STB_IMAGE_IMPLEMENTATION
+#define STBI_ONLY_PNG
+#include "stb/stb_image.h"
+
 #include <cstring>
 #include <memory>
 #include <unordered_map>
@@ -125,6 +131,18 @@ void rf_cie(float l, float *xyz) {
     xyz[2] = CIE_Z(l);
 }
 float rf_cie_y_integral() { return CIE_Y_Integral(); }
+
+// stb_image's 8-bit RGB decode of an encoded image (stbi_load_from_memory with 3 channels, as
+// ReadImage asks for); returns w * h * 3 and writes the texels when they fit `cap`, -1 on failure.
+int rf_stbi_rgb8(const unsigned char *data, int len, int *w, int *h, unsigned char *out, int cap) {
+    int n = 0;
+    unsigned char *d = stbi_load_from_memory(data, len, w, h, &n, 3);
+    if (!d) return -1;
+    const int size = *w * *h * 3;
+    if (out && size <= cap) std::memcpy(out, d, size_t(size));
+    stbi_image_free(d);
+    return size;
+}
 
 // The RGB-to-spectrum fit's inputs as the reference defines them (src/core/rgb2spec.cpp:39-74),
 // from its own colour matrices, white adaptation, D65 and CIE functions: per fine wavelength (189

@@ -33,6 +33,7 @@ def load_ref():
         ("rf_cie", None, [C.c_float, vp]),
         ("rf_cie_y_integral", C.c_float, []),
         ("rf_rgb_fit_tables", None, [vp, vp, vp]),
+        ("rf_stbi_rgb8", C.c_int, [vp, C.c_int, vp, vp, vp, C.c_int]),
         ("rf_to_xyz", None, [vp, C.c_float, C.c_int, vp]),
         ("rf_spectrum_eval", C.c_float, [vp, vp, C.c_float]),
         ("rf_spectrum_y", C.c_float, [vp, vp]),
@@ -335,10 +336,56 @@ def main():
     texture_lookup(L, out)
     shading_perturbation(L, out)
     rgb_fit_tables(L, out)
+    png_decodes(L, out)
 
     path = os.path.join(HERE, "ref_components.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")
+
+
+def stb_rgb8(L, data):
+    w, h = C.c_int(), C.c_int()
+    n = L.rf_stbi_rgb8(data, len(data), C.byref(w), C.byref(h), None, 0)
+    assert n > 0, "stb_image refused the file"
+    buf = np.zeros(n, np.uint8)
+    L.rf_stbi_rgb8(data, len(data), C.byref(w), C.byref(h), ptr(buf), n)
+    return buf.reshape(h.value, w.value, 3)
+
+
+def png_decodes(L, out):
+    """PNG files of every colour type and bit depth, plain and Adam7, with mixed row filters (the
+    test writer's, from the PNG specification), and the reference's own silk-weave texture,
+    decoded by the reference's stb_image to 8-bit RGB: the files and the texels (the large
+    texture by its SHA-256)."""
+    import hashlib
+    import tempfile
+    sys.path.insert(0, os.path.join(HERE, ".."))
+    import texscenes as ts
+    rng = np.random.default_rng(11)
+    files, texels, shapes = [], [], []
+    kinds = [(0, 1), (0, 2), (0, 4), (0, 8), (0, 16), (2, 8), (2, 16), (3, 1), (3, 2), (3, 4), (3, 8), (4, 8),
+             (4, 16), (6, 8), (6, 16)]
+    with tempfile.TemporaryDirectory() as d:
+        for ct, depth in kinds:
+            for interlace in (False, True):
+                h, w = int(rng.integers(1, 19)), int(rng.integers(1, 23))
+                nc = {0: 1, 2: 3, 3: 1, 4: 2, 6: 4}[ct]
+                palette = rng.integers(0, 256, (min(2 ** depth, 11), 3)) if ct == 3 else None
+                samples = rng.integers(0, len(palette) if ct == 3 else 2 ** depth, (h, w, nc))
+                path = os.path.join(d, "p.png")
+                ts.write_png(path, samples, ct, depth, (0, 1, 2, 3, 4, 4, 2), interlace, palette)
+                data = open(path, "rb").read()
+                files.append(np.frombuffer(data, np.uint8))
+                texels.append(stb_rgb8(L, data).reshape(-1))
+                shapes.append((h, w))
+    out["png_file_bytes"] = np.concatenate(files)
+    out["png_file_ends"] = np.cumsum([len(f) for f in files])
+    out["png_texels"] = np.concatenate(texels)
+    out["png_shapes"] = np.array(shapes, np.int32)
+    silk = open("/root/reference/scenes/diamonds/textures/silk_weave.png", "rb").read()
+    t = stb_rgb8(L, silk)
+    out["silk_shape"] = np.array(t.shape[:2], np.int32)
+    out["silk_sha256"] = np.frombuffer(hashlib.sha256(t.tobytes()).digest(), np.uint8)
 
 
 def rgb_fit_tables(L, out):

@@ -15,6 +15,8 @@ reference's compiled ImageData::Lookup and Material::PerturbShading (test_oracle
 - a reflectance-1 sphere with a white image texture under a white environment is a white furnace;
   a constant-colour texture seen under it has the sRGB -> XYZ of its colour within 2%.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -23,6 +25,8 @@ import oracle as orc
 import pathscenes as ps
 import prender_amd as pa
 import texscenes as ts
+
+REF_SILK = "/root/reference/scenes/diamonds/textures/silk_weave.png"
 
 
 @pytest.fixture(scope="module")
@@ -211,3 +215,48 @@ def test_png_srgb_decode_and_errors(host, tmpdir):
         d = dict(doc, textures={"t": {"type": "image", "file": f"{tmpdir}/{name}.png"}})
         with pytest.raises(RuntimeError, match=msg):
             es.load(host, tmpdir, "png_" + name, d)
+
+
+def test_png_decoding_equals_the_reference_decoder(host, tmpdir):
+    """The reference reads PNG through its vendored stb_image asking for 8-bit RGB
+    (src/core/image.cpp:90): the golden vectors hold that decoder's output for PNG files of every
+    colour type and bit depth, plain and interlaced, with every row filter. The loader's linear
+    texels times 255 are those bytes exactly."""
+    G = np.load(f"{os.path.dirname(os.path.abspath(__file__))}/golden/ref_components.npz")
+    ends, shapes = G["png_file_ends"], G["png_shapes"]
+    starts = np.concatenate([[0], ends[:-1]])
+    tstart = 0
+    textures, mats, objects, want = {}, {}, [], []
+    for k, (a, b) in enumerate(zip(starts, ends)):
+        path = f"{tmpdir}/golden{k}.png"
+        with open(path, "wb") as f:
+            f.write(G["png_file_bytes"][a:b].tobytes())
+        h, w = shapes[k]
+        want.append(G["png_texels"][tstart:tstart + h * w * 3])
+        tstart += h * w * 3
+        textures[f"t{k}"] = {"type": "image", "file": path, "colorspace": "linear"}
+        mats[f"m{k}"] = {"type": "diffuse", "reflectance": {"texture": f"t{k}"}}
+        objects.append({"geometry": "r", "material": f"m{k}"})
+    doc = {"textures": textures, "materials": mats, "geometry": {"r": {"type": "rect"}}, "objects": objects,
+           "cameras": {"cam": {"position": [0, 2, 3], "look_at": [0, 0, 0]}}}
+    s = es.load(host, tmpdir, "png_golden", doc)
+    for k, wk in enumerate(want):
+        off = s.spectra[s.materials[k]["albedo"]]["offset"]
+        texels = s.knots[off + 12:off + 12 + wk.size]
+        np.testing.assert_array_equal(np.round(texels * 255).astype(np.uint8), wk, err_msg=f"file {k}")
+        np.testing.assert_array_equal(texels, wk.astype(np.float32) / np.float32(255))
+
+
+@pytest.mark.skipif(not os.path.exists(REF_SILK), reason="the reference tree is not present")
+def test_reference_silk_texture_decodes_as_the_reference_decoder(host, tmpdir):
+    import hashlib
+    G = np.load(f"{os.path.dirname(os.path.abspath(__file__))}/golden/ref_components.npz")
+    doc = {"textures": {"t": {"type": "image", "file": REF_SILK, "colorspace": "linear"}},
+           "materials": {"m": {"type": "diffuse", "reflectance": {"texture": "t"}}},
+           "geometry": {"r": {"type": "rect"}}, "objects": [{"geometry": "r", "material": "m"}],
+           "cameras": {"cam": {"position": [0, 2, 3], "look_at": [0, 0, 0]}}}
+    s = es.load(host, tmpdir, "silk", doc)
+    h, w = G["silk_shape"]
+    off = s.spectra[s.materials[0]["albedo"]]["offset"]
+    texels = np.round(s.knots[off + 12:off + 12 + h * w * 3] * 255).astype(np.uint8)
+    assert hashlib.sha256(texels.tobytes()).digest() == G["silk_sha256"].tobytes()
