@@ -3,7 +3,9 @@
 // Radiance reference manual; the OpenEXR file layout document). The reference reads .hdr through
 // stb_image's float loader and .exr through tinyexr's LoadEXR (src/core/image.cpp:79-89,
 // src/core/exr_read.cpp:19-37); these restate what those produce for the files they accept:
-// RGBE texels as c * 2^(e - 136), EXR channels R, G, B by name (a one-channel file is grey).
+// RGBE texels as c * 2^(e - 136), EXR channels R, G, B by name (a one-channel file is grey; a UINT
+// channel's 32 bits are taken as a float, and a DECREASING_Y file is turned upside down, as
+// LoadEXR returns them).
 #include "image_codecs.h"
 
 #include <zlib.h>
@@ -197,7 +199,7 @@ bool DecodeEXR(const std::vector<unsigned char> &f, int *w, int *h, std::vector<
         return true;
     };
     std::vector<ExrChannel> chans;
-    int compression = -1;
+    int compression = -1, lineOrder = 0;
     int32_t box[4] = {0, 0, -1, -1};
     bool haveBox = false;
     for (;;) {
@@ -236,6 +238,8 @@ bool DecodeEXR(const std::vector<unsigned char> &f, int *w, int *h, std::vector<
             }
         } else if (name == "compression" && size >= 1) {
             compression = v[0];
+        } else if (name == "lineOrder" && size >= 1) {
+            lineOrder = v[0];
         } else if (name == "dataWindow" && type == "box2i" && size >= 16) {
             for (int k = 0; k < 4; ++k) box[k] = int32_t(LE32(v + 4 * k));
             haveBox = true;
@@ -325,7 +329,8 @@ bool DecodeEXR(const std::vector<unsigned char> &f, int *w, int *h, std::vector<
         // per scanline: each channel's W samples in turn
         size_t q = 0;
         for (int r = 0; r < rows; ++r) {
-            const size_t y = size_t(y0 - box[1] + r);
+            // LoadEXR stores a DECREASING_Y file's rows bottom up, so the reference sees it flipped
+            const size_t yr = size_t(y0 - box[1] + r), y = lineOrder == 1 ? size_t(H - 1) - yr : yr;
             for (size_t c = 0; c < chans.size(); ++c) {
                 const int t = chans[c].type;
                 for (int64_t x = 0; x < W; ++x) {
@@ -333,12 +338,9 @@ bool DecodeEXR(const std::vector<unsigned char> &f, int *w, int *h, std::vector<
                     if (t == 1) {
                         v = HalfToFloat(uint16_t(block[q] | (block[q + 1] << 8)));
                         q += 2;
-                    } else if (t == 2) {
+                    } else {  // FLOAT, and UINT as LoadEXR returns it: its 32 bits taken as a float
                         const uint32_t b = LE32(&block[q]);
                         std::memcpy(&v, &b, 4);
-                        q += 4;
-                    } else {
-                        v = float(LE32(&block[q]));
                         q += 4;
                     }
                     for (int o = 0; o < 3; ++o)

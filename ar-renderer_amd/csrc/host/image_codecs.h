@@ -18,5 +18,7 @@ bool DecodeHDR(const std::vector<unsigned char> &file, int *w, int *h, std::vect
 
 // Decodes a single-part scanline OpenEXR file (NONE, RLE, ZIPS or ZIP compression; HALF, FLOAT or
 // UINT channels) into float RGB over its data window, top row first. Channels R, G and B are read
-// by name; a file with a single channel is grey. Returns false with *err set otherwise.
+// by name; a file with a single channel is grey; a UINT channel's 32 bits are taken as a float and
+// a DECREASING_Y file comes out upside down, as tinyexr's LoadEXR (the reference's reader) returns
+// them. Returns false with *err set otherwise.
 bool DecodeEXR(const std::vector<unsigned char> &file, int *w, int *h, std::vector<float> *rgb, std::string *err);

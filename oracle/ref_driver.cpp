@@ -25,11 +25,17 @@
 
 #include "../include/prender_gpu.h"
 
-// The reference's image reader decodes PNG with stb_image asking for 3 channels
-// (src/core/image.cpp:90); the same vendored decoder, built here for the PNG golden vectors.
+// The reference's image reader decodes PNG with stb_image asking for 3 channels, Radiance HDR with
+// stb_image's float loader (src/core/image.cpp:79-101) and OpenEXR with tinyexr's LoadEXR
+// (src/core/exr_read.cpp:19-37); the same vendored decoders, built here for golden vectors.
 #define STB_IMAGE_IMPLEMENTATION
 #define STBI_ONLY_PNG
+#define STBI_ONLY_HDR
 #include "stb/stb_image.h"
+#define TINYEXR_IMPLEMENTATION
+#define TINYEXR_USE_MINIZ 1
+#define TINYEXR_USE_THREAD 0
+#include "tinyexr/tinyexr.h"
 
 #include <cstring>
 #include <memory>
@@ -131,6 +137,35 @@ void rf_cie(float l, float *xyz) {
     xyz[2] = CIE_Z(l);
 }
 float rf_cie_y_integral() { return CIE_Y_Integral(); }
+
+// stb_image's float RGB decode of a Radiance HDR image (stbi_loadf_from_memory, 3 channels);
+// returns w * h * 3 and writes the texels when they fit `cap`, -1 on failure.
+int rf_stbi_rgbf(const unsigned char *data, int len, int *w, int *h, float *out, int cap) {
+    int n = 0;
+    float *d = stbi_loadf_from_memory(data, len, w, h, &n, 3);
+    if (!d) return -1;
+    const int size = *w * *h * 3;
+    if (out && size <= cap) std::memcpy(out, d, sizeof(float) * size_t(size));
+    stbi_image_free(d);
+    return size;
+}
+
+// tinyexr's LoadEXR decode as the reference keeps it (R, G, B of its RGBA output); returns
+// w * h * 3 and writes the texels when they fit `cap`, -1 on failure.
+int rf_exr_rgb(const unsigned char *data, int len, int *w, int *h, float *out, int cap) {
+    float *rgba = nullptr;
+    const char *msg = nullptr;
+    if (LoadEXRFromMemory(&rgba, w, h, data, size_t(len), &msg) != TINYEXR_SUCCESS) {
+        if (msg) FreeEXRErrorMessage(msg);
+        return -1;
+    }
+    const int size = *w * *h * 3;
+    if (out && size <= cap)
+        for (int i = 0; i < *w * *h; ++i)
+            for (int c = 0; c < 3; ++c) out[3 * i + c] = rgba[4 * i + c];
+    free(rgba);
+    return size;
+}
 
 // stb_image's 8-bit RGB decode of an encoded image (stbi_load_from_memory with 3 channels, as
 // ReadImage asks for); returns w * h * 3 and writes the texels when they fit `cap`, -1 on failure.

@@ -34,6 +34,8 @@ def load_ref():
         ("rf_cie_y_integral", C.c_float, []),
         ("rf_rgb_fit_tables", None, [vp, vp, vp]),
         ("rf_stbi_rgb8", C.c_int, [vp, C.c_int, vp, vp, vp, C.c_int]),
+        ("rf_stbi_rgbf", C.c_int, [vp, C.c_int, vp, vp, vp, C.c_int]),
+        ("rf_exr_rgb", C.c_int, [vp, C.c_int, vp, vp, vp, C.c_int]),
         ("rf_to_xyz", None, [vp, C.c_float, C.c_int, vp]),
         ("rf_spectrum_eval", C.c_float, [vp, vp, C.c_float]),
         ("rf_spectrum_y", C.c_float, [vp, vp]),
@@ -337,6 +339,7 @@ def main():
     shading_perturbation(L, out)
     rgb_fit_tables(L, out)
     png_decodes(L, out)
+    float_image_decodes(L, out)
 
     path = os.path.join(HERE, "ref_components.npz")
     np.savez_compressed(path, **out)
@@ -386,6 +389,58 @@ def png_decodes(L, out):
     t = stb_rgb8(L, silk)
     out["silk_shape"] = np.array(t.shape[:2], np.int32)
     out["silk_sha256"] = np.frombuffer(hashlib.sha256(t.tobytes()).digest(), np.uint8)
+
+
+def ref_float_decode(L, fn, data):
+    w, h = C.c_int(), C.c_int()
+    n = fn(data, len(data), C.byref(w), C.byref(h), None, 0)
+    assert n > 0, "the reference decoder refused the file"
+    buf = np.zeros(n, np.float32)
+    fn(data, len(data), C.byref(w), C.byref(h), ptr(buf), n)
+    return buf, (h.value, w.value)
+
+
+def float_image_decodes(L, out):
+    """Radiance HDR files (run-length coded, flat, narrow) decoded by the reference's stb_image
+    float loader, and scanline OpenEXR files (NONE / RLE / ZIPS / ZIP; HALF / FLOAT / UINT; RGB,
+    RGBA and one grey channel; a data window off the origin; both line orders) decoded by its
+    tinyexr LoadEXR: the files (written by the test writers from the formats' descriptions) and
+    the float RGB texels."""
+    import tempfile
+    sys.path.insert(0, os.path.join(HERE, ".."))
+    import texscenes as ts
+    rng = np.random.default_rng(12)
+    for tag, fn in (("hdr", L.rf_stbi_rgbf), ("exr", L.rf_exr_rgb)):
+        files, texels, shapes = [], [], []
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "m." + tag)
+            if tag == "hdr":
+                cases = [((6, 13), True), ((5, 7), True), ((4, 9), False), ((3, 40), True)]
+                for (h, w), rle in cases:
+                    rgb = (rng.random((h, w, 3)) ** 3 * 80).astype(np.float32)
+                    ts.write_hdr(path, rgb, rle=rle)
+                    data = open(path, "rb").read()
+                    t, shp = ref_float_decode(L, fn, data)
+                    files.append(np.frombuffer(data, np.uint8)), texels.append(t), shapes.append(shp)
+            else:
+                for comp in (0, 1, 2, 3):
+                    for dtype, layout in ((np.float16, "RGB"), (np.float32, "RGBA"), (np.uint32, "RGB"),
+                                          (np.float16, "Y")):
+                        h, w = 21, 11
+                        base = rng.random((h, w)) * 4
+                        base[3:9] = 0.75
+                        if dtype == np.uint32:
+                            chans = {c: (base * (k + 1) * 1000).astype(np.uint32) for k, c in enumerate(layout)}
+                        else:
+                            chans = {c: (base * (k + 1)).astype(dtype) for k, c in enumerate(layout)}
+                        ts.write_exr(path, chans, comp, window=(2, -3), decreasing=comp % 2 == 1)
+                        data = open(path, "rb").read()
+                        t, shp = ref_float_decode(L, fn, data)
+                        files.append(np.frombuffer(data, np.uint8)), texels.append(t), shapes.append(shp)
+        out[f"{tag}_file_bytes"] = np.concatenate(files)
+        out[f"{tag}_file_ends"] = np.cumsum([len(f) for f in files])
+        out[f"{tag}_texels"] = np.concatenate(texels)
+        out[f"{tag}_shapes"] = np.array(shapes, np.int32)
 
 
 def rgb_fit_tables(L, out):

@@ -8,6 +8,8 @@ and what the mathematics fixes for the path loop under an image environment:
 - a reflectance-1 diffuse sphere under a constant white map is a white furnace: mean Y = 1 within
   1%, which pins the map's importance sampling and its pdf in the MIS weights.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -194,7 +196,8 @@ def test_openexr_maps(host, tmpdir, compression, dtype, layout):
     """Single-part scanline OpenEXR written by the file layout (channels in name order, NONE /
     RLE / ZIPS / ZIP with the byte predictor and interleave, 16-line ZIP blocks, a data window
     away from the origin, bottom-first chunk order for half the cases): the map's texels are the
-    R, G, B channels (a lone channel as grey), exactly."""
+    R, G, B channels (a lone channel as grey; UINT bits read as floats; a DECREASING_Y file upside
+    down, as the reference's reader returns it), exactly."""
     rng = np.random.default_rng(compression * 10 + len(layout))
     h, w = 37, 13
     base = rng.random((h, w)) * 4
@@ -206,7 +209,11 @@ def test_openexr_maps(host, tmpdir, compression, dtype, layout):
     name = f"e{compression}{np.dtype(dtype).char}{layout}"
     ts.write_exr(f"{tmpdir}/{name}.exr", chans, compression, window=(3, -2), decreasing=compression % 2 == 1)
     got = _map_texels(host, tmpdir, name, f"{tmpdir}/{name}.exr")
-    want = np.stack([chans["Y" if layout == "Y" else c].astype(np.float32) for c in "RGB"], axis=2)
+    # a UINT channel's bits are taken as a float, as the reference's LoadEXR returns them
+    as_float = (lambda a: a.view(np.float32)) if dtype == np.uint32 else (lambda a: a.astype(np.float32))
+    want = np.stack([as_float(chans["Y" if layout == "Y" else c]) for c in "RGB"], axis=2)
+    if compression % 2 == 1:  # DECREASING_Y: the reference's LoadEXR returns it upside down
+        want = want[::-1]
     np.testing.assert_array_equal(got, want)
 
 
@@ -226,3 +233,24 @@ def test_openexr_and_hdr_errors(host, tmpdir):
     open(f"{tmpdir}/bad.hdr", "wb").write(b"#?RADIANCE\n\n+Y 4 +X 4\n" + bytes(64))
     with pytest.raises(RuntimeError, match="layout"):
         es.load(host, tmpdir, "badhdr", es.env_doc(f"{tmpdir}/bad.hdr", objects=False))
+
+
+@pytest.mark.parametrize("tag", ["hdr", "exr"])
+def test_float_maps_equal_the_reference_decoders(host, tmpdir, tag):
+    """The reference reads .hdr with its vendored stb_image float loader and .exr with its vendored
+    tinyexr LoadEXR (src/core/image.cpp:79-89, src/core/exr_read.cpp:19-37). The golden vectors
+    hold those decoders' texels for Radiance files (run-length coded, flat, narrow) and scanline
+    OpenEXR files (NONE / RLE / ZIPS / ZIP, HALF / FLOAT / UINT, RGB / RGBA / one grey channel,
+    an offset data window, both line orders): the maps' texels equal them bit for bit."""
+    G = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_components.npz"))
+    ends, shapes = G[f"{tag}_file_ends"], G[f"{tag}_shapes"]
+    starts = np.concatenate([[0], ends[:-1]])
+    t0 = 0
+    for k, (a, b) in enumerate(zip(starts, ends)):
+        path = f"{tmpdir}/golden{k}.{tag}"
+        with open(path, "wb") as f:
+            f.write(G[f"{tag}_file_bytes"][a:b].tobytes())
+        h, w = shapes[k]
+        want = G[f"{tag}_texels"][t0:t0 + h * w * 3].reshape(h, w, 3)
+        t0 += h * w * 3
+        np.testing.assert_array_equal(_map_texels(host, tmpdir, f"g{tag}{k}", path), want, err_msg=f"{tag} file {k}")
