@@ -1622,7 +1622,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(Bufs b) {
             const int i = i0 + int(threadIdx.x);
             const int bin = i < hi ? f * kSortBins + (b.qk[off + i] & (kSortBins - 1)) : -1;
             const int pos = lds_append(hist, bin);
-            if (bin >= 0) b.qms[off + pos] = b.qm[off + i];
+            if (bin >= 0 && pos < n) b.qms[off + pos] = b.qm[off + i];  // positions stay in the family's queue
         }
     }
 }
@@ -1688,7 +1688,7 @@ __global__ __launch_bounds__(kBlock) void k_ray_sort_scatter(Bufs b, const int *
         const int i = i0 + int(threadIdx.x);
         const int bin = i < hi ? (b.nk[i] & (kRaySortBins - 1)) : -1;
         const int pos = lds_append(hist, bin);
-        if (bin >= 0) b.qr[pos] = src[i];
+        if (bin >= 0 && pos < n) b.qr[pos] = src[i];  // positions stay in the queue
     }
 }
 
