@@ -404,7 +404,8 @@ PG_HD void tex_rgb(const float *rec, float u, float v, float out[3]) {
 PG_HD float tex_float(const float *rec, int channel, float u, float v) {
     float c[3];
     tex_rgb(rec, u, v, c);
-    const float x = channel < 0 ? (c[0] + c[1] + c[2]) / 3 : c[channel];
+    // a select rather than c[channel]: a run-time index would put c in scratch memory
+    const float x = channel < 0 ? (c[0] + c[1] + c[2]) / 3 : (channel == 0 ? c[0] : (channel == 1 ? c[1] : c[2]));
     return x * rec[8];
 }
 

@@ -166,59 +166,62 @@ PG_HD bool prim_has_normals(const prgpu::Primitive &p) {
              p.n2.y == 0 && p.n2.z == 0);
 }
 
+// Both branches leave their results in locals and the record is written once at the end: stores
+// of different fields from the two branches, merged at the join, would make the compiler keep the
+// record in scratch memory.
 PG_HD void interaction(const prgpu::Primitive &pr, V3 ro, V3 rd, float tHit, float b1, float b2, Surf *si) {
+    V3 p, n, ns, dpdu, dpdv;
+    float u, v;
     if (pr.type == prgpu::PrimSphere) {
         V3 c = f3(pr.p0);
         float r = pr.r;
         V3 pHit = ro + rd * tHit;
         V3 rel = pHit - c;
         float l = len(rel);
-        V3 n = l > 0 ? rel / l : V3(0, 1, 0);
-        pHit = c + n * r;
-        float phi = atan2f(n.z, n.x);
+        V3 nn = l > 0 ? rel / l : V3(0, 1, 0);
+        p = c + nn * r;
+        float phi = atan2f(nn.z, nn.x);
         if (phi < 0) phi += 2 * kPi;
-        float theta = safe_acos(n.y);
-        si->p = pHit;
-        si->n = pr.flip ? -n : n;
-        si->u = phi * kInv2Pi;
-        si->v = theta * kInvPi;
-        si->wo = -normalize(rd);
-        V3 dpdu = V3(-rel.z, 0, rel.x) * (2 * kPi);
-        float cosT = n.y, sinT = safe_sqrt(1 - cosT * cosT);
-        V3 dpdv = sinT > 1e-6f ? V3(cosT * rel.x / sinT, -r * sinT, cosT * rel.z / sinT) * kPi : V3();
-        if (len2(dpdu) < 1e-12f || len2(dpdv) < 1e-12f) coordinate_system(n, &dpdu, &dpdv);
-        si->ns = si->n;
-        si->dpdus = dpdu;
-        si->dpdv = dpdv;
-        return;
+        float theta = safe_acos(nn.y);
+        n = pr.flip ? -nn : nn;
+        u = phi * kInv2Pi;
+        v = theta * kInvPi;
+        dpdu = V3(-rel.z, 0, rel.x) * (2 * kPi);
+        float cosT = nn.y, sinT = safe_sqrt(1 - cosT * cosT);
+        dpdv = sinT > 1e-6f ? V3(cosT * rel.x / sinT, -r * sinT, cosT * rel.z / sinT) * kPi : V3();
+        if (len2(dpdu) < 1e-12f || len2(dpdv) < 1e-12f) coordinate_system(nn, &dpdu, &dpdv);
+        ns = n;
+    } else {
+        V3 p0 = f3(pr.p0), p1 = f3(pr.p1), p2 = f3(pr.p2);
+        float b0 = 1 - b1 - b2;
+        p = p0 * b0 + p1 * b1 + p2 * b2;
+        n = normalize(cross_f(p1 - p0, p2 - p0));
+        u = pr.u0 * b0 + pr.u1 * b1 + pr.u2 * b2;
+        v = pr.v0 * b0 + pr.v1 * b1 + pr.v2 * b2;
+        float du02 = pr.u0 - pr.u2, dv02 = pr.v0 - pr.v2, du12 = pr.u1 - pr.u2, dv12 = pr.v1 - pr.v2;
+        V3 dp02 = p0 - p2, dp12 = p1 - p2;
+        float det = du02 * dv12 - dv02 * du12;
+        bool degenerate = fabsf(det) < 1e-9f;
+        if (!degenerate) {
+            float inv = 1 / det;
+            dpdu = (dp02 * dv12 - dp12 * dv02) * inv;
+            dpdv = (dp12 * du02 - dp02 * du12) * inv;
+            degenerate = len2(cross_d(dpdu, n)) == 0 || !is_finite(dpdu.x) || !is_finite(dpdv.x);
+        }
+        if (degenerate) coordinate_system(n, &dpdu, &dpdv);
+        ns = n;
+        if (prim_has_normals(pr)) {
+            ns = f3(pr.n0) * b0 + f3(pr.n1) * b1 + f3(pr.n2) * b2;
+            float l = len(ns);
+            ns = l > 0 ? ns / l : n;
+            n = face_forward(n, ns);
+        }
     }
-    V3 p0 = f3(pr.p0), p1 = f3(pr.p1), p2 = f3(pr.p2);
-    float b0 = 1 - b1 - b2;
-    si->p = p0 * b0 + p1 * b1 + p2 * b2;
-    V3 n = normalize(cross_f(p1 - p0, p2 - p0));
-    si->u = pr.u0 * b0 + pr.u1 * b1 + pr.u2 * b2;
-    si->v = pr.v0 * b0 + pr.v1 * b1 + pr.v2 * b2;
-    float du02 = pr.u0 - pr.u2, dv02 = pr.v0 - pr.v2, du12 = pr.u1 - pr.u2, dv12 = pr.v1 - pr.v2;
-    V3 dp02 = p0 - p2, dp12 = p1 - p2;
-    float det = du02 * dv12 - dv02 * du12;
-    V3 dpdu, dpdv;
-    bool degenerate = fabsf(det) < 1e-9f;
-    if (!degenerate) {
-        float inv = 1 / det;
-        dpdu = (dp02 * dv12 - dp12 * dv02) * inv;
-        dpdv = (dp12 * du02 - dp02 * du12) * inv;
-        degenerate = len2(cross_d(dpdu, n)) == 0 || !is_finite(dpdu.x) || !is_finite(dpdv.x);
-    }
-    if (degenerate) coordinate_system(n, &dpdu, &dpdv);
-    V3 ns = n;
-    if (prim_has_normals(pr)) {
-        ns = f3(pr.n0) * b0 + f3(pr.n1) * b1 + f3(pr.n2) * b2;
-        float l = len(ns);
-        ns = l > 0 ? ns / l : n;
-        n = face_forward(n, ns);
-    }
+    si->p = p;
     si->n = n;
     si->ns = ns;
+    si->u = u;
+    si->v = v;
     si->dpdus = dpdu;
     si->dpdv = dpdv;
     si->wo = -normalize(rd);
