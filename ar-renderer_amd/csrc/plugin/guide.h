@@ -47,16 +47,23 @@ PG_HD constexpr int guide_boff(int l) { return guide_woff(l) + guide_rows(l) * k
 constexpr int kGParams = guide_woff(kGLayers);       // 19760 (19305 used)
 constexpr int kGWeights = (4 * kGHidden + kGOutRows) * kGIn;  // bf16 weights, all layers
 
-// Feature f (0..63) of a vertex: sin / cos of 2^j pi x on each normalised axis (48), the normal
-// and the outgoing direction (6), zeros (10).
+// Feature f (0..63) of a vertex: sin / cos of 2^j pi x on each normalised axis (48), then the
+// normal's encoding (16): the normal, sin / cos of pi n on each axis, sin / cos of 2 pi n on each
+// axis, one zero. The incident radiance a diffuse vertex samples depends on where it is and how it
+// faces, not on the direction the path arrived from.
 __device__ __forceinline__ float guide_feature(int f, const float *pos, const float *nrm, const float *wo) {
+    (void)wo;
     if (f < 48) {
         const int axis = f >> 4, j = (f & 15) >> 1;
         const float arg = pos[axis] * float(1 << j);
         return (f & 1) ? cospif(arg) : sinpif(arg);
     }
     if (f < 51) return nrm[f - 48];
-    if (f < 54) return wo[f - 51];
+    if (f < 63) {
+        const int g = f - 51, axis = (g % 6) >> 1;
+        const float arg = nrm[axis] * (g < 6 ? 1.f : 2.f);
+        return (g & 1) ? cospif(arg) : sinpif(arg);
+    }
     return 0.f;
 }
 

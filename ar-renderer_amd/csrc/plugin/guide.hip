@@ -106,9 +106,11 @@ __device__ __forceinline__ s16x8 pack8(f32x4 a, f32x4 b) {
 }
 
 // Features of this lane's vertex in the K order: B[0] = axis 0 then axis 1 frequencies 2q, 2q + 1
-// (sin, cos each), B[1] = axis 2 frequencies, then the normal and outgoing direction (q = 0, 1) or
-// zeros. Feature f of the model (guide.h) is element (f & 3) + 4 ((f >> 4) & 1) of block f >> 5.
+// (sin, cos each), B[1] = axis 2 frequencies, then features 48 + 4q .. 51 + 4q of the normal's
+// encoding (guide.h: n, sin / cos of pi n per axis, sin / cos of 2 pi n per axis, a zero).
+// Feature f of the model is element (f & 3) + 4 ((f >> 4) & 1) of block f >> 5.
 __device__ __forceinline__ void features32(const float *pos, const float *nrm, const float *wo, int lane, s16x8 x[2]) {
+    (void)wo;
     const int q = lane >> 4;
     const float f0 = float(1 << (2 * q)), f1 = 2.f * f0;
     float v[3][4];
@@ -117,9 +119,17 @@ __device__ __forceinline__ void features32(const float *pos, const float *nrm, c
         sincospif(pos[a] * f0, &v[a][0], &v[a][1]);
         sincospif(pos[a] * f1, &v[a][2], &v[a][3]);
     }
-    const bool g0 = q == 0, g1 = q == 1;
-    const float e[4] = {g0 ? nrm[0] : (g1 ? wo[1] : 0.f), g0 ? nrm[1] : (g1 ? wo[2] : 0.f), g0 ? nrm[2] : 0.f,
-                        g0 ? wo[0] : 0.f};
+    float s1[3], c1[3], s2[3], c2[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        sincospif(nrm[a], &s1[a], &c1[a]);
+        sincospif(2.f * nrm[a], &s2[a], &c2[a]);
+    }
+    // features 48..63: n0 n1 n2 s1[0] | c1[0] s1[1] c1[1] s1[2] | c1[2] s2[0] c2[0] s2[1] | c2[1] s2[2] c2[2] 0
+    const float e[4] = {q == 0 ? nrm[0] : (q == 1 ? c1[0] : (q == 2 ? c1[2] : c2[1])),
+                        q == 0 ? nrm[1] : (q == 1 ? s1[1] : (q == 2 ? s2[0] : s2[2])),
+                        q == 0 ? nrm[2] : (q == 1 ? c1[1] : (q == 2 ? c2[0] : c2[2])),
+                        q == 0 ? s1[0] : (q == 1 ? s1[2] : (q == 2 ? s2[1] : 0.f))};
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         x[0][i] = f2bf(v[0][i]);

@@ -6,7 +6,8 @@
 //
 //   features (64): position normalised to the scene box, frequency-encoded
 //                  (sin, cos of 2^f pi x for f = 0..7 on each axis: 48 values), the geometric
-//                  normal (3), the outgoing direction (3), 10 zeros
+//                  normal's encoding (16: the normal, sin / cos of pi n per axis, sin / cos of
+//                  2 pi n per axis, a zero)
 //   network:       4 hidden layers of 64 ReLU units, one linear output layer of 41 values
 //   outputs:       8 von Mises-Fisher lobes (direction 3, concentration 1, weight logit 1)
 //                  and the probability s of sampling the guide instead of the BSDF
@@ -75,7 +76,13 @@ void Features(const float *rec, double *f) {
             f[k++] = std::sin(arg);
             f[k++] = std::cos(arg);
         }
-    for (int i = 0; i < 6; ++i) f[k++] = rec[3 + i];
+    for (int i = 0; i < 3; ++i) f[k++] = rec[3 + i];
+    for (int m = 1; m <= 2; ++m)
+        for (int a = 0; a < 3; ++a) {
+            double arg = kPi * m * double(rec[3 + a]);
+            f[k++] = std::sin(arg);
+            f[k++] = std::cos(arg);
+        }
     while (k < kIn) f[k++] = 0;
 }
 
