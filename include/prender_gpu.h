@@ -434,7 +434,8 @@ PRGPU_API int prgpu_guide_param_count();
 // Reads (write = 0) or replaces (write = 1; also resets the optimiser) the device's model.
 PRGPU_API int prgpu_guide_params(int deviceIndex, float *params, int count, int write);
 // The network's 41 raw outputs for n records of 9 floats (position normalised to the scene box,
-// normal, outgoing direction), evaluated with the BF16 matrix-core kernel.
+// normal, outgoing direction; the network's inputs are the position's and the normal's encodings,
+// the direction is kept in the record but not read), evaluated with the BF16 matrix-core kernel.
 PRGPU_API int prgpu_guide_eval(int deviceIndex, const float *records, int n, float *out41);
 // Gradient of the normalised training loss over n records of 16 floats (position, normal, wo,
 // sampled direction, p_bsdf, weight, 2 unused) into grad (param_count floats); stats = {summed
